@@ -1,0 +1,578 @@
+"""GPU eikonal solver driver: the counterpart of agd/Eikonal/HFM_CUDA.
+
+RunGPU(hfmIn) reads the dictIn keys (with defaults and help, as
+_Interface.py:80-122), prepares the tiled device arrays, calls the HIP
+library (libagd_hip.so) and assembles hfmOut.  Pipeline order follows
+_Interface.py:128-146:
+    SetKernelTraits -> SetGeometry -> SetArgs -> Solve('eikonal')
+    -> PostProcess (flow) -> SolveAD (reverse) -> FinalCheck.
+
+Device buffers are torch tensors on the current HIP device (allocation and
+stream plumbing only); all computation happens in the HIP kernels.
+"""
+import numbers
+import time
+from collections import OrderedDict
+
+import numpy as np
+
+from .. import FiniteDifferences as fd
+from .. import Metrics
+from .. import _lib
+from ._device import (torch, device_tensor, host_array, ptr, stream_handle, empty,
+                      full, zeros)
+
+_MODELS = {'Isotropic': _lib.ISOTROPIC, 'Diagonal': _lib.DIAGONAL,
+           'Riemann': _lib.RIEMANN, 'Rander': _lib.RANDER}
+
+_SOLVER_NAMES = {'AGSI': 'adaptive_gauss_siedel_iteration', 'FIM': 'fast_iterative_method'}
+_SOLVER_IDS = {'adaptive_gauss_siedel_iteration': _lib.AGSI,
+               'fast_iterative_method': _lib.FIM, 'global_iteration': _lib.GLOBAL}
+
+
+def default_traits(model):
+    """Tile shape, in-tile sweeps and FIM front width per model
+    (kernel_traits.py:6-137)."""
+    table = {
+        'Isotropic2': ((16, 16), 16, 4), 'Diagonal2': ((16, 16), 16, 4),
+        'Isotropic3': ((4, 4, 4), 8, 4), 'Diagonal3': ((4, 4, 4), 8, 4),
+        'Riemann2': ((8, 8), 8, 4), 'Riemann3': ((4, 4, 4), 4, 6),
+        'Rander2': ((8, 8), 12, 8), 'Rander3': ((4, 4, 4), 8, 8),
+    }
+    if model not in table:
+        raise NotImplementedError(
+            f"Model {model} is not provided by the MI355X eikonal solver "
+            f"(supported: {sorted(table)})")
+    shape_i, niter_i, fim = table[model]
+    traits = {'Scalar': np.float32, 'Int': np.int32, 'OffsetT': np.int32,
+              'shape_i': shape_i, 'niter_i': niter_i}
+    return traits, fim
+
+
+def RunGPU(hfmIn, *args, cache=None, host_outputs=False, **kwargs):
+    """Run the GPU eikonal solver (HFM_CUDA/__init__.py:10-25).
+
+    Set hfmIn['verbosity'] (0, 1, 2) to control the printed information;
+    hfmOut['keys']['help'] documents every key read.
+    """
+    if cache is not None:
+        print("Warning : gpu eikonal solver does not support caching")
+    return Interface(hfmIn, host_outputs=host_outputs).Run(*args, **kwargs)
+
+
+class Interface:
+    """Carries out RunGPU (_Interface.py:23-174).  Not meant for direct use."""
+
+    def __init__(self, hfmIn, host_outputs=False):
+        self.hfmIn = hfmIn
+        if hfmIn['arrayOrdering'] != 'RowMajor':
+            raise ValueError("Only RowMajor indexing supported")
+        self.host_outputs = host_outputs
+        self.hfmOut = {'keys': {
+            'used': ['origin', 'arrayOrdering', 'dims', 'mode', 'projective'],
+            'default': OrderedDict(), 'visited': [], 'help': OrderedDict(),
+            'kernelStats': OrderedDict()}}
+        self.verbosity = 1
+        self.verbosity = self.GetValue('verbosity', default=1,
+                                       help="Choose the amount of detail displayed on the run")
+        self.model = self.GetValue('model', help='Minimal path model to be solved.')
+        self.ndim = len(hfmIn['dims'])
+        self.stats = {key: {} for key in ('eikonal', 'flow', 'reverseAD')}
+
+    @property
+    def model_(self):
+        return self.model[:-1]
+
+    def HasValue(self, key):
+        self.hfmOut['keys']['visited'].append(key)
+        return key in self.hfmIn
+
+    def GetValue(self, key, default="_None", verbosity=2, help="Sorry : no help for this key"):
+        """Value of a key, recording its default and help (_Interface.py:80-122)."""
+        assert key not in self.hfmOut['keys']['help']
+        self.hfmOut['keys']['help'][key] = help
+        self.hfmOut['keys']['default'][key] = default
+        if key in self.hfmIn:
+            self.hfmOut['keys']['used'].append(key)
+            return self.hfmIn[key]
+        if isinstance(default, str) and default == "_None":
+            raise ValueError(f"Missing value for key {key}")
+        if verbosity <= self.verbosity:
+            if isinstance(default, str) and default == "_Dummy":
+                print(f"see out['keys']['default'][{key}] for default")
+            else:
+                print(f"key {key} defaults to {default}")
+        return default
+
+    def Warn(self, msg):
+        if self.verbosity >= -1:
+            print("---- Warning ----\n", msg, "\n-----------------\n")
+
+    def Run(self):
+        self.SetKernelTraits()
+        self.SetGeometry()
+        self.SetArgs()
+        self.Solve()
+        self.PostProcess()
+        self.SolveAD()
+        self.FinalCheck()
+        if self.extractValues or self.retself:
+            retval = [self.hfmOut]
+            if self.extractValues:
+                retval.insert(0, self.values)
+            if self.retself:
+                retval.append(self)
+            return retval
+        return self.hfmOut
+
+    # ------------------------------------------------------------ traits
+    def SetKernelTraits(self):
+        """_Kernel.py:23-92."""
+        if self.verbosity >= 1:
+            print("Setting the kernel traits.")
+        if self.model_ not in _MODELS or self.ndim not in (2, 3) or int(self.model[-1]) != self.ndim:
+            default_traits(self.model)  # raises with the list of models
+        traits, fim_default = default_traits(self.model)
+        user = self.GetValue('traits', default=traits,
+                             help="Optional trait parameters for the eikonal kernel.")
+        traits = {**traits, **(user or {})}
+        self.traits = traits
+        for key, name in (('multiprecision', 'multiprecision'), ('values_float64', 'values_float64')):
+            if self.GetValue(key, default=False, help=f"{name} arithmetic (not provided)"):
+                raise NotImplementedError(f"'{key}' is not provided by the MI355X solver")
+        if self.GetValue('factoringRadius', default=0, help="Source factorization (not provided)"):
+            raise NotImplementedError("source factorization is not provided by the MI355X solver")
+        self.order = self.GetValue('order', default=1, help="Scheme order (1 only)")
+        if self.order != 1:
+            raise NotImplementedError("only the first order scheme is provided")
+        if self.GetValue('bound_active_blocks', default=False,
+                         help="Limit the number of active blocks (not provided)"):
+            raise NotImplementedError("bound_active_blocks is not provided by the MI355X solver")
+
+        self.float_t = np.dtype(traits['Scalar']).type
+        if self.float_t not in (np.float32, np.float64):
+            raise ValueError(f"Unsupported Scalar trait {traits['Scalar']}")
+        self.scalar = _lib.F32 if self.float_t == np.float32 else _lib.F64
+        self.shape_i = tuple(int(s) for s in traits['shape_i'])
+        self.niter_i = int(traits['niter_i'])
+        if len(self.shape_i) != self.ndim:
+            raise ValueError(f"shape_i {self.shape_i} does not match dimension {self.ndim}")
+        if not _lib.lib.agd_tile_supported(_MODELS[self.model_], self.ndim, _lib.i32(self.shape_i)):
+            raise ValueError(f"No kernel compiled for tile shape {self.shape_i}")
+        self.size_i = int(np.prod(self.shape_i))
+
+        solver = 'FIM' if (self.HasValue('fim_front_width') or fim_default) else 'AGSI'
+        solver = self.GetValue('solver', default=solver,
+                               help="Choice of fixed point solver (AGSI, FIM, global_iteration)")
+        self.solver = _SOLVER_NAMES.get(solver, solver)
+        if self.solver not in _SOLVER_IDS:
+            raise ValueError(f"Unrecognized solver : {solver}")
+        self.fim_front_width = fim_default
+        if self.solver == 'fast_iterative_method':
+            self.fim_front_width = self.GetValue(
+                'fim_front_width', default=fim_default,
+                help="Dictates the max front width in the FIM variant.\n(original FIM : 2. Must be >=2.)")
+            if self.fim_front_width is None or self.fim_front_width < 2:
+                raise ValueError("Please specify an fim_front_width (integer >=2)")
+
+    # ------------------------------------------------------------ geometry
+    def SetGeometry(self):
+        """_SetGeometry.py:16-282 (metric, cost, tolerances, walls)."""
+        if self.verbosity >= 1:
+            print("Preparing the domain data (shape,metric,...)")
+        self.shape = self.hfmIn.shape
+        self.periodic = tuple(bool(p) for p in self.GetValue(
+            'periodic', default=(False,) * self.ndim,
+            help="Apply periodic boundary conditions on some axes"))
+        self.shape_o = tuple(int(s) for s in fd.round_up_ratio(self.shape, self.shape_i))
+        self.ntiles = int(np.prod(self.shape_o))
+        self.size_tot = self.ntiles * self.size_i
+        if 'gridScale' in self.hfmIn:
+            self.GetValue('gridScale', help="Scale of the computational grid")
+        else:
+            self.GetValue('gridScales', help="Axis independent scales of the computational grid")
+        self.h = np.asarray(host_array(self.hfmIn.gridScales), dtype=np.float32)
+        self.stream = stream_handle()
+        self.modelId = _MODELS[self.model_]
+
+        # Metric -> tiled geometry (computed on the device).
+        self.iso_weights = (0., 0., 0.)
+        self.geom = None
+        if self.model_ == 'Isotropic':
+            w = (self.h.astype(np.float32) ** -2).astype(np.float64)
+            self.iso_weights = tuple(w.tolist()) + (0.,) * (3 - self.ndim)
+        else:
+            metric = self.GetValue('metric', default=None, verbosity=3,
+                                   help="Metric of the minimal path model")
+            dualMetric = self.GetValue('dualMetric', default=None, verbosity=3,
+                                       help="Dual metric of the minimal path model")
+            self._set_metric_geometry(metric, dualMetric)
+
+        self.exportGeodesicFlow = self.GetValue(
+            'exportGeodesicFlow', default=False,
+            help="Export the upwind geodesic flow (direction of the geodesics)")
+        tips = self.GetValue('tips', default=None, help="Tips from which to compute the minimal geodesics")
+        if tips is not None:
+            raise NotImplementedError("geodesic backtracking ('tips') is not provided yet")
+
+        # Cost function
+        if self.HasValue('speed'):
+            speed = self.GetValue('speed', help="speed = 1/cost (scales the metric)")
+            self.cost = 1. / speed
+        else:
+            self.cost = self.GetValue('cost', default=None, help="cost = 1/speed (scales the metric)")
+            if self.cost is None:
+                self.cost = np.ones((), dtype=np.float32)
+        if self.GetValue('costVariation', default=None,
+                         help="First order variation of the cost function (not provided)") is not None:
+            raise NotImplementedError("forward AD ('costVariation') is not provided yet")
+
+        # Tolerances (_SetGeometry.py:237-253)
+        res = np.finfo(self.float_t).resolution
+        if self.HasValue('atol') and self.HasValue('rtol'):
+            tol = None
+        else:
+            tol = self.GetValue('tol', default="_Dummy",
+                                help="Convergence tolerance for the fixed point solver (determines atol, rtol)")
+            if isinstance(tol, str) and tol == "_Dummy":
+                c = self.cost
+                mean_cost = float(torch.nanmean(c.float()).item()) if hasattr(c, 'detach') \
+                    else float(np.nanmean(np.asarray(c, dtype=np.float64)))
+                tol = mean_cost * res * 5.
+                self.hfmOut['keys']['default']['tol'] = self.float_t(tol)
+        self.atol = float(self.GetValue('atol', default=tol,
+                                        help="Absolute convergence tolerance for the fixed point solver"))
+        self.rtol = float(self.GetValue('rtol', default=float(res * 5.),
+                                        help="Relative convergence tolerance for the fixed point solver"))
+
+        # Walls
+        self.walls = self.GetValue('walls', default=None, help='Obstacles in the domain')
+        self.wallDistBound = int(self.GetValue(
+            'wallDistBound', default=10,
+            help="Bound on the computed distance to the obstacles.\n(Ideally a sharp upper bound on the stencil width.)"))
+
+    def _set_metric_geometry(self, metric, dualMetric):
+        model = self.model_
+        cls = {'Diagonal': Metrics.Diagonal, 'Riemann': Metrics.Riemann, 'Rander': Metrics.Rander}[model]
+        if metric is None and dualMetric is None:
+            raise ValueError(f"Missing value for key metric (model {self.model})")
+        if metric is not None and not isinstance(metric, Metrics.Base):
+            metric = cls.from_HFM(host_array(metric))
+        if metric is None:
+            if not isinstance(dualMetric, Metrics.Base):
+                dualMetric = cls.from_HFM(host_array(dualMetric))
+            metric = dualMetric.dual()
+        if model == 'Diagonal':
+            m = metric.costs if isinstance(metric, Metrics.Diagonal) else None
+            if m is None:
+                raise ValueError("Diagonal model expects a Metrics.Diagonal metric")
+            arr, w = m, None
+            core = (self.ndim,)
+        elif model == 'Riemann':
+            metric = metric if isinstance(metric, Metrics.Riemann) else Metrics.Riemann.from_cast(metric)
+            arr, w = metric.m, None
+            core = (self.ndim, self.ndim)
+        else:
+            metric = metric if isinstance(metric, Metrics.Rander) else Metrics.Rander.from_cast(metric)
+            arr, w = metric.m, metric.w
+            core = (self.ndim, self.ndim)
+        self.metric = metric
+        arr_d, stride = self._field_tensor(arr, core, 'metric')
+        w_d = None
+        if w is not None:
+            w_d, wstride = self._field_tensor(w, (self.ndim,), 'metric drift')
+            if wstride != stride:  # broadcast the constant one
+                if stride == 1:
+                    w_d = w_d.reshape(self.ndim, 1).expand(self.ndim, int(np.prod(self.shape))).contiguous()
+                else:
+                    arr_d = arr_d.reshape(-1, 1).expand(arr_d.numel(), int(np.prod(self.shape))).contiguous()
+                    stride = 1
+        ng = {'Diagonal': self.ndim, 'Riemann': self.ndim * (self.ndim + 1) // 2,
+              'Rander': self.ndim * (self.ndim + 1) // 2 + self.ndim}[model]
+        self.geom = empty((ng, self.size_tot), self.float_t)
+        h = (ctypes_double3(self.h))
+        _lib.check(_lib.lib.agd_prepare_geometry(
+            self.modelId, self.ndim, _lib.i64(self.shape), _lib.i32(self.shape_i), self.scalar,
+            h, ptr(arr_d), ptr(w_d) if w_d is not None else None, stride, ptr(self.geom),
+            self.stream))
+        self._keep = (arr_d, w_d)
+
+    def _field_tensor(self, arr, core, name):
+        """float32 device tensor of a field of shape core + shape (or core
+        alone, broadcast), and its cell stride (1 field, 0 constant)."""
+        t = device_tensor(arr, np.float32)
+        if tuple(t.shape) == core:
+            return t.contiguous(), 0
+        if tuple(t.shape) == core + tuple(self.shape):
+            return t.contiguous(), 1
+        raise ValueError(f"Field {name} has incorrect dimensions. Found {tuple(t.shape)}, "
+                         f"expected {core} + {self.shape}")
+
+    # ------------------------------------------------------------ args
+    def SetArgs(self):
+        """_SetArgs.py:8-177: values, rhs, seeds, walls in the tiled layout."""
+        if self.verbosity >= 1:
+            print("Preparing the problem rhs (cost, seeds,...)")
+        shp, si = _lib.i64(self.shape), _lib.i32(self.shape_i)
+        esize = np.dtype(self.float_t).itemsize
+        values = self.GetValue('values', default=None, help="Initial values for the eikonal solver")
+        self.values_t = empty((self.size_tot,), self.float_t)
+        if values is None:
+            self.values_t.fill_(float('inf'))
+        else:
+            v = device_tensor(values, self.float_t)
+            if tuple(v.shape) != tuple(self.shape):
+                raise ValueError(f"Field values has incorrect dimensions {tuple(v.shape)}")
+            _lib.check(_lib.lib.agd_block_expand(self.ndim, shp, si, 1, esize, ptr(v), 1,
+                                                 ptr(self.values_t), float('inf'), self.stream))
+        self.warmStartAllTiles = bool(self.GetValue(
+            'warmStartAllTiles', default=False,
+            help="Start with every tile active (use with an initial supersolution 'values')"))
+
+        # rhs = cost, padded with +inf
+        cost = device_tensor(self.cost, self.float_t)
+        self.rhs_t = empty((self.size_tot,), self.float_t)
+        if cost.dim() == 0:
+            stride = 0
+        elif tuple(cost.shape) == tuple(self.shape):
+            stride = 1
+        else:
+            raise ValueError(f"Field cost has incorrect dimensions {tuple(cost.shape)}")
+        self.cost_t = cost
+        _lib.check(_lib.lib.agd_block_expand(self.ndim, shp, si, 1, esize, ptr(cost.contiguous()),
+                                             stride, ptr(self.rhs_t), float('inf'), self.stream))
+
+        nwords = (self.size_tot + 31) // 32
+        self.seeds_t = zeros((nwords,), np.int32)
+        self.trigger_t = zeros((nwords,), np.int32)
+
+        seeds = self.GetValue('seeds', default=None, help="Points from where the front propagation starts")
+        self.reverseAD = self.HasValue('sensitivity')
+        self.seedIndices = None
+        if seeds is not None:
+            seeds = np.asarray(host_array(seeds), dtype=self.hfmIn.float_t)
+            if seeds.ndim != 2 or seeds.shape[1] != self.ndim:
+                raise ValueError(f"seeds must have shape (n, {self.ndim})")
+            cont = self.hfmIn.PointFromIndex(seeds, to=True)
+            seedValues = self.GetValue('seedValues', default=np.zeros(len(seeds), dtype=np.float32),
+                                       help="Initial value for the front propagation")
+            seedValues = np.asarray(host_array(seedValues), dtype=np.float64).reshape(-1)
+            if len(seedValues) != len(seeds):
+                raise ValueError("seedValues must have one entry per seed")
+            if self.GetValue('seedRadius', default=0.,
+                             help="Spread the seeds over a radius given in pixels (0 only)") != 0.:
+                raise NotImplementedError("seedRadius>0 is not provided yet")
+            idx = np.round(cont).astype(np.int64)  # round half to even (_SetArgs.py:44-45)
+            for k in range(self.ndim):
+                if self.periodic[k]:
+                    idx[:, k] %= self.shape[k]
+            if np.any(idx < 0) or np.any(idx >= np.asarray(self.shape)):
+                raise ValueError("Seed outside the domain")
+            self.seedIndices = idx
+            self.seedValues = seedValues
+            tiled = fd.tiled_index(idx, self.shape_i, self.shape_o)
+            tiled_d = device_tensor(tiled, np.int64)
+            vals_d = device_tensor(seedValues, np.float64)
+            _lib.check(_lib.lib.agd_scatter_seeds(self.scalar, len(seeds), ptr(tiled_d), ptr(vals_d),
+                                                  ptr(self.rhs_t), ptr(self.seeds_t),
+                                                  ptr(self.trigger_t), self.stream))
+        elif self.solver != 'global_iteration' and not self.warmStartAllTiles:
+            raise ValueError("Missing value for key seeds")
+
+        self.wallDist_t = None
+        if self.walls is not None:
+            walls = device_tensor(self.walls, np.uint8)
+            if tuple(walls.shape) != tuple(self.shape):
+                raise ValueError(f"Field walls has incorrect dimensions {tuple(walls.shape)}")
+            walls_t = zeros((self.size_tot,), np.uint8)
+            _lib.check(_lib.lib.agd_block_expand(self.ndim, shp, si, 1, 1, ptr(walls.contiguous()), 1,
+                                                 ptr(walls_t), 0., self.stream))
+            self.wallDist_t = empty((self.size_tot,), np.uint8)
+            desc = self._desc()
+            _lib.check(_lib.lib.agd_prepare_walls(desc, ptr(walls_t), self.wallDistBound,
+                                                  ptr(self.rhs_t), ptr(self.seeds_t),
+                                                  ptr(self.wallDist_t), self.stream))
+
+        self.nitermax_o = int(self.GetValue('nitermax_o', default=2000,
+                                            help="Maximum number of iterations of the solver"))
+        self.raiseOnNonConvergence = self.GetValue('raiseOnNonConvergence', default=True,
+                                                   help="Raise an exception if a solver fails to converge")
+        self.check_interval = int(self.GetValue(
+            'check_interval', default=4,
+            help="Grid passes between two convergence checks on the host"))
+
+    def _desc(self, niter_i=None):
+        d = _lib.EikonalDesc()
+        d.model = self.modelId
+        d.ndim = self.ndim
+        d.scalar = self.scalar
+        d.solver = _SOLVER_IDS[self.solver]
+        for k in range(3):
+            d.shape[k] = int(self.shape[k]) if k < self.ndim else 1
+            d.shape_i[k] = int(self.shape_i[k]) if k < self.ndim else 1
+            d.periodic[k] = int(self.periodic[k]) if k < self.ndim else 0
+            d.iso_weights[k] = float(self.iso_weights[k])
+        d.niter_i = self.niter_i if niter_i is None else niter_i
+        d.fim_front_width = int(self.fim_front_width or 0)
+        d.atol = self.atol
+        d.rtol = self.rtol
+        d.nitermax_o = getattr(self, 'nitermax_o', 2000)
+        d.check_interval = getattr(self, 'check_interval', 4)
+        d.activate_all = int(getattr(self, 'warmStartAllTiles', False))
+        return d
+
+    def _io(self):
+        io = _lib.EikonalIO()
+        io.values = ptr(self.values_t)
+        io.geom = ptr(self.geom) if self.geom is not None else None
+        io.rhs = ptr(self.rhs_t)
+        io.seeds = ptr(self.seeds_t)
+        io.trigger = ptr(self.trigger_t)
+        io.wall_dist = ptr(self.wallDist_t) if self.wallDist_t is not None else None
+        return io
+
+    # ------------------------------------------------------------ solve
+    def Solve(self):
+        """_solvers.py:15-57."""
+        if self.verbosity >= 1:
+            print("Running the eikonal GPU kernel")
+        desc, io, st = self._desc(), self._io(), _lib.EikonalStats()
+        torch.cuda.synchronize()
+        t0 = time.time()
+        status = _lib.lib.agd_eikonal_solve(desc, io, st, self.stream)
+        wall = time.time() - t0
+        self.stats['eikonal'] = {**st.as_dict(), 'time': wall, 'gpu_time': st.gpu_ms * 1e-3}
+        if self.verbosity >= 1:
+            print(f"GPU kernel eikonal ran for {wall} seconds, and {st.niter_o} iterations.")
+        if status == _lib.AGD_NOCONV:
+            msg = (f"Solver {self.solver} for kernel eikonal did not reach convergence after "
+                   f"maximum allowed number {self.nitermax_o} of iterations")
+            if self.raiseOnNonConvergence:
+                raise ValueError(msg)
+            self.Warn(msg)
+        else:
+            _lib.check(status)
+        self._values = None
+
+    @property
+    def values(self):
+        if self._values is None:
+            out = empty(tuple(self.shape), self.float_t)
+            _lib.check(_lib.lib.agd_block_squeeze(self.ndim, _lib.i64(self.shape), _lib.i32(self.shape_i),
+                                                  1, np.dtype(self.float_t).itemsize,
+                                                  ptr(self.values_t), ptr(out), self.stream))
+            self._values = out
+        return self._output(self._values)
+
+    def _output(self, t):
+        return t.cpu().numpy() if self.host_outputs else t
+
+    def _squeeze(self, t_tiled, ncomp, dtype):
+        out = empty((ncomp,) + tuple(self.shape), dtype)
+        _lib.check(_lib.lib.agd_block_squeeze(self.ndim, _lib.i64(self.shape), _lib.i32(self.shape_i),
+                                              ncomp, np.dtype(dtype).itemsize, ptr(t_tiled),
+                                              ptr(out), self.stream))
+        return out
+
+    # ------------------------------------------------------------ flow
+    def PostProcess(self):
+        """_PostProcess.py:46-95: one global pass exporting the upwind flow."""
+        self.flow_needed = self.exportGeodesicFlow or self.reverseAD
+        if not self.flow_needed:
+            return
+        if self.verbosity >= 1:
+            print("Post-Processing")
+        nsym = self.ndim if self.model_ in ('Isotropic', 'Diagonal') else self.ndim * (self.ndim + 1) // 2
+        self.nsym = nsym
+        self.flow_weights_t = empty((nsym, self.size_tot), self.float_t)
+        self.flow_weightsum_t = empty((self.size_tot,), self.float_t)
+        self.flow_indices_t = empty((nsym, self.size_tot), np.int32)
+        self.flow_vector_t = empty((self.ndim, self.size_tot), self.float_t)
+        f = _lib.FlowIO()
+        f.flow_weights = ptr(self.flow_weights_t)
+        f.flow_weightsum = ptr(self.flow_weightsum_t)
+        f.flow_indices = ptr(self.flow_indices_t)
+        f.flow_vector = ptr(self.flow_vector_t)
+        t0 = time.time()
+        _lib.check(_lib.lib.agd_eikonal_flow(self._desc(niter_i=1), self._io(), f, self.stream))
+        torch.cuda.synchronize()
+        self.stats['flow'] = {'niter_o': 1, 'time': time.time() - t0}
+        self._values = None
+        if self.exportGeodesicFlow:
+            if self.model_ == 'Rander':
+                raise NotImplementedError("flow normalisation of the Rander model is not provided yet")
+            fv = self._squeeze(self.flow_vector_t, self.ndim, self.float_t)
+            h = torch.as_tensor(self.h, dtype=fv.dtype, device=fv.device).reshape((self.ndim,) + (1,) * self.ndim)
+            self.hfmOut['flow'] = self._output(fv * (-h))
+
+    # ------------------------------------------------------------ reverse AD
+    def SolveAD(self):
+        """_PostProcess.py:155-219 (reverse branch)."""
+        if not self.reverseAD:
+            return
+        if self.model_ == 'Rander':
+            raise NotImplementedError("reverse AD of the Rander model is not provided yet")
+        sens = self.GetValue('sensitivity', help='Reverse automatic differentiation')
+        sens = device_tensor(sens, self.float_t)
+        if tuple(sens.shape[:-1]) != tuple(self.shape):
+            raise ValueError(f"Reverse AD rhs shape {tuple(sens.shape)} does not start with {self.shape}")
+        nrhs = int(sens.shape[-1])
+        sens_c = torch.movedim(sens, -1, 0).contiguous()
+        rhs_t = empty((nrhs, self.size_tot), self.float_t)
+        esize = np.dtype(self.float_t).itemsize
+        _lib.check(_lib.lib.agd_block_expand(self.ndim, _lib.i64(self.shape), _lib.i32(self.shape_i),
+                                             nrhs, esize, ptr(sens_c), 1, ptr(rhs_t), float('nan'),
+                                             self.stream))
+        res = np.finfo(self.float_t).resolution
+        self.linear_rtol = float(self.GetValue('linear_rtol', default=float(res * 5),
+                                               help="Relative convergence tolerance for the linear systems"))
+        linear_atol = self.GetValue('linear_atol', default=None,
+                                    help="Absolute convergence tolerance for the linear systems")
+        if linear_atol is None:
+            fin = torch.isfinite(sens_c)
+            mean = float(sens_c[fin].abs().double().mean().item()) if bool(fin.any()) else 0.
+            linear_atol = self.linear_rtol * mean
+            self.hfmOut['keys']['default']['linear_atol'] = linear_atol
+        self.linear_atol = float(linear_atol)
+        lam_t = empty((nrhs, self.size_tot), self.float_t)
+        st = _lib.EikonalStats()
+        status = _lib.lib.agd_eikonal_adjoint(
+            self._desc(), ptr(self.values_t), ptr(self.flow_weights_t), ptr(self.flow_indices_t),
+            ptr(self.flow_weightsum_t), nrhs, ptr(rhs_t), ptr(lam_t), self.linear_atol,
+            self.linear_rtol, st, self.stream)
+        self.stats['reverseAD'] = {**st.as_dict(), 'time': st.gpu_ms * 1e-3}
+        if status == _lib.AGD_NOCONV:
+            msg = "Solver for kernel reverseAD did not reach convergence"
+            if self.raiseOnNonConvergence:
+                raise ValueError(msg)
+            self.Warn(msg)
+        else:
+            _lib.check(status)
+        lam = torch.movedim(self._squeeze(lam_t, nrhs, self.float_t), 0, -1).contiguous()
+        pos = tuple(torch.as_tensor(self.seedIndices[:, k], device=lam.device) for k in range(self.ndim))
+        seedSens = lam[pos].clone()
+        lam[pos] = 0
+        self.costSensitivity = lam
+        self.hfmOut['costSensitivity'] = self._output(lam)
+        # seedValues_rev is the identity (seedRadius = 0): one row per seed.
+        self.hfmOut['seedValueSensitivity'] = self._output(seedSens)
+
+    # ------------------------------------------------------------ outputs
+    def FinalCheck(self):
+        """_Interface.py:162-174."""
+        if self.GetValue('exportValues', False, help="Return the solution numerical values"):
+            self.hfmOut['values'] = self.values
+        self.extractValues = self.GetValue('extractValues', False,
+                                           help="Return the solution numerical values separately from other data")
+        self.retself = self.GetValue('retself', False, help="Return the class instance that did the work")
+        self.hfmOut['stats'] = self.stats
+        self.hfmOut['solverGPUTime'] = self.stats['eikonal']['time']
+        self.hfmOut['keys']['unused'] = list(set(self.hfmIn.keys()) - set(self.hfmOut['keys']['used']))
+        if self.verbosity >= 1 and self.hfmOut['keys']['unused']:
+            print(f"!! Warning !! Unused keys from user : {self.hfmOut['keys']['unused']}")
+
+
+def ctypes_double3(h):
+    import ctypes
+    arr = (ctypes.c_double * 3)(*([float(x) for x in h] + [1.] * (3 - len(h))))
+    return arr

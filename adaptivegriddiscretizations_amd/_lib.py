@@ -1,0 +1,161 @@
+"""ctypes binding of libagd_hip.so, the C ABI declared in include/agd_hip.h.
+
+The library is the only compute path of this package: if it cannot be
+loaded, importing this module raises (there is no CPU fallback).
+"""
+import ctypes
+import os
+from ctypes import POINTER, c_char_p, c_double, c_int32, c_int64, c_void_p
+
+from . import _build
+
+c_int32_p = POINTER(c_int32)
+c_int64_p = POINTER(c_int64)
+c_double_p = POINTER(c_double)
+c_uint8_p = POINTER(ctypes.c_uint8)
+c_uint32_p = POINTER(ctypes.c_uint32)
+
+# agd_status
+AGD_OK, AGD_EINVAL, AGD_EHIP, AGD_NOCONV, AGD_ENOMEM = 0, -1, -2, -3, -4
+# agd_model
+ISOTROPIC, DIAGONAL, RIEMANN, RANDER = 0, 1, 2, 3
+# agd_scalar
+F32, F64 = 0, 1
+# agd_solver_kind
+AGSI, FIM, GLOBAL = 0, 1, 2
+
+
+class EikonalDesc(ctypes.Structure):
+    _fields_ = [
+        ("model", c_int32), ("ndim", c_int32), ("scalar", c_int32), ("solver", c_int32),
+        ("shape", c_int64 * 3), ("shape_i", c_int32 * 3), ("periodic", c_int32 * 3),
+        ("niter_i", c_int32), ("fim_front_width", c_int32),
+        ("atol", c_double), ("rtol", c_double), ("nitermax_o", c_int64),
+        ("iso_weights", c_double * 3), ("check_interval", c_int32), ("activate_all", c_int32),
+    ]
+
+
+class EikonalIO(ctypes.Structure):
+    _fields_ = [
+        ("values", c_void_p), ("geom", c_void_p), ("rhs", c_void_p),
+        ("seeds", c_void_p), ("trigger", c_void_p), ("wall_dist", c_void_p),
+    ]
+
+
+class EikonalStats(ctypes.Structure):
+    _fields_ = [
+        ("niter_o", c_int64), ("tile_updates", c_int64), ("cell_updates", c_int64),
+        ("launches", c_int64), ("gpu_ms", c_double), ("converged", c_int32),
+        ("grid_blocks", c_int32),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class FlowIO(ctypes.Structure):
+    _fields_ = [
+        ("flow_weights", c_void_p), ("flow_weightsum", c_void_p),
+        ("flow_indices", c_void_p), ("flow_vector", c_void_p),
+    ]
+
+
+# name -> (restype, argtypes).  Kept in the order of include/agd_hip.h.
+SIGNATURES = {
+    "agd_last_error": (c_char_p, []),
+    "agd_version": (c_int32, []),
+    "agd_tile_supported": (c_int32, [c_int32, c_int32, c_int32_p]),
+    "agd_device_count": (c_int32, [c_int32_p]),
+    "agd_eikonal_solve": (c_int32, [POINTER(EikonalDesc), POINTER(EikonalIO),
+                                    POINTER(EikonalStats), c_void_p]),
+    "agd_eikonal_flow": (c_int32, [POINTER(EikonalDesc), POINTER(EikonalIO), POINTER(FlowIO),
+                                   c_void_p]),
+    "agd_eikonal_adjoint": (c_int32, [POINTER(EikonalDesc), c_void_p, c_void_p, c_void_p,
+                                      c_void_p, c_int32, c_void_p, c_void_p, c_double,
+                                      c_double, POINTER(EikonalStats), c_void_p]),
+    "agd_block_expand": (c_int32, [c_int32, c_int64_p, c_int32_p, c_int32, c_int32, c_void_p,
+                                   c_int64, c_void_p, c_double, c_void_p]),
+    "agd_block_squeeze": (c_int32, [c_int32, c_int64_p, c_int32_p, c_int32, c_int32, c_void_p,
+                                    c_void_p, c_void_p]),
+    "agd_prepare_geometry": (c_int32, [c_int32, c_int32, c_int64_p, c_int32_p, c_int32,
+                                       c_double_p, c_void_p, c_void_p, c_int64, c_void_p,
+                                       c_void_p]),
+    "agd_scatter_seeds": (c_int32, [c_int32, c_int64, c_void_p, c_void_p, c_void_p, c_void_p,
+                                    c_void_p, c_void_p]),
+    "agd_prepare_walls": (c_int32, [POINTER(EikonalDesc), c_void_p, c_int32, c_void_p,
+                                    c_void_p, c_void_p, c_void_p]),
+    "agd_stencil_export": (c_int32, [c_int32, c_int32, c_int32, c_int64, c_void_p, c_double_p,
+                                     c_void_p, c_void_p, c_void_p]),
+    "agd_malloc": (c_int32, [POINTER(c_void_p), c_int64]),
+    "agd_free": (c_int32, [c_void_p]),
+    "agd_memcpy_htod": (c_int32, [c_void_p, c_void_p, c_int64, c_void_p]),
+    "agd_memcpy_dtoh": (c_int32, [c_void_p, c_void_p, c_int64, c_void_p]),
+    "agd_memset": (c_int32, [c_void_p, c_int32, c_int64, c_void_p]),
+    "agd_stream_synchronize": (c_int32, [c_void_p]),
+}
+
+
+class AGDError(RuntimeError):
+    """A call into libagd_hip.so failed."""
+
+    def __init__(self, status, message):
+        super().__init__(f"libagd_hip status {status}: {message}")
+        self.status = status
+
+
+class NonConvergence(AGDError):
+    pass
+
+
+def hip_runtimes_loaded():
+    """Paths of the HIP runtime libraries mapped in this process."""
+    with open("/proc/self/maps") as f:
+        return sorted({line.split()[-1] for line in f if "libamdhip64" in line})
+
+
+def _load():
+    path = os.environ.get("AGD_HIP_LIB", _build.LIB_PATH)
+    if not os.path.exists(path):
+        raise ImportError(
+            f"libagd_hip.so not found at {path}: build it first "
+            "(python -c 'import __graft_entry__; __graft_entry__.build()')")
+    # torch ships its own copy of the HIP runtime under a different file name
+    # (same SONAME).  Loading torch first lets the dynamic linker bind this
+    # library to that copy; loading this library first would map a second HIP
+    # runtime, and torch's streams and events would be foreign objects to it.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+    lib = ctypes.CDLL(path)
+    runtimes = hip_runtimes_loaded()
+    if len(runtimes) > 1:
+        raise ImportError(f"several HIP runtimes are mapped in this process: {runtimes}")
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def check(status):
+    """Raise AGDError (NonConvergence for AGD_NOCONV) on a non-zero status."""
+    if status != AGD_OK:
+        msg = lib.agd_last_error().decode(errors="replace")
+        if status == AGD_NOCONV:
+            raise NonConvergence(status, msg)
+        raise AGDError(status, msg)
+    return status
+
+
+def i64(seq):
+    arr = (c_int64 * 3)(*([int(s) for s in seq] + [1] * (3 - len(seq))))
+    return arr
+
+
+def i32(seq):
+    arr = (c_int32 * 3)(*([int(s) for s in seq] + [1] * (3 - len(seq))))
+    return arr

@@ -1,0 +1,351 @@
+// adjoint.hip -- reverse-mode sensitivities of the eikonal solve.
+//
+// The converged upwind scheme, linearised, reads (D - W) du = c dc with
+// W(x, y) = flow weight of x towards its upwind neighbour y and D(x) = sum of
+// x's flow weights (1 on seeds / walls, where it vanishes).  Reverse mode
+// solves (D - W)^T lambda = s:
+//     lambda(y) = ( s(y) + sum_{x : y upwind of x} W(x, y) lambda(x) ) / D(y).
+// (reference: _PostProcess.py:155-219 SolveAD, graph_reverse.py:10-77,
+// GraphReverse.h:35-82, Kernel_LinearUpdate.h:99-216.)
+//
+// MI355X formulation:
+//  1. transpose the flow graph into CSR rows indexed by the upwind cell y:
+//     in-degree count (atomics), exclusive scan (hipCUB), scatter (atomic
+//     cursors), then each row sorted by source index so the summation order,
+//     hence the FP32 result, is deterministic;
+//  2. tile Gauss-Seidel on the transposed system with the same device
+//     worklist as the eikonal solver (AGSI activation, as the reference's
+//     SolveLinear switches FIM to AGSI, _PostProcess.py:117-118).  The tile's
+//     lambda values sit in LDS; sources outside the tile are read through
+//     the cache at every sweep.
+#include <hip/hip_runtime.h>
+
+#include <hipcub/hipcub.hpp>
+
+#include "common.hpp"
+#include "device.hpp"
+#include "worklist.hpp"
+
+namespace agd {
+
+template <class S>
+__global__ void adj_count(int64_t N, int nsym, const S* fw, const int32_t* fidx, int32_t* deg) {
+  const int64_t x = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (x >= N) return;
+  for (int k = 0; k < nsym; ++k) {
+    const int32_t y = fidx[k * N + x];
+    if (fw[k * N + x] != S(0) && y != INT32_MAX) atomicAdd(deg + y, 1);
+  }
+}
+
+template <class S>
+__global__ void adj_fill(int64_t N, int nsym, const S* fw, const int32_t* fidx, int32_t* cursor,
+                         int32_t* src, S* wt) {
+  const int64_t x = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (x >= N) return;
+  for (int k = 0; k < nsym; ++k) {
+    const int32_t y = fidx[k * N + x];
+    const S w = fw[k * N + x];
+    if (w != S(0) && y != INT32_MAX) {
+      const int32_t p = atomicAdd(cursor + y, 1);
+      src[p] = int32_t(x);
+      wt[p] = w;
+    }
+  }
+}
+
+// Sort each row by source index (rows hold a handful of entries).
+template <class S>
+__global__ void adj_sort_rows(int64_t N, const int32_t* rowptr, int32_t* src, S* wt) {
+  const int64_t y = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (y >= N) return;
+  const int32_t r0 = rowptr[y], r1 = rowptr[y + 1];
+  for (int32_t i = r0 + 1; i < r1; ++i) {
+    const int32_t s = src[i];
+    const S w = wt[i];
+    int32_t j = i - 1;
+    while (j >= r0 && src[j] > s) {
+      src[j + 1] = src[j];
+      wt[j + 1] = wt[j];
+      --j;
+    }
+    src[j + 1] = s;
+    wt[j + 1] = w;
+  }
+}
+
+// Trigger: cells of the domain (finite value) with no incoming transposed
+// edge, i.e. no cell uses them as upwind neighbour (_PostProcess.py:143-144).
+template <class S>
+__global__ void adj_trigger(int64_t N, const S* u, const int32_t* rowptr, uint32_t* trig) {
+  const int64_t y = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (y >= N) return;
+  const S v = u[y];
+  if (rowptr[y + 1] == rowptr[y] && v == v && v != inf_v<S>())
+    atomicOr(trig + (y >> 5), 1u << (y & 31));
+}
+
+template <class S> struct LinArgs {
+  S* lam;
+  const S* rhs;
+  const S* diag;
+  const int32_t* rowptr;
+  const int32_t* src;
+  const S* wt;
+  GridDesc g;
+  S atol, rtol;
+  int32_t niter_i;
+  int32_t iter;
+  const int32_t* list_in;
+  const int32_t* count_in;
+  int32_t* list_out;
+  int32_t* count_out;
+  int32_t* count_clear;
+  int32_t* stamp;
+  int32_t* hist;
+};
+
+template <class S> __global__ void fill_inf_kernel(int64_t n, S* x) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = inf_v<S>();
+}
+
+template <class S> __device__ inline S init_val(S x) { return x == inf_v<S>() ? S(0) : x; }
+
+template <class S, int D, int T0, int T1, int T2>
+__global__ void __launch_bounds__(T0* T1* T2) adj_update(LinArgs<S> a) {
+  using TL = Tile<D, T0, T1, T2>;
+  constexpr int NI = TL::NI;
+  __shared__ S l_s[NI];
+  const int tid = threadIdx.x;
+  const int count = *a.count_in;
+  if (blockIdx.x == 0 && tid == 0) {
+    *a.count_clear = 0;
+    a.hist[a.iter] = count;
+  }
+  const int32_t tag = a.iter + 1;
+  for (int li = blockIdx.x; li < count; li += gridDim.x) {
+    const int n_o = a.list_in[li];
+    int xo[3];
+    tile_pos<D>(n_o, a.g, xo);
+    const int64_t n_t = int64_t(n_o) * NI + tid;
+    const S lold = a.lam[n_t];
+    const S rhs = a.rhs[n_t];
+    const S dg = a.diag[n_t];
+    const S diag = dg == S(0) ? S(1) : dg;
+    const int32_t r0 = a.rowptr[n_t], r1 = a.rowptr[n_t + 1];
+    l_s[tid] = init_val(lold);
+    __syncthreads();
+    for (int it = 0; it < a.niter_i; ++it) {
+      S sum = rhs;
+      for (int32_t j = r0; j < r1; ++j) {
+        const int32_t s = a.src[j];
+        const S val = (s / NI == n_o) ? l_s[s % NI] : init_val(a.lam[s]);
+        sum += a.wt[j] * val;
+      }
+      const S nv = sum / diag;
+      __syncthreads();
+      const bool ch = !(nv == l_s[tid]);
+      l_s[tid] = nv;
+      if (!__syncthreads_or(ch)) break;
+    }
+    const S val = l_s[tid];
+    a.lam[n_t] = val;
+    const S tol = abs_(val) * a.rtol + a.atol;
+    const bool changed = abs_(val - lold) > tol;
+    const bool any = __syncthreads_or(changed);
+    if (any && tid < 2 * D + 1) {
+      const int nb = (tid == 2 * D) ? n_o : tile_neighbor<D>(xo, tid >> 1, 2 * (tid & 1) - 1, a.g);
+      if (nb >= 0) enqueue(nb, tag, a.stamp, a.list_out, a.count_out);
+    }
+    __syncthreads();
+  }
+}
+
+template <int NI>
+__global__ void adj_init_list(const uint32_t* trigger, int32_t ntiles, int32_t* stamp,
+                              int32_t* list, int32_t* count) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntiles) return;
+  bool on = false;
+  const int64_t b0 = int64_t(t) * NI, b1 = b0 + NI;
+  for (int64_t b = b0; b < b1 && !on; ++b) on = (trigger[b >> 5] >> (b & 31)) & 1u;
+  if (on) enqueue(t, 0, stamp, list, count);
+}
+
+template <class S> struct LinSet {
+  void (*update)(LinArgs<S>);
+  void (*init)(const uint32_t*, int32_t, int32_t*, int32_t*, int32_t*);
+  int ni;
+};
+
+template <class S, int D, int T0, int T1, int T2> LinSet<S> make_lin() {
+  LinSet<S> s;
+  s.update = adj_update<S, D, T0, T1, T2>;
+  s.init = adj_init_list<T0 * T1 * T2>;
+  s.ni = T0 * T1 * T2;
+  return s;
+}
+
+template <class S> static bool lin_lookup(int d, const int32_t* si, LinSet<S>& out) {
+  if (d == 2) {
+    const int key = si[0] * 100 + si[1];
+    switch (key) {
+      case 404: out = make_lin<S, 2, 4, 4, 1>(); return true;
+      case 808: out = make_lin<S, 2, 8, 8, 1>(); return true;
+      case 1616: out = make_lin<S, 2, 16, 16, 1>(); return true;
+      case 816: out = make_lin<S, 2, 8, 16, 1>(); return true;
+    }
+  } else if (d == 3) {
+    const int key = si[0] * 10000 + si[1] * 100 + si[2];
+    switch (key) {
+      case 40404: out = make_lin<S, 3, 4, 4, 4>(); return true;
+      case 80804: out = make_lin<S, 3, 8, 8, 4>(); return true;
+      case 40408: out = make_lin<S, 3, 4, 4, 8>(); return true;
+    }
+  }
+  return false;
+}
+
+template <class S>
+static int adjoint_impl(const agd_eikonal_desc* d, const S* values, const S* fw,
+                        const int32_t* fidx, const S* wsum, int32_t nrhs, const S* sens, S* lam,
+                        double atol, double rtol, agd_eikonal_stats* stats, hipStream_t stream) {
+  LinSet<S> ls;
+  AGD_REQUIRE(lin_lookup<S>(d->ndim, d->shape_i, ls), "unsupported tile shape");
+  AGD_REQUIRE(values && fw && fidx && wsum && sens && lam && nrhs >= 1, "null buffer");
+  GridDesc g{};
+  g.ntiles = 1;
+  for (int k = 0; k < 3; ++k) {
+    g.shape_tot[k] = k < d->ndim ? int32_t(d->shape[k]) : 1;
+    g.shape_o[k] = k < d->ndim ? int32_t(ceil_div(d->shape[k], d->shape_i[k])) : 1;
+    g.ntiles *= g.shape_o[k];
+  }
+  g.periodic = 0;
+  for (int k = 0; k < d->ndim; ++k)
+    if (d->periodic[k]) g.periodic |= (1 << k);
+  g.size_tot = int64_t(g.ntiles) * ls.ni;
+  const int64_t N = g.size_tot;
+  const int nsym = nsym_of(d->model, d->ndim);
+  const dim3 grid(unsigned(ceil_div(N, 256))), block(256);
+
+  // ---- 1. transpose
+  int32_t *deg = nullptr, *rowptr = nullptr, *src = nullptr;
+  uint32_t* trig = nullptr;
+  S* wt = nullptr;
+  void* tmp = nullptr;
+  size_t tmp_bytes = 0;
+  AGD_HIP_TRY(hipMalloc((void**)&deg, (N + 1) * sizeof(int32_t)));
+  AGD_HIP_TRY(hipMalloc((void**)&rowptr, (N + 1) * sizeof(int32_t)));
+  AGD_HIP_TRY(hipMalloc((void**)&trig, ceil_div(N, 32) * sizeof(uint32_t)));
+  AGD_HIP_TRY(hipMemsetAsync(deg, 0, (N + 1) * sizeof(int32_t), stream));
+  AGD_HIP_TRY(hipMemsetAsync(trig, 0, ceil_div(N, 32) * sizeof(uint32_t), stream));
+  hipLaunchKernelGGL(adj_count<S>, grid, block, 0, stream, N, nsym, fw, fidx, deg);
+  AGD_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, deg, rowptr, int(N + 1), stream));
+  AGD_HIP_TRY(hipMalloc(&tmp, tmp_bytes));
+  AGD_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, deg, rowptr, int(N + 1), stream));
+  int32_t nnz = 0;
+  AGD_HIP_TRY(hipMemcpyAsync(&nnz, rowptr + N, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+  AGD_HIP_TRY(hipStreamSynchronize(stream));
+  AGD_HIP_TRY(hipMalloc((void**)&src, std::max<int64_t>(nnz, 1) * sizeof(int32_t)));
+  AGD_HIP_TRY(hipMalloc((void**)&wt, std::max<int64_t>(nnz, 1) * sizeof(S)));
+  AGD_HIP_TRY(hipMemcpyAsync(deg, rowptr, (N + 1) * sizeof(int32_t), hipMemcpyDeviceToDevice, stream));
+  hipLaunchKernelGGL(adj_fill<S>, grid, block, 0, stream, N, nsym, fw, fidx, deg, src, wt);
+  hipLaunchKernelGGL(adj_sort_rows<S>, grid, block, 0, stream, N, rowptr, src, wt);
+  hipLaunchKernelGGL(adj_trigger<S>, grid, block, 0, stream, N, values, rowptr, trig);
+  AGD_HIP_TRY(hipGetLastError());
+
+  // ---- 2. tile Gauss-Seidel, one right-hand side at a time
+  const int blocks = blocks_for(g.ntiles, ls.ni);
+  int64_t niter = 0, visits = 0, passes = 0;
+  float ms = 0.f;
+  bool converged = true;
+  int status = AGD_OK;
+  for (int r = 0; r < nrhs && status == AGD_OK; ++r) {
+    LinArgs<S> a{};
+    a.lam = lam + int64_t(r) * N;
+    a.rhs = sens + int64_t(r) * N;
+    a.diag = wsum;
+    a.rowptr = rowptr;
+    a.src = src;
+    a.wt = wt;
+    a.g = g;
+    a.atol = S(atol);
+    a.rtol = S(rtol);
+    a.niter_i = d->niter_i;
+    // Dummy initialisation to +inf marks unvisited cells (read as 0).
+    hipLaunchKernelGGL(fill_inf_kernel<S>, grid, block, 0, stream, N, a.lam);
+    Workspace* wsp = nullptr;
+    auto init = [&](Workspace* ws, int32_t* list0, int32_t* counts) -> int {
+      wsp = ws;
+      a.stamp = ws->stamp;
+      a.hist = ws->hist;
+      hipLaunchKernelGGL(ls.init, dim3(unsigned(ceil_div(g.ntiles, 256))), dim3(256), 0, stream,
+                         trig, g.ntiles, ws->stamp, list0, counts + 0);
+      return AGD_OK;
+    };
+    auto launch = [&](const PassPointers& p) {
+      a.list_in = p.list_in;
+      a.count_in = p.count_in;
+      a.list_out = p.list_out;
+      a.count_out = p.count_out;
+      a.count_clear = p.count_clear;
+      a.iter = p.iter;
+      hipLaunchKernelGGL(ls.update, dim3(blocks), dim3(ls.ni), 0, stream, a);
+    };
+    WorklistResult res;
+    status = run_worklist(g.ntiles, d->nitermax_o, d->check_interval, stream, init, launch, res);
+    (void)wsp;
+    niter += res.niter;
+    visits += res.visits;
+    passes += res.passes;
+    ms += res.ms;
+    converged = converged && res.converged;
+  }
+  (void)hipStreamSynchronize(stream);
+  (void)hipFree(deg);
+  (void)hipFree(rowptr);
+  (void)hipFree(trig);
+  (void)hipFree(src);
+  (void)hipFree(wt);
+  (void)hipFree(tmp);
+  if (status) return status;
+  if (stats) {
+    stats->niter_o = niter;
+    stats->tile_updates = visits;
+    stats->cell_updates = visits * ls.ni;
+    stats->launches = passes;
+    stats->gpu_ms = ms;
+    stats->converged = converged ? 1 : 0;
+    stats->grid_blocks = blocks;
+  }
+  if (!converged) {
+    set_error("adjoint solver did not converge within nitermax_o passes");
+    return AGD_NOCONV;
+  }
+  return AGD_OK;
+}
+
+}  // namespace agd
+
+using namespace agd;
+
+extern "C" int agd_eikonal_adjoint(const agd_eikonal_desc* desc, const void* values,
+                                   const void* flow_weights, const int32_t* flow_indices,
+                                   const void* flow_weightsum, int32_t nrhs, const void* sens,
+                                   void* lambda, double atol, double rtol,
+                                   agd_eikonal_stats* stats, void* stream) {
+  AGD_REQUIRE(desc && (desc->ndim == 2 || desc->ndim == 3), "bad descriptor");
+  AGD_REQUIRE(desc->niter_i >= 1 && desc->nitermax_o >= 1, "bad iteration counts");
+  if (desc->scalar == AGD_F32)
+    return adjoint_impl<float>(desc, static_cast<const float*>(values),
+                               static_cast<const float*>(flow_weights), flow_indices,
+                               static_cast<const float*>(flow_weightsum), nrhs,
+                               static_cast<const float*>(sens), static_cast<float*>(lambda), atol,
+                               rtol, stats, as_stream(stream));
+  return adjoint_impl<double>(desc, static_cast<const double*>(values),
+                              static_cast<const double*>(flow_weights), flow_indices,
+                              static_cast<const double*>(flow_weightsum), nrhs,
+                              static_cast<const double*>(sens), static_cast<double*>(lambda), atol,
+                              rtol, stats, as_stream(stream));
+}

@@ -1,0 +1,54 @@
+// common.hpp -- host-side error handling shared by the library's translation
+// units.  Every C-ABI entry point returns an agd_status and records a
+// human-readable message retrievable with agd_last_error().
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <string>
+
+#include "../../include/agd_hip.h"
+
+namespace agd {
+
+void set_error(const std::string& msg);
+
+#define AGD_HIP_TRY(expr)                                                          \
+  do {                                                                             \
+    hipError_t e_ = (expr);                                                        \
+    if (e_ != hipSuccess) {                                                        \
+      ::agd::set_error(std::string(#expr) + " failed: " + hipGetErrorString(e_) + \
+                       " (" __FILE__ ":" + std::to_string(__LINE__) + ")");        \
+      return AGD_EHIP;                                                             \
+    }                                                                              \
+  } while (0)
+
+#define AGD_REQUIRE(cond, msg)                      \
+  do {                                              \
+    if (!(cond)) {                                  \
+      ::agd::set_error(std::string("invalid argument: ") + (msg)); \
+      return AGD_EINVAL;                            \
+    }                                               \
+  } while (0)
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// Geometry components per model (agd_model) and dimension.
+inline int ngeom_of(int model, int d) {
+  const int sd = d * (d + 1) / 2;
+  switch (model) {
+    case AGD_ISOTROPIC: return 0;
+    case AGD_DIAGONAL: return d;
+    case AGD_RIEMANN: return sd;
+    case AGD_RANDER: return sd + d;
+  }
+  return -1;
+}
+inline int nsym_of(int model, int d) {
+  return (model == AGD_ISOTROPIC || model == AGD_DIAGONAL) ? d : d * (d + 1) / 2;
+}
+
+}  // namespace agd

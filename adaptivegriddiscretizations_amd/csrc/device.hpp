@@ -1,0 +1,367 @@
+// device.hpp -- device building blocks of the eikonal tile solver (gfx950).
+//
+//  * tiled (bi-level) indexing: n_t = n_o * size_i + n_i, tile shape known at
+//    compile time so that in-tile divisions become shifts;
+//  * Selling decomposition of 2x2 / 3x3 symmetric positive definite matrices,
+//    kept entirely in registers (no dynamically indexed private arrays: the
+//    reduction loop is unrolled over its cyclic phases so every superbase
+//    access has a compile-time index);
+//  * the upwind Hopf-Lax update: sorting network on (value, weight) pairs and
+//    the active-set quadratic solve;
+//  * the wall visibility walk.
+//
+// Semantics follow the reference device code (paths relative to the AGD
+// source tree): Geometry_.h:144-156 (scal_vmv), Geometry2.h:21-48,
+// Geometry3.h:19-58 (Selling), Isotropic_.h:20-41 and Riemann_.h:16-29
+// (schemes), HFM.h:28-207 (update), NetworkSort.h:17-60 (sort networks),
+// Walls.h:13-50 (visibility), Grid.h:15-90 (indexing).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/agd_hip.h"
+
+namespace agd {
+
+template <class S> __device__ __host__ inline S inf_v();
+template <> __device__ __host__ inline float inf_v<float>() { return __builtin_huge_valf(); }
+template <> __device__ __host__ inline double inf_v<double>() { return __builtin_huge_val(); }
+
+__device__ inline float fma_(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ inline double fma_(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ inline float abs_(float a) { return __builtin_fabsf(a); }
+__device__ inline double abs_(double a) { return __builtin_fabs(a); }
+__device__ inline float sqrt_(float a) { return __builtin_sqrtf(a); }
+__device__ inline double sqrt_(double a) { return __builtin_sqrt(a); }
+__device__ inline bool isnan_(float a) { return a != a; }
+__device__ inline bool isnan_(double a) { return a != a; }
+
+// ------------------------------------------------------------------ models
+template <int MODEL, int D> struct ModelTraits {
+  static constexpr int SYMDIM = D * (D + 1) / 2;
+  static constexpr int NSYM = (MODEL == AGD_ISOTROPIC || MODEL == AGD_DIAGONAL) ? D : SYMDIM;
+  static constexpr int NGEOM = MODEL == AGD_ISOTROPIC ? 0
+                               : MODEL == AGD_DIAGONAL ? D
+                               : MODEL == AGD_RIEMANN  ? SYMDIM
+                                                       : SYMDIM + D;
+  static constexpr bool DRIFT = (MODEL == AGD_RANDER);
+};
+
+// ------------------------------------------------------------------ tiles
+template <int D, int T0, int T1, int T2> struct Tile {
+  static constexpr int NI = T0 * T1 * T2;
+  static constexpr int T[3] = {T0, T1, T2};
+  // Row-major position of cell n inside the tile (last axis fastest).
+  __device__ static inline void pos(int n, int* x) {
+    if constexpr (D == 2) {
+      x[1] = n % T1;
+      x[0] = n / T1;
+      x[2] = 0;
+    } else {
+      x[2] = n % T2;
+      x[1] = (n / T2) % T1;
+      x[0] = n / (T1 * T2);
+    }
+  }
+  __device__ static inline int index(const int* x) {
+    if constexpr (D == 2) return x[0] * T1 + x[1];
+    else return (x[0] * T1 + x[1]) * T2 + x[2];
+  }
+  __device__ static inline bool inside(const int* x) {
+    bool r = (x[0] >= 0) && (x[0] < T0) && (x[1] >= 0) && (x[1] < T1);
+    if constexpr (D == 3) r = r && (x[2] >= 0) && (x[2] < T2);
+    return r;
+  }
+};
+
+// Grid-level description passed by value to every kernel.
+struct GridDesc {
+  int32_t shape_tot[3];  // grid shape
+  int32_t shape_o[3];    // tile grid shape
+  int32_t periodic;      // bit k: axis k periodic
+  int32_t ntiles;
+  int64_t size_tot;      // ntiles * size_i
+};
+
+__device__ inline int mod_pos(int x, int n) {
+  x %= n;
+  return x >= 0 ? x : x + n;
+}
+
+// Position of tile n_o in the tile grid.
+template <int D> __device__ inline void tile_pos(int n_o, const GridDesc& g, int* xo) {
+  if constexpr (D == 2) {
+    xo[1] = n_o % g.shape_o[1];
+    xo[0] = n_o / g.shape_o[1];
+    xo[2] = 0;
+  } else {
+    xo[2] = n_o % g.shape_o[2];
+    const int r = n_o / g.shape_o[2];
+    xo[1] = r % g.shape_o[1];
+    xo[0] = r / g.shape_o[1];
+  }
+}
+
+// Grid.h:68-90 (InRange_per + Index_per on the tile grid): index of the
+// neighbouring tile of xo along axis `ax` in direction `dir`, -1 if outside.
+template <int D>
+__device__ inline int tile_neighbor(const int* xo, int ax, int dir, const GridDesc& g) {
+  int y[3] = {xo[0], xo[1], xo[2]};
+  y[ax] += dir;
+  if ((g.periodic >> ax) & 1) y[ax] = mod_pos(y[ax], g.shape_o[ax]);
+  else if (y[ax] < 0 || y[ax] >= g.shape_o[ax]) return -1;
+  if constexpr (D == 2) return y[0] * g.shape_o[1] + y[1];
+  else return (y[0] * g.shape_o[1] + y[1]) * g.shape_o[2] + y[2];
+}
+
+// Grid.h:15-35 Index_tot after the InRange_per test: tiled linear index of
+// the global position y (wrapped along periodic axes), or -1 outside.
+template <int D, class TL>
+__device__ inline int64_t index_tot(const int* y_in, const GridDesc& g) {
+  int y[3] = {y_in[0], y_in[1], y_in[2]};
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    if ((g.periodic >> k) & 1) y[k] = mod_pos(y[k], g.shape_tot[k]);
+    else if (y[k] < 0 || y[k] >= g.shape_tot[k]) return -1;
+  }
+  int yo[3], yi[3];
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    yo[k] = y[k] / TL::T[k];
+    yi[k] = y[k] % TL::T[k];
+  }
+  if constexpr (D == 2) {
+    return int64_t(yo[0] * g.shape_o[1] + yo[1]) * TL::NI + (yi[0] * TL::T[1] + yi[1]);
+  } else {
+    return int64_t((yo[0] * g.shape_o[1] + yo[1]) * g.shape_o[2] + yo[2]) * TL::NI +
+           ((yi[0] * TL::T[1] + yi[1]) * TL::T[2] + yi[2]);
+  }
+}
+
+// ------------------------------------------------------------------ Selling
+// Geometry_.h:144-156: integer products first, then one (fused) multiply-add
+// per coefficient of the lower triangle (d00, d10, d11, d20, d21, d22).
+template <class S> __device__ inline S scal_vmv2(const int* x, const S* m, const int* y) {
+  S r = S(0);
+  r = fma_(S(x[0] * y[0]), m[0], r);
+  r = fma_(S(x[1] * y[0] + x[0] * y[1]), m[1], r);
+  r = fma_(S(x[1] * y[1]), m[2], r);
+  return r;
+}
+template <class S> __device__ inline S scal_vmv3(const int* x, const S* m, const int* y) {
+  S r = S(0);
+  r = fma_(S(x[0] * y[0]), m[0], r);
+  r = fma_(S(x[1] * y[0] + x[0] * y[1]), m[1], r);
+  r = fma_(S(x[1] * y[1]), m[2], r);
+  r = fma_(S(x[2] * y[0] + x[0] * y[2]), m[3], r);
+  r = fma_(S(x[2] * y[1] + x[1] * y[2]), m[4], r);
+  r = fma_(S(x[2] * y[2]), m[5], r);
+  return r;
+}
+
+constexpr int kSellingMaxIter = 100;
+
+// One cyclic step (i,j,k) of the 2D reduction (Geometry2.h:25-33).
+template <class S, int I, int J, int K>
+__device__ inline void selling2_step(int (&sb)[3][2], const S* m, int& red) {
+  if (scal_vmv2<S>(sb[I], m, sb[J]) > S(0)) {
+    sb[K][0] = sb[I][0] - sb[J][0];
+    sb[K][1] = sb[I][1] - sb[J][1];
+    sb[I][0] = -sb[I][0];
+    sb[I][1] = -sb[I][1];
+    red = 0;
+  }
+}
+
+// Selling decomposition, 2D: weights w[3], offsets e[3][2].
+template <class S> __device__ inline void selling2(const S* m, S* w, int (&e)[3][3]) {
+  int sb[3][2] = {{1, 0}, {0, 1}, {-1, -1}};
+  int it = 0, red = 0;
+  bool done = false;
+  for (int outer = 0; outer < (kSellingMaxIter + 2) / 3 + 1 && !done; ++outer) {
+    // Phases 0,1,2 of iter%3 with compile-time indices.
+    if (!done) { if (it < kSellingMaxIter && red < 3) { selling2_step<S, 0, 1, 2>(sb, m, red); ++it; ++red; } else done = true; }
+    if (!done) { if (it < kSellingMaxIter && red < 3) { selling2_step<S, 1, 2, 0>(sb, m, red); ++it; ++red; } else done = true; }
+    if (!done) { if (it < kSellingMaxIter && red < 3) { selling2_step<S, 2, 0, 1>(sb, m, red); ++it; ++red; } else done = true; }
+  }
+  // Geometry2.h:40-48: w_r = max(0, -<b_r, D b_{r+1}>), e_r = perp(b_{r+2}).
+  const S s0 = -scal_vmv2<S>(sb[0], m, sb[1]);
+  const S s1 = -scal_vmv2<S>(sb[1], m, sb[2]);
+  const S s2 = -scal_vmv2<S>(sb[2], m, sb[0]);
+  w[0] = s0 > S(0) ? s0 : S(0);
+  w[1] = s1 > S(0) ? s1 : S(0);
+  w[2] = s2 > S(0) ? s2 : S(0);
+  e[0][0] = -sb[2][1]; e[0][1] = sb[2][0]; e[0][2] = 0;
+  e[1][0] = -sb[0][1]; e[1][1] = sb[0][0]; e[1][2] = 0;
+  e[2][0] = -sb[1][1]; e[2][1] = sb[1][0]; e[2][2] = 0;
+}
+
+// One step of the 3D reduction with permutation (i,j,k,l) (Geometry3.h:27-37).
+template <class S, int I, int J, int K, int L>
+__device__ inline void selling3_step(int (&sb)[4][3], const S* m, int& red) {
+  if (scal_vmv3<S>(sb[I], m, sb[J]) > S(0)) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      sb[K][c] += sb[I][c];
+      sb[L][c] += sb[I][c];
+      sb[I][c] = -sb[I][c];
+    }
+    red = 0;
+  }
+}
+
+template <int I, int J, int K, int L, class S>
+__device__ inline void selling3_out(const int (&sb)[4][3], const S* m, S& w, int* e) {
+  const S s = -scal_vmv3<S>(sb[I], m, sb[J]);
+  w = s > S(0) ? s : S(0);
+  e[0] = sb[K][1] * sb[L][2] - sb[K][2] * sb[L][1];
+  e[1] = sb[K][2] * sb[L][0] - sb[K][0] * sb[L][2];
+  e[2] = sb[K][0] * sb[L][1] - sb[K][1] * sb[L][0];
+}
+
+// Selling decomposition, 3D: weights w[6], offsets e[6][3].  Permutations
+// {0,1,2,3},{0,2,1,3},{0,3,1,2},{1,2,0,3},{1,3,0,2},{2,3,0,1}.
+template <class S> __device__ inline void selling3(const S* m, S* w, int (&e)[6][3]) {
+  int sb[4][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {-1, -1, -1}};
+  int it = 0, red = 0;
+  bool done = false;
+#define AGD_S3(I, J, K, L)                                                          \
+  if (!done) {                                                                      \
+    if (it < kSellingMaxIter && red < 6) { selling3_step<S, I, J, K, L>(sb, m, red); ++it; ++red; } \
+    else done = true;                                                               \
+  }
+  for (int outer = 0; outer < kSellingMaxIter / 6 + 2 && !done; ++outer) {
+    AGD_S3(0, 1, 2, 3) AGD_S3(0, 2, 1, 3) AGD_S3(0, 3, 1, 2)
+    AGD_S3(1, 2, 0, 3) AGD_S3(1, 3, 0, 2) AGD_S3(2, 3, 0, 1)
+  }
+#undef AGD_S3
+  selling3_out<0, 1, 2, 3>(sb, m, w[0], e[0]);
+  selling3_out<0, 2, 1, 3>(sb, m, w[1], e[1]);
+  selling3_out<0, 3, 1, 2>(sb, m, w[2], e[2]);
+  selling3_out<1, 2, 0, 3>(sb, m, w[3], e[3]);
+  selling3_out<1, 3, 0, 2>(sb, m, w[4], e[4]);
+  selling3_out<2, 3, 0, 1>(sb, m, w[5], e[5]);
+}
+
+// Per-cell stencil: NSYM symmetric offsets +-e_k with weights w_k, plus the
+// drift shift <omega, e_k> for Rander (FiniteDifferences.h:41-45).
+template <class S, int MODEL, int D> struct Stencil {
+  using MT = ModelTraits<MODEL, D>;
+  S w[MT::NSYM];
+  int e[MT::NSYM][3];
+  S shift[MT::NSYM];
+};
+
+template <class S, int MODEL, int D>
+__device__ inline void make_stencil(const S* g, const S* iso_w, Stencil<S, MODEL, D>& st) {
+  using MT = ModelTraits<MODEL, D>;
+  if constexpr (MODEL == AGD_ISOTROPIC || MODEL == AGD_DIAGONAL) {
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      st.w[k] = (MODEL == AGD_ISOTROPIC) ? iso_w[k] : g[k];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) st.e[k][c] = (c == k);
+      st.shift[k] = S(0);
+    }
+  } else {
+    if constexpr (D == 2) selling2<S>(g, st.w, st.e);
+    else selling3<S>(g, st.w, st.e);
+#pragma unroll
+    for (int k = 0; k < MT::NSYM; ++k) {
+      S s = S(0);
+      if constexpr (MT::DRIFT) {
+#pragma unroll
+        for (int c = 0; c < D; ++c) s += g[MT::SYMDIM + c] * S(st.e[k][c]);
+      }
+      st.shift[k] = s;
+    }
+  }
+}
+
+// ------------------------------------------------------------------ update
+// Compare-exchange of the sorting network (NetworkSort.h:17-22): position i
+// keeps its pair when v[i] < v[j], otherwise the pairs swap (ties swap).
+template <class S>
+__device__ inline void cex(S& vi, S& wi, S& vj, S& wj) {
+  const bool keep = vi < vj;
+  const S a = keep ? vi : vj, b = keep ? vj : vi;
+  const S c = keep ? wi : wj, d = keep ? wj : wi;
+  vi = a; vj = b; wi = c; wj = d;
+}
+
+template <class S, int N> __device__ inline void sort_pairs(S (&v)[N], S (&w)[N]) {
+  if constexpr (N == 2) {
+    cex(v[0], w[0], v[1], w[1]);
+  } else if constexpr (N == 3) {
+    cex(v[1], w[1], v[2], w[2]); cex(v[0], w[0], v[2], w[2]); cex(v[0], w[0], v[1], w[1]);
+  } else if constexpr (N == 6) {
+    cex(v[1], w[1], v[2], w[2]); cex(v[4], w[4], v[5], w[5]);
+    cex(v[0], w[0], v[2], w[2]); cex(v[3], w[3], v[5], w[5]);
+    cex(v[0], w[0], v[1], w[1]); cex(v[3], w[3], v[4], w[4]); cex(v[2], w[2], v[5], w[5]);
+    cex(v[0], w[0], v[3], w[3]); cex(v[1], w[1], v[4], w[4]);
+    cex(v[2], w[2], v[4], w[4]); cex(v[1], w[1], v[3], w[3]);
+    cex(v[2], w[2], v[3], w[3]);
+  }
+}
+
+// HFM.h:165-195: smallest solution of sum_k w_k (u - v_k)_+^2 = rhs^2 over
+// the sorted neighbour values, growing the active set while the quadratic is
+// non-positive at the next value.  Inputs by value (sorted copies).
+template <class S, int N>
+__device__ inline S hfm_solve(S (&v)[N], S (&w)[N], S rhs) {
+  sort_pairs<S, N>(v, w);
+  const S vmin = v[0];
+  S a = w[0], b = S(0), c = -rhs * rhs;
+  bool go = true;
+#pragma unroll
+  for (int k = 1; k < N; ++k) {
+    const S t = v[k] - vmin;
+    go = go && !((a * t - S(2) * b) * t + c > S(0));
+    if (go) {
+      a += w[k];
+      b += w[k] * t;
+      c += w[k] * t * t;
+    }
+  }
+  const S delta = b * b - a * c;
+  const S sdelta = sqrt_(delta > S(0) ? delta : S(0));
+  const S value = (a == S(0)) ? inf_v<S>() : (b + sdelta) / a;
+  return (vmin == inf_v<S>()) ? vmin : vmin + value;
+}
+
+// ------------------------------------------------------------------ walls
+__device__ inline int div_round_closest(int n, int d) {
+  return ((n < 0) ^ (d < 0)) ? ((n - d / 2) / d) : ((n + d / 2) / d);
+}
+
+// Walls.h:18-50.  wx = wall distance at x (255 = far).  `lookup(w)` returns
+// the wall distance at x + w (255 outside the domain).
+template <int D, class Lookup>
+__device__ inline bool visible(const int* e, int wx, Lookup lookup) {
+  if (wx == 255) return true;
+  int l1 = 0, linf = 0;
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    const int a = e[k] < 0 ? -e[k] : e[k];
+    l1 += a;
+    linf = a > linf ? a : linf;
+  }
+  if (l1 < wx) return true;
+  for (int t = 1; t <= linf; ++t) {
+    int w[3] = {0, 0, 0};
+    int diffl1 = 0;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      w[k] = div_round_closest(e[k] * t, linf);
+      const int r = e[k] - w[k];
+      diffl1 += r < 0 ? -r : r;
+    }
+    const int value = lookup(w);
+    if (diffl1 < value) return true;
+    if (value == 0) return false;
+  }
+  return true;
+}
+
+}  // namespace agd

@@ -1,0 +1,820 @@
+// eikonal.hip -- the eikonal tile solver for gfx950: grid-level worklist
+// driver, the fused tile-update kernel, the flow export kernel and the
+// seed / wall / stencil helpers.
+//
+// Tile update (reference: Update.h:20-227, HFMIter.h:10-86,
+// Propagation.h:18-167, FiniteDifferences.h:17-106):
+//   one workgroup per active tile, one lane per cell.  The lane loads its
+//   value, rhs, seed bit and geometry (coalesced: the tiled layout makes a
+//   tile contiguous), builds its stencil in registers (Selling for
+//   Riemann/Rander), then resolves each of its 2*nsym neighbours once: an LDS
+//   slot when the neighbour lies in the tile, otherwise the neighbour's value
+//   gathered from HBM/L2 and held in a register for the whole visit (values
+//   outside the tile do not change during the in-tile sweeps).  The tile's
+//   own values live in LDS and are relaxed niter_i times (Jacobi inside a
+//   sweep, early exit once a sweep changes nothing -- which cannot alter the
+//   result since the next sweep would read identical inputs).  The change
+//   test |u_old-u_new| <= atol + rtol|u_new| is reduced with a block-wide
+//   OR (a single wave64 ballot for 64-cell tiles) and drives tile activation.
+//
+// Worklist (replaces the host round trip of _solvers.py:122-193): the tile
+// list of pass k+1 is built on the device while pass k runs.  A tile is
+// appended at most once per pass thanks to a per-tile stamp
+// (atomicMax(stamp, k+1) < k+1 <=> first to tag it), so neither the list
+// nor the flags need clearing.  Three rotating counters: pass k reads
+// count[k%3], appends to count[(k+1)%3] and zeroes count[(k+2)%3], which no
+// later pass of this launch touches before pass k+1.  Every launch uses the
+// same grid; workgroups stride over the list.  The host reads the next
+// count every `check_interval` passes (one small copy + sync).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <mutex>
+#include <unordered_map>
+#include <vector>
+
+#include "common.hpp"
+#include "device.hpp"
+#include "worklist.hpp"
+
+namespace agd {
+
+template <class S> struct EikArgs {
+  S* u;
+  const S* geom;
+  const S* rhs;
+  const uint32_t* seeds;
+  const uint8_t* wall;
+  GridDesc g;
+  S atol, rtol;
+  S iso_w[3];
+  int32_t niter_i;
+  int32_t solver;
+  int32_t fim_front;
+  int32_t iter;
+  // worklist
+  const int32_t* list_in;
+  const int32_t* count_in;
+  int32_t* list_out;
+  int32_t* count_out;
+  int32_t* count_clear;
+  int32_t* stamp;
+  int32_t* hist;
+  // FIM scores, packed (iteration << 8) | score, two arrays by parity
+  const int32_t* score_prev;
+  int32_t* score_next;
+};
+
+__device__ inline int32_t read_score(const int32_t* arr, int32_t tile, int32_t iter) {
+  const int32_t v = arr[tile];
+  return ((v >> 8) == iter) ? (v & 255) : 0;
+}
+
+// ------------------------------------------------------------------ update
+template <class S, int MODEL, int D, int T0, int T1, int T2>
+__global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
+  using TL = Tile<D, T0, T1, T2>;
+  using MT = ModelTraits<MODEL, D>;
+  constexpr int NI = TL::NI;
+  constexpr int NS = MT::NSYM;
+  constexpr int NG = MT::NGEOM;
+  __shared__ S u_s[NI];
+  __shared__ uint8_t wall_s[NI];
+  __shared__ int32_t score_s[8];
+
+  const int tid = threadIdx.x;
+  const int count = *a.count_in;
+  if (blockIdx.x == 0 && tid == 0) {
+    if (a.count_clear) *a.count_clear = 0;
+    a.hist[a.iter] = count;
+  }
+  int xi[3];
+  TL::pos(tid, xi);
+  const int32_t tag = a.iter + 1;
+
+  for (int li = blockIdx.x; li < count; li += gridDim.x) {
+    const int n_o = a.list_in[li];
+    int xo[3];
+    tile_pos<D>(n_o, a.g, xo);
+    int xt[3] = {0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < D; ++k) xt[k] = xo[k] * TL::T[k] + xi[k];
+    const int64_t n_t = int64_t(n_o) * NI + tid;
+
+    // ---- loads (coalesced within the tile)
+    const S u_old = a.u[n_t];
+    const S rhs = a.rhs[n_t];
+    const bool seed = (a.seeds[n_t >> 5] >> (n_t & 31)) & 1u;
+    S geo[NG > 0 ? NG : 1];
+#pragma unroll
+    for (int k = 0; k < NG; ++k) geo[k] = a.geom[k * a.g.size_tot + n_t];
+    int wx = 255;
+    if (a.wall) {
+      wx = a.wall[n_t];
+      wall_s[tid] = uint8_t(wx);
+    }
+    u_s[tid] = seed ? rhs : u_old;
+
+    Stencil<S, MODEL, D> st;
+    make_stencil<S, MODEL, D>(geo, a.iso_w, st);
+    if (a.wall) __syncthreads();  // wall_s complete before the visibility walks
+
+    // ---- resolve neighbours: LDS slot (tile) or gathered value (outside)
+    int nidx[2 * NS];
+    S vo[2 * NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {
+        const int sg = 2 * s - 1;
+        int e[3] = {sg * st.e[k][0], sg * st.e[k][1], sg * st.e[k][2]};
+        const S sh = (s == 0) ? st.shift[k] : -st.shift[k];
+        bool vis = true;
+        if (a.wall && wx != 255) {
+          vis = visible<D>(e, wx, [&](const int* w) -> int {
+            int yi[3] = {xi[0] + w[0], xi[1] + w[1], xi[2] + w[2]};
+            int yt[3] = {xt[0] + w[0], xt[1] + w[1], xt[2] + w[2]};
+            bool dom = true;
+#pragma unroll
+            for (int c = 0; c < D; ++c)
+              dom = dom && (((a.g.periodic >> c) & 1) || (yt[c] >= 0 && yt[c] < a.g.shape_tot[c]));
+            if (TL::inside(yi) && dom) return wall_s[TL::index(yi)];
+            const int64_t m = index_tot<D, TL>(yt, a.g);
+            return m >= 0 ? int(a.wall[m]) : 255;
+          });
+        }
+        int yi[3] = {xi[0] + e[0], xi[1] + e[1], xi[2] + e[2]};
+        int yt[3] = {xt[0] + e[0], xt[1] + e[1], xt[2] + e[2]};
+        bool in_dom = true;
+#pragma unroll
+        for (int c = 0; c < D; ++c) in_dom = in_dom && (yt[c] >= 0 && yt[c] < a.g.shape_tot[c]);
+        const int j = 2 * k + s;
+        if (vis && TL::inside(yi) && in_dom) {
+          nidx[j] = TL::index(yi);
+          vo[j] = sh;
+        } else {
+          nidx[j] = -1;
+          const int64_t m = vis ? index_tot<D, TL>(yt, a.g) : -1;
+          vo[j] = (m >= 0) ? a.u[m] + sh : inf_v<S>();
+        }
+      }
+    }
+    __syncthreads();  // u_s complete
+
+    // ---- in-tile sweeps (HFMIter.h:25-84, DECREASING)
+    const bool active = !seed;
+    for (int it = 0; it < a.niter_i; ++it) {
+      S v[NS], w[NS];
+#pragma unroll
+      for (int k = 0; k < NS; ++k) {
+        const S a0 = nidx[2 * k] >= 0 ? u_s[nidx[2 * k]] + vo[2 * k] : vo[2 * k];
+        const S a1 = nidx[2 * k + 1] >= 0 ? u_s[nidx[2 * k + 1]] + vo[2 * k + 1] : vo[2 * k + 1];
+        v[k] = (a0 > a1) ? a1 : a0;  // Greater(v,-) keeps the -e side on ties
+        w[k] = st.w[k];
+      }
+      const S un = hfm_solve<S, NS>(v, w, rhs);
+      __syncthreads();
+      const bool ch = active && (u_s[tid] > un);
+      if (ch) u_s[tid] = un;
+      if (!__syncthreads_or(ch)) break;
+    }
+
+    // ---- store and convergence (Update.h:163-219)
+    const S un = u_s[tid];
+    a.u[n_t] = un;
+    const S diff = abs_(u_old - un);
+    const S tol = a.atol + a.rtol * abs_(un);
+    const bool changed = !(isnan_(diff) || diff <= tol);
+    const bool any = __syncthreads_or(changed);
+
+    // ---- activation (Propagation.h:103-167)
+    // (The global-iteration solver runs with AGSI tags: they only decide
+    // when to stop, every pass visits the full list.)
+    if (a.solver == AGD_AGSI) {
+      if (any && tid < 2 * D + 1) {
+        const int nb = (tid == 2 * D) ? n_o : tile_neighbor<D>(xo, tid >> 1, 2 * (tid & 1) - 1, a.g);
+        if (nb >= 0) enqueue(nb, tag, a.stamp, a.list_out, a.count_out);
+      }
+    } else {  // FIM
+      if (tid < 2 * D + 1) {
+        const int nb = (tid == 2 * D) ? n_o : tile_neighbor<D>(xo, tid >> 1, 2 * (tid & 1) - 1, a.g);
+        score_s[tid] = nb >= 0 ? read_score(a.score_prev, nb, a.iter - 1) : 0;
+      }
+      __syncthreads();
+      const int front = a.fim_front;
+      const int prev = score_s[2 * D];
+      int next;
+      bool prop;
+      if (!any) {
+        if (prev == 0 || prev == front) { next = 0; prop = false; }
+        else { next = front; prop = true; }
+      } else {
+        next = 1;
+        if (front > 2) {
+#pragma unroll
+          for (int i = 0; i < 2 * D; ++i) next = max(next, score_s[i] - 1);
+        }
+        prop = next > 1;
+      }
+      if (tid == 0) {
+        a.score_next[n_o] = (a.iter << 8) | next;
+        if (any) enqueue(n_o, tag, a.stamp, a.list_out, a.count_out);
+      }
+      if (prop && tid < 2 * D) {
+        const int nb = tile_neighbor<D>(xo, tid >> 1, 2 * (tid & 1) - 1, a.g);
+        if (nb >= 0) enqueue(nb, tag, a.stamp, a.list_out, a.count_out);
+      }
+    }
+    __syncthreads();  // LDS reuse by the next tile of this workgroup
+  }
+}
+
+// Initial list: tiles holding at least one trigger bit (or every tile).
+template <int NI>
+__global__ void eik_init_list(const uint32_t* trigger, int32_t ntiles, int32_t all,
+                              int32_t* stamp, int32_t* list, int32_t* count) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntiles) return;
+  bool on = all != 0;
+  if (!on) {
+    const int64_t b0 = int64_t(t) * NI, b1 = b0 + NI;
+    for (int64_t b = b0; b < b1 && !on; ) {
+      const int64_t wi = b >> 5;
+      const int lo = int(b & 31);
+      const int64_t lim = std::min<int64_t>(b1, (wi + 1) << 5);
+      const int hi = int(lim - (wi << 5));
+      uint32_t word = trigger[wi];
+      uint32_t mask = (hi >= 32 ? 0xffffffffu : ((1u << hi) - 1u)) & ~((1u << lo) - 1u);
+      on = (word & mask) != 0;
+      b = lim;
+    }
+  }
+  if (on) enqueue(t, 0, stamp, list, count);
+}
+
+__global__ void iota_kernel(int32_t* list, int32_t n, int32_t* count) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) list[t] = t;
+  if (t == 0) *count = n;
+}
+
+// ------------------------------------------------------------------ flow
+// One global pass with niter_i = 1 and flow export (Update.h:149-209).
+template <class S> struct FlowArgs {
+  S* u;
+  const S* geom;
+  const S* rhs;
+  const uint32_t* seeds;
+  const uint8_t* wall;
+  GridDesc g;
+  S iso_w[3];
+  S* fw;
+  S* wsum;
+  int32_t* fidx;
+  S* fvec;
+};
+
+template <class S, int MODEL, int D, int T0, int T1, int T2>
+__global__ void __launch_bounds__(T0* T1* T2) eik_flow(FlowArgs<S> a) {
+  using TL = Tile<D, T0, T1, T2>;
+  using MT = ModelTraits<MODEL, D>;
+  constexpr int NI = TL::NI;
+  constexpr int NS = MT::NSYM;
+  constexpr int NG = MT::NGEOM;
+  const int tid = threadIdx.x;
+  const int n_o = blockIdx.x;
+  int xi[3], xo[3], xt[3] = {0, 0, 0};
+  TL::pos(tid, xi);
+  tile_pos<D>(n_o, a.g, xo);
+#pragma unroll
+  for (int k = 0; k < D; ++k) xt[k] = xo[k] * TL::T[k] + xi[k];
+  const int64_t n_t = int64_t(n_o) * NI + tid;
+  const int64_t N = a.g.size_tot;
+
+  const S u_old = a.u[n_t];
+  const S rhs = a.rhs[n_t];
+  const bool seed = (a.seeds[n_t >> 5] >> (n_t & 31)) & 1u;
+  S geo[NG > 0 ? NG : 1];
+#pragma unroll
+  for (int k = 0; k < NG; ++k) geo[k] = a.geom[k * N + n_t];
+  const int wx = a.wall ? int(a.wall[n_t]) : 255;
+  Stencil<S, MODEL, D> st;
+  make_stencil<S, MODEL, D>(geo, a.iso_w, st);
+
+  S v[NS], w[NS];
+  int side[NS];
+  int64_t nb[NS];
+#pragma unroll
+  for (int k = 0; k < NS; ++k) {
+    S best = inf_v<S>();
+    int bs = 0;
+    int64_t bi = -1;
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const int sg = 2 * s - 1;
+      int e[3] = {sg * st.e[k][0], sg * st.e[k][1], sg * st.e[k][2]};
+      const S sh = (s == 0) ? st.shift[k] : -st.shift[k];
+      bool vis = true;
+      if (a.wall && wx != 255) {
+        vis = visible<D>(e, wx, [&](const int* ww) -> int {
+          int yt[3] = {xt[0] + ww[0], xt[1] + ww[1], xt[2] + ww[2]};
+          const int64_t m = index_tot<D, TL>(yt, a.g);
+          return m >= 0 ? int(a.wall[m]) : 255;
+        });
+      }
+      int yt[3] = {xt[0] + e[0], xt[1] + e[1], xt[2] + e[2]};
+      const int64_t m = vis ? index_tot<D, TL>(yt, a.g) : -1;
+      const S val = (m >= 0) ? a.u[m] + sh : inf_v<S>();
+      if (s == 0) { best = val; bs = 0; bi = m; }
+      else if (best > val) { best = val; bs = 1; bi = m; }
+    }
+    v[k] = best;
+    side[k] = bs;
+    nb[k] = bi;
+    w[k] = st.w[k];
+  }
+  S vs[NS], ws[NS];
+#pragma unroll
+  for (int k = 0; k < NS; ++k) { vs[k] = v[k]; ws[k] = w[k]; }
+  const S val = hfm_solve<S, NS>(vs, ws, rhs);
+
+  S fw[NS];
+  S sum = S(0);
+  S vec[3] = {S(0), S(0), S(0)};
+  const bool live = !seed && val != inf_v<S>();
+#pragma unroll
+  for (int k = 0; k < NS; ++k) {
+    const S diff = val - v[k];
+    fw[k] = live ? w[k] * (diff > S(0) ? diff : S(0)) : S(0);
+    sum += fw[k];
+    const int sg = 2 * side[k] - 1;
+#pragma unroll
+    for (int c = 0; c < D; ++c) vec[c] += fw[k] * S(sg * st.e[k][c]);
+  }
+  __syncthreads();  // every lane has read its neighbours before any store
+  if (!seed && val < u_old) a.u[n_t] = val;
+  if (a.fw) {
+#pragma unroll
+    for (int k = 0; k < NS; ++k) a.fw[k * N + n_t] = fw[k];
+  }
+  if (a.fidx) {
+#pragma unroll
+    for (int k = 0; k < NS; ++k)
+      a.fidx[k * N + n_t] = (fw[k] != S(0) && nb[k] >= 0) ? int32_t(nb[k]) : INT32_MAX;
+  }
+  if (a.wsum) a.wsum[n_t] = sum;
+  if (a.fvec) {
+#pragma unroll
+    for (int c = 0; c < D; ++c) a.fvec[c * N + n_t] = vec[c];
+  }
+}
+
+// ------------------------------------------------------------------ stencil export
+template <class S, int MODEL, int D>
+__global__ void stencil_export_kernel(int64_t n, const S* geom, S iw0, S iw1, S iw2, S* wout,
+                                      int32_t* eout) {
+  using MT = ModelTraits<MODEL, D>;
+  constexpr int NS = MT::NSYM;
+  constexpr int NG = MT::NGEOM;
+  const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  S geo[NG > 0 ? NG : 1];
+#pragma unroll
+  for (int k = 0; k < NG; ++k) geo[k] = geom[k * n + c];
+  S iw[3] = {iw0, iw1, iw2};
+  Stencil<S, MODEL, D> st;
+  make_stencil<S, MODEL, D>(geo, iw, st);
+#pragma unroll
+  for (int k = 0; k < NS; ++k) {
+    wout[c * NS + k] = st.w[k];
+#pragma unroll
+    for (int l = 0; l < D; ++l) eout[(c * NS + k) * D + l] = st.e[k][l];
+  }
+}
+
+// ------------------------------------------------------------------ seeds / walls
+template <class S>
+__global__ void scatter_seeds_kernel(int64_t n, const int64_t* idx, const double* val, S* rhs,
+                                     uint32_t* seeds, uint32_t* trigger) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t m = idx[i];
+  rhs[m] = S(val[i]);
+  atomicOr(seeds + (m >> 5), 1u << (m & 31));
+  if (trigger) atomicOr(trigger + (m >> 5), 1u << (m & 31));
+}
+
+template <class S>
+__global__ void walls_mark_kernel(int64_t N, const uint8_t* walls, S* rhs, uint32_t* seeds,
+                                  uint8_t* dist) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const bool w = walls[i] != 0;
+  dist[i] = w ? 0 : 255;
+  if (w) {
+    rhs[i] = inf_v<S>();
+    atomicOr(seeds + (i >> 5), 1u << (i & 31));
+  }
+}
+
+// One min-plus pass with the L1 unit kernel (Kernel_InfConvolution.h:66-91
+// with distance_kernel(1, ndim, ord=1)), saturated at 255.  Operates on the
+// tiled layout; padded cells stay at 255.
+template <int D, int T0, int T1, int T2>
+__global__ void wall_dist_pass(GridDesc g, const uint8_t* in, uint8_t* out) {
+  using TL = Tile<D, T0, T1, T2>;
+  const int n_o = blockIdx.x;
+  const int tid = threadIdx.x;
+  int xi[3], xo[3], xt[3] = {0, 0, 0};
+  TL::pos(tid, xi);
+  tile_pos<D>(n_o, g, xo);
+  bool dom = true;
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    xt[k] = xo[k] * TL::T[k] + xi[k];
+    dom = dom && xt[k] < g.shape_tot[k];
+  }
+  const int64_t n_t = int64_t(n_o) * TL::NI + tid;
+  if (!dom) { out[n_t] = 255; return; }
+  int best = in[n_t];
+  // 3^D neighbourhood, L1 kernel weights.
+  const int r2 = (D == 3) ? 1 : 0;
+  for (int d0 = -1; d0 <= 1; ++d0)
+    for (int d1 = -1; d1 <= 1; ++d1)
+      for (int d2 = -r2; d2 <= r2; ++d2) {
+        int y[3] = {xt[0] + d0, xt[1] + d1, xt[2] + d2};
+        const int64_t m = index_tot<D, TL>(y, g);
+        if (m < 0) continue;
+        const int kw = (d0 < 0 ? -d0 : d0) + (d1 < 0 ? -d1 : d1) + (d2 < 0 ? -d2 : d2);
+        const int vv = in[m];
+        const int p = (vv >= 255 - kw) ? 255 : vv + kw;
+        best = p < best ? p : best;
+      }
+  out[n_t] = uint8_t(best);
+}
+
+__global__ void wall_dist_cap(int64_t N, uint8_t* d, int bound) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < N && d[i] > bound) d[i] = 255;
+}
+
+// ------------------------------------------------------------------ dispatch
+template <class S> struct KernelSet {
+  void (*update)(EikArgs<S>);
+  void (*flow)(FlowArgs<S>);
+  void (*init)(const uint32_t*, int32_t, int32_t, int32_t*, int32_t*, int32_t*);
+  void (*wallpass)(GridDesc, const uint8_t*, uint8_t*);
+  int ni;
+};
+
+template <class S, int MODEL, int D, int T0, int T1, int T2>
+KernelSet<S> make_set() {
+  KernelSet<S> k;
+  k.update = eik_update<S, MODEL, D, T0, T1, T2>;
+  k.flow = eik_flow<S, MODEL, D, T0, T1, T2>;
+  k.init = eik_init_list<T0 * T1 * T2>;
+  k.wallpass = wall_dist_pass<D, T0, T1, T2>;
+  k.ni = T0 * T1 * T2;
+  return k;
+}
+
+static int tile_key(int d, const int32_t* si) {
+  return d == 2 ? (si[0] * 100 + si[1]) : (si[0] * 10000 + si[1] * 100 + si[2]);
+}
+
+template <class S, int MODEL>
+static bool lookup_model(int d, const int32_t* si, KernelSet<S>& out) {
+  const int key = tile_key(d, si);
+  if (d == 2) {
+    switch (key) {
+      case 404: out = make_set<S, MODEL, 2, 4, 4, 1>(); return true;
+      case 808: out = make_set<S, MODEL, 2, 8, 8, 1>(); return true;
+      case 1616: out = make_set<S, MODEL, 2, 16, 16, 1>(); return true;
+      case 816: out = make_set<S, MODEL, 2, 8, 16, 1>(); return true;
+    }
+  } else if (d == 3) {
+    switch (key) {
+      case 40404: out = make_set<S, MODEL, 3, 4, 4, 4>(); return true;
+      case 80804: out = make_set<S, MODEL, 3, 8, 8, 4>(); return true;
+      case 40408: out = make_set<S, MODEL, 3, 4, 4, 8>(); return true;
+    }
+  }
+  return false;
+}
+
+template <class S>
+static bool lookup(int model, int d, const int32_t* si, KernelSet<S>& out) {
+  switch (model) {
+    case AGD_ISOTROPIC: return lookup_model<S, AGD_ISOTROPIC>(d, si, out);
+    case AGD_DIAGONAL: return lookup_model<S, AGD_DIAGONAL>(d, si, out);
+    case AGD_RIEMANN: return lookup_model<S, AGD_RIEMANN>(d, si, out);
+    case AGD_RANDER: return lookup_model<S, AGD_RANDER>(d, si, out);
+  }
+  return false;
+}
+
+static int check_desc(const agd_eikonal_desc* d) {
+  AGD_REQUIRE(d != nullptr, "null descriptor");
+  AGD_REQUIRE(d->ndim == 2 || d->ndim == 3, "ndim must be 2 or 3");
+  AGD_REQUIRE(d->model >= AGD_ISOTROPIC && d->model <= AGD_RANDER, "unknown model");
+  AGD_REQUIRE(d->scalar == AGD_F32 || d->scalar == AGD_F64, "unknown scalar type");
+  for (int k = 0; k < d->ndim; ++k) {
+    AGD_REQUIRE(d->shape[k] >= 1 && d->shape[k] < (1 << 30), "shape out of range");
+    AGD_REQUIRE(d->shape_i[k] >= 1, "tile shape must be positive");
+  }
+  AGD_REQUIRE(d->niter_i >= 1, "niter_i must be >= 1");
+  AGD_REQUIRE(d->nitermax_o >= 1, "nitermax_o must be >= 1");
+  return AGD_OK;
+}
+
+static GridDesc make_grid(const agd_eikonal_desc* d) {
+  GridDesc g{};
+  g.ntiles = 1;
+  int64_t ni = 1;
+  for (int k = 0; k < 3; ++k) {
+    g.shape_tot[k] = k < d->ndim ? int32_t(d->shape[k]) : 1;
+    g.shape_o[k] = k < d->ndim ? int32_t(ceil_div(d->shape[k], d->shape_i[k])) : 1;
+    if (k < d->ndim) {
+      g.ntiles *= g.shape_o[k];
+      ni *= d->shape_i[k];
+    }
+  }
+  g.periodic = 0;
+  for (int k = 0; k < d->ndim; ++k)
+    if (d->periodic[k]) g.periodic |= (1 << k);
+  g.size_tot = int64_t(g.ntiles) * ni;
+  return g;
+}
+
+// Solver workspace, cached per device and grown on demand.
+static std::mutex g_ws_mutex;
+static std::unordered_map<int, Workspace> g_ws;
+
+int get_workspace(int64_t ntiles, int64_t nhist, Workspace*& out) {
+  int dev = 0;
+  AGD_HIP_TRY(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(g_ws_mutex);
+  Workspace& w = g_ws[dev];
+  if (!w.ev0) {
+    AGD_HIP_TRY(hipEventCreate(&w.ev0));
+    AGD_HIP_TRY(hipEventCreate(&w.ev1));
+    AGD_HIP_TRY(hipHostMalloc((void**)&w.host_count, 64, hipHostMallocDefault));
+    AGD_HIP_TRY(hipMalloc((void**)&w.counts, 8 * sizeof(int32_t)));
+  }
+  if (w.cap < ntiles) {
+    if (w.stamp) {
+      AGD_HIP_TRY(hipFree(w.stamp));
+      AGD_HIP_TRY(hipFree(w.lists));
+      AGD_HIP_TRY(hipFree(w.score));
+    }
+    const int64_t cap = std::max<int64_t>(ntiles, 1024);
+    AGD_HIP_TRY(hipMalloc((void**)&w.stamp, cap * sizeof(int32_t)));
+    AGD_HIP_TRY(hipMalloc((void**)&w.lists, 2 * cap * sizeof(int32_t)));
+    AGD_HIP_TRY(hipMalloc((void**)&w.score, 2 * cap * sizeof(int32_t)));
+    w.cap = cap;
+  }
+  if (w.hist_cap < nhist) {
+    if (w.hist) AGD_HIP_TRY(hipFree(w.hist));
+    const int64_t cap = std::max<int64_t>(nhist, 4096);
+    AGD_HIP_TRY(hipMalloc((void**)&w.hist, cap * sizeof(int32_t)));
+    w.hist_cap = cap;
+  }
+  out = &w;
+  return AGD_OK;
+}
+
+int blocks_for(int ntiles, int lanes_per_block) {
+  static int cus = 0;
+  if (cus == 0) {
+    int dev = 0;
+    hipDeviceProp_t p;
+    cus = (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&p, dev) == hipSuccess)
+              ? p.multiProcessorCount
+              : 256;
+  }
+  // Enough resident waves per CU to hide the neighbour-gather latency; the
+  // workgroups stride over the tile list.
+  const int waves_per_block = (lanes_per_block + 63) / 64;
+  const int target = cus * std::max(1, 32 / waves_per_block);
+  return std::max(1, std::min(ntiles, target));
+}
+
+template <class S>
+static int solve_impl(const agd_eikonal_desc* d, const agd_eikonal_io* io,
+                      agd_eikonal_stats* stats, hipStream_t stream) {
+  KernelSet<S> ks;
+  AGD_REQUIRE(lookup<S>(d->model, d->ndim, d->shape_i, ks), "unsupported tile shape for this model");
+  AGD_REQUIRE(io && io->values && io->rhs && io->seeds, "values, rhs and seeds are required");
+  AGD_REQUIRE(d->model == AGD_ISOTROPIC || io->geom, "geometry required for this model");
+  AGD_REQUIRE(d->solver != AGD_FIM || d->fim_front_width >= 2, "fim_front_width must be >= 2");
+  const GridDesc g = make_grid(d);
+  const int ni = ks.ni;
+  const int blocks = blocks_for(g.ntiles, ni);
+  const bool global = d->solver == AGD_GLOBAL;
+
+  EikArgs<S> a{};
+  a.u = static_cast<S*>(io->values);
+  a.geom = static_cast<const S*>(io->geom);
+  a.rhs = static_cast<const S*>(io->rhs);
+  a.seeds = io->seeds;
+  a.wall = io->wall_dist;
+  a.g = g;
+  a.atol = S(d->atol);
+  a.rtol = S(d->rtol);
+  for (int k = 0; k < 3; ++k) a.iso_w[k] = S(d->iso_weights[k]);
+  a.niter_i = d->niter_i;
+  a.solver = global ? AGD_AGSI : d->solver;
+  a.fim_front = d->fim_front_width;
+
+  Workspace* wsp = nullptr;
+  auto init = [&](Workspace* ws, int32_t* list0, int32_t* counts) -> int {
+    wsp = ws;
+    a.stamp = ws->stamp;
+    a.hist = ws->hist;
+    AGD_HIP_TRY(hipMemsetAsync(ws->score, 0x7f, size_t(2 * ws->cap) * sizeof(int32_t), stream));
+    if (global) {
+      // Every pass visits the full list (kept in the score buffer, unused
+      // by this solver); the AGSI tags only decide when to stop.
+      hipLaunchKernelGGL(iota_kernel, dim3(unsigned(ceil_div(g.ntiles, 256))), dim3(256), 0,
+                         stream, ws->score, g.ntiles, counts + 7);
+      hipLaunchKernelGGL(iota_kernel, dim3(unsigned(ceil_div(g.ntiles, 256))), dim3(256), 0,
+                         stream, list0, g.ntiles, counts + 0);
+    } else {
+      const uint32_t* trig = io->trigger ? io->trigger : io->seeds;
+      hipLaunchKernelGGL(ks.init, dim3(unsigned(ceil_div(g.ntiles, 256))), dim3(256), 0, stream,
+                         trig, g.ntiles, d->activate_all, ws->stamp, list0, counts + 0);
+    }
+    return AGD_OK;
+  };
+  auto launch = [&](const PassPointers& p) {
+    a.list_in = global ? wsp->score : p.list_in;
+    a.count_in = global ? wsp->counts + 7 : p.count_in;
+    a.list_out = p.list_out;
+    a.count_out = p.count_out;
+    a.count_clear = p.count_clear;
+    a.iter = p.iter;
+    a.score_prev = wsp->score + ((p.iter + 1) % 2) * wsp->cap;
+    a.score_next = wsp->score + (p.iter % 2) * wsp->cap;
+    hipLaunchKernelGGL(ks.update, dim3(blocks), dim3(ni), 0, stream, a);
+  };
+  WorklistResult r;
+  int st = run_worklist(g.ntiles, d->nitermax_o, d->check_interval, stream, init, launch, r);
+  if (st) return st;
+  if (stats) {
+    stats->niter_o = r.niter;
+    stats->tile_updates = r.visits;
+    stats->cell_updates = r.visits * ni;
+    stats->launches = r.passes;
+    stats->gpu_ms = r.ms;
+    stats->converged = r.converged ? 1 : 0;
+    stats->grid_blocks = blocks;
+  }
+  if (!r.converged) {
+    set_error("eikonal solver did not converge within nitermax_o passes");
+    return AGD_NOCONV;
+  }
+  return AGD_OK;
+}
+
+template <class S>
+static int flow_impl(const agd_eikonal_desc* d, const agd_eikonal_io* io, const agd_flow_io* f,
+                     hipStream_t stream) {
+  KernelSet<S> ks;
+  AGD_REQUIRE(lookup<S>(d->model, d->ndim, d->shape_i, ks), "unsupported tile shape for this model");
+  AGD_REQUIRE(io && io->values && io->rhs && io->seeds && f, "missing buffers");
+  const GridDesc g = make_grid(d);
+  FlowArgs<S> a{};
+  a.u = static_cast<S*>(io->values);
+  a.geom = static_cast<const S*>(io->geom);
+  a.rhs = static_cast<const S*>(io->rhs);
+  a.seeds = io->seeds;
+  a.wall = io->wall_dist;
+  a.g = g;
+  for (int k = 0; k < 3; ++k) a.iso_w[k] = S(d->iso_weights[k]);
+  a.fw = static_cast<S*>(f->flow_weights);
+  a.wsum = static_cast<S*>(f->flow_weightsum);
+  a.fidx = f->flow_indices;
+  a.fvec = static_cast<S*>(f->flow_vector);
+  hipLaunchKernelGGL(ks.flow, dim3(g.ntiles), dim3(ks.ni), 0, stream, a);
+  AGD_HIP_TRY(hipGetLastError());
+  return AGD_OK;
+}
+
+}  // namespace agd
+
+using namespace agd;
+
+extern "C" {
+
+int agd_tile_supported(int32_t model, int32_t ndim, const int32_t* shape_i) {
+  KernelSet<float> ks;
+  return (shape_i && lookup<float>(model, ndim, shape_i, ks)) ? 1 : 0;
+}
+
+int agd_eikonal_solve(const agd_eikonal_desc* desc, const agd_eikonal_io* io,
+                      agd_eikonal_stats* stats, void* stream) {
+  int st = check_desc(desc);
+  if (st) return st;
+  if (desc->scalar == AGD_F32) return solve_impl<float>(desc, io, stats, as_stream(stream));
+  return solve_impl<double>(desc, io, stats, as_stream(stream));
+}
+
+int agd_eikonal_flow(const agd_eikonal_desc* desc, const agd_eikonal_io* io,
+                     const agd_flow_io* flow, void* stream) {
+  int st = check_desc(desc);
+  if (st) return st;
+  if (desc->scalar == AGD_F32) return flow_impl<float>(desc, io, flow, as_stream(stream));
+  return flow_impl<double>(desc, io, flow, as_stream(stream));
+}
+
+int agd_stencil_export(int32_t model, int32_t ndim, int32_t scalar, int64_t n, const void* geom,
+                       const double* iso_weights, void* weights, int32_t* offsets, void* stream) {
+  AGD_REQUIRE(ndim == 2 || ndim == 3, "ndim must be 2 or 3");
+  AGD_REQUIRE(n >= 0 && weights && offsets, "bad buffers");
+  AGD_REQUIRE(model == AGD_ISOTROPIC || geom, "geometry required");
+  if (n == 0) return AGD_OK;
+  const double iw[3] = {iso_weights ? iso_weights[0] : 0., iso_weights ? iso_weights[1] : 0.,
+                        iso_weights ? iso_weights[2] : 0.};
+  hipStream_t s = as_stream(stream);
+  const dim3 grid(unsigned(ceil_div(n, 256))), block(256);
+#define AGD_EXPORT(S, M, D)                                                                  \
+  hipLaunchKernelGGL((stencil_export_kernel<S, M, D>), grid, block, 0, s, n,                 \
+                     static_cast<const S*>(geom), S(iw[0]), S(iw[1]), S(iw[2]),              \
+                     static_cast<S*>(weights), offsets)
+#define AGD_EXPORT_M(S)                                                         \
+  switch (model * 10 + ndim) {                                                  \
+    case AGD_ISOTROPIC * 10 + 2: AGD_EXPORT(S, AGD_ISOTROPIC, 2); break;        \
+    case AGD_ISOTROPIC * 10 + 3: AGD_EXPORT(S, AGD_ISOTROPIC, 3); break;        \
+    case AGD_DIAGONAL * 10 + 2: AGD_EXPORT(S, AGD_DIAGONAL, 2); break;          \
+    case AGD_DIAGONAL * 10 + 3: AGD_EXPORT(S, AGD_DIAGONAL, 3); break;          \
+    case AGD_RIEMANN * 10 + 2: AGD_EXPORT(S, AGD_RIEMANN, 2); break;            \
+    case AGD_RIEMANN * 10 + 3: AGD_EXPORT(S, AGD_RIEMANN, 3); break;            \
+    case AGD_RANDER * 10 + 2: AGD_EXPORT(S, AGD_RANDER, 2); break;              \
+    case AGD_RANDER * 10 + 3: AGD_EXPORT(S, AGD_RANDER, 3); break;              \
+    default: set_error("invalid argument: unknown model"); return AGD_EINVAL;   \
+  }
+  if (scalar == AGD_F32) { AGD_EXPORT_M(float) }
+  else { AGD_EXPORT_M(double) }
+#undef AGD_EXPORT_M
+#undef AGD_EXPORT
+  AGD_HIP_TRY(hipGetLastError());
+  return AGD_OK;
+}
+
+int agd_scatter_seeds(int32_t scalar, int64_t nseeds, const int64_t* tiled_index,
+                      const double* seed_values, void* rhs, uint32_t* seeds, uint32_t* trigger,
+                      void* stream) {
+  AGD_REQUIRE(nseeds >= 0, "negative seed count");
+  if (nseeds == 0) return AGD_OK;
+  AGD_REQUIRE(tiled_index && seed_values && rhs && seeds, "null buffer");
+  hipStream_t s = as_stream(stream);
+  const dim3 grid(unsigned(ceil_div(nseeds, 256))), block(256);
+  if (scalar == AGD_F32)
+    hipLaunchKernelGGL(scatter_seeds_kernel<float>, grid, block, 0, s, nseeds, tiled_index,
+                       seed_values, static_cast<float*>(rhs), seeds, trigger);
+  else
+    hipLaunchKernelGGL(scatter_seeds_kernel<double>, grid, block, 0, s, nseeds, tiled_index,
+                       seed_values, static_cast<double*>(rhs), seeds, trigger);
+  AGD_HIP_TRY(hipGetLastError());
+  return AGD_OK;
+}
+
+int agd_prepare_walls(const agd_eikonal_desc* desc, const uint8_t* walls, int32_t bound,
+                      void* rhs, uint32_t* seeds, uint8_t* wall_dist, void* stream) {
+  int st = check_desc(desc);
+  if (st) return st;
+  AGD_REQUIRE(walls && rhs && seeds && wall_dist, "null buffer");
+  AGD_REQUIRE(bound >= 0 && bound < 255, "wallDistBound must lie in [0,255)");
+  KernelSet<float> ks;
+  AGD_REQUIRE(lookup<float>(desc->model, desc->ndim, desc->shape_i, ks), "unsupported tile shape");
+  const GridDesc g = make_grid(desc);
+  hipStream_t s = as_stream(stream);
+  const int64_t N = g.size_tot;
+  const dim3 grid(unsigned(ceil_div(N, 256))), block(256);
+  if (desc->scalar == AGD_F32)
+    hipLaunchKernelGGL(walls_mark_kernel<float>, grid, block, 0, s, N, walls,
+                       static_cast<float*>(rhs), seeds, wall_dist);
+  else
+    hipLaunchKernelGGL(walls_mark_kernel<double>, grid, block, 0, s, N, walls,
+                       static_cast<double*>(rhs), seeds, wall_dist);
+  AGD_HIP_TRY(hipGetLastError());
+  if (bound > 0) {
+    uint8_t* tmp = nullptr;
+    AGD_HIP_TRY(hipMalloc((void**)&tmp, N));
+    uint8_t* src = wall_dist;
+    uint8_t* dst = tmp;
+    for (int i = 0; i < bound; ++i) {
+      hipLaunchKernelGGL(ks.wallpass, dim3(g.ntiles), dim3(ks.ni), 0, s, g, src, dst);
+      std::swap(src, dst);
+    }
+    if (src != wall_dist)
+      AGD_HIP_TRY(hipMemcpyAsync(wall_dist, src, N, hipMemcpyDeviceToDevice, s));
+    AGD_HIP_TRY(hipStreamSynchronize(s));
+    AGD_HIP_TRY(hipFree(tmp));
+  }
+  hipLaunchKernelGGL(wall_dist_cap, grid, block, 0, s, N, wall_dist, bound);
+  AGD_HIP_TRY(hipGetLastError());
+  return AGD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,135 @@
+"""Synthetic eikonal workloads used by the benchmark and the tests.
+
+Each generator returns a Workload: the model name, the box and grid, the
+float32 input fields and the seed points, plus `dict_in()` building the
+matching Eikonal.dictIn.  Random fields come from numpy's PCG64 with the
+seed given, so a workload is reproducible on any host.
+
+  isotropic2_bumps   2D isotropic, cost 1 + 0.5 tanh(sum of 32 Gaussian
+                     bumps), seed at the origin            (512^2 headline)
+  riemann2_aniso     2D Riemannian, M = R(theta)^T diag(1, a^2) R(theta),
+                     theta and log a smooth random fields   (2000^2 headline)
+  riemann3_aniso     3D Riemannian, M = Q diag(l1,l2,l3) Q^T, Q from a smooth
+                     quaternion field, log l smooth in [0, ln 5]  (200^3)
+"""
+from dataclasses import dataclass, field
+
+import numpy as np
+from scipy.ndimage import gaussian_filter
+
+
+@dataclass
+class Workload:
+    name: str
+    model: str
+    sides: list
+    dims: tuple
+    seeds: np.ndarray                 # physical seed points (n, d)
+    metric: np.ndarray = None         # (d, d, *dims) float32 primal metric
+    cost: np.ndarray = None           # (*dims) float32 or None (=1)
+    receivers: np.ndarray = None      # (n, d) integer cells
+    extra: dict = field(default_factory=dict)
+
+    @property
+    def ndim(self):
+        return len(self.dims)
+
+    @property
+    def h(self):
+        s = np.asarray(self.sides, dtype=np.float64)
+        return (s[:, 1] - s[:, 0]) / np.asarray(self.dims, dtype=np.float64)
+
+    @property
+    def ncells(self):
+        return int(np.prod(self.dims))
+
+    def dict_in(self, **extra):
+        from .Eikonal import dictIn
+        from . import Metrics
+        d = {'model': self.model, 'seeds': self.seeds, 'verbosity': 0}
+        if self.metric is not None:
+            d['metric'] = Metrics.Riemann(self.metric)
+        if self.cost is not None:
+            d['cost'] = self.cost
+        d.update(extra)
+        hfmIn = dictIn(d)
+        hfmIn.SetRect(self.sides, dims=list(self.dims))
+        return hfmIn
+
+
+def _smooth_uniform(rng, shape, sigma, lo, hi):
+    """Gaussian-filtered uniform noise, min-max mapped to [lo, hi)."""
+    f = gaussian_filter(rng.uniform(size=shape), sigma=sigma, mode='wrap')
+    f = (f - f.min()) / max(f.max() - f.min(), 1e-30)
+    return lo + (hi - lo) * f * (1 - 1e-7)
+
+
+def isotropic2_bumps(n=512, seed=0, nbumps=32):
+    """2D isotropic eikonal on [-1,1]^2, cost c = 1 + 0.5 tanh(g), g a sum of
+    Gaussian bumps (centres uniform, widths 0.05-0.2, amplitudes N(0,1)),
+    single seed at the origin."""
+    rng = np.random.default_rng(seed)
+    centers = rng.uniform(-1, 1, (nbumps, 2))
+    widths = rng.uniform(0.05, 0.2, nbumps)
+    amps = rng.normal(size=nbumps)
+    x = -1 + (np.arange(n) + 0.5) * (2. / n)
+    X = np.stack(np.meshgrid(x, x, indexing='ij'))
+    g = np.zeros((n, n))
+    for c, s, a in zip(centers, widths, amps):
+        g += a * np.exp(-((X[0] - c[0]) ** 2 + (X[1] - c[1]) ** 2) / (2 * s ** 2))
+    cost = (1 + 0.5 * np.tanh(g)).astype(np.float32)
+    return Workload('isotropic2_bumps', 'Isotropic2', [[-1, 1], [-1, 1]], (n, n),
+                    np.zeros((1, 2), dtype=np.float32), cost=cost)
+
+
+def riemann2_aniso(n=2000, seed=1, nseeds=4, sigma=20., amax=10.):
+    """2D Riemannian eikonal on [-1,1]^2: M = R(theta)^T diag(1, a^2) R(theta),
+    theta smooth in [0, pi), a log-uniform in [1, amax] and smoothed (sigma in
+    cells); nseeds seeds at random cell centres."""
+    rng = np.random.default_rng(seed)
+    theta = _smooth_uniform(rng, (n, n), sigma, 0., np.pi)
+    loga = _smooth_uniform(rng, (n, n), sigma, 0., np.log(amax))
+    a2 = np.exp(2 * loga)
+    c, s = np.cos(theta), np.sin(theta)
+    # R = [[c, -s], [s, c]];  M = R^T diag(1, a^2) R
+    m00 = c * c + a2 * s * s
+    m01 = -c * s + a2 * s * c
+    m11 = s * s + a2 * c * c
+    metric = np.array([[m00, m01], [m01, m11]], dtype=np.float32)
+    h = 2. / n
+    cells = rng.integers(0, n, (nseeds, 2))
+    seeds = (-1 + (cells + 0.5) * h).astype(np.float32)
+    return Workload('riemann2_aniso', 'Riemann2', [[-1, 1], [-1, 1]], (n, n), seeds,
+                    metric=metric, extra={'seed_cells': cells})
+
+
+def rotation_from_quaternion(q):
+    """Rotation matrices of unit quaternions q[4, ...] (r, i, j, k)."""
+    r, i, j, k = q
+    return np.array([
+        [1 - 2 * (j * j + k * k), 2 * (i * j - k * r), 2 * (i * k + j * r)],
+        [2 * (i * j + k * r), 1 - 2 * (i * i + k * k), 2 * (j * k - i * r)],
+        [2 * (i * k - j * r), 2 * (j * k + i * r), 1 - 2 * (i * i + j * j)]])
+
+
+def riemann3_aniso(n=200, seed=2, sigma=10., lmax=5., nreceivers=64):
+    """3D Riemannian eikonal on [-1,1]^3: M = Q diag(l) Q^T, Q from a
+    Gaussian-smoothed quaternion field (4 N(0,1) channels, normalised), log l
+    uniform in [0, ln lmax] and smoothed; one seed at the centre cell;
+    nreceivers receivers at random boundary cells."""
+    rng = np.random.default_rng(seed)
+    shape = (n, n, n)
+    q = np.stack([gaussian_filter(rng.normal(size=shape), sigma=sigma, mode='wrap') for _ in range(4)])
+    q /= np.linalg.norm(q, axis=0)
+    Q = rotation_from_quaternion(q)
+    lam = np.stack([np.exp(_smooth_uniform(rng, shape, sigma, 0., np.log(lmax))) for _ in range(3)])
+    metric = np.einsum('ik...,k...,jk...->ij...', Q, lam, Q).astype(np.float32)
+    h = 2. / n
+    centre = np.array([[n // 2] * 3])
+    seeds = (-1 + (centre + 0.5) * h).astype(np.float32)
+    rec = rng.integers(0, n, (nreceivers, 3))
+    axis = rng.integers(0, 3, nreceivers)
+    side = rng.integers(0, 2, nreceivers)
+    rec[np.arange(nreceivers), axis] = side * (n - 1)
+    return Workload('riemann3_aniso', 'Riemann3', [[-1, 1]] * 3, shape, seeds, metric=metric,
+                    receivers=rec, extra={'seed_cells': centre})

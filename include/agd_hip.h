@@ -1,0 +1,197 @@
+/* agd_hip.h -- C ABI of the MI355X (gfx950) eikonal solver library
+ * (libagd_hip.so).
+ *
+ * The library replaces the device side of the reference's GPU eikonal path:
+ * the NVRTC-compiled `Update` kernel and the cupy host loops that drive it.
+ * Every entry point takes plain pointers and sizes; device pointers are
+ * caller-owned buffers in HBM (the library only owns its internal
+ * workspaces).  All work is enqueued on the stream passed in (a hipStream_t,
+ * passed as void*; NULL = default stream).  Functions return 0 on success and
+ * a negative agd_status otherwise; agd_last_error() describes the failure.
+ *
+ * Data layout ("tiled"): a grid of shape shape[0..ndim) is padded up to a
+ * multiple of the tile shape shape_i and stored tile after tile:
+ *   n_t = n_o * size_i + n_i,
+ * n_o the row-major index of the tile in the tile grid shape_o, n_i the
+ * row-major index of the cell inside its tile (last axis fastest).  Padded
+ * cells carry +inf (values, rhs, geometry).  Multi-component fields
+ * (geometry, flow) are component-major: field[k * size_tot + n_t].
+ * This is the reference's bi-level layout (Update.h:60-66, Grid.h:15-35,
+ * FiniteDifferences.py:164-222).
+ */
+#ifndef AGD_HIP_H
+#define AGD_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ enums */
+enum agd_status {
+  AGD_OK = 0,
+  AGD_EINVAL = -1,  /* invalid argument (shape, model, tile, pointer)      */
+  AGD_EHIP = -2,    /* a HIP runtime call failed                            */
+  AGD_NOCONV = -3,  /* nitermax_o reached before convergence                */
+  AGD_ENOMEM = -4,  /* workspace allocation failed                          */
+};
+
+enum agd_model {
+  AGD_ISOTROPIC = 0, /* Isotropic_.h: unit offsets, weights h^-2 (iso_weights) */
+  AGD_DIAGONAL = 1,  /* Isotropic_.h with geometry: per-cell weights        */
+  AGD_RIEMANN = 2,   /* Riemann_.h: Selling decomposition of the dual metric */
+  AGD_RANDER = 3,    /* Riemann_.h + drift (FiniteDifferences.h:41-45)      */
+};
+
+enum agd_scalar { AGD_F32 = 0, AGD_F64 = 1 };
+
+enum agd_solver_kind {
+  AGD_AGSI = 0,   /* adaptive Gauss-Seidel: tag self + faces on change
+                     (_solvers.py:122-193, Propagation.h:160-164)          */
+  AGD_FIM = 1,    /* fast iterative method with front score
+                     (_solvers.py:94-120, Propagation.h:127-159)           */
+  AGD_GLOBAL = 2, /* every tile on every pass (_solvers.py:77-92)          */
+};
+
+/* ------------------------------------------------------------ descriptors */
+typedef struct agd_eikonal_desc {
+  int32_t model;          /* agd_model                                   */
+  int32_t ndim;           /* 2 or 3                                      */
+  int32_t scalar;         /* agd_scalar                                  */
+  int32_t solver;         /* agd_solver_kind                             */
+  int64_t shape[3];       /* grid shape (unused axes = 1)                */
+  int32_t shape_i[3];     /* tile shape; supported: see agd_tile_supported */
+  int32_t periodic[3];    /* per-axis periodic boundary                  */
+  int32_t niter_i;        /* in-tile sweeps per tile visit               */
+  int32_t fim_front_width;/* FIM score of a freshly stabilised tile (>=2) */
+  double atol, rtol;      /* |u_old-u_new| <= atol + rtol |u_new|        */
+  int64_t nitermax_o;     /* maximum number of grid-level passes         */
+  double iso_weights[3];  /* AGD_ISOTROPIC: per-axis weights (h^-2)      */
+  int32_t check_interval; /* passes between host convergence checks (>=1) */
+  int32_t activate_all;   /* 1: start with every tile active (warm start) */
+} agd_eikonal_desc;
+
+/* Device pointers, tiled layout.  Scalars are float (AGD_F32) or double. */
+typedef struct agd_eikonal_io {
+  void* values;               /* in/out: initial supersolution, then u    */
+  const void* geom;           /* ngeom components (NULL for isotropic)    */
+  const void* rhs;            /* cost; seed value on seeds; +inf in walls */
+  const uint32_t* seeds;      /* 1 bit per cell (bit n_t%32 of word n_t/32): fixed cells */
+  const uint32_t* trigger;    /* 1 bit per cell: cells whose tiles start active (NULL: seeds) */
+  const uint8_t* wall_dist;   /* saturated L1 distance to walls, 255 = far; NULL = no walls */
+} agd_eikonal_io;
+
+typedef struct agd_eikonal_stats {
+  int64_t niter_o;        /* grid-level passes with a non-empty tile list */
+  int64_t tile_updates;   /* tile visits summed over passes               */
+  int64_t cell_updates;   /* tile_updates * size_i                         */
+  int64_t launches;       /* kernel launches issued (incl. empty passes)  */
+  double gpu_ms;          /* HIP-event time of the solve loop             */
+  int32_t converged;      /* 1 if the tile list emptied                   */
+  int32_t grid_blocks;    /* workgroups per launch                         */
+} agd_eikonal_stats;
+
+/* Flow export (reference: Update.h:171-209 compiled with FLOW_* macros,
+ * driven by _PostProcess.py:46-95).  Any output may be NULL. */
+typedef struct agd_flow_io {
+  void* flow_weights;     /* [nsym][size_tot]  w_k (u - v_k)_+           */
+  void* flow_weightsum;   /* [size_tot]                                  */
+  int32_t* flow_indices;  /* [nsym][size_tot] tiled index of the upwind neighbour, INT32_MAX if weight 0 */
+  void* flow_vector;      /* [ndim][size_tot] sum_k fw_k (+-e_k)          */
+} agd_flow_io;
+
+/* ------------------------------------------------------------ library */
+const char* agd_last_error(void);
+int agd_version(void);
+/* 1 if (model, ndim, shape_i) has a compiled kernel. */
+int agd_tile_supported(int32_t model, int32_t ndim, const int32_t* shape_i);
+int agd_device_count(int32_t* count);
+
+/* Solve the eikonal equation in place on `values` (replaces Solve('eikonal'),
+ * _solvers.py:15-57, and the Update kernel, Update.h:20-227).  Returns
+ * AGD_NOCONV (and still fills stats) if nitermax_o passes did not converge. */
+int agd_eikonal_solve(const agd_eikonal_desc* desc, const agd_eikonal_io* io,
+                      agd_eikonal_stats* stats, void* stream);
+
+/* One global pass exporting the upwind flow (replaces Solve('flow'),
+ * _PostProcess.py:79, Update.h:171-209).  Updates `values` like the
+ * reference's flow pass (DECREASING rule).  */
+int agd_eikonal_flow(const agd_eikonal_desc* desc, const agd_eikonal_io* io,
+                     const agd_flow_io* flow, void* stream);
+
+/* Reverse-mode sensitivities (replaces SolveAD's reverse branch,
+ * _PostProcess.py:194-219: graph_reverse.py:10-77 + Kernel_LinearUpdate.h).
+ * Solves (D - W)^T lambda = sens for nrhs right-hand sides with the tile
+ * Gauss-Seidel solver, where W is the flow-weight graph (flow_weights,
+ * flow_indices as produced by agd_eikonal_flow) and D = flow_weightsum
+ * (1 where it is 0).  sens and lambda: [nrhs][size_tot] tiled; lambda is
+ * overwritten.  atol/rtol as in _PostProcess.py:126-138. */
+int agd_eikonal_adjoint(const agd_eikonal_desc* desc, const void* values,
+                        const void* flow_weights, const int32_t* flow_indices,
+                        const void* flow_weightsum, int32_t nrhs, const void* sens,
+                        void* lambda, double atol, double rtol,
+                        agd_eikonal_stats* stats, void* stream);
+
+/* ------------------------------------------------------------ layout */
+/* block_expand (FiniteDifferences.py:164-195): src cell-major
+ * [ncomp][shape], dst tiled [ncomp][size_tot], padded with pad.
+ * elem_size: 1, 4 or 8 bytes.  src_cell_stride = 0 broadcasts one value per
+ * component to the whole grid. */
+int agd_block_expand(int32_t ndim, const int64_t* shape, const int32_t* shape_i,
+                     int32_t ncomp, int32_t elem_size, const void* src,
+                     int64_t src_cell_stride, void* dst, double pad, void* stream);
+/* block_squeeze (FiniteDifferences.py:197-222): inverse of the above. */
+int agd_block_squeeze(int32_t ndim, const int64_t* shape, const int32_t* shape_i,
+                      int32_t ncomp, int32_t elem_size, const void* src, void* dst,
+                      void* stream);
+
+/* Geometry preparation (_SetGeometry.py:127-142 for Riemann / Rander,
+ * riemann.py:38-40,68-73, rander.py:61-68, misc.py:8-15): from a cell-major
+ * metric m[d][d][cells] (and drift w[d][cells] for Rander), grid scales h[d],
+ * write the tiled geometry of the solver:
+ *   Riemann: flat((diag(h) m diag(h))^-1)                    (symdim comps)
+ *   Rander : [flat((diag(h) m diag(h))^-1), h * w]           (symdim + d)
+ *   Diagonal: m holds costs[d][cells]; geometry = (costs*h)^-2 (d comps)
+ * The inverse is computed in double precision.  cell_stride = 0 broadcasts
+ * a constant metric.  dst element type follows `scalar`. */
+int agd_prepare_geometry(int32_t model, int32_t ndim, const int64_t* shape,
+                         const int32_t* shape_i, int32_t scalar, const double* h,
+                         const float* m, const float* w, int64_t cell_stride,
+                         void* geom, void* stream);
+
+/* Seeds (_SetArgs.py:116-167): for each seed index (tiled cell index),
+ * set rhs = seed value, set the seed bit and the trigger bit.  Bits are
+ * OR-ed in, so callers zero the bit arrays first. */
+int agd_scatter_seeds(int32_t scalar, int64_t nseeds, const int64_t* tiled_index,
+                      const double* seed_values, void* rhs, uint32_t* seeds,
+                      uint32_t* trigger, void* stream);
+
+/* Walls (_SetGeometry.py:262-282, _SetArgs.py:124-126): wall cells get
+ * rhs = +inf and their seed bit; wall_dist = saturated L1 distance to the
+ * walls (computed by `bound` min-plus passes, Kernel_InfConvolution.h),
+ * 255 beyond `bound`.  walls: tiled uint8 mask (padding = 0). */
+int agd_prepare_walls(const agd_eikonal_desc* desc, const uint8_t* walls, int32_t bound,
+                      void* rhs, uint32_t* seeds, uint8_t* wall_dist, void* stream);
+
+/* Debug export of the per-cell stencil (weights, offsets) computed by the
+ * device code from a component-major geometry of n cells (cell-major, not
+ * tiled): weights[n][nsym], offsets[n][nsym][ndim].  Used to check that the
+ * Selling offsets are bit-identical to the oracle's
+ * (Geometry2.h:40-48, Geometry3.h:43-58). */
+int agd_stencil_export(int32_t model, int32_t ndim, int32_t scalar, int64_t n,
+                       const void* geom, const double* iso_weights, void* weights,
+                       int32_t* offsets, void* stream);
+
+/* ------------------------------------------------------------ memory */
+int agd_malloc(void** ptr, int64_t bytes);
+int agd_free(void* ptr);
+int agd_memcpy_htod(void* dst, const void* src, int64_t bytes, void* stream);
+int agd_memcpy_dtoh(void* dst, const void* src, int64_t bytes, void* stream);
+int agd_memset(void* dst, int32_t value, int64_t bytes, void* stream);
+int agd_stream_synchronize(void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* AGD_HIP_H */

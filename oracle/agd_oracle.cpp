@@ -1,0 +1,688 @@
+// agd_oracle.cpp -- CPU restatement of the reference eikonal scheme.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the oracle against which the HIP
+// product path is checked.  Only tests/, __graft_entry__.smoke() and the
+// cpu_baseline leg of bench.py may load it; the product package never does.
+//
+// What it restates (reference paths relative to the AGD source tree):
+//   * Selling decomposition, 2D/3D, CUDA ordering:
+//       agd/Eikonal/HFM_CUDA/cuda/Geometry2.h:21-48, Geometry3.h:23-58,
+//       scal_vmv summation order Geometry_.h:144-156, canonical superbase
+//       Geometry_.h:237-243.
+//   * Scheme per model: Isotropic_.h:20-41 (isotropic / diagonal),
+//       Riemann_.h:16-29 (Riemann, Rander = Riemann + drift),
+//       drift shift FiniteDifferences.h:41-45 (fact[0]+=s, fact[1]-=s).
+//   * Neighbour values (min over -e/+e, ties keep -e), network sort and the
+//       upwind quadratic update: HFM.h:28-207, NetworkSort.h:17-60.
+//   * Convergence test |u_old-u_new| <= atol + rtol|u_new|: Update.h:211-219.
+//   * Flow weights w_k (u - v_k)_+ and flow vector: HFM.h:197-207,
+//       Update.h:171-209.
+//   * Reverse AD system (D - W)^T lambda = s: _PostProcess.py:155-219,
+//       Kernel_LinearUpdate.h:99-216 (solved here exactly, in causal order).
+//   * Walls: visibility walk Walls.h:18-50 and L1 wall distance
+//       _SetGeometry.py:262-282 / Kernel_InfConvolution.h:66-91.
+//
+// Solvers:
+//   * orc_solve_fm     : Dijkstra-like fast marching in FP64 -- the exact
+//                        fixed point of the causal upwind scheme (restated
+//                        from Notebooks_NonDiv/EikonalEulerian.ipynb cell 35).
+//   * orc_solve_jacobi : strict global Jacobi iteration (FP32 or FP64) with the
+//                        reference stopping rule; used to show that the fixed
+//                        point does not depend on the schedule.
+//
+// Arithmetic conventions: scal_vmv accumulates with explicit fused
+// multiply-add (the contracted form that the reference's JIT compiler emits
+// with its default fmad=true); the product kernels do the same, so integer
+// offsets agree bit for bit.  Arrays are cell-major, row-major (last axis
+// fastest).  Geometry is component-major: geom[k*N + n].
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <queue>
+#include <vector>
+
+namespace orc {
+
+template <class S> static inline S Inf() { return std::numeric_limits<S>::infinity(); }
+static inline float fmaT(float a, float b, float c) { return std::fmaf(a, b, c); }
+static inline double fmaT(double a, double b, double c) { return std::fma(a, b, c); }
+
+// ---------------------------------------------------------------- Selling
+// scal_vmv(x, m, y) = sum_{i>=j} (i==j ? x_i y_i : x_i y_j + x_j y_i) m_ij,
+// integer products first, then one fma per matrix coefficient
+// (Geometry_.h:144-156).
+template <int D, class S>
+static S scal_vmv(const int* x, const S* m, const int* y) {
+  S r = 0;
+  int k = 0;
+  for (int i = 0; i < D; ++i)
+    for (int j = 0; j <= i; ++j, ++k) {
+      const int p = (i == j) ? x[i] * y[i] : (x[i] * y[j] + x[j] * y[i]);
+      r = fmaT(S(p), m[k], r);
+    }
+  return r;
+}
+
+static const int kSellingMaxIter = 100;  // Geometry2.h:13 / Geometry3.h:10
+
+// Geometry2.h:21-36: cyclic (i,j,k); if <b_i,D b_j> > 0 then b_k = b_i - b_j,
+// b_i = -b_i; stop after 3 consecutive non-reducing steps.
+template <class S>
+static void obtuse_superbase2(const S* m, int sb[3][2]) {
+  sb[0][0] = 1; sb[0][1] = 0;
+  sb[1][0] = 0; sb[1][1] = 1;
+  sb[2][0] = -1; sb[2][1] = -1;
+  for (int iter = 0, red = 0; iter < kSellingMaxIter && red < 3; ++iter, ++red) {
+    const int i = iter % 3, j = (iter + 1) % 3, k = (iter + 2) % 3;
+    if (scal_vmv<2, S>(sb[i], m, sb[j]) > 0) {
+      for (int l = 0; l < 2; ++l) sb[k][l] = sb[i][l] - sb[j][l];
+      for (int l = 0; l < 2; ++l) sb[i][l] = -sb[i][l];
+      red = 0;
+    }
+  }
+}
+
+// Geometry2.h:40-48: w_r = max(0, -<b_i,D b_j>), e_r = perp(b_k), perp(x)=(-x1,x0).
+template <class S>
+static void selling2(const S* m, S* w, int e[][3]) {
+  int sb[3][2];
+  obtuse_superbase2<S>(m, sb);
+  for (int r = 0; r < 3; ++r) {
+    const int i = r, j = (r + 1) % 3, k = (r + 2) % 3;
+    const S s = -scal_vmv<2, S>(sb[i], m, sb[j]);
+    w[r] = s > S(0) ? s : S(0);
+    e[r][0] = -sb[k][1];
+    e[r][1] = sb[k][0];
+    e[r][2] = 0;
+  }
+}
+
+// Geometry3.h:19-20: pairs (i,j) range over all 6 pairs, (k,l) the complement.
+static const int kPerm3[6][4] = {{0, 1, 2, 3}, {0, 2, 1, 3}, {0, 3, 1, 2},
+                                 {1, 2, 0, 3}, {1, 3, 0, 2}, {2, 3, 0, 1}};
+
+// Geometry3.h:23-39: if <b_i,D b_j> > 0 then b_k += b_i, b_l += b_i, b_i = -b_i.
+template <class S>
+static void obtuse_superbase3(const S* m, int sb[4][3]) {
+  for (int a = 0; a < 4; ++a)
+    for (int b = 0; b < 3; ++b) sb[a][b] = (a == 3) ? -1 : (a == b ? 1 : 0);
+  for (int iter = 0, red = 0; iter < kSellingMaxIter && red < 6; ++iter, ++red) {
+    const int* p = kPerm3[iter % 6];
+    const int i = p[0], j = p[1];
+    if (scal_vmv<3, S>(sb[i], m, sb[j]) > 0) {
+      const int k = p[2], l = p[3];
+      for (int c = 0; c < 3; ++c) {
+        sb[k][c] += sb[i][c];
+        sb[l][c] += sb[i][c];
+        sb[i][c] = -sb[i][c];
+      }
+      red = 0;
+    }
+  }
+}
+
+// Geometry3.h:43-58: w_r = max(0,-<b_i,D b_j>), e_r = b_k x b_l.
+template <class S>
+static void selling3(const S* m, S* w, int e[][3]) {
+  int sb[4][3];
+  obtuse_superbase3<S>(m, sb);
+  for (int r = 0; r < 6; ++r) {
+    const int* p = kPerm3[r];
+    const int i = p[0], j = p[1], k = p[2], l = p[3];
+    const S s = -scal_vmv<3, S>(sb[i], m, sb[j]);
+    w[r] = s > S(0) ? s : S(0);
+    for (int c = 0; c < 3; ++c) {
+      const int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+      e[r][c] = sb[k][c1] * sb[l][c2] - sb[k][c2] * sb[l][c1];
+    }
+  }
+}
+
+// ---------------------------------------------------------------- models
+enum Model { ISOTROPIC = 0, DIAGONAL = 1, RIEMANN = 2, RANDER = 3 };
+
+static int symdim(int d) { return d * (d + 1) / 2; }
+static int nsym_of(int model, int d) {
+  return (model == ISOTROPIC || model == DIAGONAL) ? d : symdim(d);
+}
+static int ngeom_of(int model, int d) {
+  switch (model) {
+    case ISOTROPIC: return 0;
+    case DIAGONAL: return d;
+    case RIEMANN: return symdim(d);
+    case RANDER: return symdim(d) + d;
+  }
+  return 0;
+}
+
+template <class S>
+struct Stencil {
+  int nsym;
+  S w[6];
+  int e[6][3];
+  S drift[3];
+  bool has_drift;
+};
+
+// Isotropic_.h:20-41 and Riemann_.h:16-29.
+template <class S>
+static void make_stencil(int model, int d, const S* g, const S* iso_w, Stencil<S>& st) {
+  st.has_drift = false;
+  st.nsym = nsym_of(model, d);
+  for (int k = 0; k < 6; ++k) {
+    st.w[k] = 0;
+    st.e[k][0] = st.e[k][1] = st.e[k][2] = 0;
+  }
+  st.drift[0] = st.drift[1] = st.drift[2] = 0;
+  if (model == ISOTROPIC || model == DIAGONAL) {
+    for (int k = 0; k < d; ++k) {
+      st.w[k] = (model == ISOTROPIC) ? iso_w[k] : g[k];
+      st.e[k][k] = 1;
+    }
+    return;
+  }
+  if (d == 2) selling2<S>(g, st.w, st.e);
+  else selling3<S>(g, st.w, st.e);
+  if (model == RANDER) {
+    st.has_drift = true;
+    for (int k = 0; k < d; ++k) st.drift[k] = g[symdim(d) + k];
+  }
+}
+
+// ---------------------------------------------------------------- sorting
+// Sorting networks of NetworkSort.h:17-60 (n = 2, 3, 6); the comparison keeps
+// order[i] when values[order[i]] < values[order[j]] and swaps otherwise.
+static const int kNet2[][2] = {{0, 1}};
+static const int kNet3[][2] = {{1, 2}, {0, 2}, {0, 1}};
+static const int kNet6[][2] = {{1, 2}, {4, 5}, {0, 2}, {3, 5}, {0, 1}, {3, 4},
+                               {2, 5}, {0, 3}, {1, 4}, {2, 4}, {1, 3}, {2, 3}};
+
+template <class S>
+static void network_sort(int n, const S* v, int* order) {
+  for (int i = 0; i < n; ++i) order[i] = i;
+  const int(*net)[2] = nullptr;
+  int m = 0;
+  if (n == 2) { net = kNet2; m = 1; }
+  else if (n == 3) { net = kNet3; m = 3; }
+  else if (n == 6) { net = kNet6; m = 12; }
+  for (int c = 0; c < m; ++c) {
+    const int i = net[c][0], j = net[c][1];
+    const int oi = order[i], oj = order[j];
+    const bool b = v[oi] < v[oj];
+    order[i] = b ? oi : oj;
+    order[j] = b ? oj : oi;
+  }
+}
+
+// HFM.h:165-207: active-set quadratic.  v[k] are the (already minimised)
+// neighbour values, w[k] the weights.  Returns the new value; fills flow
+// weights w_k (val - v_k)_+ when fw != nullptr.
+template <class S>
+static S hfm_solve(int nact, const S* w, const S* v, S rhs, S* fw) {
+  int order[6];
+  network_sort<S>(nact, v, order);
+  const S vmin = v[order[0]];
+  if (vmin == Inf<S>()) {
+    if (fw) for (int k = 0; k < nact; ++k) fw[k] = 0;
+    return vmin;
+  }
+  S a = w[order[0]], b = 0, c = -rhs * rhs;
+  for (int kk = 1; kk < nact; ++kk) {
+    const int k = order[kk];
+    const S t = v[k] - vmin;
+    if ((a * t - 2 * b) * t + c > 0) break;
+    a += w[k];
+    b += w[k] * t;
+    c += w[k] * t * t;
+  }
+  const S delta = b * b - a * c;
+  const S sdelta = std::sqrt(delta > S(0) ? delta : S(0));
+  const S value = (a == S(0)) ? Inf<S>() : (b + sdelta) / a;
+  const S val = vmin + value;
+  if (fw) {
+    if (val == Inf<S>()) {
+      for (int k = 0; k < nact; ++k) fw[k] = 0;
+    } else {
+      for (int k = 0; k < nact; ++k) {
+        const S diff = val - v[k];
+        fw[k] = w[k] * (diff > S(0) ? diff : S(0));
+      }
+    }
+  }
+  return val;
+}
+
+// ---------------------------------------------------------------- grid
+struct Grid {
+  int d;
+  int64_t shape[3];
+  int periodic[3];
+  int64_t N;
+  int64_t stride[3];
+  void init(int d_, const int64_t* sh, const int* per) {
+    d = d_;
+    N = 1;
+    for (int k = 0; k < 3; ++k) { shape[k] = 1; periodic[k] = 0; }
+    for (int k = 0; k < d; ++k) { shape[k] = sh[k]; periodic[k] = per ? per[k] : 0; N *= sh[k]; }
+    int64_t s = 1;
+    for (int k = d - 1; k >= 0; --k) { stride[k] = s; s *= shape[k]; }
+  }
+  void pos(int64_t n, int64_t* x) const {
+    for (int k = d - 1; k >= 0; --k) { x[k] = n % shape[k]; n /= shape[k]; }
+  }
+  // Grid.h:68-80 InRange_per / Index_per: periodic axes wrap, others reject.
+  int64_t index(const int64_t* y) const {
+    int64_t n = 0;
+    for (int k = 0; k < d; ++k) {
+      int64_t yk = y[k];
+      if (periodic[k]) { yk %= shape[k]; if (yk < 0) yk += shape[k]; }
+      else if (yk < 0 || yk >= shape[k]) return -1;
+      n += yk * stride[k];
+    }
+    return n;
+  }
+};
+
+// Walls.h:13-14: integer division rounding half away from zero (d > 0 here).
+static inline int div_round_closest(int n, int d) {
+  return ((n < 0) ^ (d < 0)) ? ((n - d / 2) / d) : ((n + d / 2) / d);
+}
+
+// Walls.h:18-50: is the segment [x, x+e] free of walls?  wallDist is the
+// saturated L1 distance to the walls (255 = far).  The walk samples
+// w_k = round(e k / |e|_inf), k = 1..|e|_inf, and concludes as soon as the
+// remaining L1 length |e - w_k|_1 is below the wall distance at x + w_k.
+static bool visible(const Grid& G, const int64_t* x, const int* e,
+                    const uint8_t* wallDist) {
+  if (!wallDist) return true;
+  const uint8_t wx = wallDist[G.index(x)];
+  if (wx == 255) return true;
+  int l1 = 0, linf = 0;
+  for (int k = 0; k < G.d; ++k) { l1 += std::abs(e[k]); linf = std::max(linf, std::abs(e[k])); }
+  if (l1 < wx) return true;
+  for (int t = 1; t <= linf; ++t) {
+    int64_t y[3];
+    int diffl1 = 0;
+    for (int k = 0; k < G.d; ++k) {
+      const int w = div_round_closest(e[k] * t, linf);
+      diffl1 += std::abs(e[k] - w);
+      y[k] = x[k] + w;
+    }
+    const int64_t ny = G.index(y);
+    const int value = (ny >= 0) ? wallDist[ny] : 255;
+    if (diffl1 < value) return true;
+    if (value == 0) return false;
+  }
+  return true;
+}
+
+struct Problem {
+  int model, d;
+  Grid G;
+  const double* geom;       // ngeom * N, component-major
+  const double* rhs;        // N: cost, seed value at seeds, +inf in walls
+  const uint8_t* seeds;     // N: fixed cells (seeds and walls)
+  const uint8_t* wallDist;  // N or null
+  double iso_w[3];
+};
+
+// Gather neighbour values for cell n (HFM.h:39-77, FiniteDifferences.h:20-106).
+template <class S, class Get>
+static void neighbours(const Problem& P, const Stencil<S>& st, int64_t n, Get get,
+                       S* v, int* side, int64_t* nidx) {
+  int64_t x[3];
+  P.G.pos(n, x);
+  for (int k = 0; k < st.nsym; ++k) {
+    S shift = 0;
+    if (st.has_drift) {
+      for (int c = 0; c < P.d; ++c) shift += st.drift[c] * S(st.e[k][c]);
+    }
+    S best = Inf<S>();
+    int bs = 0;
+    int64_t bi = -1;
+    for (int s = 0; s < 2; ++s) {
+      const int eps = 2 * s - 1;
+      int ee[3] = {eps * st.e[k][0], eps * st.e[k][1], eps * st.e[k][2]};
+      int64_t y[3];
+      for (int c = 0; c < P.d; ++c) y[c] = x[c] + ee[c];
+      S val = Inf<S>();
+      int64_t ny = -1;
+      if (visible(P.G, x, ee, P.wallDist)) {
+        ny = P.G.index(y);
+        if (ny >= 0) val = get(ny) + (s == 0 ? shift : -shift);
+      }
+      if (s == 0) { best = val; bs = 0; bi = ny; }
+      else if (best > val) { best = val; bs = 1; bi = ny; }
+    }
+    v[k] = best;
+    if (side) side[k] = bs;
+    if (nidx) nidx[k] = bi;
+  }
+}
+
+template <class S>
+static void stencil_at(const Problem& P, int64_t n, Stencil<S>& st) {
+  const int ng = ngeom_of(P.model, P.d);
+  S g[9];
+  for (int k = 0; k < ng; ++k) g[k] = S(P.geom[k * P.G.N + n]);
+  S iw[3] = {S(P.iso_w[0]), S(P.iso_w[1]), S(P.iso_w[2])};
+  make_stencil<S>(P.model, P.d, g, iw, st);
+}
+
+}  // namespace orc
+
+using namespace orc;
+
+static void fill_problem(Problem& P, int model, int d, const int64_t* shape,
+                         const int* periodic, const double* geom, const double* rhs,
+                         const uint8_t* seeds, const uint8_t* wallDist,
+                         const double* iso_w) {
+  P.model = model;
+  P.d = d;
+  P.G.init(d, shape, periodic);
+  P.geom = geom;
+  P.rhs = rhs;
+  P.seeds = seeds;
+  P.wallDist = wallDist;
+  for (int k = 0; k < 3; ++k) P.iso_w[k] = (iso_w && k < d) ? iso_w[k] : 0.;
+}
+
+// Strict (Jacobi) global iteration with the reference stopping rule
+// (Update.h:211-219, DECREASING enabled).  u[N] is in/out (initial guess, a
+// supersolution; +inf for a cold start).  Returns the iteration count
+// (nitermax if not converged).
+template <class S>
+static int64_t jacobi(const Problem& P, double atol, double rtol, int64_t nitermax, double* uio) {
+  const int64_t N = P.G.N;
+  const int ns = nsym_of(P.model, P.d);
+  std::vector<Stencil<S>> st(N);
+  for (int64_t n = 0; n < N; ++n) stencil_at<S>(P, n, st[n]);
+  std::vector<S> u(N), un(N);
+  for (int64_t n = 0; n < N; ++n) u[n] = P.seeds[n] ? S(P.rhs[n]) : S(uio[n]);
+  int64_t it = 0;
+  for (; it < nitermax; ++it) {
+    bool changed = false;
+    for (int64_t n = 0; n < N; ++n) {
+      if (P.seeds[n]) { un[n] = u[n]; continue; }
+      S v[6];
+      neighbours<S>(P, st[n], n, [&](int64_t m) { return u[m]; }, v, nullptr, nullptr);
+      const S val = hfm_solve<S>(ns, st[n].w, v, S(P.rhs[n]), nullptr);
+      const S nv = (val < u[n]) ? val : u[n];
+      const S diff = std::fabs(u[n] - nv);
+      const S tol = S(atol) + S(rtol) * std::fabs(nv);
+      if (!(std::isnan(diff) || diff <= tol)) changed = true;
+      un[n] = nv;
+    }
+    u.swap(un);
+    if (!changed) break;
+  }
+  for (int64_t n = 0; n < N; ++n) uio[n] = double(u[n]);
+  return it;
+}
+
+extern "C" {
+
+int orc_version() { return 3; }
+
+// Selling decomposition of n symmetric matrices (row n: symdim coefficients,
+// lower triangle (d00, d10, d11, d20, d21, d22)).  Outputs weights[n][symdim]
+// and offsets[n][symdim][d].  use_float selects FP32 arithmetic.
+void orc_selling(int d, int64_t n, const double* m, double* w, int32_t* e, int use_float) {
+  const int sd = symdim(d);
+  for (int64_t c = 0; c < n; ++c) {
+    int ee[6][3];
+    if (use_float) {
+      float mf[6], wf[6];
+      for (int k = 0; k < sd; ++k) mf[k] = float(m[c * sd + k]);
+      if (d == 2) selling2<float>(mf, wf, ee); else selling3<float>(mf, wf, ee);
+      for (int k = 0; k < sd; ++k) w[c * sd + k] = wf[k];
+    } else {
+      if (d == 2) selling2<double>(m + c * sd, w + c * sd, ee);
+      else selling3<double>(m + c * sd, w + c * sd, ee);
+    }
+    for (int k = 0; k < sd; ++k)
+      for (int l = 0; l < d; ++l) e[(c * sd + k) * d + l] = ee[k][l];
+  }
+}
+
+// The upwind update for n independent cells: w[n][nact], v[n][nact] (already
+// minimised over +-e), rhs[n] -> u[n], flow weights fw[n][nact] (nullable).
+void orc_hfm_update(int nact, int64_t n, const double* w, const double* v,
+                    const double* rhs, double* u, double* fw, int use_float) {
+  for (int64_t c = 0; c < n; ++c) {
+    if (use_float) {
+      float wf[6], vf[6], f[6];
+      for (int k = 0; k < nact; ++k) { wf[k] = float(w[c * nact + k]); vf[k] = float(v[c * nact + k]); }
+      u[c] = hfm_solve<float>(nact, wf, vf, float(rhs[c]), fw ? f : nullptr);
+      if (fw) for (int k = 0; k < nact; ++k) fw[c * nact + k] = f[k];
+    } else {
+      u[c] = hfm_solve<double>(nact, w + c * nact, v + c * nact, rhs[c],
+                               fw ? fw + c * nact : nullptr);
+    }
+  }
+}
+
+// Per-cell stencil export (weights, offsets) for a whole problem, FP32 or FP64.
+void orc_stencils(int model, int d, const int64_t* shape, const double* geom,
+                  const double* iso_w, int use_float, double* w, int32_t* e) {
+  Problem P;
+  fill_problem(P, model, d, shape, nullptr, geom, nullptr, nullptr, nullptr, iso_w);
+  const int ns = nsym_of(model, d);
+  for (int64_t n = 0; n < P.G.N; ++n) {
+    int ee[6][3];
+    if (use_float) {
+      Stencil<float> st;
+      stencil_at<float>(P, n, st);
+      for (int k = 0; k < ns; ++k) { w[n * ns + k] = st.w[k]; for (int l = 0; l < d; ++l) ee[k][l] = st.e[k][l]; }
+    } else {
+      Stencil<double> st;
+      stencil_at<double>(P, n, st);
+      for (int k = 0; k < ns; ++k) { w[n * ns + k] = st.w[k]; for (int l = 0; l < d; ++l) ee[k][l] = st.e[k][l]; }
+    }
+    for (int k = 0; k < ns; ++k)
+      for (int l = 0; l < d; ++l) e[(n * ns + k) * d + l] = ee[k][l];
+  }
+}
+
+// Exact fixed point by fast marching (FP64).  u[N] receives the values.
+// Returns the number of accepted cells.
+int64_t orc_solve_fm(int model, int d, const int64_t* shape, const int* periodic,
+                     const double* geom, const double* rhs, const uint8_t* seeds,
+                     const uint8_t* wallDist, const double* iso_w, double* u) {
+  Problem P;
+  fill_problem(P, model, d, shape, periodic, geom, rhs, seeds, wallDist, iso_w);
+  const int64_t N = P.G.N;
+  const int ns = nsym_of(model, d);
+  // Stencils and reverse adjacency (cells whose update reads a given cell).
+  std::vector<Stencil<double>> st(N);
+  std::vector<int64_t> deg(N + 1, 0);
+  std::vector<int64_t> nbr((size_t)N * 2 * ns, -1);
+  for (int64_t n = 0; n < N; ++n) {
+    stencil_at<double>(P, n, st[n]);
+    int64_t x[3];
+    P.G.pos(n, x);
+    for (int k = 0; k < ns; ++k)
+      for (int s = 0; s < 2; ++s) {
+        const int eps = 2 * s - 1;
+        int ee[3] = {eps * st[n].e[k][0], eps * st[n].e[k][1], eps * st[n].e[k][2]};
+        int64_t y[3];
+        for (int c = 0; c < d; ++c) y[c] = x[c] + ee[c];
+        const int64_t ny = visible(P.G, x, ee, wallDist) ? P.G.index(y) : -1;
+        nbr[(size_t)n * 2 * ns + 2 * k + s] = ny;
+        if (ny >= 0) deg[ny + 1]++;
+      }
+  }
+  for (int64_t n = 0; n < N; ++n) deg[n + 1] += deg[n];
+  std::vector<int64_t> rev(deg[N]);
+  {
+    std::vector<int64_t> cur(deg.begin(), deg.end() - 1);
+    for (int64_t n = 0; n < N; ++n)
+      for (int j = 0; j < 2 * ns; ++j) {
+        const int64_t ny = nbr[(size_t)n * 2 * ns + j];
+        if (ny >= 0) rev[cur[ny]++] = n;
+      }
+  }
+  std::vector<uint8_t> acc(N, 0);
+  for (int64_t n = 0; n < N; ++n) u[n] = Inf<double>();
+  typedef std::pair<double, int64_t> QE;
+  std::priority_queue<QE, std::vector<QE>, std::greater<QE>> heap;
+  for (int64_t n = 0; n < N; ++n)
+    if (seeds[n]) {
+      u[n] = rhs[n];
+      if (u[n] < Inf<double>()) heap.push(QE(u[n], n));
+      else acc[n] = 1;  // walls: fixed at +inf
+    }
+  int64_t naccepted = 0;
+  auto get_acc = [&](int64_t m) { return acc[m] ? u[m] : Inf<double>(); };
+  while (!heap.empty()) {
+    const QE top = heap.top();
+    heap.pop();
+    const int64_t n = top.second;
+    if (acc[n] || top.first > u[n]) continue;
+    acc[n] = 1;
+    ++naccepted;
+    for (int64_t r = deg[n]; r < deg[n + 1]; ++r) {
+      const int64_t m = rev[r];
+      if (acc[m] || seeds[m]) continue;
+      double v[6];
+      neighbours<double>(P, st[m], m, get_acc, v, nullptr, nullptr);
+      const double val = hfm_solve<double>(ns, st[m].w, v, rhs[m], nullptr);
+      if (val < u[m]) {
+        u[m] = val;
+        heap.push(QE(val, m));
+      }
+    }
+  }
+  return naccepted;
+}
+
+int64_t orc_solve_jacobi(int model, int d, const int64_t* shape, const int* periodic,
+                         const double* geom, const double* rhs, const uint8_t* seeds,
+                         const uint8_t* wallDist, const double* iso_w, double atol,
+                         double rtol, int64_t nitermax, int use_float, double* u) {
+  Problem P;
+  fill_problem(P, model, d, shape, periodic, geom, rhs, seeds, wallDist, iso_w);
+  return use_float ? jacobi<float>(P, atol, rtol, nitermax, u)
+                   : jacobi<double>(P, atol, rtol, nitermax, u);
+}
+
+// Flow export from converged values (Update.h:171-209 with FLOW_* on): one
+// update per non-seed cell; writes the updated value back (the reference's
+// flow pass is one more global sweep that also stores u), the flow weights
+// fw[N][nsym], neighbour linear indices nidx[N][nsym] (-1 when the weight is
+// zero), flow vector fv[d][N] = sum_k fw_k (+-e_k), and the weight sum.
+int orc_flow(int model, int d, const int64_t* shape, const int* periodic, const double* geom,
+             const double* rhs, const uint8_t* seeds, const uint8_t* wallDist,
+             const double* iso_w, int use_float, double* u, double* fw, int64_t* nidx,
+             double* fv, double* wsum) {
+  Problem P;
+  fill_problem(P, model, d, shape, periodic, geom, rhs, seeds, wallDist, iso_w);
+  const int64_t N = P.G.N;
+  const int ns = nsym_of(model, d);
+  std::vector<double> u0(u, u + N);
+  for (int64_t n = 0; n < N; ++n) {
+    double f[6] = {0, 0, 0, 0, 0, 0};
+    int side[6] = {0, 0, 0, 0, 0, 0};
+    int64_t ni[6] = {-1, -1, -1, -1, -1, -1};
+    int e[6][3];
+    if (!seeds[n]) {
+      if (use_float) {
+        Stencil<float> st;
+        stencil_at<float>(P, n, st);
+        float v[6], ff[6];
+        neighbours<float>(P, st, n, [&](int64_t m) { return float(u0[m]); }, v, side, ni);
+        const float val = hfm_solve<float>(ns, st.w, v, float(rhs[n]), ff);
+        if (val < float(u0[n])) u[n] = val;
+        for (int k = 0; k < ns; ++k) { f[k] = ff[k]; for (int c = 0; c < 3; ++c) e[k][c] = st.e[k][c]; }
+      } else {
+        Stencil<double> st;
+        stencil_at<double>(P, n, st);
+        double v[6];
+        neighbours<double>(P, st, n, [&](int64_t m) { return u0[m]; }, v, side, ni);
+        const double val = hfm_solve<double>(ns, st.w, v, rhs[n], f);
+        if (val < u0[n]) u[n] = val;
+        for (int k = 0; k < ns; ++k) for (int c = 0; c < 3; ++c) e[k][c] = st.e[k][c];
+      }
+    } else {
+      for (int k = 0; k < ns; ++k) for (int c = 0; c < 3; ++c) e[k][c] = 0;
+    }
+    double s = 0;
+    double vec[3] = {0, 0, 0};
+    for (int k = 0; k < ns; ++k) {
+      const int eps = 2 * side[k] - 1;
+      fw[n * ns + k] = f[k];
+      nidx[n * ns + k] = (f[k] != 0) ? ni[k] : -1;
+      s += f[k];
+      for (int c = 0; c < d; ++c) vec[c] += f[k] * double(eps * e[k][c]);
+    }
+    wsum[n] = s;
+    for (int c = 0; c < d; ++c) fv[c * N + n] = vec[c];
+  }
+  return 0;
+}
+
+// Reverse-mode sensitivity: solves (D - W)^T lam = sens exactly (FP64) by
+// visiting cells in decreasing order of u (the upwind graph is acyclic in that
+// order: a positive flow weight from x to y implies u(x) > u(y)).  D = sum of
+// flow weights (1 where it vanishes: seeds, walls).  nrhs right-hand sides,
+// sens/lam layout [nrhs][N].
+int orc_adjoint(int64_t N, int nact, const double* u, const double* fw, const int64_t* nidx,
+                int nrhs, const double* sens, double* lam) {
+  std::vector<int64_t> order;
+  order.reserve(N);
+  for (int64_t n = 0; n < N; ++n)
+    if (std::isfinite(u[n])) order.push_back(n);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int64_t a, int64_t b) { return u[a] > u[b]; });
+  for (int r = 0; r < nrhs; ++r) {
+    double* L = lam + (int64_t)r * N;
+    const double* Sv = sens + (int64_t)r * N;
+    std::vector<double> acc(N, 0.);
+    for (int64_t n = 0; n < N; ++n) L[n] = 0.;
+    for (int64_t n : order) {
+      double diag = 0;
+      for (int k = 0; k < nact; ++k) diag += fw[n * nact + k];
+      if (diag == 0) diag = 1;
+      const double l = (Sv[n] + acc[n]) / diag;
+      L[n] = l;
+      for (int k = 0; k < nact; ++k) {
+        const double w = fw[n * nact + k];
+        const int64_t m = nidx[n * nact + k];
+        if (w != 0 && m >= 0) acc[m] += w * l;
+      }
+    }
+  }
+  return 0;
+}
+
+// L1 distance to walls, saturated (_SetGeometry.py:262-282): 0 on walls,
+// k for cells at L1 distance k <= bound, 255 beyond the bound.
+void orc_wall_distance(int d, const int64_t* shape, const int* periodic, const uint8_t* walls,
+                       int bound, uint8_t* dist) {
+  Grid G;
+  G.init(d, shape, periodic);
+  std::vector<int> cur(G.N), nxt(G.N);
+  for (int64_t n = 0; n < G.N; ++n) cur[n] = walls[n] ? 0 : 255;
+  for (int it = 0; it < bound; ++it) {
+    for (int64_t n = 0; n < G.N; ++n) {
+      int best = cur[n];
+      int64_t x[3];
+      G.pos(n, x);
+      for (int k = 0; k < d; ++k)
+        for (int s = -1; s <= 1; s += 2) {
+          int64_t y[3] = {x[0], x[1], x[2]};
+          y[k] += s;
+          const int64_t m = G.index(y);
+          if (m >= 0) best = std::min(best, std::min(255, cur[m] + 1));
+        }
+      nxt[n] = best;
+    }
+    cur.swap(nxt);
+  }
+  for (int64_t n = 0; n < G.N; ++n) dist[n] = (uint8_t)(cur[n] > bound ? 255 : cur[n]);
+}
+
+}  // extern "C"

@@ -1,0 +1,261 @@
+"""Python side of the CPU oracle -- TEST INFRASTRUCTURE ONLY.
+
+Loads liboracle.so (agd_oracle.cpp, built by the Makefile next to this
+file) and restates, in numpy FP64, the reference's host preprocessing of the
+GPU eikonal path:
+  * grid geometry: cell centres origin + (i+1/2) h, continuous index
+    (x - origin)/h - 1/2, seeds rounded half to even
+    (agd/Eikonal/DictIn.py:404-453, HFM_CUDA/_SetArgs.py:44-45);
+  * metric preparation: D = (diag(h) M diag(h))^-1 flattened lower-triangular
+    for Riemann, [flat(inv(h m h)), h w] for Rander, (c h)^-2 for Diagonal,
+    h^-2 for Isotropic (HFM_CUDA/_SetGeometry.py:127-142,
+    HFM_CUDA/_Kernel.py:316-317, Metrics/riemann.py:38-73,
+    Metrics/rander.py:61-68, Metrics/misc.py:8-15);
+  * rhs = cost with rhs[seed] = seed value, walls -> rhs = inf and fixed
+    (_SetArgs.py:116-126).
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it.
+"""
+import ctypes
+import os
+import subprocess
+from ctypes import POINTER, c_double, c_int, c_int64, c_void_p
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+ISOTROPIC, DIAGONAL, RIEMANN, RANDER = 0, 1, 2, 3
+MODEL_IDS = {'Isotropic': ISOTROPIC, 'Diagonal': DIAGONAL, 'Riemann': RIEMANN, 'Rander': RANDER}
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+def _load():
+    src = os.path.join(HERE, "agd_oracle.cpp")
+    if not os.path.exists(LIB_PATH) or os.path.getmtime(LIB_PATH) < os.path.getmtime(src):
+        build()
+    lib = ctypes.CDLL(LIB_PATH)
+    P = c_void_p
+    lib.orc_version.restype = c_int
+    lib.orc_selling.argtypes = [c_int, c_int64, P, P, P, c_int]
+    lib.orc_hfm_update.argtypes = [c_int, c_int64, P, P, P, P, P, c_int]
+    lib.orc_stencils.argtypes = [c_int, c_int, P, P, P, c_int, P, P]
+    lib.orc_solve_fm.argtypes = [c_int, c_int, P, P, P, P, P, P, P, P]
+    lib.orc_solve_fm.restype = c_int64
+    lib.orc_solve_jacobi.argtypes = [c_int, c_int, P, P, P, P, P, P, P, c_double, c_double,
+                                     c_int64, c_int, P]
+    lib.orc_solve_jacobi.restype = c_int64
+    lib.orc_flow.argtypes = [c_int, c_int, P, P, P, P, P, P, P, c_int, P, P, P, P, P]
+    lib.orc_adjoint.argtypes = [c_int64, c_int, P, P, P, c_int, P, P]
+    lib.orc_wall_distance.argtypes = [c_int, P, P, P, c_int, P]
+    return lib
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        _lib = _load()
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(c_void_p)
+
+
+def _c(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+# ---------------------------------------------------------------- kernels
+def selling(m, use_float=False):
+    """Selling decomposition of matrices m[n, symdim] (lower triangle).
+    Returns weights [n, symdim] and offsets [n, symdim, d]."""
+    m = _c(m, np.float64)
+    n, sd = m.shape
+    d = {3: 2, 6: 3}[sd]
+    w = np.zeros((n, sd))
+    e = np.zeros((n, sd, d), dtype=np.int32)
+    lib().orc_selling(d, n, _p(m), _p(w), _p(e), int(use_float))
+    return w, e
+
+
+def hfm_update(w, v, rhs, use_float=False, flow=False):
+    """Upwind update of independent cells: w, v [n, nact], rhs [n]."""
+    w, v = _c(w, np.float64), _c(v, np.float64)
+    rhs = _c(np.broadcast_to(np.asarray(rhs, dtype=np.float64), (w.shape[0],)), np.float64)
+    n, nact = w.shape
+    u = np.zeros(n)
+    fw = np.zeros((n, nact)) if flow else None
+    lib().orc_hfm_update(nact, n, _p(w), _p(v), _p(rhs), _p(u), _p(fw), int(use_float))
+    return (u, fw) if flow else u
+
+
+# ---------------------------------------------------------------- problems
+class Problem:
+    """A discretised eikonal problem in the oracle's cell-major layout."""
+
+    def __init__(self, model, shape, geom, rhs, seeds, iso_w, wall_dist=None, periodic=None):
+        self.model = model if isinstance(model, int) else MODEL_IDS[model]
+        self.shape = tuple(int(s) for s in shape)
+        self.d = len(self.shape)
+        self.N = int(np.prod(self.shape))
+        self.geom = None if geom is None else _c(np.asarray(geom).reshape(-1, self.N), np.float64)
+        self.rhs = _c(np.asarray(rhs).reshape(self.N), np.float64)
+        self.seeds = _c(np.asarray(seeds).reshape(self.N), np.uint8)
+        self.iso_w = _c(list(iso_w) + [0.] * (3 - len(iso_w)), np.float64)
+        self.wall_dist = None if wall_dist is None else _c(np.asarray(wall_dist).reshape(self.N), np.uint8)
+        self.periodic = _c(list(periodic or [0] * self.d) + [0] * (3 - self.d), np.int32)
+        self._shape = _c(list(self.shape) + [1] * (3 - self.d), np.int64)
+
+    @property
+    def nsym(self):
+        return self.d if self.model in (ISOTROPIC, DIAGONAL) else self.d * (self.d + 1) // 2
+
+    def cast(self, dtype):
+        """Same problem with geometry and rhs rounded to dtype (the FP32
+        inputs the GPU path sees)."""
+        g = None if self.geom is None else self.geom.astype(dtype).astype(np.float64)
+        p = Problem(self.model, self.shape, g, self.rhs.astype(dtype).astype(np.float64), self.seeds,
+                    np.asarray(self.iso_w[:self.d]).astype(dtype).astype(np.float64),
+                    self.wall_dist, list(self.periodic[:self.d]))
+        return p
+
+    def solve_fm(self):
+        u = np.zeros(self.N)
+        lib().orc_solve_fm(self.model, self.d, _p(self._shape), _p(self.periodic), _p(self.geom),
+                           _p(self.rhs), _p(self.seeds), _p(self.wall_dist), _p(self.iso_w), _p(u))
+        return u.reshape(self.shape)
+
+    def solve_jacobi(self, atol, rtol, nitermax=100000, use_float=False, u0=None):
+        u = np.full(self.N, np.inf) if u0 is None else _c(np.asarray(u0).reshape(self.N), np.float64).copy()
+        it = lib().orc_solve_jacobi(self.model, self.d, _p(self._shape), _p(self.periodic),
+                                    _p(self.geom), _p(self.rhs), _p(self.seeds), _p(self.wall_dist),
+                                    _p(self.iso_w), float(atol), float(rtol), int(nitermax),
+                                    int(use_float), _p(u))
+        return u.reshape(self.shape), int(it)
+
+    def stencils(self, use_float=False):
+        ns = self.nsym
+        w = np.zeros((self.N, ns))
+        e = np.zeros((self.N, ns, self.d), dtype=np.int32)
+        lib().orc_stencils(self.model, self.d, _p(self._shape), _p(self.geom), _p(self.iso_w),
+                           int(use_float), _p(w), _p(e))
+        return w, e
+
+    def flow(self, u, use_float=False):
+        """One flow pass from values u: returns dict(values, weights [N,ns],
+        indices [N,ns] (-1 for zero weight), vector [d,*shape], weightsum)."""
+        ns = self.nsym
+        uu = _c(np.asarray(u).reshape(self.N), np.float64).copy()
+        fw = np.zeros((self.N, ns))
+        ni = np.zeros((self.N, ns), dtype=np.int64)
+        fv = np.zeros((self.d, self.N))
+        ws = np.zeros(self.N)
+        lib().orc_flow(self.model, self.d, _p(self._shape), _p(self.periodic), _p(self.geom),
+                       _p(self.rhs), _p(self.seeds), _p(self.wall_dist), _p(self.iso_w),
+                       int(use_float), _p(uu), _p(fw), _p(ni), _p(fv), _p(ws))
+        return dict(values=uu.reshape(self.shape), weights=fw, indices=ni,
+                    vector=fv.reshape((self.d,) + self.shape), weightsum=ws.reshape(self.shape))
+
+    def adjoint(self, u, flow, sens):
+        """Exact solution of (D - W)^T lam = sens, sens [*shape, nrhs]."""
+        sens = np.asarray(sens, dtype=np.float64)
+        nrhs = sens.shape[-1]
+        s = _c(np.moveaxis(sens, -1, 0).reshape(nrhs, self.N), np.float64)
+        lam = np.zeros((nrhs, self.N))
+        uu = _c(np.asarray(u).reshape(self.N), np.float64)
+        lib().orc_adjoint(self.N, self.nsym, _p(uu), _p(_c(flow['weights'], np.float64)),
+                          _p(_c(flow['indices'], np.int64)), nrhs, _p(s), _p(lam))
+        return np.moveaxis(lam.reshape((nrhs,) + self.shape), 0, -1)
+
+
+def wall_distance(walls, bound=10, periodic=None):
+    walls = _c(walls, np.uint8)
+    shape = walls.shape
+    d = len(shape)
+    out = np.zeros(shape, dtype=np.uint8)
+    sh = _c(list(shape) + [1] * (3 - d), np.int64)
+    per = _c(list(periodic or [0] * d) + [0] * (3 - d), np.int32)
+    lib().orc_wall_distance(d, _p(sh), _p(per), _p(walls), int(bound), _p(out))
+    return out
+
+
+# ---------------------------------------------------------------- preprocessing
+def flatten_lower(m):
+    d = m.shape[0]
+    return np.stack([m[i, j] for i in range(d) for j in range(i + 1)], axis=0)
+
+
+def grid_h(sides, dims):
+    """Cell-centred grid on the box `sides` with `dims` points per axis
+    (SetRect with dims=..., sampleBoundary=False): returns h, origin."""
+    sides = np.asarray(sides, dtype=np.float64)
+    dims = np.asarray(dims, dtype=np.float64)
+    h = (sides[:, 1] - sides[:, 0]) / dims
+    return h, sides[:, 0]
+
+
+def seed_indices(seeds, origin, h, dtype=np.float32):
+    """Continuous index (x - origin)/h - 1/2 in dtype, rounded half to even."""
+    seeds = np.asarray(seeds, dtype=dtype)
+    start = np.asarray(origin, dtype=dtype) + dtype(0.5) * np.asarray(h, dtype=dtype)
+    cont = (seeds - start) / np.asarray(h, dtype=dtype)
+    return np.round(cont).astype(np.int64)
+
+
+def make_problem(model, shape, h, metric=None, drift=None, cost=1., seeds_idx=(), seed_values=None,
+                 walls=None, wall_bound=10, periodic=None):
+    """Cell-major FP64 problem, restating _SetGeometry.py / _SetArgs.py.
+
+    metric: Riemann/Rander primal matrices m[d, d, *shape] (or [d, d]);
+    Diagonal: costs[d, *shape]; ignored for Isotropic.  drift: Rander w[d, ...].
+    """
+    shape = tuple(int(s) for s in shape)
+    d = len(shape)
+    N = int(np.prod(shape))
+    h = np.asarray(h, dtype=np.float64)
+    mid = model if isinstance(model, int) else MODEL_IDS[model]
+    geom = None
+    iso_w = [0.] * d
+    if mid == ISOTROPIC:
+        iso_w = list(h ** -2)
+    elif mid == DIAGONAL:
+        c = np.asarray(metric, dtype=np.float64)
+        if c.ndim == 1:
+            c = c.reshape((d,) + (1,) * d)
+        c = np.broadcast_to(c, (d,) + shape)
+        geom = (c * h.reshape((d,) + (1,) * d)) ** -2
+    else:
+        m = np.asarray(metric, dtype=np.float64)
+        if m.ndim == 2:
+            m = np.broadcast_to(m.reshape(d, d, *(1,) * d), (d, d) + shape)
+        mh = m * (h[:, None] * h[None, :]).reshape((d, d) + (1,) * d)
+        inv = np.moveaxis(np.linalg.inv(np.moveaxis(mh, (0, 1), (-2, -1))), (-2, -1), (0, 1))
+        geom = flatten_lower(inv)
+        if mid == RANDER:
+            w = np.asarray(drift, dtype=np.float64)
+            if w.ndim == 1:
+                w = np.broadcast_to(w.reshape(d, *(1,) * d), (d,) + shape)
+            geom = np.concatenate([geom, w * h.reshape((d,) + (1,) * d)], axis=0)
+    rhs = np.broadcast_to(np.asarray(cost, dtype=np.float64), shape).copy()
+    seeds = np.zeros(shape, dtype=np.uint8)
+    idx = np.asarray(seeds_idx, dtype=np.int64).reshape(-1, d)
+    vals = np.zeros(len(idx)) if seed_values is None else np.asarray(seed_values, dtype=np.float64)
+    for i, s in enumerate(idx):
+        rhs[tuple(s)] = vals[i]
+        seeds[tuple(s)] = 1
+    wd = None
+    if walls is not None:
+        walls = np.asarray(walls, dtype=bool)
+        rhs[walls] = np.inf
+        seeds[walls] = 1
+        wd = wall_distance(walls, wall_bound, periodic)
+    return Problem(mid, shape, None if geom is None else geom.reshape(-1, N), rhs, seeds, iso_w,
+                   wd, periodic)

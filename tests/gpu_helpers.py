@@ -1,0 +1,41 @@
+"""Shared helpers of the GPU parity tests: build the same problem for the
+HIP path (through Eikonal.dictIn) and for the CPU oracle."""
+import sys
+
+import numpy as np
+
+
+def log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def oracle_problem(O, wl, walls=None, seed_values=None):
+    """Oracle problem matching workload `wl` (float32 inputs, FP64 arithmetic)."""
+    sides = np.asarray(wl.sides, dtype=np.float64)
+    h = wl.h
+    idx = O.seed_indices(wl.seeds, sides[:, 0], h)
+    model = wl.model[:-1]
+    kw = {}
+    if model in ('Riemann', 'Rander'):
+        kw['metric'] = wl.metric.astype(np.float64)
+    if model == 'Rander':
+        kw['drift'] = wl.extra['drift'].astype(np.float64)
+    if model == 'Diagonal':
+        kw['metric'] = wl.extra['costs'].astype(np.float64)
+    cost = 1. if wl.cost is None else wl.cost.astype(np.float64)
+    P = O.make_problem(model, wl.dims, h, cost=cost, seeds_idx=idx, seed_values=seed_values,
+                       walls=walls, **kw)
+    return P.cast(np.float32), idx
+
+
+def run_gpu(wl, **extra):
+    hfmIn = wl.dict_in(exportValues=True, **extra)
+    out = hfmIn.Run()
+    return out['values'].cpu().numpy().astype(np.float64), out
+
+
+def rel_err(u, ref):
+    fin = np.isfinite(ref)
+    assert np.array_equal(fin, np.isfinite(u)), "finite masks differ"
+    return np.max(np.abs(u[fin] - ref[fin])) / np.max(np.abs(ref[fin]))
