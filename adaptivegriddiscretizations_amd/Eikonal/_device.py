@@ -33,15 +33,19 @@ def device_tensor(x, dtype):
     dev = device()
     if isinstance(x, torch.Tensor):
         return x.detach().to(device=dev, dtype=tdtype(dtype)).contiguous()
-    arr = np.asarray(x, dtype=dtype)
+    arr = np.asarray(host_array(x), dtype=dtype)
     if not arr.flags.c_contiguous:  # (np.ascontiguousarray would turn 0-d into 1-d)
         arr = arr.copy(order='C')
     return torch.from_numpy(arr).to(dev)
 
 
 def host_array(x):
+    """numpy view/copy of x (torch tensors, and arrays of other device
+    libraries that expose .get(), are copied to the host)."""
     if isinstance(x, torch.Tensor):
         return x.detach().cpu().numpy()
+    if not isinstance(x, np.ndarray) and hasattr(x, 'get') and hasattr(x, 'shape'):
+        return np.asarray(x.get())
     return np.asarray(x)
 
 

@@ -32,6 +32,7 @@ class EikonalDesc(ctypes.Structure):
         ("niter_i", c_int32), ("fim_front_width", c_int32),
         ("atol", c_double), ("rtol", c_double), ("nitermax_o", c_int64),
         ("iso_weights", c_double * 3), ("check_interval", c_int32), ("activate_all", c_int32),
+        ("time_launches", c_int32), ("reserved", c_int32),
     ]
 
 
@@ -46,7 +47,7 @@ class EikonalStats(ctypes.Structure):
     _fields_ = [
         ("niter_o", c_int64), ("tile_updates", c_int64), ("cell_updates", c_int64),
         ("launches", c_int64), ("gpu_ms", c_double), ("converged", c_int32),
-        ("grid_blocks", c_int32),
+        ("grid_blocks", c_int32), ("kernel_ms", c_double),
     ]
 
     def as_dict(self):

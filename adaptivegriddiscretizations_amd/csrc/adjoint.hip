@@ -259,6 +259,7 @@ static int adjoint_impl(const agd_eikonal_desc* d, const S* values, const S* fw,
   const int blocks = blocks_for(g.ntiles, ls.ni);
   int64_t niter = 0, visits = 0, passes = 0;
   float ms = 0.f;
+  double kernel_ms = 0.;
   bool converged = true;
   int status = AGD_OK;
   for (int r = 0; r < nrhs && status == AGD_OK; ++r) {
@@ -294,7 +295,9 @@ static int adjoint_impl(const agd_eikonal_desc* d, const S* values, const S* fw,
       hipLaunchKernelGGL(ls.update, dim3(blocks), dim3(ls.ni), 0, stream, a);
     };
     WorklistResult res;
-    status = run_worklist(g.ntiles, d->nitermax_o, d->check_interval, stream, init, launch, res);
+    status = run_worklist(g.ntiles, d->nitermax_o, d->check_interval, stream, init, launch, res,
+                          d->time_launches != 0);
+    kernel_ms += res.kernel_ms;
     (void)wsp;
     niter += res.niter;
     visits += res.visits;
@@ -318,6 +321,7 @@ static int adjoint_impl(const agd_eikonal_desc* d, const S* values, const S* fw,
     stats->gpu_ms = ms;
     stats->converged = converged ? 1 : 0;
     stats->grid_blocks = blocks;
+    stats->kernel_ms = kernel_ms;
   }
   if (!converged) {
     set_error("adjoint solver did not converge within nitermax_o passes");

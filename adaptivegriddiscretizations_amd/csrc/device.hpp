@@ -105,26 +105,37 @@ template <int D> __device__ inline void tile_pos(int n_o, const GridDesc& g, int
 
 // Grid.h:68-90 (InRange_per + Index_per on the tile grid): index of the
 // neighbouring tile of xo along axis `ax` in direction `dir`, -1 if outside.
+// (Axis selection by comparison keeps every array access compile-time.)
 template <int D>
 __device__ inline int tile_neighbor(const int* xo, int ax, int dir, const GridDesc& g) {
   int y[3] = {xo[0], xo[1], xo[2]};
-  y[ax] += dir;
-  if ((g.periodic >> ax) & 1) y[ax] = mod_pos(y[ax], g.shape_o[ax]);
-  else if (y[ax] < 0 || y[ax] >= g.shape_o[ax]) return -1;
+  bool ok = true;
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    if (k == ax) {
+      y[k] += dir;
+      if ((g.periodic >> k) & 1) y[k] = mod_pos(y[k], g.shape_o[k]);
+      else ok = y[k] >= 0 && y[k] < g.shape_o[k];
+    }
+  }
+  if (!ok) return -1;
   if constexpr (D == 2) return y[0] * g.shape_o[1] + y[1];
   else return (y[0] * g.shape_o[1] + y[1]) * g.shape_o[2] + y[2];
 }
 
 // Grid.h:15-35 Index_tot after the InRange_per test: tiled linear index of
-// the global position y (wrapped along periodic axes), or -1 outside.
+// the global position y (wrapped along periodic axes), or -1 outside.  Tiled
+// indices fit in 32 bits (the host rejects grids of 2^31 cells or more).
 template <int D, class TL>
-__device__ inline int64_t index_tot(const int* y_in, const GridDesc& g) {
+__device__ inline int index_tot(const int* y_in, const GridDesc& g) {
   int y[3] = {y_in[0], y_in[1], y_in[2]};
+  bool ok = true;
 #pragma unroll
   for (int k = 0; k < D; ++k) {
     if ((g.periodic >> k) & 1) y[k] = mod_pos(y[k], g.shape_tot[k]);
-    else if (y[k] < 0 || y[k] >= g.shape_tot[k]) return -1;
+    else ok = ok && y[k] >= 0 && y[k] < g.shape_tot[k];
   }
+  if (!ok) return -1;
   int yo[3], yi[3];
 #pragma unroll
   for (int k = 0; k < D; ++k) {
@@ -132,9 +143,9 @@ __device__ inline int64_t index_tot(const int* y_in, const GridDesc& g) {
     yi[k] = y[k] % TL::T[k];
   }
   if constexpr (D == 2) {
-    return int64_t(yo[0] * g.shape_o[1] + yo[1]) * TL::NI + (yi[0] * TL::T[1] + yi[1]);
+    return (yo[0] * g.shape_o[1] + yo[1]) * TL::NI + (yi[0] * TL::T[1] + yi[1]);
   } else {
-    return int64_t((yo[0] * g.shape_o[1] + yo[1]) * g.shape_o[2] + yo[2]) * TL::NI +
+    return ((yo[0] * g.shape_o[1] + yo[1]) * g.shape_o[2] + yo[2]) * TL::NI +
            ((yi[0] * TL::T[1] + yi[1]) * TL::T[2] + yi[2]);
   }
 }

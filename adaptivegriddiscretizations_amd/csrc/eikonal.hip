@@ -71,16 +71,27 @@ __device__ inline int32_t read_score(const int32_t* arr, int32_t tile, int32_t i
 }
 
 // ------------------------------------------------------------------ update
-template <class S, int MODEL, int D, int T0, int T1, int T2>
+// Block-wide "any": one ballot for single-wave tiles, an LDS reduction
+// otherwise.
+template <int NI> __device__ inline bool block_any(bool p) {
+  if constexpr (NI <= 64) {
+    __syncthreads();
+    return __ballot(p) != 0;
+  } else {
+    return __syncthreads_or(p) != 0;
+  }
+}
+
+template <class S, int MODEL, int D, int T0, int T1, int T2, bool WALLS>
 __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
   using TL = Tile<D, T0, T1, T2>;
   using MT = ModelTraits<MODEL, D>;
   constexpr int NI = TL::NI;
   constexpr int NS = MT::NSYM;
   constexpr int NG = MT::NGEOM;
+  constexpr int NJ = 2 * NS;
   __shared__ S u_s[NI];
-  __shared__ uint8_t wall_s[NI];
-  __shared__ int32_t score_s[8];
+  __shared__ uint8_t wall_s[WALLS ? NI : 1];
 
   const int tid = threadIdx.x;
   const int count = *a.count_in;
@@ -91,6 +102,7 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
   int xi[3];
   TL::pos(tid, xi);
   const int32_t tag = a.iter + 1;
+  const bool fim = a.solver == AGD_FIM;
 
   for (int li = blockIdx.x; li < count; li += gridDim.x) {
     const int n_o = a.list_in[li];
@@ -99,17 +111,28 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
     int xt[3] = {0, 0, 0};
 #pragma unroll
     for (int k = 0; k < D; ++k) xt[k] = xo[k] * TL::T[k] + xi[k];
-    const int64_t n_t = int64_t(n_o) * NI + tid;
+    const int n_t = n_o * NI + tid;
 
-    // ---- loads (coalesced within the tile)
+    // ---- activation inputs first: the tile and its face neighbours
+    // (lanes 0..2D of wave 0), and their FIM scores of the previous pass.
+    int nb = -1, sc = 0;
+    if (tid < 2 * D + 1) {
+      nb = (tid == 2 * D) ? n_o : tile_neighbor<D>(xo, tid >> 1, 2 * (tid & 1) - 1, a.g);
+      if (fim && nb >= 0) sc = read_score(a.score_prev, nb, a.iter - 1);
+    }
+
+    // ---- cell loads (coalesced: a tile is contiguous)
     const S u_old = a.u[n_t];
     const S rhs = a.rhs[n_t];
     const bool seed = (a.seeds[n_t >> 5] >> (n_t & 31)) & 1u;
     S geo[NG > 0 ? NG : 1];
 #pragma unroll
-    for (int k = 0; k < NG; ++k) geo[k] = a.geom[k * a.g.size_tot + n_t];
+    for (int k = 0; k < NG; ++k) {
+      const S* __restrict__ gk = a.geom + k * a.g.size_tot;  // uniform base
+      geo[k] = gk[n_t];
+    }
     int wx = 255;
-    if (a.wall) {
+    if constexpr (WALLS) {
       wx = a.wall[n_t];
       wall_s[tid] = uint8_t(wx);
     }
@@ -117,46 +140,61 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
 
     Stencil<S, MODEL, D> st;
     make_stencil<S, MODEL, D>(geo, a.iso_w, st);
-    if (a.wall) __syncthreads();  // wall_s complete before the visibility walks
+    if constexpr (WALLS) __syncthreads();  // wall_s complete before the visibility walks
 
-    // ---- resolve neighbours: LDS slot (tile) or gathered value (outside)
-    int nidx[2 * NS];
-    S vo[2 * NS];
+    // ---- resolve the 2*nsym neighbours: LDS slot inside the tile, else one
+    // gathered value.  All gathers are issued before any is consumed (safe
+    // address = own cell when no load is needed), so a visit costs a single
+    // memory round trip.
+    int nidx[NJ];
+    int gidx[NJ];
+    bool gval[NJ];
 #pragma unroll
     for (int k = 0; k < NS; ++k) {
 #pragma unroll
       for (int s = 0; s < 2; ++s) {
+        const int j = 2 * k + s;
         const int sg = 2 * s - 1;
         int e[3] = {sg * st.e[k][0], sg * st.e[k][1], sg * st.e[k][2]};
-        const S sh = (s == 0) ? st.shift[k] : -st.shift[k];
         bool vis = true;
-        if (a.wall && wx != 255) {
-          vis = visible<D>(e, wx, [&](const int* w) -> int {
-            int yi[3] = {xi[0] + w[0], xi[1] + w[1], xi[2] + w[2]};
-            int yt[3] = {xt[0] + w[0], xt[1] + w[1], xt[2] + w[2]};
-            bool dom = true;
+        if constexpr (WALLS) {
+          if (wx != 255) {
+            vis = visible<D>(e, wx, [&](const int* w) -> int {
+              int yi[3] = {xi[0] + w[0], xi[1] + w[1], xi[2] + w[2]};
+              int yt[3] = {xt[0] + w[0], xt[1] + w[1], xt[2] + w[2]};
+              bool dom = true;
 #pragma unroll
-            for (int c = 0; c < D; ++c)
-              dom = dom && (((a.g.periodic >> c) & 1) || (yt[c] >= 0 && yt[c] < a.g.shape_tot[c]));
-            if (TL::inside(yi) && dom) return wall_s[TL::index(yi)];
-            const int64_t m = index_tot<D, TL>(yt, a.g);
-            return m >= 0 ? int(a.wall[m]) : 255;
-          });
+              for (int c = 0; c < D; ++c)
+                dom = dom && (((a.g.periodic >> c) & 1) || (yt[c] >= 0 && yt[c] < a.g.shape_tot[c]));
+              if (TL::inside(yi) && dom) return wall_s[TL::index(yi)];
+              const int m = index_tot<D, TL>(yt, a.g);
+              return m >= 0 ? int(a.wall[m]) : 255;
+            });
+          }
         }
         int yi[3] = {xi[0] + e[0], xi[1] + e[1], xi[2] + e[2]};
         int yt[3] = {xt[0] + e[0], xt[1] + e[1], xt[2] + e[2]};
         bool in_dom = true;
 #pragma unroll
         for (int c = 0; c < D; ++c) in_dom = in_dom && (yt[c] >= 0 && yt[c] < a.g.shape_tot[c]);
+        const bool local = vis && TL::inside(yi) && in_dom;
+        nidx[j] = local ? TL::index(yi) : -1;
+        const int m = (vis && !local) ? index_tot<D, TL>(yt, a.g) : -1;
+        gval[j] = m >= 0;
+        gidx[j] = m >= 0 ? m : n_t;
+      }
+    }
+    S g[NJ];
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) g[j] = a.u[gidx[j]];
+    S vo[NJ];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {
         const int j = 2 * k + s;
-        if (vis && TL::inside(yi) && in_dom) {
-          nidx[j] = TL::index(yi);
-          vo[j] = sh;
-        } else {
-          nidx[j] = -1;
-          const int64_t m = vis ? index_tot<D, TL>(yt, a.g) : -1;
-          vo[j] = (m >= 0) ? a.u[m] + sh : inf_v<S>();
-        }
+        const S sh = (s == 0) ? st.shift[k] : -st.shift[k];
+        vo[j] = nidx[j] >= 0 ? sh : (gval[j] ? g[j] + sh : inf_v<S>());
       }
     }
     __syncthreads();  // u_s complete
@@ -176,7 +214,7 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
       __syncthreads();
       const bool ch = active && (u_s[tid] > un);
       if (ch) u_s[tid] = un;
-      if (!__syncthreads_or(ch)) break;
+      if (!block_any<NI>(ch)) break;
     }
 
     // ---- store and convergence (Update.h:163-219)
@@ -185,44 +223,35 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
     const S diff = abs_(u_old - un);
     const S tol = a.atol + a.rtol * abs_(un);
     const bool changed = !(isnan_(diff) || diff <= tol);
-    const bool any = __syncthreads_or(changed);
+    const bool any = block_any<NI>(changed);
 
-    // ---- activation (Propagation.h:103-167)
-    // (The global-iteration solver runs with AGSI tags: they only decide
-    // when to stop, every pass visits the full list.)
-    if (a.solver == AGD_AGSI) {
-      if (any && tid < 2 * D + 1) {
-        const int nb = (tid == 2 * D) ? n_o : tile_neighbor<D>(xo, tid >> 1, 2 * (tid & 1) - 1, a.g);
-        if (nb >= 0) enqueue(nb, tag, a.stamp, a.list_out, a.count_out);
-      }
-    } else {  // FIM
-      if (tid < 2 * D + 1) {
-        const int nb = (tid == 2 * D) ? n_o : tile_neighbor<D>(xo, tid >> 1, 2 * (tid & 1) - 1, a.g);
-        score_s[tid] = nb >= 0 ? read_score(a.score_prev, nb, a.iter - 1) : 0;
-      }
-      __syncthreads();
-      const int front = a.fim_front;
-      const int prev = score_s[2 * D];
-      int next;
-      bool prop;
-      if (!any) {
-        if (prev == 0 || prev == front) { next = 0; prop = false; }
-        else { next = front; prop = true; }
+    // ---- activation (Propagation.h:103-167), wave 0 only.  The
+    // global-iteration solver runs with AGSI tags: they only decide when to
+    // stop, every pass visits the full list.
+    if (tid < 64) {
+      if (!fim) {
+        if (any && nb >= 0) enqueue(nb, tag, a.stamp, a.list_out, a.count_out);
       } else {
-        next = 1;
-        if (front > 2) {
+        const int front = a.fim_front;
+        const int prev = __shfl(sc, 2 * D);
+        int mx = 0;
 #pragma unroll
-          for (int i = 0; i < 2 * D; ++i) next = max(next, score_s[i] - 1);
+        for (int i = 0; i < 2 * D; ++i) mx = max(mx, __shfl(sc, i));
+        int next;
+        bool prop;
+        if (!any) {
+          if (prev == 0 || prev == front) { next = 0; prop = false; }
+          else { next = front; prop = true; }
+        } else {
+          next = 1;
+          if (front > 2) next = max(next, mx - 1);
+          prop = next > 1;
         }
-        prop = next > 1;
-      }
-      if (tid == 0) {
-        a.score_next[n_o] = (a.iter << 8) | next;
-        if (any) enqueue(n_o, tag, a.stamp, a.list_out, a.count_out);
-      }
-      if (prop && tid < 2 * D) {
-        const int nb = tile_neighbor<D>(xo, tid >> 1, 2 * (tid & 1) - 1, a.g);
-        if (nb >= 0) enqueue(nb, tag, a.stamp, a.list_out, a.count_out);
+        if (tid == 2 * D) {
+          a.score_next[n_o] = (a.iter << 8) | next;
+          if (any) enqueue(n_o, tag, a.stamp, a.list_out, a.count_out);
+        }
+        if (prop && tid < 2 * D && nb >= 0) enqueue(nb, tag, a.stamp, a.list_out, a.count_out);
       }
     }
     __syncthreads();  // LDS reuse by the next tile of this workgroup
@@ -461,6 +490,7 @@ __global__ void wall_dist_cap(int64_t N, uint8_t* d, int bound) {
 // ------------------------------------------------------------------ dispatch
 template <class S> struct KernelSet {
   void (*update)(EikArgs<S>);
+  void (*update_walls)(EikArgs<S>);
   void (*flow)(FlowArgs<S>);
   void (*init)(const uint32_t*, int32_t, int32_t, int32_t*, int32_t*, int32_t*);
   void (*wallpass)(GridDesc, const uint8_t*, uint8_t*);
@@ -470,7 +500,8 @@ template <class S> struct KernelSet {
 template <class S, int MODEL, int D, int T0, int T1, int T2>
 KernelSet<S> make_set() {
   KernelSet<S> k;
-  k.update = eik_update<S, MODEL, D, T0, T1, T2>;
+  k.update = eik_update<S, MODEL, D, T0, T1, T2, false>;
+  k.update_walls = eik_update<S, MODEL, D, T0, T1, T2, true>;
   k.flow = eik_flow<S, MODEL, D, T0, T1, T2>;
   k.init = eik_init_list<T0 * T1 * T2>;
   k.wallpass = wall_dist_pass<D, T0, T1, T2>;
@@ -524,6 +555,9 @@ static int check_desc(const agd_eikonal_desc* d) {
   }
   AGD_REQUIRE(d->niter_i >= 1, "niter_i must be >= 1");
   AGD_REQUIRE(d->nitermax_o >= 1, "nitermax_o must be >= 1");
+  int64_t size_tot = 1;
+  for (int k = 0; k < d->ndim; ++k) size_tot *= ceil_div(d->shape[k], d->shape_i[k]) * d->shape_i[k];
+  AGD_REQUIRE(size_tot < (int64_t(1) << 31), "grids of 2^31 padded cells or more are not supported");
   return AGD_OK;
 }
 
@@ -655,12 +689,15 @@ static int solve_impl(const agd_eikonal_desc* d, const agd_eikonal_io* io,
     a.iter = p.iter;
     a.score_prev = wsp->score + ((p.iter + 1) % 2) * wsp->cap;
     a.score_next = wsp->score + (p.iter % 2) * wsp->cap;
-    hipLaunchKernelGGL(ks.update, dim3(blocks), dim3(ni), 0, stream, a);
+    hipLaunchKernelGGL(io->wall_dist ? ks.update_walls : ks.update, dim3(blocks), dim3(ni), 0,
+                       stream, a);
   };
   WorklistResult r;
-  int st = run_worklist(g.ntiles, d->nitermax_o, d->check_interval, stream, init, launch, r);
+  int st = run_worklist(g.ntiles, d->nitermax_o, d->check_interval, stream, init, launch, r,
+                        d->time_launches != 0);
   if (st) return st;
   if (stats) {
+    stats->kernel_ms = r.kernel_ms;
     stats->niter_o = r.niter;
     stats->tile_updates = r.visits;
     stats->cell_updates = r.visits * ni;

@@ -58,6 +58,7 @@ struct WorklistResult {
   int64_t niter = 0;       // passes with a non-empty list
   int64_t visits = 0;      // tile visits
   float ms = 0.f;
+  double kernel_ms = 0.;   // sum of per-launch durations (time_launches only)
   bool converged = false;
 };
 
@@ -67,7 +68,7 @@ struct WorklistResult {
 // histogram are reset here.
 template <class Init, class Launch>
 int run_worklist(int32_t ntiles, int64_t nitermax, int check_interval, hipStream_t stream,
-                 Init init, Launch launch, WorklistResult& res) {
+                 Init init, Launch launch, WorklistResult& res, bool time_launches = false) {
   Workspace* ws = nullptr;
   int st = get_workspace(ntiles, nitermax + 8, ws);
   if (st) return st;
@@ -82,6 +83,24 @@ int run_worklist(int32_t ntiles, int64_t nitermax, int check_interval, hipStream
   AGD_HIP_TRY(hipGetLastError());
   AGD_HIP_TRY(hipEventRecord(ws->ev0, stream));
   const int ci = std::max(1, check_interval);
+  // Optional per-launch timing: one event pair per pass, read back at every
+  // convergence check (the stream is synchronised there anyway).
+  std::vector<hipEvent_t> evs;
+  if (time_launches) {
+    evs.resize(2 * size_t(ci));
+    for (auto& e : evs) AGD_HIP_TRY(hipEventCreate(&e));
+  }
+  res.kernel_ms = 0.;
+  int pending = 0;
+  auto drain = [&]() -> int {
+    for (int i = 0; i < pending; ++i) {
+      float t = 0.f;
+      AGD_HIP_TRY(hipEventElapsedTime(&t, evs[2 * i], evs[2 * i + 1]));
+      res.kernel_ms += t;
+    }
+    pending = 0;
+    return AGD_OK;
+  };
   int64_t k = 0;
   bool converged = false;
   while (k < nitermax) {
@@ -92,12 +111,20 @@ int run_worklist(int32_t ntiles, int64_t nitermax, int check_interval, hipStream
     p.count_out = counts + ((k + 1) % 3);
     p.count_clear = counts + ((k + 2) % 3);
     p.iter = int32_t(k);
+    if (time_launches) AGD_HIP_TRY(hipEventRecord(evs[2 * pending], stream));
     launch(p);
+    if (time_launches) AGD_HIP_TRY(hipEventRecord(evs[2 * pending + 1], stream));
+    ++pending;
     ++k;
     if (k % ci == 0 || k == nitermax) {
       AGD_HIP_TRY(hipMemcpyAsync(ws->host_count, counts + (k % 3), sizeof(int32_t),
                                  hipMemcpyDeviceToHost, stream));
       AGD_HIP_TRY(hipStreamSynchronize(stream));
+      if (time_launches) {
+        const int st2 = drain();
+        if (st2) return st2;
+      }
+      pending = 0;
       if (*ws->host_count == 0) {
         converged = true;
         break;
@@ -108,6 +135,11 @@ int run_worklist(int32_t ntiles, int64_t nitermax, int check_interval, hipStream
   AGD_HIP_TRY(hipEventRecord(ws->ev1, stream));
   AGD_HIP_TRY(hipEventSynchronize(ws->ev1));
   AGD_HIP_TRY(hipEventElapsedTime(&res.ms, ws->ev0, ws->ev1));
+  if (time_launches) {
+    const int st2 = drain();
+    if (st2) return st2;
+    for (auto& e : evs) AGD_HIP_TRY(hipEventDestroy(e));
+  }
   std::vector<int32_t> hist(k);
   if (k > 0)
     AGD_HIP_TRY(hipMemcpy(hist.data(), ws->hist, size_t(k) * sizeof(int32_t), hipMemcpyDeviceToHost));

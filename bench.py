@@ -1,0 +1,212 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X eikonal solver.
+
+Workload (default): 3D Riemannian eikonal on a 200^3 grid over [-1,1]^3,
+M = Q diag(l) Q^T with Q from a smoothed quaternion field and log l smooth in
+[0, ln 5], one seed at the centre, FP32, default tolerances
+(atol = rtol = 5e-6 mean cost), FIM activation, 4x4x4 tiles, 4 in-tile sweeps.
+A step is one cold-start solve to convergence of that grid (values reset to
++inf, then the device-side worklist runs until no tile is active), with all
+inputs (tiled geometry, rhs, seed bits) resident in HBM.
+
+Reported:
+  value         converged cell-updates/s: sum over the timed steps of
+                (tile visits x 64 cells) / wall time of the steps;
+  roofline      the tile-update kernel's algorithmic HBM bytes
+                (36.125 B per cell-update: u read+write 8, rhs 4, metric 24,
+                seed bit 1/8) over its summed per-launch HIP-event time;
+  cpu_baseline  the oracle's single-thread FP64 fast marching on a 100^3
+                sample of the same generator (solved cells/s).
+
+Usage: python bench.py [--steps K] [--warmup W] [--config riemann3|riemann2|isotropic2]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_CELL_UPDATE = {'Isotropic2': 12.125, 'Riemann2': 24.125, 'Riemann3': 36.125}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_workload(name, small=False):
+    from adaptivegriddiscretizations_amd import workloads
+    if name == 'riemann3':
+        return workloads.riemann3_aniso(n=100 if small else 200)
+    if name == 'riemann2':
+        return workloads.riemann2_aniso(n=1000 if small else 2000)
+    if name == 'isotropic2':
+        return workloads.isotropic2_bumps(n=512)
+    raise ValueError(name)
+
+
+def prepare(wl, **extra):
+    from adaptivegriddiscretizations_amd.Eikonal import HFM_GPU
+    hfmIn = wl.dict_in(**extra)
+    iface = HFM_GPU.Interface(hfmIn)
+    iface.SetKernelTraits()
+    iface.SetGeometry()
+    iface.SetArgs()
+    return iface
+
+
+def solve_once(iface, desc):
+    import torch
+    from adaptivegriddiscretizations_amd import _lib
+    iface.values_t.fill_(float('inf'))
+    iface._values = None
+    st = _lib.EikonalStats()
+    _lib.check(_lib.lib.agd_eikonal_solve(desc, iface._io(), st, torch.cuda.current_stream().cuda_stream))
+    return st
+
+
+def cpu_baseline(name, budget_s=30.):
+    """Single-thread FP64 fast marching of the oracle on a reduced sample."""
+    sys.path.insert(0, os.path.join(ROOT, 'oracle'))
+    import numpy as np
+    import oracle as O
+    wl = make_workload(name, small=True)
+    sides = np.asarray(wl.sides, dtype=np.float64)
+    h = wl.h.astype(np.float32).astype(np.float64)
+    idx = O.seed_indices(wl.seeds, sides[:, 0], h)
+    kw = {'metric': wl.metric.astype(np.float64)} if wl.metric is not None else {}
+    cost = 1. if wl.cost is None else wl.cost.astype(np.float64)
+    P = O.make_problem(wl.model[:-1], wl.dims, h, cost=cost, seeds_idx=idx, **kw).cast(np.float32)
+    t0 = time.perf_counter()
+    u = P.solve_fm()
+    dt = time.perf_counter() - t0
+    assert np.isfinite(u).all()
+    return {"value": wl.ncells / dt, "unit": "solved cells/s", "cores": 1, "kind": "port",
+            "sample": f"{wl.name} {'x'.join(map(str, wl.dims))} ({wl.ncells} cells, same generator), "
+                      f"FP64 fast marching incl. stencil construction, {dt:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=10)
+    ap.add_argument('--warmup', type=int, default=2)
+    ap.add_argument('--config', default='riemann3', choices=['riemann3', 'riemann2', 'isotropic2'])
+    ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--no-fp64', action='store_true')
+    ap.add_argument('--copy-calibration', type=int, default=0, metavar='MIB',
+                    help='only run a plain device copy of MIB MiB three times (reference for '
+                         'the FETCH_SIZE / WRITE_SIZE counters) and exit')
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the solver has no "
+              "multi-GPU path in this build", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from adaptivegriddiscretizations_amd import _lib
+
+    torch.cuda.set_device(0)
+    if args.copy_calibration:
+        n = args.copy_calibration * (1 << 20) // 4
+        src = torch.rand(n, device='cuda')
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": n * 4}), flush=True)
+        return
+    t0 = time.time()
+    wl = make_workload(args.config)
+    log(f"workload {wl.name} {wl.dims} generated in {time.time() - t0:.1f} s")
+    iface = prepare(wl)
+    desc = iface._desc()
+    desc.time_launches = 1
+    log(f"tiles {iface.shape_i} x {iface.ntiles}, niter_i {iface.niter_i}, solver {iface.solver}, "
+        f"atol {iface.atol:.3g} rtol {iface.rtol:.3g}")
+
+    for i in range(args.warmup):
+        st = solve_once(iface, desc)
+        log(f"warmup {i}: {st.gpu_ms:.2f} ms, passes {st.launches}, cell-updates {st.cell_updates}")
+    torch.cuda.synchronize()
+    stats = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        stats.append(solve_once(iface, desc))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    for i, st in enumerate(stats):
+        log(f"step {i}: {st.gpu_ms:.2f} ms, niter_o {st.niter_o}, passes {st.launches}, "
+            f"cell-updates {st.cell_updates}, kernel {st.kernel_ms:.2f} ms")
+
+    cell_updates = sum(st.cell_updates for st in stats)
+    kernel_ms = sum(st.kernel_ms for st in stats)
+    launches = sum(st.launches for st in stats)
+    bpc = BYTES_PER_CELL_UPDATE[wl.model]
+    achieved = cell_updates * bpc / (kernel_ms * 1e-3) / 1e9
+    u32 = iface.values.float().cpu().numpy()
+    assert np.isfinite(u32).all(), "non-finite values in the benchmark solve"
+
+    fp64_err = None
+    if not args.no_fp64:
+        import numpy as _np
+        iface64 = prepare(wl, traits={'Scalar': _np.float64, 'shape_i': iface.shape_i,
+                                      'niter_i': iface.niter_i},
+                          atol=iface.atol, rtol=iface.rtol)
+        st64 = solve_once(iface64, iface64._desc())
+        u64 = iface64.values.cpu().numpy()
+        fp64_err = float(np.max(np.abs(u32 - u64)) / np.max(np.abs(u64)))
+        log(f"fp64 run: {st64.gpu_ms:.1f} ms, max|u32-u64|/max u64 = {fp64_err:.3e}")
+        del iface64
+
+    result = {
+        "metric": "converged cell-updates/s and %HBM roofline; time-to-solution warm vs cold @1e-6",
+        "value": cell_updates / wall,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{wl.name} {'x'.join(map(str, wl.dims))}, one seed, cold start to convergence",
+            "model": wl.model, "grid": list(wl.dims), "tile": list(iface.shape_i),
+            "niter_i": iface.niter_i, "solver": iface.solver, "atol": iface.atol, "rtol": iface.rtol,
+            "parallelism": "single GPU",
+        },
+        "solved_cells_per_s": wl.ncells * args.steps / wall,
+        "cell_updates_per_step": cell_updates / args.steps,
+        "passes_per_step": launches / args.steps,
+        "fp64_max_rel_err": fp64_err,
+        "roofline": {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "kernel": "eik_update", "bytes_per_cell_update": bpc,
+            "avg_launch_us": kernel_ms / launches * 1e3,
+        },
+        "cpu_baseline": None,
+    }
+    tpath = os.path.join(ROOT, 'profiles', f'eik_update_{args.config}_{wl.dims[0]}_traffic.json')
+    if os.path.exists(tpath):
+        # HBM bytes per dispatch from the committed FETCH_SIZE / WRITE_SIZE runs
+        # (scaled by the device-copy calibration) for this workload.
+        t = json.load(open(tpath))
+        result["roofline"]["traffic"] = t["hbm_bytes_per_solve"] / t["dispatches_per_solve"]
+        result["roofline"]["traffic_source"] = os.path.relpath(tpath, ROOT)
+        result["roofline"]["algorithmic_bytes_per_dispatch"] = cell_updates * bpc / launches
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        result["cpu_baseline"] = cpu_baseline(args.config)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == '__main__':
+    main()

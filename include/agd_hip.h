@@ -70,6 +70,8 @@ typedef struct agd_eikonal_desc {
   double iso_weights[3];  /* AGD_ISOTROPIC: per-axis weights (h^-2)      */
   int32_t check_interval; /* passes between host convergence checks (>=1) */
   int32_t activate_all;   /* 1: start with every tile active (warm start) */
+  int32_t time_launches;  /* 1: time every pass with an event pair (kernel_ms) */
+  int32_t reserved;
 } agd_eikonal_desc;
 
 /* Device pointers, tiled layout.  Scalars are float (AGD_F32) or double. */
@@ -90,6 +92,7 @@ typedef struct agd_eikonal_stats {
   double gpu_ms;          /* HIP-event time of the solve loop             */
   int32_t converged;      /* 1 if the tile list emptied                   */
   int32_t grid_blocks;    /* workgroups per launch                         */
+  double kernel_ms;       /* sum of per-pass kernel times (time_launches)  */
 } agd_eikonal_stats;
 
 /* Flow export (reference: Update.h:171-209 compiled with FLOW_* macros,

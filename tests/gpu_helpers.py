@@ -10,10 +10,12 @@ def log(msg):
     sys.stdout.flush()
 
 
-def oracle_problem(O, wl, walls=None, seed_values=None):
-    """Oracle problem matching workload `wl` (float32 inputs, FP64 arithmetic)."""
+def oracle_problem(O, wl, walls=None, seed_values=None, cast=True):
+    """Oracle problem matching workload `wl` (FP64 arithmetic).  With cast,
+    geometry and rhs are rounded to float32 as the FP32 device path stores
+    them; the FP64 device path keeps them in double."""
     sides = np.asarray(wl.sides, dtype=np.float64)
-    h = wl.h
+    h = wl.h.astype(np.float32).astype(np.float64)  # grid scales are float32 in GPU mode
     idx = O.seed_indices(wl.seeds, sides[:, 0], h)
     model = wl.model[:-1]
     kw = {}
@@ -26,7 +28,7 @@ def oracle_problem(O, wl, walls=None, seed_values=None):
     cost = 1. if wl.cost is None else wl.cost.astype(np.float64)
     P = O.make_problem(model, wl.dims, h, cost=cost, seeds_idx=idx, seed_values=seed_values,
                        walls=walls, **kw)
-    return P.cast(np.float32), idx
+    return (P.cast(np.float32) if cast else P), idx
 
 
 def run_gpu(wl, **extra):

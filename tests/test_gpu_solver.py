@@ -58,7 +58,7 @@ def test_riemann2_float64_tight(O):
     wl = workloads.riemann2_aniso(n=96, seed=6, sigma=4., amax=6.)
     u, out = run_gpu(wl, traits={'Scalar': np.float64, 'shape_i': (8, 8), 'niter_i': 8},
                      atol=1e-13, rtol=1e-13)
-    P, _ = oracle_problem(O, wl)
+    P, _ = oracle_problem(O, wl, cast=False)
     ref = P.solve_fm()
     err = rel_err(u, ref)
     log(f"riemann2 fp64 96^2 rel err {err:.2e}")
