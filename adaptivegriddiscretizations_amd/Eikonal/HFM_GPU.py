@@ -398,8 +398,11 @@ class Interface:
         self.raiseOnNonConvergence = self.GetValue('raiseOnNonConvergence', default=True,
                                                    help="Raise an exception if a solver fails to converge")
         self.check_interval = int(self.GetValue(
-            'check_interval', default=4,
+            'check_interval', default=8,
             help="Grid passes between two convergence checks on the host"))
+        self.use_graph = bool(self.GetValue(
+            'hipGraph', default=True,
+            help="Capture check_interval grid passes once as a hipGraph and replay it"))
 
     def _desc(self, niter_i=None):
         d = _lib.EikonalDesc()
@@ -419,6 +422,7 @@ class Interface:
         d.nitermax_o = getattr(self, 'nitermax_o', 2000)
         d.check_interval = getattr(self, 'check_interval', 4)
         d.activate_all = int(getattr(self, 'warmStartAllTiles', False))
+        d.use_graph = int(getattr(self, 'use_graph', True))
         return d
 
     def _io(self):

@@ -95,14 +95,7 @@ template <class S> struct LinArgs {
   GridDesc g;
   S atol, rtol;
   int32_t niter_i;
-  int32_t iter;
-  const int32_t* list_in;
-  const int32_t* count_in;
-  int32_t* list_out;
-  int32_t* count_out;
-  int32_t* count_clear;
-  int32_t* stamp;
-  int32_t* hist;
+  WorkCtx w;
 };
 
 template <class S> __global__ void fill_inf_kernel(int64_t n, S* x) {
@@ -118,14 +111,11 @@ __global__ void __launch_bounds__(T0* T1* T2) adj_update(LinArgs<S> a) {
   constexpr int NI = TL::NI;
   __shared__ S l_s[NI];
   const int tid = threadIdx.x;
-  const int count = *a.count_in;
-  if (blockIdx.x == 0 && tid == 0) {
-    *a.count_clear = 0;
-    a.hist[a.iter] = count;
-  }
-  const int32_t tag = a.iter + 1;
+  const PassView pv = pass_begin(a.w);
+  const int count = pv.count;
+  const int32_t tag = pv.iter + 1;
   for (int li = blockIdx.x; li < count; li += gridDim.x) {
-    const int n_o = a.list_in[li];
+    const int n_o = pv.list_in[li];
     int xo[3];
     tile_pos<D>(n_o, a.g, xo);
     const int64_t n_t = int64_t(n_o) * NI + tid;
@@ -156,7 +146,7 @@ __global__ void __launch_bounds__(T0* T1* T2) adj_update(LinArgs<S> a) {
     const bool any = __syncthreads_or(changed);
     if (any && tid < 2 * D + 1) {
       const int nb = (tid == 2 * D) ? n_o : tile_neighbor<D>(xo, tid >> 1, 2 * (tid & 1) - 1, a.g);
-      if (nb >= 0) enqueue(nb, tag, a.stamp, a.list_out, a.count_out);
+      if (nb >= 0) enqueue(nb, tag, a.w.stamp, pv.list_out, pv.count_out);
     }
     __syncthreads();
   }
@@ -276,29 +266,26 @@ static int adjoint_impl(const agd_eikonal_desc* d, const S* values, const S* fw,
     a.niter_i = d->niter_i;
     // Dummy initialisation to +inf marks unvisited cells (read as 0).
     hipLaunchKernelGGL(fill_inf_kernel<S>, grid, block, 0, stream, N, a.lam);
-    Workspace* wsp = nullptr;
     auto init = [&](Workspace* ws, int32_t* list0, int32_t* counts) -> int {
-      wsp = ws;
-      a.stamp = ws->stamp;
-      a.hist = ws->hist;
+      a.w.lists = ws->lists;
+      a.w.counts = counts;
+      a.w.stamp = ws->stamp;
+      a.w.hist = ws->hist;
+      a.w.score = ws->score;
+      a.w.cap = int32_t(ws->cap);
+      a.w.fixed = nullptr;
       hipLaunchKernelGGL(ls.init, dim3(unsigned(ceil_div(g.ntiles, 256))), dim3(256), 0, stream,
                          trig, g.ntiles, ws->stamp, list0, counts + 0);
       return AGD_OK;
     };
-    auto launch = [&](const PassPointers& p) {
-      a.list_in = p.list_in;
-      a.count_in = p.count_in;
-      a.list_out = p.list_out;
-      a.count_out = p.count_out;
-      a.count_clear = p.count_clear;
-      a.iter = p.iter;
-      hipLaunchKernelGGL(ls.update, dim3(blocks), dim3(ls.ni), 0, stream, a);
+    auto launch = [&](int32_t koff, hipStream_t s) {
+      a.w.koff = koff;
+      hipLaunchKernelGGL(ls.update, dim3(blocks), dim3(ls.ni), 0, s, a);
     };
     WorklistResult res;
     status = run_worklist(g.ntiles, d->nitermax_o, d->check_interval, stream, init, launch, res,
-                          d->time_launches != 0);
+                          d->time_launches != 0, d->use_graph != 0);
     kernel_ms += res.kernel_ms;
-    (void)wsp;
     niter += res.niter;
     visits += res.visits;
     passes += res.passes;

@@ -89,6 +89,28 @@ __device__ inline int mod_pos(int x, int n) {
   return x >= 0 ? x : x + n;
 }
 
+// Worklist entries are packed tile coordinates (10 bits per axis in 3D, 15
+// in 2D), so a pass decodes its tile with shifts instead of divisions.
+template <int D> __device__ __host__ inline int pack_tile(const int* xo) {
+  if constexpr (D == 2) return xo[0] | (xo[1] << 15);
+  else return xo[0] | (xo[1] << 10) | (xo[2] << 20);
+}
+template <int D> __device__ inline void unpack_tile(int p, int* xo) {
+  if constexpr (D == 2) {
+    xo[0] = p & 0x7fff;
+    xo[1] = (p >> 15) & 0x7fff;
+    xo[2] = 0;
+  } else {
+    xo[0] = p & 0x3ff;
+    xo[1] = (p >> 10) & 0x3ff;
+    xo[2] = (p >> 20) & 0x3ff;
+  }
+}
+template <int D> __device__ inline int tile_linear(const int* xo, const GridDesc& g) {
+  if constexpr (D == 2) return xo[0] * g.shape_o[1] + xo[1];
+  else return (xo[0] * g.shape_o[1] + xo[1]) * g.shape_o[2] + xo[2];
+}
+
 // Position of tile n_o in the tile grid.
 template <int D> __device__ inline void tile_pos(int n_o, const GridDesc& g, int* xo) {
   if constexpr (D == 2) {
@@ -107,7 +129,8 @@ template <int D> __device__ inline void tile_pos(int n_o, const GridDesc& g, int
 // neighbouring tile of xo along axis `ax` in direction `dir`, -1 if outside.
 // (Axis selection by comparison keeps every array access compile-time.)
 template <int D>
-__device__ inline int tile_neighbor(const int* xo, int ax, int dir, const GridDesc& g) {
+__device__ inline int tile_neighbor(const int* xo, int ax, int dir, const GridDesc& g,
+                                    int* packed = nullptr) {
   int y[3] = {xo[0], xo[1], xo[2]};
   bool ok = true;
 #pragma unroll
@@ -119,20 +142,21 @@ __device__ inline int tile_neighbor(const int* xo, int ax, int dir, const GridDe
     }
   }
   if (!ok) return -1;
-  if constexpr (D == 2) return y[0] * g.shape_o[1] + y[1];
-  else return (y[0] * g.shape_o[1] + y[1]) * g.shape_o[2] + y[2];
+  if (packed) *packed = pack_tile<D>(y);
+  return tile_linear<D>(y, g);
 }
 
 // Grid.h:15-35 Index_tot after the InRange_per test: tiled linear index of
-// the global position y (wrapped along periodic axes), or -1 outside.  Tiled
-// indices fit in 32 bits (the host rejects grids of 2^31 cells or more).
-template <int D, class TL>
+// the global position y (wrapped along periodic axes when PER), or -1
+// outside.  Tiled indices fit in 32 bits (the host rejects grids of 2^31
+// cells or more).
+template <int D, class TL, bool PER = true>
 __device__ inline int index_tot(const int* y_in, const GridDesc& g) {
   int y[3] = {y_in[0], y_in[1], y_in[2]};
   bool ok = true;
 #pragma unroll
   for (int k = 0; k < D; ++k) {
-    if ((g.periodic >> k) & 1) y[k] = mod_pos(y[k], g.shape_tot[k]);
+    if (PER && ((g.periodic >> k) & 1)) y[k] = mod_pos(y[k], g.shape_tot[k]);
     else ok = ok && y[k] >= 0 && y[k] < g.shape_tot[k];
   }
   if (!ok) return -1;
@@ -208,18 +232,24 @@ template <class S> __device__ inline void selling2(const S* m, S* w, int (&e)[3]
   e[2][0] = -sb[1][1]; e[2][1] = sb[1][0]; e[2][2] = 0;
 }
 
-// One step of the 3D reduction with permutation (i,j,k,l) (Geometry3.h:27-37).
+// One step of the 3D reduction with permutation (i,j,k,l) (Geometry3.h:27-37),
+// predicated: the step takes effect only in lanes still iterating (`live`),
+// which keeps the loop free of divergent branches (and of the registers
+// their merges cost).
 template <class S, int I, int J, int K, int L>
-__device__ inline void selling3_step(int (&sb)[4][3], const S* m, int& red) {
-  if (scal_vmv3<S>(sb[I], m, sb[J]) > S(0)) {
+__device__ inline void selling3_step(int (&sb)[4][3], const S* m, int& it, int& red) {
+  const bool live = it < kSellingMaxIter && red < 6;
+  const bool acute = scal_vmv3<S>(sb[I], m, sb[J]) > S(0);
+  const bool flip = live && acute;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      sb[K][c] += sb[I][c];
-      sb[L][c] += sb[I][c];
-      sb[I][c] = -sb[I][c];
-    }
-    red = 0;
+  for (int c = 0; c < 3; ++c) {
+    const int bi = sb[I][c];
+    sb[K][c] = flip ? sb[K][c] + bi : sb[K][c];
+    sb[L][c] = flip ? sb[L][c] + bi : sb[L][c];
+    sb[I][c] = flip ? -bi : bi;
   }
+  red = flip ? 1 : (live ? red + 1 : red);
+  it = live ? it + 1 : it;
 }
 
 template <int I, int J, int K, int L, class S>
@@ -232,21 +262,21 @@ __device__ inline void selling3_out(const int (&sb)[4][3], const S* m, S& w, int
 }
 
 // Selling decomposition, 3D: weights w[6], offsets e[6][3].  Permutations
-// {0,1,2,3},{0,2,1,3},{0,3,1,2},{1,2,0,3},{1,3,0,2},{2,3,0,1}.
+// {0,1,2,3},{0,2,1,3},{0,3,1,2},{1,2,0,3},{1,3,0,2},{2,3,0,1}.  The loop runs
+// in rounds of the 6 phases until no lane of the wave is still iterating.
 template <class S> __device__ inline void selling3(const S* m, S* w, int (&e)[6][3]) {
   int sb[4][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {-1, -1, -1}};
   int it = 0, red = 0;
-  bool done = false;
-#define AGD_S3(I, J, K, L)                                                          \
-  if (!done) {                                                                      \
-    if (it < kSellingMaxIter && red < 6) { selling3_step<S, I, J, K, L>(sb, m, red); ++it; ++red; } \
-    else done = true;                                                               \
+#pragma unroll 1
+  for (int round = 0; round < (kSellingMaxIter + 5) / 6; ++round) {
+    selling3_step<S, 0, 1, 2, 3>(sb, m, it, red);
+    selling3_step<S, 0, 2, 1, 3>(sb, m, it, red);
+    selling3_step<S, 0, 3, 1, 2>(sb, m, it, red);
+    selling3_step<S, 1, 2, 0, 3>(sb, m, it, red);
+    selling3_step<S, 1, 3, 0, 2>(sb, m, it, red);
+    selling3_step<S, 2, 3, 0, 1>(sb, m, it, red);
+    if (!__any(it < kSellingMaxIter && red < 6)) break;
   }
-  for (int outer = 0; outer < kSellingMaxIter / 6 + 2 && !done; ++outer) {
-    AGD_S3(0, 1, 2, 3) AGD_S3(0, 2, 1, 3) AGD_S3(0, 3, 1, 2)
-    AGD_S3(1, 2, 0, 3) AGD_S3(1, 3, 0, 2) AGD_S3(2, 3, 0, 1)
-  }
-#undef AGD_S3
   selling3_out<0, 1, 2, 3>(sb, m, w[0], e[0]);
   selling3_out<0, 2, 1, 3>(sb, m, w[1], e[1]);
   selling3_out<0, 3, 1, 2>(sb, m, w[2], e[2]);

@@ -51,18 +51,8 @@ template <class S> struct EikArgs {
   int32_t niter_i;
   int32_t solver;
   int32_t fim_front;
-  int32_t iter;
-  // worklist
-  const int32_t* list_in;
-  const int32_t* count_in;
-  int32_t* list_out;
-  int32_t* count_out;
-  int32_t* count_clear;
-  int32_t* stamp;
-  int32_t* hist;
-  // FIM scores, packed (iteration << 8) | score, two arrays by parity
-  const int32_t* score_prev;
-  int32_t* score_next;
+  // worklist; FIM scores are packed (pass << 8) | score, two arrays by parity
+  WorkCtx w;
 };
 
 __device__ inline int32_t read_score(const int32_t* arr, int32_t tile, int32_t iter) {
@@ -82,8 +72,11 @@ template <int NI> __device__ inline bool block_any(bool p) {
   }
 }
 
-template <class S, int MODEL, int D, int T0, int T1, int T2, bool WALLS>
+// GENERAL = walls and/or periodic axes; the fast variant assumes neither, so
+// its neighbour indexing carries no wrap-around code.
+template <class S, int MODEL, int D, int T0, int T1, int T2, bool GENERAL>
 __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
+  constexpr bool WALLS = GENERAL;
   using TL = Tile<D, T0, T1, T2>;
   using MT = ModelTraits<MODEL, D>;
   constexpr int NI = TL::NI;
@@ -94,20 +87,18 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
   __shared__ uint8_t wall_s[WALLS ? NI : 1];
 
   const int tid = threadIdx.x;
-  const int count = *a.count_in;
-  if (blockIdx.x == 0 && tid == 0) {
-    if (a.count_clear) *a.count_clear = 0;
-    a.hist[a.iter] = count;
-  }
+  const PassView pv = pass_begin(a.w);
+  const int count = pv.count;
   int xi[3];
   TL::pos(tid, xi);
-  const int32_t tag = a.iter + 1;
+  const int32_t tag = pv.iter + 1;
   const bool fim = a.solver == AGD_FIM;
 
   for (int li = blockIdx.x; li < count; li += gridDim.x) {
-    const int n_o = a.list_in[li];
+    const int pk = pv.list_in[li];
     int xo[3];
-    tile_pos<D>(n_o, a.g, xo);
+    unpack_tile<D>(pk, xo);
+    const int n_o = tile_linear<D>(xo, a.g);
     int xt[3] = {0, 0, 0};
 #pragma unroll
     for (int k = 0; k < D; ++k) xt[k] = xo[k] * TL::T[k] + xi[k];
@@ -115,10 +106,10 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
 
     // ---- activation inputs first: the tile and its face neighbours
     // (lanes 0..2D of wave 0), and their FIM scores of the previous pass.
-    int nb = -1, sc = 0;
+    int nb = -1, nbp = pk, sc = 0;
     if (tid < 2 * D + 1) {
-      nb = (tid == 2 * D) ? n_o : tile_neighbor<D>(xo, tid >> 1, 2 * (tid & 1) - 1, a.g);
-      if (fim && nb >= 0) sc = read_score(a.score_prev, nb, a.iter - 1);
+      nb = (tid == 2 * D) ? n_o : tile_neighbor<D>(xo, tid >> 1, 2 * (tid & 1) - 1, a.g, &nbp);
+      if (fim && nb >= 0) sc = read_score(pv.score_prev, nb, pv.iter - 1);
     }
 
     // ---- cell loads (coalesced: a tile is contiguous)
@@ -133,7 +124,7 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
     }
     int wx = 255;
     if constexpr (WALLS) {
-      wx = a.wall[n_t];
+      if (a.wall) wx = a.wall[n_t];
       wall_s[tid] = uint8_t(wx);
     }
     u_s[tid] = seed ? rhs : u_old;
@@ -142,13 +133,13 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
     make_stencil<S, MODEL, D>(geo, a.iso_w, st);
     if constexpr (WALLS) __syncthreads();  // wall_s complete before the visibility walks
 
-    // ---- resolve the 2*nsym neighbours: LDS slot inside the tile, else one
-    // gathered value.  All gathers are issued before any is consumed (safe
-    // address = own cell when no load is needed), so a visit costs a single
-    // memory round trip.
-    int nidx[NJ];
-    int gidx[NJ];
-    bool gval[NJ];
+    // ---- resolve the 2*nsym neighbours into one code each:
+    //   code >= 0 : tiled index of a neighbour outside the tile (gathered),
+    //   code <= -2: LDS slot (-2 - code) inside the tile,
+    //   code == -1: no neighbour (outside the domain or hidden by a wall).
+    // All gathers are issued before any is consumed (safe address = own
+    // cell), so a visit costs a single memory round trip.
+    int code[NJ];
 #pragma unroll
     for (int k = 0; k < NS; ++k) {
 #pragma unroll
@@ -156,9 +147,31 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
         const int j = 2 * k + s;
         const int sg = 2 * s - 1;
         int e[3] = {sg * st.e[k][0], sg * st.e[k][1], sg * st.e[k][2]};
-        bool vis = true;
-        if constexpr (WALLS) {
-          if (wx != 255) {
+        if constexpr (!GENERAL) {
+          // No walls, no periodic axes: unsigned range checks, tile
+          // coordinates by shifts, tile-local test against the tile's own
+          // (uniform) coordinates.
+          int yt[3] = {xt[0] + e[0], xt[1] + e[1], xt[2] + e[2]};
+          bool dom = true;
+#pragma unroll
+          for (int c = 0; c < D; ++c) dom = dom && unsigned(yt[c]) < unsigned(a.g.shape_tot[c]);
+          int yo[3] = {0, 0, 0}, yi[3] = {0, 0, 0};
+          bool local = true;
+#pragma unroll
+          for (int c = 0; c < D; ++c) {
+            const unsigned u = unsigned(yt[c]);
+            yo[c] = int(u / unsigned(TL::T[c]));
+            yi[c] = int(u % unsigned(TL::T[c]));
+            local = local && yo[c] == xo[c];
+          }
+          const int li = TL::index(yi);
+          int to = yo[0];
+#pragma unroll
+          for (int c = 1; c < D; ++c) to = to * a.g.shape_o[c] + yo[c];
+          code[j] = !dom ? -1 : (local ? -2 - li : to * NI + li);
+        } else {
+          bool vis = true;
+          if (a.wall && wx != 255) {
             vis = visible<D>(e, wx, [&](const int* w) -> int {
               int yi[3] = {xi[0] + w[0], xi[1] + w[1], xi[2] + w[2]};
               int yt[3] = {xt[0] + w[0], xt[1] + w[1], xt[2] + w[2]};
@@ -171,22 +184,20 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
               return m >= 0 ? int(a.wall[m]) : 255;
             });
           }
-        }
-        int yi[3] = {xi[0] + e[0], xi[1] + e[1], xi[2] + e[2]};
-        int yt[3] = {xt[0] + e[0], xt[1] + e[1], xt[2] + e[2]};
-        bool in_dom = true;
+          int yi[3] = {xi[0] + e[0], xi[1] + e[1], xi[2] + e[2]};
+          int yt[3] = {xt[0] + e[0], xt[1] + e[1], xt[2] + e[2]};
+          bool in_dom = true;
 #pragma unroll
-        for (int c = 0; c < D; ++c) in_dom = in_dom && (yt[c] >= 0 && yt[c] < a.g.shape_tot[c]);
-        const bool local = vis && TL::inside(yi) && in_dom;
-        nidx[j] = local ? TL::index(yi) : -1;
-        const int m = (vis && !local) ? index_tot<D, TL>(yt, a.g) : -1;
-        gval[j] = m >= 0;
-        gidx[j] = m >= 0 ? m : n_t;
+          for (int c = 0; c < D; ++c) in_dom = in_dom && (yt[c] >= 0 && yt[c] < a.g.shape_tot[c]);
+          const bool local = vis && TL::inside(yi) && in_dom;
+          const int m = (vis && !local) ? index_tot<D, TL>(yt, a.g) : -1;
+          code[j] = local ? -2 - TL::index(yi) : m;
+        }
       }
     }
     S g[NJ];
 #pragma unroll
-    for (int j = 0; j < NJ; ++j) g[j] = a.u[gidx[j]];
+    for (int j = 0; j < NJ; ++j) g[j] = a.u[code[j] >= 0 ? code[j] : n_t];
     S vo[NJ];
 #pragma unroll
     for (int k = 0; k < NS; ++k) {
@@ -194,7 +205,7 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
       for (int s = 0; s < 2; ++s) {
         const int j = 2 * k + s;
         const S sh = (s == 0) ? st.shift[k] : -st.shift[k];
-        vo[j] = nidx[j] >= 0 ? sh : (gval[j] ? g[j] + sh : inf_v<S>());
+        vo[j] = code[j] <= -2 ? sh : (code[j] >= 0 ? g[j] + sh : inf_v<S>());
       }
     }
     __syncthreads();  // u_s complete
@@ -205,8 +216,8 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
       S v[NS], w[NS];
 #pragma unroll
       for (int k = 0; k < NS; ++k) {
-        const S a0 = nidx[2 * k] >= 0 ? u_s[nidx[2 * k]] + vo[2 * k] : vo[2 * k];
-        const S a1 = nidx[2 * k + 1] >= 0 ? u_s[nidx[2 * k + 1]] + vo[2 * k + 1] : vo[2 * k + 1];
+        const S a0 = code[2 * k] <= -2 ? u_s[-2 - code[2 * k]] + vo[2 * k] : vo[2 * k];
+        const S a1 = code[2 * k + 1] <= -2 ? u_s[-2 - code[2 * k + 1]] + vo[2 * k + 1] : vo[2 * k + 1];
         v[k] = (a0 > a1) ? a1 : a0;  // Greater(v,-) keeps the -e side on ties
         w[k] = st.w[k];
       }
@@ -230,7 +241,7 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
     // stop, every pass visits the full list.
     if (tid < 64) {
       if (!fim) {
-        if (any && nb >= 0) enqueue(nb, tag, a.stamp, a.list_out, a.count_out);
+        enqueue_wave(any && nb >= 0, nb, nbp, tag, a.w.stamp, pv.list_out, pv.count_out);
       } else {
         const int front = a.fim_front;
         const int prev = __shfl(sc, 2 * D);
@@ -247,11 +258,9 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
           if (front > 2) next = max(next, mx - 1);
           prop = next > 1;
         }
-        if (tid == 2 * D) {
-          a.score_next[n_o] = (a.iter << 8) | next;
-          if (any) enqueue(n_o, tag, a.stamp, a.list_out, a.count_out);
-        }
-        if (prop && tid < 2 * D && nb >= 0) enqueue(nb, tag, a.stamp, a.list_out, a.count_out);
+        if (tid == 2 * D) pv.score_next[n_o] = (pv.iter << 8) | next;
+        const bool want = (tid == 2 * D) ? any : (prop && tid < 2 * D && nb >= 0);
+        enqueue_wave(want, nb, nbp, tag, a.w.stamp, pv.list_out, pv.count_out);
       }
     }
     __syncthreads();  // LDS reuse by the next tile of this workgroup
@@ -259,11 +268,11 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
 }
 
 // Initial list: tiles holding at least one trigger bit (or every tile).
-template <int NI>
-__global__ void eik_init_list(const uint32_t* trigger, int32_t ntiles, int32_t all,
+template <int D, int NI>
+__global__ void eik_init_list(const uint32_t* trigger, GridDesc g, int32_t all,
                               int32_t* stamp, int32_t* list, int32_t* count) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= ntiles) return;
+  if (t >= g.ntiles) return;
   bool on = all != 0;
   if (!on) {
     const int64_t b0 = int64_t(t) * NI, b1 = b0 + NI;
@@ -278,13 +287,21 @@ __global__ void eik_init_list(const uint32_t* trigger, int32_t ntiles, int32_t a
       b = lim;
     }
   }
-  if (on) enqueue(t, 0, stamp, list, count);
+  int xo[3];
+  tile_pos<D>(t, g, xo);
+  if (on) enqueue(t, 0, stamp, list, count, pack_tile<D>(xo));
 }
 
-__global__ void iota_kernel(int32_t* list, int32_t n, int32_t* count) {
+// Every tile, as packed coordinates (global-iteration passes).
+template <int D>
+__global__ void all_tiles_kernel(GridDesc g, int32_t* list, int32_t* count) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < n) list[t] = t;
-  if (t == 0) *count = n;
+  if (t < g.ntiles) {
+    int xo[3];
+    tile_pos<D>(t, g, xo);
+    list[t] = pack_tile<D>(xo);
+  }
+  if (t == 0) *count = g.ntiles;
 }
 
 // ------------------------------------------------------------------ flow
@@ -492,7 +509,8 @@ template <class S> struct KernelSet {
   void (*update)(EikArgs<S>);
   void (*update_walls)(EikArgs<S>);
   void (*flow)(FlowArgs<S>);
-  void (*init)(const uint32_t*, int32_t, int32_t, int32_t*, int32_t*, int32_t*);
+  void (*init)(const uint32_t*, GridDesc, int32_t, int32_t*, int32_t*, int32_t*);
+  void (*all_tiles)(GridDesc, int32_t*, int32_t*);
   void (*wallpass)(GridDesc, const uint8_t*, uint8_t*);
   int ni;
 };
@@ -503,7 +521,8 @@ KernelSet<S> make_set() {
   k.update = eik_update<S, MODEL, D, T0, T1, T2, false>;
   k.update_walls = eik_update<S, MODEL, D, T0, T1, T2, true>;
   k.flow = eik_flow<S, MODEL, D, T0, T1, T2>;
-  k.init = eik_init_list<T0 * T1 * T2>;
+  k.init = eik_init_list<D, T0 * T1 * T2>;
+  k.all_tiles = all_tiles_kernel<D>;
   k.wallpass = wall_dist_pass<D, T0, T1, T2>;
   k.ni = T0 * T1 * T2;
   return k;
@@ -558,6 +577,9 @@ static int check_desc(const agd_eikonal_desc* d) {
   int64_t size_tot = 1;
   for (int k = 0; k < d->ndim; ++k) size_tot *= ceil_div(d->shape[k], d->shape_i[k]) * d->shape_i[k];
   AGD_REQUIRE(size_tot < (int64_t(1) << 31), "grids of 2^31 padded cells or more are not supported");
+  for (int k = 0; k < d->ndim; ++k)
+    AGD_REQUIRE(ceil_div(d->shape[k], d->shape_i[k]) <= (d->ndim == 2 ? 32768 : 1024),
+                "too many tiles along an axis (2D: 32768, 3D: 1024)");
   return AGD_OK;
 }
 
@@ -593,6 +615,7 @@ int get_workspace(int64_t ntiles, int64_t nhist, Workspace*& out) {
     AGD_HIP_TRY(hipEventCreate(&w.ev0));
     AGD_HIP_TRY(hipEventCreate(&w.ev1));
     AGD_HIP_TRY(hipHostMalloc((void**)&w.host_count, 64, hipHostMallocDefault));
+    AGD_HIP_TRY(hipStreamCreateWithFlags(&w.capture_stream, hipStreamNonBlocking));
     AGD_HIP_TRY(hipMalloc((void**)&w.counts, 8 * sizeof(int32_t)));
   }
   if (w.cap < ntiles) {
@@ -660,41 +683,36 @@ static int solve_impl(const agd_eikonal_desc* d, const agd_eikonal_io* io,
   a.solver = global ? AGD_AGSI : d->solver;
   a.fim_front = d->fim_front_width;
 
-  Workspace* wsp = nullptr;
   auto init = [&](Workspace* ws, int32_t* list0, int32_t* counts) -> int {
-    wsp = ws;
-    a.stamp = ws->stamp;
-    a.hist = ws->hist;
+    a.w.lists = ws->lists;
+    a.w.counts = counts;
+    a.w.stamp = ws->stamp;
+    a.w.hist = ws->hist;
+    a.w.score = ws->score;
+    a.w.cap = int32_t(ws->cap);
+    a.w.fixed = nullptr;
     AGD_HIP_TRY(hipMemsetAsync(ws->score, 0x7f, size_t(2 * ws->cap) * sizeof(int32_t), stream));
     if (global) {
       // Every pass visits the full list (kept in the score buffer, unused
       // by this solver); the AGSI tags only decide when to stop.
-      hipLaunchKernelGGL(iota_kernel, dim3(unsigned(ceil_div(g.ntiles, 256))), dim3(256), 0,
-                         stream, ws->score, g.ntiles, counts + 7);
-      hipLaunchKernelGGL(iota_kernel, dim3(unsigned(ceil_div(g.ntiles, 256))), dim3(256), 0,
-                         stream, list0, g.ntiles, counts + 0);
+      hipLaunchKernelGGL(ks.all_tiles, dim3(unsigned(ceil_div(g.ntiles, 256))), dim3(256), 0,
+                         stream, g, ws->score, counts + 3);
+      a.w.fixed = ws->score;
     } else {
       const uint32_t* trig = io->trigger ? io->trigger : io->seeds;
       hipLaunchKernelGGL(ks.init, dim3(unsigned(ceil_div(g.ntiles, 256))), dim3(256), 0, stream,
-                         trig, g.ntiles, d->activate_all, ws->stamp, list0, counts + 0);
+                         trig, g, d->activate_all, ws->stamp, list0, counts + 0);
     }
     return AGD_OK;
   };
-  auto launch = [&](const PassPointers& p) {
-    a.list_in = global ? wsp->score : p.list_in;
-    a.count_in = global ? wsp->counts + 7 : p.count_in;
-    a.list_out = p.list_out;
-    a.count_out = p.count_out;
-    a.count_clear = p.count_clear;
-    a.iter = p.iter;
-    a.score_prev = wsp->score + ((p.iter + 1) % 2) * wsp->cap;
-    a.score_next = wsp->score + (p.iter % 2) * wsp->cap;
-    hipLaunchKernelGGL(io->wall_dist ? ks.update_walls : ks.update, dim3(blocks), dim3(ni), 0,
-                       stream, a);
+  auto launch = [&](int32_t koff, hipStream_t s) {
+    a.w.koff = koff;
+    hipLaunchKernelGGL((io->wall_dist || g.periodic) ? ks.update_walls : ks.update, dim3(blocks),
+                       dim3(ni), 0, s, a);
   };
   WorklistResult r;
   int st = run_worklist(g.ntiles, d->nitermax_o, d->check_interval, stream, init, launch, r,
-                        d->time_launches != 0);
+                        d->time_launches != 0, d->use_graph != 0);
   if (st) return st;
   if (stats) {
     stats->kernel_ms = r.kernel_ms;

@@ -29,29 +29,91 @@ struct Workspace {
   int32_t* host_count = nullptr;  // pinned
   int64_t cap = 0, hist_cap = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // Private stream used only to record pass chunks into a graph (the legacy
+  // null stream, torch's default, cannot be captured).  Nothing executes on
+  // it: the instantiated graph is launched on the caller's stream.
+  hipStream_t capture_stream = nullptr;
 };
 
 // Per-device cached workspace (grown on demand).  Defined in eikonal.hip.
 int get_workspace(int64_t ntiles, int64_t nhist, Workspace*& out);
 int blocks_for(int ntiles, int lanes_per_block);
 
+// Per-lane append of `entry` (default: the tile index) if `tile` was not yet
+// stamped with `tag`.
 __device__ inline void enqueue(int32_t tile, int32_t tag, int32_t* stamp, int32_t* list,
-                               int32_t* count) {
+                               int32_t* count, int32_t entry = -1) {
   const int32_t old = atomicMax(stamp + tile, tag);
   if (old < tag) {
     const int32_t pos = atomicAdd(count, 1);
-    list[pos] = tile;
+    list[pos] = entry >= 0 ? entry : tile;
   }
 }
 
-struct PassPointers {
+// Wave-aggregated append: every lane of the calling wave must execute this
+// (lanes with want == false contribute nothing).  Lanes whose tile was not yet
+// stamped for this pass append `entry`; one atomicAdd on the counter per wave.
+__device__ inline void enqueue_wave(bool want, int32_t tile, int32_t entry, int32_t tag,
+                                    int32_t* stamp, int32_t* list, int32_t* count) {
+  const bool first = want && atomicMax(stamp + tile, tag) < tag;
+  const uint64_t mask = __ballot(first);
+  if (mask == 0) return;
+  const int lane = __lane_id();
+  const int leader = __ffsll((unsigned long long)mask) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(count, __popcll(mask));
+  base = __shfl(base, leader);
+  if (first) list[base + __popcll(mask & ((uint64_t(1) << lane) - 1))] = entry;
+}
+
+// Worklist state seen by a pass kernel.  counts[0..2] are the rotating list
+// lengths, counts[3] the length of a fixed list (global iteration),
+// counts[4] the pass base.  A kernel runs pass iter = counts[4] + koff, so
+// a chunk of passes with koff = 0..G-1 followed by an advance of counts[4]
+// by G can be captured once as a graph and replayed.
+struct WorkCtx {
+  int32_t* lists;          // 2 * cap entries
+  int32_t* counts;         // 8
+  int32_t* stamp;          // ntiles
+  int32_t* hist;           // pass histogram
+  int32_t* score;          // 2 * cap (FIM scores)
+  const int32_t* fixed;    // non-null: every pass reads this list (length counts[3])
+  int32_t cap;
+  int32_t koff;
+};
+
+struct PassView {
+  int32_t iter, count;
   const int32_t* list_in;
-  const int32_t* count_in;
   int32_t* list_out;
   int32_t* count_out;
-  int32_t* count_clear;
-  int32_t iter;
+  const int32_t* score_prev;
+  int32_t* score_next;
 };
+
+// One scalar load brings the counters and the pass base together.  Block 0
+// zeroes the counter of pass iter+2 and records the list length.
+__device__ inline PassView pass_begin(const WorkCtx& c) {
+  int4 q0 = *reinterpret_cast<const int4*>(c.counts);
+  int4 q1 = *reinterpret_cast<const int4*>(c.counts + 4);
+  const int cnt[5] = {q0.x, q0.y, q0.z, q0.w, q1.x};
+  PassView v;
+  v.iter = cnt[4] + c.koff;
+  const int r = v.iter % 3;
+  v.count = c.fixed ? cnt[3] : (r == 0 ? cnt[0] : (r == 1 ? cnt[1] : cnt[2]));
+  v.list_in = c.fixed ? c.fixed : c.lists + (v.iter & 1) * c.cap;
+  v.list_out = c.lists + ((v.iter + 1) & 1) * c.cap;
+  v.count_out = c.counts + (v.iter + 1) % 3;
+  v.score_prev = c.score + ((v.iter + 1) & 1) * c.cap;
+  v.score_next = c.score + (v.iter & 1) * c.cap;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    c.counts[(v.iter + 2) % 3] = 0;
+    c.hist[v.iter] = v.count;
+  }
+  return v;
+}
+
+__global__ inline void advance_kernel(int32_t* counts, int32_t g) { counts[4] += g; }
 
 struct WorklistResult {
   int64_t passes = 0;      // launches issued
@@ -63,31 +125,43 @@ struct WorklistResult {
 };
 
 // Runs passes until the list empties or nitermax passes were issued.
-// `init(ws)` enqueues the initial list (stamp 0) into list0/counts[0];
-// `launch(PassPointers)` enqueues one pass.  Stamps, counters and the pass
-// histogram are reset here.
+// `init(ws, list0, counts)` enqueues the initial list into list0 /
+// counts[0]; `launch(koff, stream)` enqueues one pass of iteration
+// counts[4] + koff on `stream`.
+// With `graph`, chunks of `check_interval` passes are captured once into a
+// hipGraph and replayed; the host reads the next list length after each
+// chunk.  Optional per-pass timing records an event pair around every pass
+// (and then launches eagerly).
 template <class Init, class Launch>
 int run_worklist(int32_t ntiles, int64_t nitermax, int check_interval, hipStream_t stream,
-                 Init init, Launch launch, WorklistResult& res, bool time_launches = false) {
+                 Init init, Launch launch, WorklistResult& res, bool time_launches = false,
+                 bool graph = false) {
   Workspace* ws = nullptr;
-  int st = get_workspace(ntiles, nitermax + 8, ws);
+  const int ci = std::max(1, check_interval);
+  int st = get_workspace(ntiles, nitermax + ci + 8, ws);
   if (st) return st;
   int32_t* list0 = ws->lists;
-  int32_t* list1 = ws->lists + ws->cap;
   int32_t* counts = ws->counts;
   AGD_HIP_TRY(hipMemsetAsync(ws->stamp, 0xff, size_t(ntiles) * sizeof(int32_t), stream));
   AGD_HIP_TRY(hipMemsetAsync(counts, 0, 8 * sizeof(int32_t), stream));
-  AGD_HIP_TRY(hipMemsetAsync(ws->hist, 0, size_t(nitermax + 8) * sizeof(int32_t), stream));
+  AGD_HIP_TRY(hipMemsetAsync(ws->hist, 0, size_t(nitermax + ci + 8) * sizeof(int32_t), stream));
   st = init(ws, list0, counts);
   if (st) return st;
   AGD_HIP_TRY(hipGetLastError());
-  AGD_HIP_TRY(hipEventRecord(ws->ev0, stream));
-  const int ci = std::max(1, check_interval);
-  // Optional per-launch timing: one event pair per pass, read back at every
-  // convergence check (the stream is synchronised there anyway).
-  std::vector<hipEvent_t> evs;
+
+  // Events and the graph are released on every return path.
+  struct Owned {
+    std::vector<hipEvent_t> evs;
+    hipGraphExec_t exec = nullptr;
+    ~Owned() {
+      for (auto e : evs)
+        if (e) hipEventDestroy(e);
+      if (exec) hipGraphExecDestroy(exec);
+    }
+  } own;
+  std::vector<hipEvent_t>& evs = own.evs;
   if (time_launches) {
-    evs.resize(2 * size_t(ci));
+    evs.assign(2 * size_t(ci), nullptr);
     for (auto& e : evs) AGD_HIP_TRY(hipEventCreate(&e));
   }
   res.kernel_ms = 0.;
@@ -101,22 +175,46 @@ int run_worklist(int32_t ntiles, int64_t nitermax, int check_interval, hipStream
     pending = 0;
     return AGD_OK;
   };
+
+  // Per-launch timing needs event pairs around every launch, which a graph
+  // does not record: timed runs launch eagerly.
+  graph = graph && !time_launches;
+  hipGraphExec_t& exec = own.exec;
+  if (graph) {
+    hipGraph_t g = nullptr;
+    hipStream_t cs = ws->capture_stream;
+    AGD_HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < ci; ++i) launch(i, cs);
+    hipLaunchKernelGGL(advance_kernel, dim3(1), dim3(1), 0, cs, counts, ci);
+    // End the capture whatever happened inside it, so the stream is usable again.
+    const hipError_t cap_err = hipGetLastError();
+    const hipError_t end_err = hipStreamEndCapture(cs, &g);
+    if (cap_err != hipSuccess || end_err != hipSuccess) {
+      if (g) hipGraphDestroy(g);
+      AGD_HIP_TRY(cap_err);
+      AGD_HIP_TRY(end_err);
+    }
+    const hipError_t inst_err = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    AGD_HIP_TRY(inst_err);
+  }
+
+  AGD_HIP_TRY(hipEventRecord(ws->ev0, stream));
   int64_t k = 0;
   bool converged = false;
   while (k < nitermax) {
-    PassPointers p;
-    p.list_in = (k % 2 == 0) ? list0 : list1;
-    p.list_out = (k % 2 == 0) ? list1 : list0;
-    p.count_in = counts + (k % 3);
-    p.count_out = counts + ((k + 1) % 3);
-    p.count_clear = counts + ((k + 2) % 3);
-    p.iter = int32_t(k);
-    if (time_launches) AGD_HIP_TRY(hipEventRecord(evs[2 * pending], stream));
-    launch(p);
-    if (time_launches) AGD_HIP_TRY(hipEventRecord(evs[2 * pending + 1], stream));
-    ++pending;
-    ++k;
-    if (k % ci == 0 || k == nitermax) {
+    if (graph) {
+      AGD_HIP_TRY(hipGraphLaunch(exec, stream));
+      pending = ci;
+      k += ci;
+    } else {
+      if (time_launches) AGD_HIP_TRY(hipEventRecord(evs[2 * pending], stream));
+      launch(int32_t(k), stream);
+      if (time_launches) AGD_HIP_TRY(hipEventRecord(evs[2 * pending + 1], stream));
+      ++pending;
+      ++k;
+    }
+    if (graph || k % ci == 0 || k >= nitermax) {
       AGD_HIP_TRY(hipMemcpyAsync(ws->host_count, counts + (k % 3), sizeof(int32_t),
                                  hipMemcpyDeviceToHost, stream));
       AGD_HIP_TRY(hipStreamSynchronize(stream));
@@ -138,7 +236,6 @@ int run_worklist(int32_t ntiles, int64_t nitermax, int check_interval, hipStream
   if (time_launches) {
     const int st2 = drain();
     if (st2) return st2;
-    for (auto& e : evs) AGD_HIP_TRY(hipEventDestroy(e));
   }
   std::vector<int32_t> hist(k);
   if (k > 0)

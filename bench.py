@@ -97,6 +97,8 @@ def main():
     ap.add_argument('--config', default='riemann3', choices=['riemann3', 'riemann2', 'isotropic2'])
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--no-fp64', action='store_true')
+    ap.add_argument('--no-graph', action='store_true',
+                    help='launch every pass eagerly instead of replaying captured pass chunks')
     ap.add_argument('--copy-calibration', type=int, default=0, metavar='MIB',
                     help='only run a plain device copy of MIB MiB three times (reference for '
                          'the FETCH_SIZE / WRITE_SIZE counters) and exit')
@@ -125,7 +127,8 @@ def main():
     log(f"workload {wl.name} {wl.dims} generated in {time.time() - t0:.1f} s")
     iface = prepare(wl)
     desc = iface._desc()
-    desc.time_launches = 1
+    desc.use_graph = 0 if args.no_graph else 1
+    desc.time_launches = 0
     log(f"tiles {iface.shape_i} x {iface.ntiles}, niter_i {iface.niter_i}, solver {iface.solver}, "
         f"atol {iface.atol:.3g} rtol {iface.rtol:.3g}")
 
@@ -141,13 +144,18 @@ def main():
     wall = time.perf_counter() - t0
     for i, st in enumerate(stats):
         log(f"step {i}: {st.gpu_ms:.2f} ms, niter_o {st.niter_o}, passes {st.launches}, "
-            f"cell-updates {st.cell_updates}, kernel {st.kernel_ms:.2f} ms")
-
+            f"cell-updates {st.cell_updates}")
     cell_updates = sum(st.cell_updates for st in stats)
-    kernel_ms = sum(st.kernel_ms for st in stats)
     launches = sum(st.launches for st in stats)
+
+    # Roofline of the update kernel: one more solve, outside the timed region,
+    # with an event pair around every pass launch (launched eagerly).
+    desc.time_launches = 1
+    st_t = solve_once(iface, desc)
+    log(f"timed-launch solve: {st_t.gpu_ms:.2f} ms, passes {st_t.launches}, cell-updates "
+        f"{st_t.cell_updates}, sum of launch durations {st_t.kernel_ms:.2f} ms")
     bpc = BYTES_PER_CELL_UPDATE[wl.model]
-    achieved = cell_updates * bpc / (kernel_ms * 1e-3) / 1e9
+    achieved = st_t.cell_updates * bpc / (st_t.kernel_ms * 1e-3) / 1e9
     u32 = iface.values.float().cpu().numpy()
     assert np.isfinite(u32).all(), "non-finite values in the benchmark solve"
 
@@ -180,7 +188,7 @@ def main():
             "workload": f"{wl.name} {'x'.join(map(str, wl.dims))}, one seed, cold start to convergence",
             "model": wl.model, "grid": list(wl.dims), "tile": list(iface.shape_i),
             "niter_i": iface.niter_i, "solver": iface.solver, "atol": iface.atol, "rtol": iface.rtol,
-            "parallelism": "single GPU",
+            "parallelism": "single GPU", "pass_launch": "eager" if args.no_graph else "hipGraph",
         },
         "solved_cells_per_s": wl.ncells * args.steps / wall,
         "cell_updates_per_step": cell_updates / args.steps,
@@ -190,7 +198,7 @@ def main():
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": None,
             "kernel": "eik_update", "bytes_per_cell_update": bpc,
-            "avg_launch_us": kernel_ms / launches * 1e3,
+            "avg_launch_us": st_t.kernel_ms / st_t.launches * 1e3,
         },
         "cpu_baseline": None,
     }
@@ -201,7 +209,7 @@ def main():
         t = json.load(open(tpath))
         result["roofline"]["traffic"] = t["hbm_bytes_per_solve"] / t["dispatches_per_solve"]
         result["roofline"]["traffic_source"] = os.path.relpath(tpath, ROOT)
-        result["roofline"]["algorithmic_bytes_per_dispatch"] = cell_updates * bpc / launches
+        result["roofline"]["algorithmic_bytes_per_dispatch"] = st_t.cell_updates * bpc / st_t.launches
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         result["cpu_baseline"] = cpu_baseline(args.config)

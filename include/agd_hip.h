@@ -71,7 +71,7 @@ typedef struct agd_eikonal_desc {
   int32_t check_interval; /* passes between host convergence checks (>=1) */
   int32_t activate_all;   /* 1: start with every tile active (warm start) */
   int32_t time_launches;  /* 1: time every pass with an event pair (kernel_ms) */
-  int32_t reserved;
+  int32_t use_graph;      /* 1: capture check_interval passes once as a hipGraph and replay it */
 } agd_eikonal_desc;
 
 /* Device pointers, tiled layout.  Scalars are float (AGD_F32) or double. */

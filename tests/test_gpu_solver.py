@@ -140,3 +140,16 @@ def test_warm_start_supersolution_same_fixed_point(O):
     ref = P.solve_fm()
     assert rel_err(u_warm, ref) <= TOL32
     assert rel_err(u_cold, ref) <= TOL32
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_graph_replay_matches_eager_launches(O, graph):
+    """Pass chunks replayed from a captured hipGraph and eager launches reach
+    the same fixed point (several replays per solve, two solves in a row)."""
+    wl = workloads.riemann2_aniso(n=100, seed=17, sigma=4., amax=6.)
+    P, _ = oracle_problem(O, wl)
+    ref = P.solve_fm()
+    for _ in range(2):
+        u, out = run_gpu(wl, hipGraph=graph, check_interval=3)
+        assert out['stats']['eikonal']['launches'] >= 6
+        assert rel_err(u, ref) <= TOL32
