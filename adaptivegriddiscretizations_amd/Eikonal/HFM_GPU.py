@@ -393,6 +393,23 @@ class Interface:
                                                   ptr(self.rhs_t), ptr(self.seeds_t),
                                                   ptr(self.wallDist_t), self.stream))
 
+        # Precomputed scheme (_SetGeometry.py:198-207): stencil weights and
+        # resolved neighbour indices stored once, instead of a Selling
+        # decomposition at every tile visit.  The reference stores them only
+        # when the geometry is shared along some axes; here HBM is plentiful
+        # and the default is on wherever a precomputed variant exists.
+        self.scheme_w_t = self.scheme_c_t = None
+        self.precompute_scheme = False
+        if _lib.lib.agd_scheme_supported(self.modelId, self.ndim, _lib.i32(self.shape_i)):
+            self.precompute_scheme = bool(self.GetValue(
+                'precompute_scheme', default=True,
+                help="Precompute and store the finite difference scheme stencils"))
+        if self.precompute_scheme:
+            nsym = self.ndim * (self.ndim + 1) // 2
+            self.scheme_w_t = empty((nsym, self.size_tot), self.float_t)
+            self.scheme_c_t = empty((2 * nsym, self.size_tot), np.int32)
+            self.compute_scheme()
+
         self.nitermax_o = int(self.GetValue('nitermax_o', default=2000,
                                             help="Maximum number of iterations of the solver"))
         self.raiseOnNonConvergence = self.GetValue('raiseOnNonConvergence', default=True,
@@ -425,6 +442,10 @@ class Interface:
         d.use_graph = int(getattr(self, 'use_graph', True))
         return d
 
+    def compute_scheme(self):
+        """Fill the precomputed scheme from the tiled geometry and walls."""
+        _lib.check(_lib.lib.agd_eikonal_scheme(self._desc(), self._io(), self.stream))
+
     def _io(self):
         io = _lib.EikonalIO()
         io.values = ptr(self.values_t)
@@ -433,6 +454,8 @@ class Interface:
         io.seeds = ptr(self.seeds_t)
         io.trigger = ptr(self.trigger_t)
         io.wall_dist = ptr(self.wallDist_t) if self.wallDist_t is not None else None
+        io.scheme_weights = ptr(getattr(self, 'scheme_w_t', None))
+        io.scheme_codes = ptr(getattr(self, 'scheme_c_t', None))
         return io
 
     # ------------------------------------------------------------ solve

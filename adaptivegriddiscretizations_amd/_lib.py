@@ -40,6 +40,7 @@ class EikonalIO(ctypes.Structure):
     _fields_ = [
         ("values", c_void_p), ("geom", c_void_p), ("rhs", c_void_p),
         ("seeds", c_void_p), ("trigger", c_void_p), ("wall_dist", c_void_p),
+        ("scheme_weights", c_void_p), ("scheme_codes", c_void_p),
     ]
 
 
@@ -71,6 +72,8 @@ SIGNATURES = {
                                     POINTER(EikonalStats), c_void_p]),
     "agd_eikonal_flow": (c_int32, [POINTER(EikonalDesc), POINTER(EikonalIO), POINTER(FlowIO),
                                    c_void_p]),
+    "agd_scheme_supported": (c_int32, [c_int32, c_int32, c_int32_p]),
+    "agd_eikonal_scheme": (c_int32, [POINTER(EikonalDesc), POINTER(EikonalIO), c_void_p]),
     "agd_eikonal_adjoint": (c_int32, [POINTER(EikonalDesc), c_void_p, c_void_p, c_void_p,
                                       c_void_p, c_int32, c_void_p, c_void_p, c_double,
                                       c_double, POINTER(EikonalStats), c_void_p]),

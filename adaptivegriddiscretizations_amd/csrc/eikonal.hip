@@ -42,6 +42,8 @@ namespace agd {
 template <class S> struct EikArgs {
   S* u;
   const S* geom;
+  const S* sw;          // precomputed stencil weights (NS components) or null
+  const int32_t* scode; // precomputed neighbour codes (2*NS components) or null
   const S* rhs;
   const uint32_t* seeds;
   const uint8_t* wall;
@@ -72,10 +74,82 @@ template <int NI> __device__ inline bool block_any(bool p) {
   }
 }
 
-// GENERAL = walls and/or periodic axes; the fast variant assumes neither, so
-// its neighbour indexing carries no wrap-around code.
-template <class S, int MODEL, int D, int T0, int T1, int T2, bool GENERAL>
+// Resolves the 2*nsym neighbours x +- e_k of a cell into one code each:
+//   code >= 0 : tiled index of a neighbour outside the tile (gathered),
+//   code <= -2: LDS slot (-2 - code) inside the tile,
+//   code == -1: no neighbour (outside the domain or hidden by a wall).
+template <class S, int MODEL, int D, class TL, bool GENERAL>
+__device__ inline void resolve_codes(const Stencil<S, MODEL, D>& st, const int* xi, const int* xt,
+                                     const int* xo, int wx, const uint8_t* wall_s,
+                                     const uint8_t* wall, const GridDesc& g, int* code) {
+  constexpr int NS = ModelTraits<MODEL, D>::NSYM;
+  constexpr int NI = TL::NI;
+#pragma unroll
+  for (int k = 0; k < NS; ++k) {
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const int j = 2 * k + s;
+      const int sg = 2 * s - 1;
+      int e[3] = {sg * st.e[k][0], sg * st.e[k][1], sg * st.e[k][2]};
+      if constexpr (!GENERAL) {
+        // No walls, no periodic axes: unsigned range checks, tile
+        // coordinates by shifts, tile-local test against the tile's own
+        // (uniform) coordinates.
+        int yt[3] = {xt[0] + e[0], xt[1] + e[1], xt[2] + e[2]};
+        bool dom = true;
+#pragma unroll
+        for (int c = 0; c < D; ++c) dom = dom && unsigned(yt[c]) < unsigned(g.shape_tot[c]);
+        int yo[3] = {0, 0, 0}, yi[3] = {0, 0, 0};
+        bool local = true;
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+          const unsigned u = unsigned(yt[c]);
+          yo[c] = int(u / unsigned(TL::T[c]));
+          yi[c] = int(u % unsigned(TL::T[c]));
+          local = local && yo[c] == xo[c];
+        }
+        const int li = TL::index(yi);
+        int to = yo[0];
+#pragma unroll
+        for (int c = 1; c < D; ++c) to = to * g.shape_o[c] + yo[c];
+        code[j] = !dom ? -1 : (local ? -2 - li : to * NI + li);
+      } else {
+        bool vis = true;
+        if (wall && wx != 255) {
+          vis = visible<D>(e, wx, [&](const int* w) -> int {
+            int yi[3] = {xi[0] + w[0], xi[1] + w[1], xi[2] + w[2]};
+            int yt[3] = {xt[0] + w[0], xt[1] + w[1], xt[2] + w[2]};
+            bool dom = true;
+#pragma unroll
+            for (int c = 0; c < D; ++c)
+              dom = dom && (((g.periodic >> c) & 1) || (yt[c] >= 0 && yt[c] < g.shape_tot[c]));
+            if (TL::inside(yi) && dom) return wall_s[TL::index(yi)];
+            const int m = index_tot<D, TL>(yt, g);
+            return m >= 0 ? int(wall[m]) : 255;
+          });
+        }
+        int yi[3] = {xi[0] + e[0], xi[1] + e[1], xi[2] + e[2]};
+        int yt[3] = {xt[0] + e[0], xt[1] + e[1], xt[2] + e[2]};
+        bool in_dom = true;
+#pragma unroll
+        for (int c = 0; c < D; ++c) in_dom = in_dom && (yt[c] >= 0 && yt[c] < g.shape_tot[c]);
+        const bool local = vis && TL::inside(yi) && in_dom;
+        const int m = (vis && !local) ? index_tot<D, TL>(yt, g) : -1;
+        code[j] = local ? -2 - TL::index(yi) : m;
+      }
+    }
+  }
+}
+
+// Kernel variants: VAR_FAST assumes no walls and no periodic axes, so its
+// neighbour indexing carries no wrap-around code; VAR_GENERAL handles both;
+// VAR_PRE reads stencil weights and neighbour codes precomputed once per
+// solve (any walls / periodicity are already resolved in the codes).
+enum { VAR_FAST = 0, VAR_GENERAL = 1, VAR_PRE = 2 };
+
+template <class S, int MODEL, int D, int T0, int T1, int T2, int VAR>
 __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
+  constexpr bool GENERAL = VAR == VAR_GENERAL;
   constexpr bool WALLS = GENERAL;
   using TL = Tile<D, T0, T1, T2>;
   using MT = ModelTraits<MODEL, D>;
@@ -116,12 +190,6 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
     const S u_old = a.u[n_t];
     const S rhs = a.rhs[n_t];
     const bool seed = (a.seeds[n_t >> 5] >> (n_t & 31)) & 1u;
-    S geo[NG > 0 ? NG : 1];
-#pragma unroll
-    for (int k = 0; k < NG; ++k) {
-      const S* __restrict__ gk = a.geom + k * a.g.size_tot;  // uniform base
-      geo[k] = gk[n_t];
-    }
     int wx = 255;
     if constexpr (WALLS) {
       if (a.wall) wx = a.wall[n_t];
@@ -129,72 +197,36 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
     }
     u_s[tid] = seed ? rhs : u_old;
 
-    Stencil<S, MODEL, D> st;
-    make_stencil<S, MODEL, D>(geo, a.iso_w, st);
-    if constexpr (WALLS) __syncthreads();  // wall_s complete before the visibility walks
-
-    // ---- resolve the 2*nsym neighbours into one code each:
-    //   code >= 0 : tiled index of a neighbour outside the tile (gathered),
-    //   code <= -2: LDS slot (-2 - code) inside the tile,
-    //   code == -1: no neighbour (outside the domain or hidden by a wall).
-    // All gathers are issued before any is consumed (safe address = own
-    // cell), so a visit costs a single memory round trip.
     int code[NJ];
+    S sw[NS], shift[NS];
+    if constexpr (VAR == VAR_PRE) {
+      // Stencil weights and neighbour codes precomputed for the whole solve.
 #pragma unroll
-    for (int k = 0; k < NS; ++k) {
+      for (int k = 0; k < NS; ++k) {
+        sw[k] = a.sw[k * a.g.size_tot + n_t];
+        shift[k] = S(0);
+      }
 #pragma unroll
-      for (int s = 0; s < 2; ++s) {
-        const int j = 2 * k + s;
-        const int sg = 2 * s - 1;
-        int e[3] = {sg * st.e[k][0], sg * st.e[k][1], sg * st.e[k][2]};
-        if constexpr (!GENERAL) {
-          // No walls, no periodic axes: unsigned range checks, tile
-          // coordinates by shifts, tile-local test against the tile's own
-          // (uniform) coordinates.
-          int yt[3] = {xt[0] + e[0], xt[1] + e[1], xt[2] + e[2]};
-          bool dom = true;
+      for (int j = 0; j < NJ; ++j) code[j] = a.scode[j * a.g.size_tot + n_t];
+    } else {
+      S geo[NG > 0 ? NG : 1];
 #pragma unroll
-          for (int c = 0; c < D; ++c) dom = dom && unsigned(yt[c]) < unsigned(a.g.shape_tot[c]);
-          int yo[3] = {0, 0, 0}, yi[3] = {0, 0, 0};
-          bool local = true;
+      for (int k = 0; k < NG; ++k) {
+        const S* __restrict__ gk = a.geom + k * a.g.size_tot;  // uniform base
+        geo[k] = gk[n_t];
+      }
+      Stencil<S, MODEL, D> st;
+      make_stencil<S, MODEL, D>(geo, a.iso_w, st);
+      if constexpr (WALLS) __syncthreads();  // wall_s complete before the visibility walks
+      resolve_codes<S, MODEL, D, TL, GENERAL>(st, xi, xt, xo, wx, wall_s, a.wall, a.g, code);
 #pragma unroll
-          for (int c = 0; c < D; ++c) {
-            const unsigned u = unsigned(yt[c]);
-            yo[c] = int(u / unsigned(TL::T[c]));
-            yi[c] = int(u % unsigned(TL::T[c]));
-            local = local && yo[c] == xo[c];
-          }
-          const int li = TL::index(yi);
-          int to = yo[0];
-#pragma unroll
-          for (int c = 1; c < D; ++c) to = to * a.g.shape_o[c] + yo[c];
-          code[j] = !dom ? -1 : (local ? -2 - li : to * NI + li);
-        } else {
-          bool vis = true;
-          if (a.wall && wx != 255) {
-            vis = visible<D>(e, wx, [&](const int* w) -> int {
-              int yi[3] = {xi[0] + w[0], xi[1] + w[1], xi[2] + w[2]};
-              int yt[3] = {xt[0] + w[0], xt[1] + w[1], xt[2] + w[2]};
-              bool dom = true;
-#pragma unroll
-              for (int c = 0; c < D; ++c)
-                dom = dom && (((a.g.periodic >> c) & 1) || (yt[c] >= 0 && yt[c] < a.g.shape_tot[c]));
-              if (TL::inside(yi) && dom) return wall_s[TL::index(yi)];
-              const int m = index_tot<D, TL>(yt, a.g);
-              return m >= 0 ? int(a.wall[m]) : 255;
-            });
-          }
-          int yi[3] = {xi[0] + e[0], xi[1] + e[1], xi[2] + e[2]};
-          int yt[3] = {xt[0] + e[0], xt[1] + e[1], xt[2] + e[2]};
-          bool in_dom = true;
-#pragma unroll
-          for (int c = 0; c < D; ++c) in_dom = in_dom && (yt[c] >= 0 && yt[c] < a.g.shape_tot[c]);
-          const bool local = vis && TL::inside(yi) && in_dom;
-          const int m = (vis && !local) ? index_tot<D, TL>(yt, a.g) : -1;
-          code[j] = local ? -2 - TL::index(yi) : m;
-        }
+      for (int k = 0; k < NS; ++k) {
+        sw[k] = st.w[k];
+        shift[k] = st.shift[k];
       }
     }
+    // All gathers are issued before any is consumed (safe address = own
+    // cell), so a visit costs a single memory round trip.
     S g[NJ];
 #pragma unroll
     for (int j = 0; j < NJ; ++j) g[j] = a.u[code[j] >= 0 ? code[j] : n_t];
@@ -204,7 +236,7 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
 #pragma unroll
       for (int s = 0; s < 2; ++s) {
         const int j = 2 * k + s;
-        const S sh = (s == 0) ? st.shift[k] : -st.shift[k];
+        const S sh = (s == 0) ? shift[k] : -shift[k];
         vo[j] = code[j] <= -2 ? sh : (code[j] >= 0 ? g[j] + sh : inf_v<S>());
       }
     }
@@ -219,7 +251,7 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
         const S a0 = code[2 * k] <= -2 ? u_s[-2 - code[2 * k]] + vo[2 * k] : vo[2 * k];
         const S a1 = code[2 * k + 1] <= -2 ? u_s[-2 - code[2 * k + 1]] + vo[2 * k + 1] : vo[2 * k + 1];
         v[k] = (a0 > a1) ? a1 : a0;  // Greater(v,-) keeps the -e side on ties
-        w[k] = st.w[k];
+        w[k] = sw[k];
       }
       const S un = hfm_solve<S, NS>(v, w, rhs);
       __syncthreads();
@@ -265,6 +297,56 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
     }
     __syncthreads();  // LDS reuse by the next tile of this workgroup
   }
+}
+
+// ------------------------------------------------------------------ scheme
+// Stencil weights and neighbour codes of every cell, computed once per solve
+// for the VAR_PRE update (the counterpart of the reference's
+// precompute_scheme export kernel, _Kernel.py:254-268 and 346-359, with
+// offsets stored already resolved against the tiled grid).
+template <class S> struct SchemeArgs {
+  const S* geom;
+  const uint8_t* wall;
+  GridDesc g;
+  S iso_w[3];
+  S* sw;
+  int32_t* scode;
+};
+
+template <class S, int MODEL, int D, int T0, int T1, int T2, bool GENERAL>
+__global__ void __launch_bounds__(T0* T1* T2) eik_scheme(SchemeArgs<S> a) {
+  using TL = Tile<D, T0, T1, T2>;
+  using MT = ModelTraits<MODEL, D>;
+  constexpr int NI = TL::NI;
+  constexpr int NS = MT::NSYM;
+  constexpr int NG = MT::NGEOM;
+  __shared__ uint8_t wall_s[GENERAL ? NI : 1];
+  const int tid = threadIdx.x;
+  const int n_o = blockIdx.x;
+  int xi[3], xo[3], xt[3] = {0, 0, 0};
+  TL::pos(tid, xi);
+  tile_pos<D>(n_o, a.g, xo);
+#pragma unroll
+  for (int k = 0; k < D; ++k) xt[k] = xo[k] * TL::T[k] + xi[k];
+  const int64_t N = a.g.size_tot;
+  const int64_t n_t = int64_t(n_o) * NI + tid;
+  int wx = 255;
+  if constexpr (GENERAL) {
+    if (a.wall) wx = a.wall[n_t];
+    wall_s[tid] = uint8_t(wx);
+    __syncthreads();
+  }
+  S geo[NG > 0 ? NG : 1];
+#pragma unroll
+  for (int k = 0; k < NG; ++k) geo[k] = a.geom[k * N + n_t];
+  Stencil<S, MODEL, D> st;
+  make_stencil<S, MODEL, D>(geo, a.iso_w, st);
+  int code[2 * NS];
+  resolve_codes<S, MODEL, D, TL, GENERAL>(st, xi, xt, xo, wx, wall_s, a.wall, a.g, code);
+#pragma unroll
+  for (int k = 0; k < NS; ++k) a.sw[k * N + n_t] = st.w[k];
+#pragma unroll
+  for (int j = 0; j < 2 * NS; ++j) a.scode[j * N + n_t] = code[j];
 }
 
 // Initial list: tiles holding at least one trigger bit (or every tile).
@@ -508,6 +590,9 @@ __global__ void wall_dist_cap(int64_t N, uint8_t* d, int bound) {
 template <class S> struct KernelSet {
   void (*update)(EikArgs<S>);
   void (*update_walls)(EikArgs<S>);
+  void (*update_pre)(EikArgs<S>);          // null: no precomputed-scheme variant
+  void (*scheme)(SchemeArgs<S>);
+  void (*scheme_general)(SchemeArgs<S>);
   void (*flow)(FlowArgs<S>);
   void (*init)(const uint32_t*, GridDesc, int32_t, int32_t*, int32_t*, int32_t*);
   void (*all_tiles)(GridDesc, int32_t*, int32_t*);
@@ -518,8 +603,19 @@ template <class S> struct KernelSet {
 template <class S, int MODEL, int D, int T0, int T1, int T2>
 KernelSet<S> make_set() {
   KernelSet<S> k;
-  k.update = eik_update<S, MODEL, D, T0, T1, T2, false>;
-  k.update_walls = eik_update<S, MODEL, D, T0, T1, T2, true>;
+  k.update = eik_update<S, MODEL, D, T0, T1, T2, VAR_FAST>;
+  k.update_walls = eik_update<S, MODEL, D, T0, T1, T2, VAR_GENERAL>;
+  // Precomputed schemes pay off where the stencil is costly to build
+  // (Selling decomposition); the drift of Rander is not stored.
+  if constexpr (MODEL == AGD_RIEMANN) {
+    k.update_pre = eik_update<S, MODEL, D, T0, T1, T2, VAR_PRE>;
+    k.scheme = eik_scheme<S, MODEL, D, T0, T1, T2, false>;
+    k.scheme_general = eik_scheme<S, MODEL, D, T0, T1, T2, true>;
+  } else {
+    k.update_pre = nullptr;
+    k.scheme = nullptr;
+    k.scheme_general = nullptr;
+  }
   k.flow = eik_flow<S, MODEL, D, T0, T1, T2>;
   k.init = eik_init_list<D, T0 * T1 * T2>;
   k.all_tiles = all_tiles_kernel<D>;
@@ -669,9 +765,15 @@ static int solve_impl(const agd_eikonal_desc* d, const agd_eikonal_io* io,
   const int blocks = blocks_for(g.ntiles, ni);
   const bool global = d->solver == AGD_GLOBAL;
 
+  const bool pre = io->scheme_weights != nullptr || io->scheme_codes != nullptr;
+  AGD_REQUIRE(!pre || (io->scheme_weights && io->scheme_codes),
+              "scheme_weights and scheme_codes go together");
+  AGD_REQUIRE(!pre || ks.update_pre, "no precomputed-scheme variant for this model");
   EikArgs<S> a{};
   a.u = static_cast<S*>(io->values);
   a.geom = static_cast<const S*>(io->geom);
+  a.sw = static_cast<const S*>(io->scheme_weights);
+  a.scode = io->scheme_codes;
   a.rhs = static_cast<const S*>(io->rhs);
   a.seeds = io->seeds;
   a.wall = io->wall_dist;
@@ -707,8 +809,8 @@ static int solve_impl(const agd_eikonal_desc* d, const agd_eikonal_io* io,
   };
   auto launch = [&](int32_t koff, hipStream_t s) {
     a.w.koff = koff;
-    hipLaunchKernelGGL((io->wall_dist || g.periodic) ? ks.update_walls : ks.update, dim3(blocks),
-                       dim3(ni), 0, s, a);
+    auto kern = pre ? ks.update_pre : ((io->wall_dist || g.periodic) ? ks.update_walls : ks.update);
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(ni), 0, s, a);
   };
   WorklistResult r;
   int st = run_worklist(g.ntiles, d->nitermax_o, d->check_interval, stream, init, launch, r,
@@ -755,11 +857,44 @@ static int flow_impl(const agd_eikonal_desc* d, const agd_eikonal_io* io, const 
   return AGD_OK;
 }
 
+template <class S>
+static int scheme_impl(const agd_eikonal_desc* d, const agd_eikonal_io* io, hipStream_t stream) {
+  KernelSet<S> ks;
+  AGD_REQUIRE(lookup<S>(d->model, d->ndim, d->shape_i, ks), "unsupported tile shape for this model");
+  AGD_REQUIRE(ks.scheme, "no precomputed-scheme variant for this model");
+  AGD_REQUIRE(io && io->geom && io->scheme_weights && io->scheme_codes,
+              "geom, scheme_weights and scheme_codes are required");
+  const GridDesc g = make_grid(d);
+  SchemeArgs<S> a{};
+  a.geom = static_cast<const S*>(io->geom);
+  a.wall = io->wall_dist;
+  a.g = g;
+  for (int k = 0; k < 3; ++k) a.iso_w[k] = S(d->iso_weights[k]);
+  a.sw = static_cast<S*>(io->scheme_weights);
+  a.scode = io->scheme_codes;
+  hipLaunchKernelGGL((io->wall_dist || g.periodic) ? ks.scheme_general : ks.scheme, dim3(g.ntiles),
+                     dim3(ks.ni), 0, stream, a);
+  AGD_HIP_TRY(hipGetLastError());
+  return AGD_OK;
+}
+
 }  // namespace agd
 
 using namespace agd;
 
 extern "C" {
+
+int agd_scheme_supported(int32_t model, int32_t ndim, const int32_t* shape_i) {
+  KernelSet<float> ks;
+  return (shape_i && lookup<float>(model, ndim, shape_i, ks) && ks.scheme) ? 1 : 0;
+}
+
+int agd_eikonal_scheme(const agd_eikonal_desc* desc, const agd_eikonal_io* io, void* stream) {
+  int st = check_desc(desc);
+  if (st) return st;
+  if (desc->scalar == AGD_F32) return scheme_impl<float>(desc, io, as_stream(stream));
+  return scheme_impl<double>(desc, io, as_stream(stream));
+}
 
 int agd_tile_supported(int32_t model, int32_t ndim, const int32_t* shape_i) {
   KernelSet<float> ks;

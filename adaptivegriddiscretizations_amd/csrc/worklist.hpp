@@ -155,8 +155,8 @@ int run_worklist(int32_t ntiles, int64_t nitermax, int check_interval, hipStream
     hipGraphExec_t exec = nullptr;
     ~Owned() {
       for (auto e : evs)
-        if (e) hipEventDestroy(e);
-      if (exec) hipGraphExecDestroy(exec);
+        if (e) (void)hipEventDestroy(e);
+      if (exec) (void)hipGraphExecDestroy(exec);
     }
   } own;
   std::vector<hipEvent_t>& evs = own.evs;
@@ -190,12 +190,12 @@ int run_worklist(int32_t ntiles, int64_t nitermax, int check_interval, hipStream
     const hipError_t cap_err = hipGetLastError();
     const hipError_t end_err = hipStreamEndCapture(cs, &g);
     if (cap_err != hipSuccess || end_err != hipSuccess) {
-      if (g) hipGraphDestroy(g);
+      if (g) (void)hipGraphDestroy(g);
       AGD_HIP_TRY(cap_err);
       AGD_HIP_TRY(end_err);
     }
     const hipError_t inst_err = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
-    hipGraphDestroy(g);
+    (void)hipGraphDestroy(g);
     AGD_HIP_TRY(inst_err);
   }
 

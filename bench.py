@@ -30,7 +30,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# Compulsory HBM bytes per cell-update: u read+write 8, rhs 4, seed bit 1/8,
+# plus the scheme inputs: 4 B per geometry float when the stencil is rebuilt
+# at every visit, or 4 B per stored weight + 4 B per stored neighbour index
+# (2 per weight) with a precomputed scheme.
 BYTES_PER_CELL_UPDATE = {'Isotropic2': 12.125, 'Riemann2': 24.125, 'Riemann3': 36.125}
+BYTES_PER_CELL_UPDATE_PRE = {'Riemann2': 12.125 + 3 * 12, 'Riemann3': 12.125 + 6 * 12}
 
 
 def log(msg):
@@ -59,10 +64,14 @@ def prepare(wl, **extra):
 
 
 def solve_once(iface, desc):
+    """One step: reset values, rebuild the precomputed scheme (if used) from
+    the resident geometry, solve to convergence."""
     import torch
     from adaptivegriddiscretizations_amd import _lib
     iface.values_t.fill_(float('inf'))
     iface._values = None
+    if iface.precompute_scheme:
+        iface.compute_scheme()
     st = _lib.EikonalStats()
     _lib.check(_lib.lib.agd_eikonal_solve(desc, iface._io(), st, torch.cuda.current_stream().cuda_stream))
     return st
@@ -97,6 +106,8 @@ def main():
     ap.add_argument('--config', default='riemann3', choices=['riemann3', 'riemann2', 'isotropic2'])
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--no-fp64', action='store_true')
+    ap.add_argument('--no-precompute', action='store_true',
+                    help='rebuild the stencil at every tile visit instead of storing it once per solve')
     ap.add_argument('--no-graph', action='store_true',
                     help='launch every pass eagerly instead of replaying captured pass chunks')
     ap.add_argument('--copy-calibration', type=int, default=0, metavar='MIB',
@@ -125,7 +136,7 @@ def main():
     t0 = time.time()
     wl = make_workload(args.config)
     log(f"workload {wl.name} {wl.dims} generated in {time.time() - t0:.1f} s")
-    iface = prepare(wl)
+    iface = prepare(wl, **({'precompute_scheme': False} if args.no_precompute else {}))
     desc = iface._desc()
     desc.use_graph = 0 if args.no_graph else 1
     desc.time_launches = 0
@@ -154,7 +165,7 @@ def main():
     st_t = solve_once(iface, desc)
     log(f"timed-launch solve: {st_t.gpu_ms:.2f} ms, passes {st_t.launches}, cell-updates "
         f"{st_t.cell_updates}, sum of launch durations {st_t.kernel_ms:.2f} ms")
-    bpc = BYTES_PER_CELL_UPDATE[wl.model]
+    bpc = (BYTES_PER_CELL_UPDATE_PRE if iface.precompute_scheme else BYTES_PER_CELL_UPDATE)[wl.model]
     achieved = st_t.cell_updates * bpc / (st_t.kernel_ms * 1e-3) / 1e9
     u32 = iface.values.float().cpu().numpy()
     assert np.isfinite(u32).all(), "non-finite values in the benchmark solve"
@@ -189,6 +200,7 @@ def main():
             "model": wl.model, "grid": list(wl.dims), "tile": list(iface.shape_i),
             "niter_i": iface.niter_i, "solver": iface.solver, "atol": iface.atol, "rtol": iface.rtol,
             "parallelism": "single GPU", "pass_launch": "eager" if args.no_graph else "hipGraph",
+            "precompute_scheme": bool(iface.precompute_scheme),
         },
         "solved_cells_per_s": wl.ncells * args.steps / wall,
         "cell_updates_per_step": cell_updates / args.steps,
@@ -202,7 +214,8 @@ def main():
         },
         "cpu_baseline": None,
     }
-    tpath = os.path.join(ROOT, 'profiles', f'eik_update_{args.config}_{wl.dims[0]}_traffic.json')
+    variant = "_pre" if iface.precompute_scheme else ""
+    tpath = os.path.join(ROOT, "profiles", f"eik_update_{args.config}_{wl.dims[0]}{variant}_traffic.json")
     if os.path.exists(tpath):
         # HBM bytes per dispatch from the committed FETCH_SIZE / WRITE_SIZE runs
         # (scaled by the device-copy calibration) for this workload.

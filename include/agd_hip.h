@@ -82,6 +82,14 @@ typedef struct agd_eikonal_io {
   const uint32_t* seeds;      /* 1 bit per cell (bit n_t%32 of word n_t/32): fixed cells */
   const uint32_t* trigger;    /* 1 bit per cell: cells whose tiles start active (NULL: seeds) */
   const uint8_t* wall_dist;   /* saturated L1 distance to walls, 255 = far; NULL = no walls */
+  /* Precomputed scheme (both NULL: the stencil is rebuilt at every tile
+   * visit).  Filled by agd_eikonal_scheme from geom and wall_dist:
+   * scheme_weights nsym components of the scalar type, scheme_codes 2*nsym
+   * int32 components (neighbour x - e_k then x + e_k, resolved against the
+   * tiled grid: >= 0 tiled index, <= -2 slot -2-code of the same tile,
+   * -1 none). */
+  void* scheme_weights;
+  int32_t* scheme_codes;
 } agd_eikonal_io;
 
 typedef struct agd_eikonal_stats {
@@ -122,6 +130,14 @@ int agd_eikonal_solve(const agd_eikonal_desc* desc, const agd_eikonal_io* io,
  * reference's flow pass (DECREASING rule).  */
 int agd_eikonal_flow(const agd_eikonal_desc* desc, const agd_eikonal_io* io,
                      const agd_flow_io* flow, void* stream);
+
+/* Precompute the scheme of every cell into io->scheme_weights /
+ * io->scheme_codes (replaces the precompute_scheme export kernel,
+ * _Kernel.py:254-268,346-359, Update.h:76-80 import_scheme_macro).  Needs
+ * geom (and wall_dist if walls are present).  Riemann only, as
+ * agd_scheme_supported reports. */
+int agd_scheme_supported(int32_t model, int32_t ndim, const int32_t* shape_i);
+int agd_eikonal_scheme(const agd_eikonal_desc* desc, const agd_eikonal_io* io, void* stream);
 
 /* Reverse-mode sensitivities (replaces SolveAD's reverse branch,
  * _PostProcess.py:194-219: graph_reverse.py:10-77 + Kernel_LinearUpdate.h).

@@ -38,7 +38,7 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(_lib.EikonalDesc) == 136
     assert _lib.EikonalDesc.atol.offset == 72
     assert _lib.EikonalDesc.nitermax_o.offset == 88
-    assert ctypes.sizeof(_lib.EikonalIO) == 48
+    assert ctypes.sizeof(_lib.EikonalIO) == 64
     assert ctypes.sizeof(_lib.EikonalStats) == 56
 
 

@@ -42,11 +42,11 @@ def test_isotropic2_512_seed_on_tie(O):
     assert err <= TOL32
 
 
-@pytest.mark.parametrize("n,solver", [(128, "FIM"), (150, "AGSI")])
-def test_riemann2_aniso(O, n, solver):
+@pytest.mark.parametrize("n,solver,pre", [(128, "FIM", True), (150, "AGSI", True), (128, "FIM", False)])
+def test_riemann2_aniso(O, n, solver, pre):
     wl = workloads.riemann2_aniso(n=n, seed=5, sigma=4., amax=10.)
-    log(f"riemann2_aniso {n}^2 solver={solver}")
-    u, out = run_gpu(wl, solver=solver)
+    log(f"riemann2_aniso {n}^2 solver={solver} precompute_scheme={pre}")
+    u, out = run_gpu(wl, solver=solver, precompute_scheme=pre)
     P, _ = oracle_problem(O, wl)
     ref = P.solve_fm()
     err = rel_err(u, ref)
@@ -65,15 +65,16 @@ def test_riemann2_float64_tight(O):
     assert err <= 1e-9
 
 
+@pytest.mark.parametrize("pre", [True, False])
 @pytest.mark.parametrize("dims", [(24, 24, 24), (30, 17, 21)])
-def test_riemann3_aniso(O, dims):
+def test_riemann3_aniso(O, dims, pre):
     wl = workloads.riemann3_aniso(n=dims[0], seed=4, sigma=3.)
     if dims != (dims[0],) * 3:
         wl.metric = np.ascontiguousarray(wl.metric[:, :, :dims[0], :dims[1], :dims[2]])
         wl.dims = dims
         wl.sides = [[-1, -1 + 2. * d / dims[0]] for d in dims]
-    log(f"riemann3_aniso {dims}")
-    u, out = run_gpu(wl)
+    log(f"riemann3_aniso {dims} precompute_scheme={pre}")
+    u, out = run_gpu(wl, precompute_scheme=pre)
     P, _ = oracle_problem(O, wl)
     ref = P.solve_fm()
     err = rel_err(u, ref)
@@ -101,11 +102,12 @@ def test_walls(O):
     assert rel_err(u, ref) <= TOL32
 
 
-def test_riemann2_walls_visibility(O):
+@pytest.mark.parametrize("pre", [True, False])
+def test_riemann2_walls_visibility(O, pre):
     wl = workloads.riemann2_aniso(n=96, seed=12, sigma=4., nseeds=1, amax=8.)
     walls = np.zeros(wl.dims, dtype=bool)
     walls[30:33, 5:70] = True
-    u, _ = run_gpu(wl, walls=walls)
+    u, _ = run_gpu(wl, walls=walls, precompute_scheme=pre)
     P, _ = oracle_problem(O, wl, walls=walls)
     assert rel_err(u, P.solve_fm()) <= TOL32
 
