@@ -136,6 +136,9 @@ class dictIn(MutableMapping):
             self.update(store)
 
     def array_float_caster(self, x):
+        from ..AutomaticDifferentiation import Dense
+        if isinstance(x, Dense.denseAD):  # cast the value and the coefficients alike
+            return Dense.denseAD(self.array_float_caster(x.value), self.array_float_caster(x.coef))
         if is_tensor(x):
             import torch
             return x.to(dtype=torch.float32 if self.float_t == np.float32 else torch.float64)

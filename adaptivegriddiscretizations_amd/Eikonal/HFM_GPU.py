@@ -16,6 +16,7 @@ from collections import OrderedDict
 
 import numpy as np
 
+from .. import AutomaticDifferentiation as ad
 from .. import FiniteDifferences as fd
 from .. import Metrics
 from .. import _lib
@@ -215,17 +216,23 @@ class Interface:
         if tips is not None:
             raise NotImplementedError("geodesic backtracking ('tips') is not provided yet")
 
-        # Cost function
+        # Cost function (_SetGeometry.py:221-233): a denseAD cost, or a
+        # separate costVariation, requests forward AD.
         if self.HasValue('speed'):
             speed = self.GetValue('speed', help="speed = 1/cost (scales the metric)")
+            if ad.is_ad(speed):
+                raise NotImplementedError("forward AD through 'speed' is not provided; use 'cost'")
             self.cost = 1. / speed
         else:
-            self.cost = self.GetValue('cost', default=None, help="cost = 1/speed (scales the metric)")
+            self.cost = self.GetValue('cost', default=None, help="cost = 1/speed (scales the metric, accepts AD)")
             if self.cost is None:
                 self.cost = np.ones((), dtype=np.float32)
-        if self.GetValue('costVariation', default=None,
-                         help="First order variation of the cost function (not provided)") is not None:
-            raise NotImplementedError("forward AD ('costVariation') is not provided yet")
+        self.cost_coef = None
+        if ad.is_ad(self.cost):
+            self.cost, self.cost_coef = self.cost.value, self.cost.coef
+        else:
+            self.cost_coef = self.GetValue('costVariation', default=None,
+                                           help="First order variation of the cost function")
 
         # Tolerances (_SetGeometry.py:237-253)
         res = np.finfo(self.float_t).resolution
@@ -349,6 +356,7 @@ class Interface:
         seeds = self.GetValue('seeds', default=None, help="Points from where the front propagation starts")
         self.reverseAD = self.HasValue('sensitivity')
         self.seedIndices = None
+        self.seed_coef = None
         if seeds is not None:
             seeds = np.asarray(host_array(seeds), dtype=self.hfmIn.float_t)
             if seeds.ndim != 2 or seeds.shape[1] != self.ndim:
@@ -356,6 +364,13 @@ class Interface:
             cont = self.hfmIn.PointFromIndex(seeds, to=True)
             seedValues = self.GetValue('seedValues', default=np.zeros(len(seeds), dtype=np.float32),
                                        help="Initial value for the front propagation")
+            if ad.is_ad(seedValues):
+                seedValues, self.seed_coef = seedValues.value, seedValues.coef
+            else:
+                svv = self.GetValue('seedValueVariation', default=None,
+                                    help="First order variation of the seed values")
+                if svv is not None:  # (nfwd, nseeds) -> (nseeds, nfwd)
+                    self.seed_coef = np.asarray(host_array(svv)).reshape(-1, len(seeds)).T
             seedValues = np.asarray(host_array(seedValues), dtype=np.float64).reshape(-1)
             if len(seedValues) != len(seeds):
                 raise ValueError("seedValues must have one entry per seed")
@@ -378,6 +393,9 @@ class Interface:
                                                   ptr(self.trigger_t), self.stream))
         elif self.solver != 'global_iteration' and not self.warmStartAllTiles:
             raise ValueError("Missing value for key seeds")
+
+        self.forwardAD = self.cost_coef is not None or self.seed_coef is not None
+        self.values_coef = None
 
         self.wallDist_t = None
         if self.walls is not None:
@@ -489,6 +507,8 @@ class Interface:
                                                   1, np.dtype(self.float_t).itemsize,
                                                   ptr(self.values_t), ptr(out), self.stream))
             self._values = out
+        if self.values_coef is not None:
+            return ad.denseAD(self._output(self._values), self._output(self.values_coef))
         return self._output(self._values)
 
     def _output(self, t):
@@ -504,7 +524,8 @@ class Interface:
     # ------------------------------------------------------------ flow
     def PostProcess(self):
         """_PostProcess.py:46-95: one global pass exporting the upwind flow."""
-        self.flow_needed = self.exportGeodesicFlow or self.reverseAD
+        self.flow_needed = (self.exportGeodesicFlow or self.reverseAD or self.forwardAD
+                            or getattr(self, 'keep_flow', False))
         if not self.flow_needed:
             return
         if self.verbosity >= 1:
@@ -532,57 +553,110 @@ class Interface:
             h = torch.as_tensor(self.h, dtype=fv.dtype, device=fv.device).reshape((self.ndim,) + (1,) * self.ndim)
             self.hfmOut['flow'] = self._output(fv * (-h))
 
-    # ------------------------------------------------------------ reverse AD
+    # ------------------------------------------------------------ AD
     def SolveAD(self):
-        """_PostProcess.py:155-219 (reverse branch)."""
-        if not self.reverseAD:
+        """_PostProcess.py:155-219: forward then reverse differentiation."""
+        if not (self.forwardAD or self.reverseAD):
             return
         if self.model_ == 'Rander':
-            raise NotImplementedError("reverse AD of the Rander model is not provided yet")
-        sens = self.GetValue('sensitivity', help='Reverse automatic differentiation')
-        sens = device_tensor(sens, self.float_t)
-        if tuple(sens.shape[:-1]) != tuple(self.shape):
-            raise ValueError(f"Reverse AD rhs shape {tuple(sens.shape)} does not start with {self.shape}")
-        nrhs = int(sens.shape[-1])
-        sens_c = torch.movedim(sens, -1, 0).contiguous()
-        rhs_t = empty((nrhs, self.size_tot), self.float_t)
-        esize = np.dtype(self.float_t).itemsize
+            raise NotImplementedError("automatic differentiation of the Rander model is not provided yet")
+        if self.forwardAD:
+            self.values_coef = self.forward_variation()
+        if self.reverseAD:
+            sens = self.GetValue('sensitivity', help='Reverse automatic differentiation')
+            lam, seedSens = self.reverse(sens)
+            self.costSensitivity = lam
+            self.hfmOut['costSensitivity'] = self._output(lam)
+            # seedValues_rev is the identity (seedRadius = 0): one row per seed.
+            self.hfmOut['seedValueSensitivity'] = self._output(seedSens)
+
+    def _expand_components(self, x_c, pad):
+        """(n, *shape) cell-major device tensor -> (n, size_tot) tiled."""
+        n = int(x_c.shape[0])
+        out = empty((n, self.size_tot), self.float_t)
         _lib.check(_lib.lib.agd_block_expand(self.ndim, _lib.i64(self.shape), _lib.i32(self.shape_i),
-                                             nrhs, esize, ptr(sens_c), 1, ptr(rhs_t), float('nan'),
-                                             self.stream))
-        res = np.finfo(self.float_t).resolution
-        self.linear_rtol = float(self.GetValue('linear_rtol', default=float(res * 5),
-                                               help="Relative convergence tolerance for the linear systems"))
-        linear_atol = self.GetValue('linear_atol', default=None,
-                                    help="Absolute convergence tolerance for the linear systems")
-        if linear_atol is None:
-            fin = torch.isfinite(sens_c)
-            mean = float(sens_c[fin].abs().double().mean().item()) if bool(fin.any()) else 0.
-            linear_atol = self.linear_rtol * mean
-            self.hfmOut['keys']['default']['linear_atol'] = linear_atol
-        self.linear_atol = float(linear_atol)
-        lam_t = empty((nrhs, self.size_tot), self.float_t)
+                                             n, np.dtype(self.float_t).itemsize, ptr(x_c), 1,
+                                             ptr(out), pad, self.stream))
+        return out
+
+    def _linear_tolerances(self, rhs_c):
+        """linear_rtol / linear_atol, set by the first linear solve
+        (_PostProcess.py:126-138)."""
+        if not hasattr(self, 'linear_rtol'):
+            res = np.finfo(self.float_t).resolution
+            self.linear_rtol = float(self.GetValue('linear_rtol', default=float(res * 5),
+                                                   help="Relative convergence tolerance for the linear systems"))
+        if not hasattr(self, 'linear_atol'):
+            linear_atol = self.GetValue('linear_atol', default=None,
+                                        help="Absolute convergence tolerance for the linear systems")
+            if linear_atol is None:
+                fin = torch.isfinite(rhs_c)
+                mean = float(rhs_c[fin].abs().double().mean().item()) if bool(fin.any()) else 0.
+                linear_atol = self.linear_rtol * mean
+                self.hfmOut['keys']['default']['linear_atol'] = linear_atol
+            self.linear_atol = float(linear_atol)
+
+    def _linear_solve(self, entry, name, rhs_c):
+        """Tile Gauss-Seidel solve of the linearised scheme for the
+        right-hand sides rhs_c (n, *shape); returns (*shape, n)."""
+        nrhs = int(rhs_c.shape[0])
+        rhs_t = self._expand_components(rhs_c, float('nan'))
+        self._linear_tolerances(rhs_c)
+        sol_t = empty((nrhs, self.size_tot), self.float_t)
         st = _lib.EikonalStats()
-        status = _lib.lib.agd_eikonal_adjoint(
-            self._desc(), ptr(self.values_t), ptr(self.flow_weights_t), ptr(self.flow_indices_t),
-            ptr(self.flow_weightsum_t), nrhs, ptr(rhs_t), ptr(lam_t), self.linear_atol,
-            self.linear_rtol, st, self.stream)
-        self.stats['reverseAD'] = {**st.as_dict(), 'time': st.gpu_ms * 1e-3}
+        status = entry(self._desc(), ptr(self.values_t), ptr(self.flow_weights_t),
+                       ptr(self.flow_indices_t), ptr(self.flow_weightsum_t), nrhs, ptr(rhs_t),
+                       ptr(sol_t), self.linear_atol, self.linear_rtol, st, self.stream)
+        self.stats[name] = {**st.as_dict(), 'time': st.gpu_ms * 1e-3}
         if status == _lib.AGD_NOCONV:
-            msg = "Solver for kernel reverseAD did not reach convergence"
+            msg = f"Solver for kernel {name} did not reach convergence"
             if self.raiseOnNonConvergence:
                 raise ValueError(msg)
             self.Warn(msg)
         else:
             _lib.check(status)
-        lam = torch.movedim(self._squeeze(lam_t, nrhs, self.float_t), 0, -1).contiguous()
-        pos = tuple(torch.as_tensor(self.seedIndices[:, k], device=lam.device) for k in range(self.ndim))
+        return torch.movedim(self._squeeze(sol_t, nrhs, self.float_t), 0, -1).contiguous()
+
+    def _seed_pos(self, dev):
+        return tuple(torch.as_tensor(self.seedIndices[:, k], device=dev) for k in range(self.ndim))
+
+    def forward_variation(self):
+        """First-order variation of the values (_PostProcess.py:181-191):
+        solves (D - W) du = r with r = cost * costVariation off the seeds,
+        the seed value variation on the seeds and 0 in the walls."""
+        cc = None if self.cost_coef is None else device_tensor(self.cost_coef, self.float_t)
+        sc = None if self.seed_coef is None else device_tensor(self.seed_coef, self.float_t)
+        nf = int((cc if cc is not None else sc).shape[-1])
+        if cc is not None and sc is not None and int(sc.shape[-1]) != nf:
+            raise ValueError("costVariation and seedValueVariation have different AD sizes")
+        if cc is not None:
+            if tuple(cc.shape) != tuple(self.shape) + (nf,):
+                raise ValueError(f"costVariation shape {tuple(cc.shape)} must be {self.shape} + (n,)")
+            rhs = cc * self.cost_t.reshape(self.cost_t.shape + (1,))
+        else:
+            rhs = zeros(tuple(self.shape) + (nf,), self.float_t)
+        if self.seedIndices is not None:
+            rhs[self._seed_pos(rhs.device)] = sc if sc is not None else 0.
+        if self.walls is not None:
+            rhs[device_tensor(self.walls, np.bool_)] = 0.
+        rhs_c = torch.movedim(rhs, -1, 0).contiguous()
+        return self._linear_solve(_lib.lib.agd_eikonal_forward_ad, 'forwardAD', rhs_c)
+
+    def reverse(self, sens):
+        """Reverse sensitivities for sens of shape (*shape, nrhs)
+        (_PostProcess.py:194-219): returns (costSensitivity, with the seeds
+        zeroed, and the seeds' rows)."""
+        sens = device_tensor(sens, self.float_t)
+        if tuple(sens.shape[:-1]) != tuple(self.shape):
+            raise ValueError(f"Reverse AD rhs shape {tuple(sens.shape)} does not start with {self.shape}")
+        sens_c = torch.movedim(sens, -1, 0).contiguous()
+        lam = self._linear_solve(_lib.lib.agd_eikonal_adjoint, 'reverseAD', sens_c)
+        if self.seedIndices is None:
+            return lam, lam.new_zeros((0, int(sens.shape[-1])))
+        pos = self._seed_pos(lam.device)
         seedSens = lam[pos].clone()
         lam[pos] = 0
-        self.costSensitivity = lam
-        self.hfmOut['costSensitivity'] = self._output(lam)
-        # seedValues_rev is the identity (seedRadius = 0): one row per seed.
-        self.hfmOut['seedValueSensitivity'] = self._output(seedSens)
+        return lam, seedSens
 
     # ------------------------------------------------------------ outputs
     def FinalCheck(self):

@@ -9,6 +9,6 @@
 The compute path is libagd_hip.so (hand-written HIP kernels for gfx950),
 reached through the C ABI of include/agd_hip.h.
 """
-from . import Eikonal, Metrics, FiniteDifferences  # noqa: F401
+from . import AutomaticDifferentiation, Eikonal, Metrics, FiniteDifferences  # noqa: F401
 
 __version__ = "0.1.0"

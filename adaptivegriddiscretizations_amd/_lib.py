@@ -77,6 +77,9 @@ SIGNATURES = {
     "agd_eikonal_adjoint": (c_int32, [POINTER(EikonalDesc), c_void_p, c_void_p, c_void_p,
                                       c_void_p, c_int32, c_void_p, c_void_p, c_double,
                                       c_double, POINTER(EikonalStats), c_void_p]),
+    "agd_eikonal_forward_ad": (c_int32, [POINTER(EikonalDesc), c_void_p, c_void_p, c_void_p,
+                                         c_void_p, c_int32, c_void_p, c_void_p, c_double,
+                                         c_double, POINTER(EikonalStats), c_void_p]),
     "agd_block_expand": (c_int32, [c_int32, c_int64_p, c_int32_p, c_int32, c_int32, c_void_p,
                                    c_int64, c_void_p, c_double, c_void_p]),
     "agd_block_squeeze": (c_int32, [c_int32, c_int64_p, c_int32_p, c_int32, c_int32, c_void_p,

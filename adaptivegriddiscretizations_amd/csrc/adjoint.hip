@@ -74,8 +74,37 @@ __global__ void adj_sort_rows(int64_t N, const int32_t* rowptr, int32_t* src, S*
   }
 }
 
-// Trigger: cells of the domain (finite value) with no incoming transposed
-// edge, i.e. no cell uses them as upwind neighbour (_PostProcess.py:143-144).
+// Forward system: row x holds x's own upwind neighbours in stencil order
+// (already deterministic, no sort needed).
+template <class S>
+__global__ void fwd_count(int64_t N, int nsym, const S* fw, const int32_t* fidx, int32_t* deg) {
+  const int64_t x = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (x >= N) return;
+  int c = 0;
+  for (int k = 0; k < nsym; ++k) c += (fw[k * N + x] != S(0) && fidx[k * N + x] != INT32_MAX);
+  deg[x] = c;
+}
+
+template <class S>
+__global__ void fwd_fill(int64_t N, int nsym, const S* fw, const int32_t* fidx,
+                         const int32_t* rowptr, int32_t* src, S* wt) {
+  const int64_t x = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (x >= N) return;
+  int32_t p = rowptr[x];
+  for (int k = 0; k < nsym; ++k) {
+    const int32_t y = fidx[k * N + x];
+    const S w = fw[k * N + x];
+    if (w != S(0) && y != INT32_MAX) {
+      src[p] = y;
+      wt[p] = w;
+      ++p;
+    }
+  }
+}
+
+// Trigger: cells of the domain (finite value) whose row is empty
+// (_PostProcess.py:143-144): in reverse mode no cell uses them as upwind
+// neighbour, in forward mode they have no upwind neighbour (seeds).
 template <class S>
 __global__ void adj_trigger(int64_t N, const S* u, const int32_t* rowptr, uint32_t* trig) {
   const int64_t y = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -198,9 +227,9 @@ template <class S> static bool lin_lookup(int d, const int32_t* si, LinSet<S>& o
 }
 
 template <class S>
-static int adjoint_impl(const agd_eikonal_desc* d, const S* values, const S* fw,
-                        const int32_t* fidx, const S* wsum, int32_t nrhs, const S* sens, S* lam,
-                        double atol, double rtol, agd_eikonal_stats* stats, hipStream_t stream) {
+static int linear_impl(bool forward, const agd_eikonal_desc* d, const S* values, const S* fw,
+                       const int32_t* fidx, const S* wsum, int32_t nrhs, const S* sens, S* lam,
+                       double atol, double rtol, agd_eikonal_stats* stats, hipStream_t stream) {
   LinSet<S> ls;
   AGD_REQUIRE(lin_lookup<S>(d->ndim, d->shape_i, ls), "unsupported tile shape");
   AGD_REQUIRE(values && fw && fidx && wsum && sens && lam && nrhs >= 1, "null buffer");
@@ -219,7 +248,7 @@ static int adjoint_impl(const agd_eikonal_desc* d, const S* values, const S* fw,
   const int nsym = nsym_of(d->model, d->ndim);
   const dim3 grid(unsigned(ceil_div(N, 256))), block(256);
 
-  // ---- 1. transpose
+  // ---- 1. the system's rows (reverse: transposed flow graph)
   int32_t *deg = nullptr, *rowptr = nullptr, *src = nullptr;
   uint32_t* trig = nullptr;
   S* wt = nullptr;
@@ -230,7 +259,10 @@ static int adjoint_impl(const agd_eikonal_desc* d, const S* values, const S* fw,
   AGD_HIP_TRY(hipMalloc((void**)&trig, ceil_div(N, 32) * sizeof(uint32_t)));
   AGD_HIP_TRY(hipMemsetAsync(deg, 0, (N + 1) * sizeof(int32_t), stream));
   AGD_HIP_TRY(hipMemsetAsync(trig, 0, ceil_div(N, 32) * sizeof(uint32_t), stream));
-  hipLaunchKernelGGL(adj_count<S>, grid, block, 0, stream, N, nsym, fw, fidx, deg);
+  if (forward)
+    hipLaunchKernelGGL(fwd_count<S>, grid, block, 0, stream, N, nsym, fw, fidx, deg);
+  else
+    hipLaunchKernelGGL(adj_count<S>, grid, block, 0, stream, N, nsym, fw, fidx, deg);
   AGD_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, deg, rowptr, int(N + 1), stream));
   AGD_HIP_TRY(hipMalloc(&tmp, tmp_bytes));
   AGD_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, deg, rowptr, int(N + 1), stream));
@@ -239,9 +271,14 @@ static int adjoint_impl(const agd_eikonal_desc* d, const S* values, const S* fw,
   AGD_HIP_TRY(hipStreamSynchronize(stream));
   AGD_HIP_TRY(hipMalloc((void**)&src, std::max<int64_t>(nnz, 1) * sizeof(int32_t)));
   AGD_HIP_TRY(hipMalloc((void**)&wt, std::max<int64_t>(nnz, 1) * sizeof(S)));
-  AGD_HIP_TRY(hipMemcpyAsync(deg, rowptr, (N + 1) * sizeof(int32_t), hipMemcpyDeviceToDevice, stream));
-  hipLaunchKernelGGL(adj_fill<S>, grid, block, 0, stream, N, nsym, fw, fidx, deg, src, wt);
-  hipLaunchKernelGGL(adj_sort_rows<S>, grid, block, 0, stream, N, rowptr, src, wt);
+  if (forward) {
+    hipLaunchKernelGGL(fwd_fill<S>, grid, block, 0, stream, N, nsym, fw, fidx, rowptr, src, wt);
+  } else {
+    AGD_HIP_TRY(hipMemcpyAsync(deg, rowptr, (N + 1) * sizeof(int32_t), hipMemcpyDeviceToDevice,
+                               stream));
+    hipLaunchKernelGGL(adj_fill<S>, grid, block, 0, stream, N, nsym, fw, fidx, deg, src, wt);
+    hipLaunchKernelGGL(adj_sort_rows<S>, grid, block, 0, stream, N, rowptr, src, wt);
+  }
   hipLaunchKernelGGL(adj_trigger<S>, grid, block, 0, stream, N, values, rowptr, trig);
   AGD_HIP_TRY(hipGetLastError());
 
@@ -311,7 +348,8 @@ static int adjoint_impl(const agd_eikonal_desc* d, const S* values, const S* fw,
     stats->kernel_ms = kernel_ms;
   }
   if (!converged) {
-    set_error("adjoint solver did not converge within nitermax_o passes");
+    set_error(forward ? "forward AD solver did not converge within nitermax_o passes"
+                      : "adjoint solver did not converge within nitermax_o passes");
     return AGD_NOCONV;
   }
   return AGD_OK;
@@ -321,22 +359,41 @@ static int adjoint_impl(const agd_eikonal_desc* d, const S* values, const S* fw,
 
 using namespace agd;
 
+namespace {
+int linear_entry(bool forward, const agd_eikonal_desc* desc, const void* values,
+                 const void* flow_weights, const int32_t* flow_indices, const void* flow_weightsum,
+                 int32_t nrhs, const void* rhs, void* sol, double atol, double rtol,
+                 agd_eikonal_stats* stats, void* stream) {
+  AGD_REQUIRE(desc && (desc->ndim == 2 || desc->ndim == 3), "bad descriptor");
+  AGD_REQUIRE(desc->niter_i >= 1 && desc->nitermax_o >= 1, "bad iteration counts");
+  if (desc->scalar == AGD_F32)
+    return linear_impl<float>(forward, desc, static_cast<const float*>(values),
+                              static_cast<const float*>(flow_weights), flow_indices,
+                              static_cast<const float*>(flow_weightsum), nrhs,
+                              static_cast<const float*>(rhs), static_cast<float*>(sol), atol, rtol,
+                              stats, as_stream(stream));
+  return linear_impl<double>(forward, desc, static_cast<const double*>(values),
+                             static_cast<const double*>(flow_weights), flow_indices,
+                             static_cast<const double*>(flow_weightsum), nrhs,
+                             static_cast<const double*>(rhs), static_cast<double*>(sol), atol, rtol,
+                             stats, as_stream(stream));
+}
+}  // namespace
+
 extern "C" int agd_eikonal_adjoint(const agd_eikonal_desc* desc, const void* values,
                                    const void* flow_weights, const int32_t* flow_indices,
                                    const void* flow_weightsum, int32_t nrhs, const void* sens,
                                    void* lambda, double atol, double rtol,
                                    agd_eikonal_stats* stats, void* stream) {
-  AGD_REQUIRE(desc && (desc->ndim == 2 || desc->ndim == 3), "bad descriptor");
-  AGD_REQUIRE(desc->niter_i >= 1 && desc->nitermax_o >= 1, "bad iteration counts");
-  if (desc->scalar == AGD_F32)
-    return adjoint_impl<float>(desc, static_cast<const float*>(values),
-                               static_cast<const float*>(flow_weights), flow_indices,
-                               static_cast<const float*>(flow_weightsum), nrhs,
-                               static_cast<const float*>(sens), static_cast<float*>(lambda), atol,
-                               rtol, stats, as_stream(stream));
-  return adjoint_impl<double>(desc, static_cast<const double*>(values),
-                              static_cast<const double*>(flow_weights), flow_indices,
-                              static_cast<const double*>(flow_weightsum), nrhs,
-                              static_cast<const double*>(sens), static_cast<double*>(lambda), atol,
-                              rtol, stats, as_stream(stream));
+  return linear_entry(false, desc, values, flow_weights, flow_indices, flow_weightsum, nrhs, sens,
+                      lambda, atol, rtol, stats, stream);
+}
+
+extern "C" int agd_eikonal_forward_ad(const agd_eikonal_desc* desc, const void* values,
+                                      const void* flow_weights, const int32_t* flow_indices,
+                                      const void* flow_weightsum, int32_t nrhs, const void* rhs,
+                                      void* dvalues, double atol, double rtol,
+                                      agd_eikonal_stats* stats, void* stream) {
+  return linear_entry(true, desc, values, flow_weights, flow_indices, flow_weightsum, nrhs, rhs,
+                      dvalues, atol, rtol, stats, stream);
 }

@@ -152,6 +152,17 @@ int agd_eikonal_adjoint(const agd_eikonal_desc* desc, const void* values,
                         void* lambda, double atol, double rtol,
                         agd_eikonal_stats* stats, void* stream);
 
+/* Forward-mode variation of the solution (replaces SolveAD's forward
+ * branch, _PostProcess.py:181-191): solves (D - W) dvalues = rhs with the
+ * same flow graph, untransposed; rhs = cost * dcost off the seeds and the
+ * seed value variation on the seeds.  rhs and dvalues: [nrhs][size_tot]
+ * tiled; dvalues is overwritten. */
+int agd_eikonal_forward_ad(const agd_eikonal_desc* desc, const void* values,
+                           const void* flow_weights, const int32_t* flow_indices,
+                           const void* flow_weightsum, int32_t nrhs, const void* rhs,
+                           void* dvalues, double atol, double rtol,
+                           agd_eikonal_stats* stats, void* stream);
+
 /* ------------------------------------------------------------ layout */
 /* block_expand (FiniteDifferences.py:164-195): src cell-major
  * [ncomp][shape], dst tiled [ncomp][size_tot], padded with pad.

@@ -658,6 +658,36 @@ int orc_adjoint(int64_t N, int nact, const double* u, const double* fw, const in
   return 0;
 }
 
+// Forward-mode variation: solves (D - W) du = r exactly (FP64) by visiting
+// cells in increasing order of u (upwind neighbours come first), the
+// untransposed counterpart of orc_adjoint (_PostProcess.py:181-191).
+// r/du layout [nrhs][N]; cells with a non-finite u get du = 0.
+int orc_forward_ad(int64_t N, int nact, const double* u, const double* fw, const int64_t* nidx,
+                   int nrhs, const double* r, double* du) {
+  std::vector<int64_t> order;
+  order.reserve(N);
+  for (int64_t n = 0; n < N; ++n)
+    if (std::isfinite(u[n])) order.push_back(n);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int64_t a, int64_t b) { return u[a] < u[b]; });
+  for (int q = 0; q < nrhs; ++q) {
+    double* X = du + (int64_t)q * N;
+    const double* R = r + (int64_t)q * N;
+    for (int64_t n = 0; n < N; ++n) X[n] = 0.;
+    for (int64_t n : order) {
+      double diag = 0, acc = R[n];
+      for (int k = 0; k < nact; ++k) {
+        const double w = fw[n * nact + k];
+        const int64_t m = nidx[n * nact + k];
+        diag += w;
+        if (w != 0 && m >= 0) acc += w * X[m];
+      }
+      X[n] = acc / (diag == 0 ? 1. : diag);
+    }
+  }
+  return 0;
+}
+
 // L1 distance to walls, saturated (_SetGeometry.py:262-282): 0 on walls,
 // k for cells at L1 distance k <= bound, 255 beyond the bound.
 void orc_wall_distance(int d, const int64_t* shape, const int* periodic, const uint8_t* walls,

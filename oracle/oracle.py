@@ -51,6 +51,7 @@ def _load():
     lib.orc_solve_jacobi.restype = c_int64
     lib.orc_flow.argtypes = [c_int, c_int, P, P, P, P, P, P, P, c_int, P, P, P, P, P]
     lib.orc_adjoint.argtypes = [c_int64, c_int, P, P, P, c_int, P, P]
+    lib.orc_forward_ad.argtypes = [c_int64, c_int, P, P, P, c_int, P, P]
     lib.orc_wall_distance.argtypes = [c_int, P, P, P, c_int, P]
     return lib
 
@@ -174,6 +175,17 @@ class Problem:
         lib().orc_adjoint(self.N, self.nsym, _p(uu), _p(_c(flow['weights'], np.float64)),
                           _p(_c(flow['indices'], np.int64)), nrhs, _p(s), _p(lam))
         return np.moveaxis(lam.reshape((nrhs,) + self.shape), 0, -1)
+
+    def forward_ad(self, u, flow, rhs):
+        """Exact solution of (D - W) du = rhs, rhs [*shape, nrhs]."""
+        rhs = np.asarray(rhs, dtype=np.float64)
+        nrhs = rhs.shape[-1]
+        r = _c(np.moveaxis(rhs, -1, 0).reshape(nrhs, self.N), np.float64)
+        du = np.zeros((nrhs, self.N))
+        uu = _c(np.asarray(u).reshape(self.N), np.float64)
+        lib().orc_forward_ad(self.N, self.nsym, _p(uu), _p(_c(flow['weights'], np.float64)),
+                             _p(_c(flow['indices'], np.int64)), nrhs, _p(r), _p(du))
+        return np.moveaxis(du.reshape((nrhs,) + self.shape), 0, -1)
 
 
 def wall_distance(walls, bound=10, periodic=None):

@@ -193,3 +193,60 @@ def test_walls_block_the_front(O):
     # behind the wall, right below the seed: path must detour via column 25
     direct = 20 * h
     assert u[25, 5] > direct + 10 * h
+
+
+def test_forward_ad_matches_finite_differences(O):
+    """du along a cost and seed-value perturbation = central FD of the FP64
+    solve (rhs = cost * dcost off the seeds, dseed on the seeds)."""
+    n = 20
+    rng = np.random.default_rng(5)
+    cost = 1 + 0.3 * rng.uniform(size=(n, n))
+    h = 1. / n
+    M = np.array([[2., 0.5], [0.5, 1.]])
+    seeds = [[3, 4], [15, 12]]
+    sv = np.array([0., 0.2])
+    dc = rng.normal(size=(n, n))
+    dsv = np.array([0.3, -0.5])
+
+    def solve(c, s):
+        P = O.make_problem('Riemann', (n, n), [h, h], metric=M, cost=c, seeds_idx=seeds, seed_values=s)
+        return P.solve_fm(), P
+
+    u, P = solve(cost, sv)
+    r = cost * dc
+    for (i, j), d in zip(seeds, dsv):
+        r[i, j] = d
+    du = P.forward_ad(u, P.flow(u), r[..., None])[..., 0]
+    eps = 1e-6
+    fd = (solve(cost + eps * dc, sv + eps * dsv)[0] - solve(cost - eps * dc, sv - eps * dsv)[0]) / (2 * eps)
+    np.testing.assert_allclose(du, fd, rtol=1e-4, atol=1e-7)
+
+
+def test_forward_ad_homogeneity(O):
+    """u(l c, l s) = l u(c, s), so the variation along (c, s) is u itself
+    (the sum rule of the seed/cost sensitivities)."""
+    n = 18
+    rng = np.random.default_rng(6)
+    cost = 1 + 0.5 * rng.uniform(size=(n, n, n // 2))
+    P = O.make_problem('Riemann', (n, n, n // 2), [1. / n] * 3, metric=np.diag([1., 2., 3.]), cost=cost,
+                       seeds_idx=[[2, 3, 4]], seed_values=[0.25])
+    u = P.solve_fm()
+    r = cost * cost
+    r[2, 3, 4] = 0.25
+    du = P.forward_ad(u, P.flow(u), r[..., None])[..., 0]
+    np.testing.assert_allclose(du, u, rtol=1e-10)
+
+
+def test_forward_and_reverse_are_transposes(O):
+    """<s, du> computed forward equals <lambda, r> computed in reverse."""
+    n = 16
+    rng = np.random.default_rng(8)
+    cost = 1 + 0.5 * rng.uniform(size=(n, n))
+    P = O.make_problem('Isotropic', (n, n), [1. / n] * 2, cost=cost, seeds_idx=[[4, 9]])
+    u = P.solve_fm()
+    fl = P.flow(u)
+    r = rng.normal(size=(n, n, 1))
+    s = rng.normal(size=(n, n, 1))
+    du = P.forward_ad(u, fl, r)
+    lam = P.adjoint(u, fl, s)
+    assert np.sum(s * du) == pytest.approx(np.sum(lam * r), rel=1e-12)
