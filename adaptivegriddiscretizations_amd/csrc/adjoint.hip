@@ -136,6 +136,7 @@ template <class S> __device__ inline S init_val(S x) { return x == inf_v<S>() ? 
 
 template <class S, int D, int T0, int T1, int T2>
 __global__ void __launch_bounds__(T0* T1* T2) adj_update(LinArgs<S> a) {
+  static_assert((T0 * T1 * T2) % 64 == 0, "tiles are whole waves (pass_begin scans per wave)");
   using TL = Tile<D, T0, T1, T2>;
   constexpr int NI = TL::NI;
   __shared__ S l_s[NI];
@@ -144,7 +145,7 @@ __global__ void __launch_bounds__(T0* T1* T2) adj_update(LinArgs<S> a) {
   const int count = pv.count;
   const int32_t tag = pv.iter + 1;
   for (int li = blockIdx.x; li < count; li += gridDim.x) {
-    const int n_o = pv.list_in[li];
+    const int n_o = pass_entry(pv, li);
     int xo[3];
     tile_pos<D>(n_o, a.g, xo);
     const int64_t n_t = int64_t(n_o) * NI + tid;
@@ -182,19 +183,23 @@ __global__ void __launch_bounds__(T0* T1* T2) adj_update(LinArgs<S> a) {
 }
 
 template <int NI>
-__global__ void adj_init_list(const uint32_t* trigger, int32_t ntiles, int32_t* stamp,
-                              int32_t* list, int32_t* count) {
+__global__ void __launch_bounds__(64) adj_init_list(const uint32_t* trigger, int32_t ntiles,
+                                                   int32_t* stamp, int32_t* lists,
+                                                   int32_t* counts, int32_t shard_cap) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= ntiles) return;
   bool on = false;
-  const int64_t b0 = int64_t(t) * NI, b1 = b0 + NI;
-  for (int64_t b = b0; b < b1 && !on; ++b) on = (trigger[b >> 5] >> (b & 31)) & 1u;
-  if (on) enqueue(t, 0, stamp, list, count);
+  if (t < ntiles) {
+    const int64_t b0 = int64_t(t) * NI, b1 = b0 + NI;
+    for (int64_t b = b0; b < b1 && !on; ++b) on = (trigger[b >> 5] >> (b & 31)) & 1u;
+  }
+  const int shard = blockIdx.x % kShards;
+  enqueue_wave(on, on ? t : 0, t, 0, stamp, lists + int64_t(shard) * shard_cap,
+               counts + shard * kCountStride);
 }
 
 template <class S> struct LinSet {
   void (*update)(LinArgs<S>);
-  void (*init)(const uint32_t*, int32_t, int32_t*, int32_t*, int32_t*);
+  void (*init)(const uint32_t*, int32_t, int32_t*, int32_t*, int32_t*, int32_t);
   int ni;
 };
 
@@ -210,7 +215,6 @@ template <class S> static bool lin_lookup(int d, const int32_t* si, LinSet<S>& o
   if (d == 2) {
     const int key = si[0] * 100 + si[1];
     switch (key) {
-      case 404: out = make_lin<S, 2, 4, 4, 1>(); return true;
       case 808: out = make_lin<S, 2, 8, 8, 1>(); return true;
       case 1616: out = make_lin<S, 2, 16, 16, 1>(); return true;
       case 816: out = make_lin<S, 2, 8, 16, 1>(); return true;
@@ -303,16 +307,17 @@ static int linear_impl(bool forward, const agd_eikonal_desc* d, const S* values,
     a.niter_i = d->niter_i;
     // Dummy initialisation to +inf marks unvisited cells (read as 0).
     hipLaunchKernelGGL(fill_inf_kernel<S>, grid, block, 0, stream, N, a.lam);
-    auto init = [&](Workspace* ws, int32_t* list0, int32_t* counts) -> int {
+    auto init = [&](Workspace* ws) -> int {
       a.w.lists = ws->lists;
-      a.w.counts = counts;
+      a.w.counts = ws->counts;
       a.w.stamp = ws->stamp;
       a.w.hist = ws->hist;
       a.w.score = ws->score;
-      a.w.cap = int32_t(ws->cap);
+      a.w.tile_cap = int32_t(ws->tile_cap);
+      a.w.shard_cap = int32_t(ws->shard_cap);
       a.w.fixed = nullptr;
-      hipLaunchKernelGGL(ls.init, dim3(unsigned(ceil_div(g.ntiles, 256))), dim3(256), 0, stream,
-                         trig, g.ntiles, ws->stamp, list0, counts + 0);
+      hipLaunchKernelGGL(ls.init, dim3(unsigned(ceil_div(g.ntiles, 64))), dim3(64), 0, stream,
+                         trig, g.ntiles, ws->stamp, ws->lists, ws->counts, a.w.shard_cap);
       return AGD_OK;
     };
     auto launch = [&](int32_t koff, hipStream_t s) {
@@ -320,8 +325,8 @@ static int linear_impl(bool forward, const agd_eikonal_desc* d, const S* values,
       hipLaunchKernelGGL(ls.update, dim3(blocks), dim3(ls.ni), 0, s, a);
     };
     WorklistResult res;
-    status = run_worklist(g.ntiles, d->nitermax_o, d->check_interval, stream, init, launch, res,
-                          d->time_launches != 0, d->use_graph != 0);
+    status = run_worklist(g.ntiles, blocks, d->nitermax_o, d->check_interval, stream, init, launch,
+                          res, d->time_launches != 0, d->use_graph != 0);
     kernel_ms += res.kernel_ms;
     niter += res.niter;
     visits += res.visits;

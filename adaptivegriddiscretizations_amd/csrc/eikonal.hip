@@ -151,6 +151,7 @@ template <class S, int MODEL, int D, int T0, int T1, int T2, int VAR>
 __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
   constexpr bool GENERAL = VAR == VAR_GENERAL;
   constexpr bool WALLS = GENERAL;
+  static_assert((T0 * T1 * T2) % 64 == 0, "tiles are whole waves (pass_begin scans per wave)");
   using TL = Tile<D, T0, T1, T2>;
   using MT = ModelTraits<MODEL, D>;
   constexpr int NI = TL::NI;
@@ -169,7 +170,7 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
   const bool fim = a.solver == AGD_FIM;
 
   for (int li = blockIdx.x; li < count; li += gridDim.x) {
-    const int pk = pv.list_in[li];
+    const int pk = pass_entry(pv, li);
     int xo[3];
     unpack_tile<D>(pk, xo);
     const int n_o = tile_linear<D>(xo, a.g);
@@ -350,13 +351,17 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_scheme(SchemeArgs<S> a) {
 }
 
 // Initial list: tiles holding at least one trigger bit (or every tile).
+// 64-lane workgroups, one tile per lane, wave-aggregated appends into
+// sub-list blockIdx % kShards of pass 0.
 template <int D, int NI>
-__global__ void eik_init_list(const uint32_t* trigger, GridDesc g, int32_t all,
-                              int32_t* stamp, int32_t* list, int32_t* count) {
+__global__ void __launch_bounds__(64) eik_init_list(const uint32_t* trigger, GridDesc g, int32_t all,
+                                                   int32_t* stamp, int32_t* lists, int32_t* counts,
+                                                   int32_t shard_cap) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= g.ntiles) return;
-  bool on = all != 0;
-  if (!on) {
+  bool on = false;
+  int xo[3] = {0, 0, 0};
+  if (t < g.ntiles) {
+    on = all != 0;
     const int64_t b0 = int64_t(t) * NI, b1 = b0 + NI;
     for (int64_t b = b0; b < b1 && !on; ) {
       const int64_t wi = b >> 5;
@@ -368,10 +373,11 @@ __global__ void eik_init_list(const uint32_t* trigger, GridDesc g, int32_t all,
       on = (word & mask) != 0;
       b = lim;
     }
+    tile_pos<D>(t, g, xo);
   }
-  int xo[3];
-  tile_pos<D>(t, g, xo);
-  if (on) enqueue(t, 0, stamp, list, count, pack_tile<D>(xo));
+  const int shard = blockIdx.x % kShards;
+  enqueue_wave(on, on ? t : 0, pack_tile<D>(xo), 0, stamp, lists + int64_t(shard) * shard_cap,
+               counts + shard * kCountStride);
 }
 
 // Every tile, as packed coordinates (global-iteration passes).
@@ -594,7 +600,7 @@ template <class S> struct KernelSet {
   void (*scheme)(SchemeArgs<S>);
   void (*scheme_general)(SchemeArgs<S>);
   void (*flow)(FlowArgs<S>);
-  void (*init)(const uint32_t*, GridDesc, int32_t, int32_t*, int32_t*, int32_t*);
+  void (*init)(const uint32_t*, GridDesc, int32_t, int32_t*, int32_t*, int32_t*, int32_t);
   void (*all_tiles)(GridDesc, int32_t*, int32_t*);
   void (*wallpass)(GridDesc, const uint8_t*, uint8_t*);
   int ni;
@@ -633,7 +639,6 @@ static bool lookup_model(int d, const int32_t* si, KernelSet<S>& out) {
   const int key = tile_key(d, si);
   if (d == 2) {
     switch (key) {
-      case 404: out = make_set<S, MODEL, 2, 4, 4, 1>(); return true;
       case 808: out = make_set<S, MODEL, 2, 8, 8, 1>(); return true;
       case 1616: out = make_set<S, MODEL, 2, 16, 16, 1>(); return true;
       case 816: out = make_set<S, MODEL, 2, 8, 16, 1>(); return true;
@@ -702,7 +707,7 @@ static GridDesc make_grid(const agd_eikonal_desc* d) {
 static std::mutex g_ws_mutex;
 static std::unordered_map<int, Workspace> g_ws;
 
-int get_workspace(int64_t ntiles, int64_t nhist, Workspace*& out) {
+int get_workspace(int64_t ntiles, int64_t shard_cap, int64_t nhist, Workspace*& out) {
   int dev = 0;
   AGD_HIP_TRY(hipGetDevice(&dev));
   std::lock_guard<std::mutex> lock(g_ws_mutex);
@@ -710,21 +715,25 @@ int get_workspace(int64_t ntiles, int64_t nhist, Workspace*& out) {
   if (!w.ev0) {
     AGD_HIP_TRY(hipEventCreate(&w.ev0));
     AGD_HIP_TRY(hipEventCreate(&w.ev1));
-    AGD_HIP_TRY(hipHostMalloc((void**)&w.host_count, 64, hipHostMallocDefault));
+    AGD_HIP_TRY(hipHostMalloc((void**)&w.host_counts, kRow * sizeof(int32_t), hipHostMallocDefault));
     AGD_HIP_TRY(hipStreamCreateWithFlags(&w.capture_stream, hipStreamNonBlocking));
-    AGD_HIP_TRY(hipMalloc((void**)&w.counts, 8 * sizeof(int32_t)));
+    AGD_HIP_TRY(hipMalloc((void**)&w.counts, kCountsLen * sizeof(int32_t)));
   }
-  if (w.cap < ntiles) {
+  if (w.tile_cap < ntiles) {
     if (w.stamp) {
       AGD_HIP_TRY(hipFree(w.stamp));
-      AGD_HIP_TRY(hipFree(w.lists));
       AGD_HIP_TRY(hipFree(w.score));
     }
     const int64_t cap = std::max<int64_t>(ntiles, 1024);
     AGD_HIP_TRY(hipMalloc((void**)&w.stamp, cap * sizeof(int32_t)));
-    AGD_HIP_TRY(hipMalloc((void**)&w.lists, 2 * cap * sizeof(int32_t)));
     AGD_HIP_TRY(hipMalloc((void**)&w.score, 2 * cap * sizeof(int32_t)));
-    w.cap = cap;
+    w.tile_cap = cap;
+  }
+  if (w.shard_cap < shard_cap) {
+    if (w.lists) AGD_HIP_TRY(hipFree(w.lists));
+    const int64_t cap = std::max<int64_t>(shard_cap, 256);
+    AGD_HIP_TRY(hipMalloc((void**)&w.lists, 2 * kShards * cap * sizeof(int32_t)));
+    w.shard_cap = cap;
   }
   if (w.hist_cap < nhist) {
     if (w.hist) AGD_HIP_TRY(hipFree(w.hist));
@@ -785,25 +794,26 @@ static int solve_impl(const agd_eikonal_desc* d, const agd_eikonal_io* io,
   a.solver = global ? AGD_AGSI : d->solver;
   a.fim_front = d->fim_front_width;
 
-  auto init = [&](Workspace* ws, int32_t* list0, int32_t* counts) -> int {
+  auto init = [&](Workspace* ws) -> int {
     a.w.lists = ws->lists;
-    a.w.counts = counts;
+    a.w.counts = ws->counts;
     a.w.stamp = ws->stamp;
     a.w.hist = ws->hist;
     a.w.score = ws->score;
-    a.w.cap = int32_t(ws->cap);
+    a.w.tile_cap = int32_t(ws->tile_cap);
+    a.w.shard_cap = int32_t(ws->shard_cap);
     a.w.fixed = nullptr;
-    AGD_HIP_TRY(hipMemsetAsync(ws->score, 0x7f, size_t(2 * ws->cap) * sizeof(int32_t), stream));
+    AGD_HIP_TRY(hipMemsetAsync(ws->score, 0x7f, size_t(2 * ws->tile_cap) * sizeof(int32_t), stream));
     if (global) {
       // Every pass visits the full list (kept in the score buffer, unused
       // by this solver); the AGSI tags only decide when to stop.
       hipLaunchKernelGGL(ks.all_tiles, dim3(unsigned(ceil_div(g.ntiles, 256))), dim3(256), 0,
-                         stream, g, ws->score, counts + 3);
+                         stream, g, ws->score, ws->counts + kFixedIdx);
       a.w.fixed = ws->score;
     } else {
       const uint32_t* trig = io->trigger ? io->trigger : io->seeds;
-      hipLaunchKernelGGL(ks.init, dim3(unsigned(ceil_div(g.ntiles, 256))), dim3(256), 0, stream,
-                         trig, g, d->activate_all, ws->stamp, list0, counts + 0);
+      hipLaunchKernelGGL(ks.init, dim3(unsigned(ceil_div(g.ntiles, 64))), dim3(64), 0, stream, trig,
+                         g, d->activate_all, ws->stamp, ws->lists, ws->counts, a.w.shard_cap);
     }
     return AGD_OK;
   };
@@ -813,8 +823,8 @@ static int solve_impl(const agd_eikonal_desc* d, const agd_eikonal_io* io,
     hipLaunchKernelGGL(kern, dim3(blocks), dim3(ni), 0, s, a);
   };
   WorklistResult r;
-  int st = run_worklist(g.ntiles, d->nitermax_o, d->check_interval, stream, init, launch, r,
-                        d->time_launches != 0, d->use_graph != 0);
+  int st = run_worklist(g.ntiles, blocks, d->nitermax_o, d->check_interval, stream, init, launch,
+                        r, d->time_launches != 0, d->use_graph != 0);
   if (st) return st;
   if (stats) {
     stats->kernel_ms = r.kernel_ms;

@@ -20,14 +20,25 @@
 
 namespace agd {
 
+// The list of a pass is split into kShards sub-lists: a workgroup appends to
+// sub-list blockIdx % kShards.  One counter shared by the whole grid
+// serialises its atomics (about 11 ns each on MI355X, i.e. ~100 us for 8k
+// appending waves); 64 counters on separate cache lines do not.
+constexpr int kShards = 64;
+constexpr int kCountStride = 16;                         // one 64-byte line per counter
+constexpr int kRow = kShards * kCountStride;             // one pass's counters
+constexpr int kFixedIdx = 3 * kRow;                      // fixed-list length (global iteration)
+constexpr int kBaseIdx = 3 * kRow + kCountStride;        // pass base
+constexpr int kCountsLen = 3 * kRow + 2 * kCountStride;
+
 struct Workspace {
   int32_t* stamp = nullptr;
-  int32_t* lists = nullptr;   // 2 * cap
-  int32_t* counts = nullptr;  // 8
+  int32_t* lists = nullptr;   // 2 * kShards * shard_cap
+  int32_t* counts = nullptr;  // kCountsLen
   int32_t* hist = nullptr;    // hist_cap: list length of every pass
-  int32_t* score = nullptr;   // 2 * cap (FIM scores)
-  int32_t* host_count = nullptr;  // pinned
-  int64_t cap = 0, hist_cap = 0;
+  int32_t* score = nullptr;   // 2 * tile_cap (FIM scores; fixed list of global iteration)
+  int32_t* host_counts = nullptr;  // pinned, kRow
+  int64_t tile_cap = 0, shard_cap = 0, hist_cap = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // Private stream used only to record pass chunks into a graph (the legacy
   // null stream, torch's default, cannot be captured).  Nothing executes on
@@ -36,8 +47,20 @@ struct Workspace {
 };
 
 // Per-device cached workspace (grown on demand).  Defined in eikonal.hip.
-int get_workspace(int64_t ntiles, int64_t nhist, Workspace*& out);
+int get_workspace(int64_t ntiles, int64_t shard_cap, int64_t nhist, Workspace*& out);
 int blocks_for(int ntiles, int lanes_per_block);
+
+// Capacity of one sub-list.  A pass appends at most 2d+1 <= 7 tiles per
+// tile visit and the workgroups b = s mod kShards of a grid of G visit at
+// most ceil(G/kShards) * ceil(n/G) tiles; the initial list (64-lane
+// workgroups over the tiles) puts at most ntiles/kShards + 64 in a
+// sub-list; no sub-list exceeds ntiles (one append per tile and pass).
+inline int64_t shard_capacity(int64_t ntiles, int64_t grid) {
+  auto cd = [](int64_t a, int64_t b) { return (a + b - 1) / b; };
+  grid = std::max<int64_t>(grid, 1);
+  const int64_t visits = cd(ntiles, kShards) + cd(grid, kShards) + cd(ntiles, grid) + 1;
+  return std::min<int64_t>(ntiles, 7 * visits + 64);
+}
 
 // Per-lane append of `entry` (default: the tile index) if `tile` was not yet
 // stamped with `tag`.
@@ -66,54 +89,84 @@ __device__ inline void enqueue_wave(bool want, int32_t tile, int32_t entry, int3
   if (first) list[base + __popcll(mask & ((uint64_t(1) << lane) - 1))] = entry;
 }
 
-// Worklist state seen by a pass kernel.  counts[0..2] are the rotating list
-// lengths, counts[3] the length of a fixed list (global iteration),
-// counts[4] the pass base.  A kernel runs pass iter = counts[4] + koff, so
-// a chunk of passes with koff = 0..G-1 followed by an advance of counts[4]
-// by G can be captured once as a graph and replayed.
+// Worklist state seen by a pass kernel.  counts holds three rows of kShards
+// rotating sub-list lengths, the fixed-list length and the pass base.  A
+// kernel runs pass iter = counts[kBaseIdx] + koff, so a chunk of passes with
+// koff = 0..G-1 followed by an advance of the base by G can be captured once
+// as a graph and replayed.
 struct WorkCtx {
-  int32_t* lists;          // 2 * cap entries
-  int32_t* counts;         // 8
+  int32_t* lists;          // 2 * kShards * shard_cap entries
+  int32_t* counts;         // kCountsLen
   int32_t* stamp;          // ntiles
   int32_t* hist;           // pass histogram
-  int32_t* score;          // 2 * cap (FIM scores)
-  const int32_t* fixed;    // non-null: every pass reads this list (length counts[3])
-  int32_t cap;
+  int32_t* score;          // 2 * tile_cap (FIM scores)
+  const int32_t* fixed;    // non-null: every pass reads this list (length counts[kFixedIdx])
+  int32_t tile_cap;
+  int32_t shard_cap;
   int32_t koff;
 };
 
 struct PassView {
-  int32_t iter, count;
-  const int32_t* list_in;
-  int32_t* list_out;
-  int32_t* count_out;
+  int32_t iter, count;     // pass number, total list length
+  int32_t incl;            // this lane's inclusive prefix of the sub-list lengths
+  const int32_t* list_in;  // sub-lists of this pass (or the fixed list)
+  bool fixed;
+  int32_t shard_cap;
+  int32_t* list_out;       // this workgroup's sub-list for the next pass
+  int32_t* count_out;      // ... and its counter
   const int32_t* score_prev;
   int32_t* score_next;
 };
 
-// One scalar load brings the counters and the pass base together.  Block 0
-// zeroes the counter of pass iter+2 and records the list length.
+// Every wave loads the three counter rows and the pass base at once (no
+// dependent load), keeps its pass's row as a prefix sum across lanes (lane
+// s = sub-list s).  Block 0 zeroes the row of pass iter+2 and records the
+// list length.  Must be called by every thread of the block.
 __device__ inline PassView pass_begin(const WorkCtx& c) {
-  int4 q0 = *reinterpret_cast<const int4*>(c.counts);
-  int4 q1 = *reinterpret_cast<const int4*>(c.counts + 4);
-  const int cnt[5] = {q0.x, q0.y, q0.z, q0.w, q1.x};
+  const int lane = threadIdx.x & 63;
+  const int c0 = c.counts[0 * kRow + lane * kCountStride];
+  const int c1 = c.counts[1 * kRow + lane * kCountStride];
+  const int c2 = c.counts[2 * kRow + lane * kCountStride];
+  const int base = c.counts[kBaseIdx];
+  const int fixed_n = c.counts[kFixedIdx];
   PassView v;
-  v.iter = cnt[4] + c.koff;
+  v.iter = base + c.koff;
   const int r = v.iter % 3;
-  v.count = c.fixed ? cnt[3] : (r == 0 ? cnt[0] : (r == 1 ? cnt[1] : cnt[2]));
-  v.list_in = c.fixed ? c.fixed : c.lists + (v.iter & 1) * c.cap;
-  v.list_out = c.lists + ((v.iter + 1) & 1) * c.cap;
-  v.count_out = c.counts + (v.iter + 1) % 3;
-  v.score_prev = c.score + ((v.iter + 1) & 1) * c.cap;
-  v.score_next = c.score + (v.iter & 1) * c.cap;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    c.counts[(v.iter + 2) % 3] = 0;
-    c.hist[v.iter] = v.count;
+  const int mine = r == 0 ? c0 : (r == 1 ? c1 : c2);
+  int incl = mine;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int o = __shfl_up(incl, d);
+    if (lane >= d) incl += o;
+  }
+  v.incl = incl;
+  v.fixed = c.fixed != nullptr;
+  v.count = v.fixed ? fixed_n : __shfl(incl, 63);
+  v.shard_cap = c.shard_cap;
+  v.list_in = v.fixed ? c.fixed : c.lists + int64_t(v.iter & 1) * kShards * c.shard_cap;
+  const int shard = blockIdx.x % kShards;
+  v.list_out = c.lists + int64_t((v.iter + 1) & 1) * kShards * c.shard_cap +
+               int64_t(shard) * c.shard_cap;
+  v.count_out = c.counts + ((v.iter + 1) % 3) * kRow + shard * kCountStride;
+  v.score_prev = c.score + ((v.iter + 1) & 1) * c.tile_cap;
+  v.score_next = c.score + (v.iter & 1) * c.tile_cap;
+  if (blockIdx.x == 0 && threadIdx.x < 64) {
+    c.counts[((v.iter + 2) % 3) * kRow + lane * kCountStride] = 0;
+    if (lane == 0) c.hist[v.iter] = v.count;
   }
   return v;
 }
 
-__global__ inline void advance_kernel(int32_t* counts, int32_t g) { counts[4] += g; }
+// Entry li (0 <= li < count, the same in every lane of the wave) of the
+// pass's list: sub-list s = number of lanes whose inclusive prefix is <= li.
+__device__ inline int32_t pass_entry(const PassView& v, int li) {
+  if (v.fixed) return v.list_in[li];
+  const int s = __popcll(__ballot(v.incl <= li));
+  const int excl = s == 0 ? 0 : __shfl(v.incl, s - 1);
+  return v.list_in[int64_t(s) * v.shard_cap + (li - excl)];
+}
+
+__global__ inline void advance_kernel(int32_t* counts, int32_t g) { counts[kBaseIdx] += g; }
 
 struct WorklistResult {
   int64_t passes = 0;      // launches issued
@@ -125,27 +178,26 @@ struct WorklistResult {
 };
 
 // Runs passes until the list empties or nitermax passes were issued.
-// `init(ws, list0, counts)` enqueues the initial list into list0 /
-// counts[0]; `launch(koff, stream)` enqueues one pass of iteration
-// counts[4] + koff on `stream`.
+// `init(ws)` enqueues the initial list into the sub-lists of pass 0 (list
+// row 0, counter row 0); `launch(koff, stream)` enqueues one pass of
+// iteration base + koff on `stream`, on a grid of `grid` workgroups.
 // With `graph`, chunks of `check_interval` passes are captured once into a
 // hipGraph and replayed; the host reads the next list length after each
 // chunk.  Optional per-pass timing records an event pair around every pass
 // (and then launches eagerly).
 template <class Init, class Launch>
-int run_worklist(int32_t ntiles, int64_t nitermax, int check_interval, hipStream_t stream,
-                 Init init, Launch launch, WorklistResult& res, bool time_launches = false,
-                 bool graph = false) {
+int run_worklist(int32_t ntiles, int32_t grid, int64_t nitermax, int check_interval,
+                 hipStream_t stream, Init init, Launch launch, WorklistResult& res,
+                 bool time_launches = false, bool graph = false) {
   Workspace* ws = nullptr;
   const int ci = std::max(1, check_interval);
-  int st = get_workspace(ntiles, nitermax + ci + 8, ws);
+  int st = get_workspace(ntiles, shard_capacity(ntiles, grid), nitermax + ci + 8, ws);
   if (st) return st;
-  int32_t* list0 = ws->lists;
   int32_t* counts = ws->counts;
   AGD_HIP_TRY(hipMemsetAsync(ws->stamp, 0xff, size_t(ntiles) * sizeof(int32_t), stream));
-  AGD_HIP_TRY(hipMemsetAsync(counts, 0, 8 * sizeof(int32_t), stream));
+  AGD_HIP_TRY(hipMemsetAsync(counts, 0, kCountsLen * sizeof(int32_t), stream));
   AGD_HIP_TRY(hipMemsetAsync(ws->hist, 0, size_t(nitermax + ci + 8) * sizeof(int32_t), stream));
-  st = init(ws, list0, counts);
+  st = init(ws);
   if (st) return st;
   AGD_HIP_TRY(hipGetLastError());
 
@@ -215,15 +267,17 @@ int run_worklist(int32_t ntiles, int64_t nitermax, int check_interval, hipStream
       ++k;
     }
     if (graph || k % ci == 0 || k >= nitermax) {
-      AGD_HIP_TRY(hipMemcpyAsync(ws->host_count, counts + (k % 3), sizeof(int32_t),
+      AGD_HIP_TRY(hipMemcpyAsync(ws->host_counts, counts + (k % 3) * kRow, kRow * sizeof(int32_t),
                                  hipMemcpyDeviceToHost, stream));
       AGD_HIP_TRY(hipStreamSynchronize(stream));
+      int64_t next = 0;
+      for (int sh = 0; sh < kShards; ++sh) next += ws->host_counts[sh * kCountStride];
       if (time_launches) {
         const int st2 = drain();
         if (st2) return st2;
       }
       pending = 0;
-      if (*ws->host_count == 0) {
+      if (next == 0) {
         converged = true;
         break;
       }
