@@ -44,6 +44,11 @@ class EikonalIO(ctypes.Structure):
     ]
 
 
+class BatchInstance(ctypes.Structure):
+    _fields_ = [("shape", c_int64 * 3), ("iso_weights", c_double * 3), ("atol", c_double),
+                ("rtol", c_double)]
+
+
 class EikonalStats(ctypes.Structure):
     _fields_ = [
         ("niter_o", c_int64), ("tile_updates", c_int64), ("cell_updates", c_int64),
@@ -72,6 +77,8 @@ SIGNATURES = {
                                     POINTER(EikonalStats), c_void_p]),
     "agd_eikonal_flow": (c_int32, [POINTER(EikonalDesc), POINTER(EikonalIO), POINTER(FlowIO),
                                    c_void_p]),
+    "agd_eikonal_solve_batched": (c_int32, [POINTER(EikonalDesc), c_int32, POINTER(BatchInstance),
+                                            POINTER(EikonalIO), POINTER(EikonalStats), c_void_p]),
     "agd_scheme_supported": (c_int32, [c_int32, c_int32, c_int32_p]),
     "agd_eikonal_scheme": (c_int32, [POINTER(EikonalDesc), POINTER(EikonalIO), c_void_p]),
     "agd_eikonal_adjoint": (c_int32, [POINTER(EikonalDesc), c_void_p, c_void_p, c_void_p,

@@ -30,6 +30,7 @@
 
 #include <algorithm>
 #include <mutex>
+#include <type_traits>
 #include <unordered_map>
 #include <vector>
 
@@ -38,6 +39,16 @@
 #include "worklist.hpp"
 
 namespace agd {
+
+// One instance of a batched solve: its own grid and constants; its tiles
+// occupy [tile_base, tile_base + g.ntiles) of the batch's tile space.
+struct InstDesc {
+  GridDesc g;
+  int32_t tile_base;
+  int32_t pad;
+  double iso_w[3];
+  double atol, rtol;
+};
 
 template <class S> struct EikArgs {
   S* u;
@@ -55,6 +66,9 @@ template <class S> struct EikArgs {
   int32_t fim_front;
   // worklist; FIM scores are packed (pass << 8) | score, two arrays by parity
   WorkCtx w;
+  // batched solves: instance table and tile -> instance map
+  const InstDesc* inst;
+  const int32_t* tile_inst;
 };
 
 __device__ inline int32_t read_score(const int32_t* arr, int32_t tile, int32_t iter) {
@@ -81,7 +95,8 @@ template <int NI> __device__ inline bool block_any(bool p) {
 template <class S, int MODEL, int D, class TL, bool GENERAL>
 __device__ inline void resolve_codes(const Stencil<S, MODEL, D>& st, const int* xi, const int* xt,
                                      const int* xo, int wx, const uint8_t* wall_s,
-                                     const uint8_t* wall, const GridDesc& g, int* code) {
+                                     const uint8_t* wall, const GridDesc& g, int* code,
+                                     int tile_base = 0) {
   constexpr int NS = ModelTraits<MODEL, D>::NSYM;
   constexpr int NI = TL::NI;
 #pragma unroll
@@ -112,7 +127,7 @@ __device__ inline void resolve_codes(const Stencil<S, MODEL, D>& st, const int* 
         int to = yo[0];
 #pragma unroll
         for (int c = 1; c < D; ++c) to = to * g.shape_o[c] + yo[c];
-        code[j] = !dom ? -1 : (local ? -2 - li : to * NI + li);
+        code[j] = !dom ? -1 : (local ? -2 - li : (tile_base + to) * NI + li);
       } else {
         bool vis = true;
         if (wall && wx != 255) {
@@ -144,13 +159,16 @@ __device__ inline void resolve_codes(const Stencil<S, MODEL, D>& st, const int* 
 // Kernel variants: VAR_FAST assumes no walls and no periodic axes, so its
 // neighbour indexing carries no wrap-around code; VAR_GENERAL handles both;
 // VAR_PRE reads stencil weights and neighbour codes precomputed once per
-// solve (any walls / periodicity are already resolved in the codes).
-enum { VAR_FAST = 0, VAR_GENERAL = 1, VAR_PRE = 2 };
+// solve (any walls / periodicity are already resolved in the codes);
+// VAR_BATCH is VAR_FAST over the concatenated tiles of several instances,
+// each tile looking up its instance's grid and constants.
+enum { VAR_FAST = 0, VAR_GENERAL = 1, VAR_PRE = 2, VAR_BATCH = 3 };
 
 template <class S, int MODEL, int D, int T0, int T1, int T2, int VAR>
 __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
   constexpr bool GENERAL = VAR == VAR_GENERAL;
   constexpr bool WALLS = GENERAL;
+  constexpr bool BATCH = VAR == VAR_BATCH;
   static_assert((T0 * T1 * T2) % 64 == 0, "tiles are whole waves (pass_begin scans per wave)");
   using TL = Tile<D, T0, T1, T2>;
   using MT = ModelTraits<MODEL, D>;
@@ -172,8 +190,32 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
   for (int li = blockIdx.x; li < count; li += gridDim.x) {
     const int pk = pass_entry(pv, li);
     int xo[3];
-    unpack_tile<D>(pk, xo);
-    const int n_o = tile_linear<D>(xo, a.g);
+    // Grid and constants of the tile's instance (the solve's own when not
+    // batched).  Batched list entries are tile ids, else packed coordinates.
+    GridDesc gi;
+    int tbase = 0;
+    S atol, rtol, iso_w[3];
+    int n_o;
+    if constexpr (BATCH) {
+      n_o = __builtin_amdgcn_readfirstlane(pk);
+      const int ii = __builtin_amdgcn_readfirstlane(a.tile_inst[n_o]);
+      const InstDesc* I = a.inst + ii;
+      gi = I->g;
+      tbase = I->tile_base;
+      atol = S(I->atol);
+      rtol = S(I->rtol);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) iso_w[k] = S(I->iso_w[k]);
+      tile_pos<D>(n_o - tbase, gi, xo);
+    } else {
+      gi = a.g;
+      atol = a.atol;
+      rtol = a.rtol;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) iso_w[k] = a.iso_w[k];
+      unpack_tile<D>(pk, xo);
+      n_o = tile_linear<D>(xo, a.g);
+    }
     int xt[3] = {0, 0, 0};
 #pragma unroll
     for (int k = 0; k < D; ++k) xt[k] = xo[k] * TL::T[k] + xi[k];
@@ -183,7 +225,15 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
     // (lanes 0..2D of wave 0), and their FIM scores of the previous pass.
     int nb = -1, nbp = pk, sc = 0;
     if (tid < 2 * D + 1) {
-      nb = (tid == 2 * D) ? n_o : tile_neighbor<D>(xo, tid >> 1, 2 * (tid & 1) - 1, a.g, &nbp);
+      if (tid == 2 * D) {
+        nb = n_o;
+      } else {
+        nb = tile_neighbor<D>(xo, tid >> 1, 2 * (tid & 1) - 1, gi, &nbp);
+        if (BATCH && nb >= 0) {
+          nb += tbase;
+          nbp = nb;
+        }
+      }
       if (fim && nb >= 0) sc = read_score(pv.score_prev, nb, pv.iter - 1);
     }
 
@@ -217,9 +267,9 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
         geo[k] = gk[n_t];
       }
       Stencil<S, MODEL, D> st;
-      make_stencil<S, MODEL, D>(geo, a.iso_w, st);
+      make_stencil<S, MODEL, D>(geo, iso_w, st);
       if constexpr (WALLS) __syncthreads();  // wall_s complete before the visibility walks
-      resolve_codes<S, MODEL, D, TL, GENERAL>(st, xi, xt, xo, wx, wall_s, a.wall, a.g, code);
+      resolve_codes<S, MODEL, D, TL, GENERAL>(st, xi, xt, xo, wx, wall_s, a.wall, gi, code, tbase);
 #pragma unroll
       for (int k = 0; k < NS; ++k) {
         sw[k] = st.w[k];
@@ -265,7 +315,7 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
     const S un = u_s[tid];
     a.u[n_t] = un;
     const S diff = abs_(u_old - un);
-    const S tol = a.atol + a.rtol * abs_(un);
+    const S tol = atol + rtol * abs_(un);
     const bool changed = !(isnan_(diff) || diff <= tol);
     const bool any = block_any<NI>(changed);
 
@@ -378,6 +428,47 @@ __global__ void __launch_bounds__(64) eik_init_list(const uint32_t* trigger, Gri
   const int shard = blockIdx.x % kShards;
   enqueue_wave(on, on ? t : 0, pack_tile<D>(xo), 0, stamp, lists + int64_t(shard) * shard_cap,
                counts + shard * kCountStride);
+}
+
+// Batched solves: the same with tile ids as entries.
+template <int NI>
+__global__ void __launch_bounds__(64) init_list_ids(const uint32_t* trigger, int32_t ntiles, int32_t all,
+                                                   int32_t* stamp, int32_t* lists, int32_t* counts,
+                                                   int32_t shard_cap) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  bool on = false;
+  if (t < ntiles) {
+    on = all != 0;
+    const int64_t b0 = int64_t(t) * NI, b1 = b0 + NI;
+    for (int64_t b = b0; b < b1 && !on; b += 32) {
+      // NI is a multiple of 64: whole words
+      on = trigger[b >> 5] != 0u;
+    }
+  }
+  const int shard = blockIdx.x % kShards;
+  enqueue_wave(on, on ? t : 0, t, 0, stamp, lists + int64_t(shard) * shard_cap,
+               counts + shard * kCountStride);
+}
+
+// Every tile id (batched global-iteration passes).
+__global__ void all_ids_kernel(int32_t ntiles, int32_t* list, int32_t* count) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < ntiles) list[t] = t;
+  if (t == 0) *count = ntiles;
+}
+
+// tile -> instance map: the last instance whose tile_base is <= t.
+__global__ void tile_inst_kernel(int32_t ninst, const InstDesc* inst, int32_t ntiles,
+                                 int32_t* tile_inst) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntiles) return;
+  int lo = 0, hi = ninst - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (inst[mid].tile_base <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  tile_inst[t] = lo;
 }
 
 // Every tile, as packed coordinates (global-iteration passes).
@@ -597,6 +688,8 @@ template <class S> struct KernelSet {
   void (*update)(EikArgs<S>);
   void (*update_walls)(EikArgs<S>);
   void (*update_pre)(EikArgs<S>);          // null: no precomputed-scheme variant
+  void (*update_batch)(EikArgs<S>);        // null: no batched variant (FP64)
+  void (*init_ids)(const uint32_t*, int32_t, int32_t, int32_t*, int32_t*, int32_t*, int32_t);
   void (*scheme)(SchemeArgs<S>);
   void (*scheme_general)(SchemeArgs<S>);
   void (*flow)(FlowArgs<S>);
@@ -611,6 +704,11 @@ KernelSet<S> make_set() {
   KernelSet<S> k;
   k.update = eik_update<S, MODEL, D, T0, T1, T2, VAR_FAST>;
   k.update_walls = eik_update<S, MODEL, D, T0, T1, T2, VAR_GENERAL>;
+  if constexpr (std::is_same<S, float>::value)
+    k.update_batch = eik_update<S, MODEL, D, T0, T1, T2, VAR_BATCH>;
+  else
+    k.update_batch = nullptr;
+  k.init_ids = init_list_ids<T0 * T1 * T2>;
   // Precomputed schemes pay off where the stencil is costly to build
   // (Selling decomposition); the drift of Rander is not stored.
   if constexpr (MODEL == AGD_RIEMANN) {
@@ -867,6 +965,143 @@ static int flow_impl(const agd_eikonal_desc* d, const agd_eikonal_io* io, const 
   return AGD_OK;
 }
 
+// Per-device cache of the instance table and tile -> instance map.
+struct BatchBuffers {
+  InstDesc* inst = nullptr;
+  int32_t* tile_inst = nullptr;
+  int64_t inst_cap = 0, tile_cap = 0;
+};
+static std::mutex g_batch_mutex;
+static std::unordered_map<int, BatchBuffers> g_batch;
+
+template <class S>
+static int solve_batched_impl(const agd_eikonal_desc* d, int32_t ninst,
+                              const agd_batch_instance* bi, const agd_eikonal_io* io,
+                              agd_eikonal_stats* stats, hipStream_t stream) {
+  KernelSet<S> ks;
+  AGD_REQUIRE(lookup<S>(d->model, d->ndim, d->shape_i, ks), "unsupported tile shape for this model");
+  AGD_REQUIRE(ks.update_batch, "batched solves are FP32 only");
+  AGD_REQUIRE(ninst >= 1 && bi, "at least one instance is required");
+  AGD_REQUIRE(io && io->values && io->rhs && io->seeds, "values, rhs and seeds are required");
+  AGD_REQUIRE(d->model == AGD_ISOTROPIC || io->geom, "geometry required for this model");
+  AGD_REQUIRE(!io->wall_dist && !io->scheme_weights && !io->scheme_codes,
+              "batched solves take no walls and no precomputed scheme");
+  AGD_REQUIRE(d->solver != AGD_FIM || d->fim_front_width >= 2, "fim_front_width must be >= 2");
+  for (int k = 0; k < d->ndim; ++k) AGD_REQUIRE(!d->periodic[k], "batched solves are not periodic");
+  const int ni = ks.ni;
+  std::vector<InstDesc> h(static_cast<size_t>(ninst));
+  int64_t tiles = 0;
+  for (int i = 0; i < ninst; ++i) {
+    InstDesc& I = h[size_t(i)];
+    I = InstDesc{};
+    I.g.ntiles = 1;
+    for (int k = 0; k < 3; ++k) {
+      const int64_t n = k < d->ndim ? bi[i].shape[k] : 1;
+      AGD_REQUIRE(n >= 1 && n < (1 << 30), "instance shape out of range");
+      const int64_t no = ceil_div(n, k < d->ndim ? d->shape_i[k] : 1);
+      AGD_REQUIRE(no <= (d->ndim == 2 ? 32768 : 1024), "too many tiles along an axis");
+      I.g.shape_tot[k] = int32_t(n);
+      I.g.shape_o[k] = int32_t(no);
+      I.g.ntiles *= int32_t(no);
+      I.iso_w[k] = bi[i].iso_weights[k];
+    }
+    I.g.periodic = 0;
+    I.g.size_tot = int64_t(I.g.ntiles) * ni;
+    I.tile_base = int32_t(tiles);
+    I.atol = bi[i].atol;
+    I.rtol = bi[i].rtol;
+    tiles += I.g.ntiles;
+    AGD_REQUIRE(tiles * ni < (int64_t(1) << 31), "batches of 2^31 padded cells or more are not supported");
+  }
+  GridDesc gb{};
+  gb.ntiles = int32_t(tiles);
+  gb.size_tot = tiles * ni;
+  for (int k = 0; k < 3; ++k) gb.shape_tot[k] = gb.shape_o[k] = 1;
+
+  BatchBuffers* bb = nullptr;
+  {
+    int dev = 0;
+    AGD_HIP_TRY(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(g_batch_mutex);
+    bb = &g_batch[dev];
+    if (bb->inst_cap < ninst) {
+      if (bb->inst) AGD_HIP_TRY(hipFree(bb->inst));
+      AGD_HIP_TRY(hipMalloc((void**)&bb->inst, size_t(ninst) * sizeof(InstDesc)));
+      bb->inst_cap = ninst;
+    }
+    if (bb->tile_cap < tiles) {
+      if (bb->tile_inst) AGD_HIP_TRY(hipFree(bb->tile_inst));
+      AGD_HIP_TRY(hipMalloc((void**)&bb->tile_inst, size_t(tiles) * sizeof(int32_t)));
+      bb->tile_cap = tiles;
+    }
+  }
+  AGD_HIP_TRY(hipMemcpyAsync(bb->inst, h.data(), size_t(ninst) * sizeof(InstDesc),
+                             hipMemcpyHostToDevice, stream));
+  hipLaunchKernelGGL(tile_inst_kernel, dim3(unsigned(ceil_div(tiles, 256))), dim3(256), 0, stream,
+                     ninst, bb->inst, int32_t(tiles), bb->tile_inst);
+  AGD_HIP_TRY(hipGetLastError());
+  AGD_HIP_TRY(hipStreamSynchronize(stream));  // h is released on return
+
+  const int blocks = blocks_for(int(tiles), ni);
+  const bool global = d->solver == AGD_GLOBAL;
+  EikArgs<S> a{};
+  a.u = static_cast<S*>(io->values);
+  a.geom = static_cast<const S*>(io->geom);
+  a.rhs = static_cast<const S*>(io->rhs);
+  a.seeds = io->seeds;
+  a.g = gb;
+  a.niter_i = d->niter_i;
+  a.solver = global ? AGD_AGSI : d->solver;
+  a.fim_front = d->fim_front_width;
+  a.inst = bb->inst;
+  a.tile_inst = bb->tile_inst;
+  auto init = [&](Workspace* ws) -> int {
+    a.w.lists = ws->lists;
+    a.w.counts = ws->counts;
+    a.w.stamp = ws->stamp;
+    a.w.hist = ws->hist;
+    a.w.score = ws->score;
+    a.w.tile_cap = int32_t(ws->tile_cap);
+    a.w.shard_cap = int32_t(ws->shard_cap);
+    a.w.fixed = nullptr;
+    AGD_HIP_TRY(hipMemsetAsync(ws->score, 0x7f, size_t(2 * ws->tile_cap) * sizeof(int32_t), stream));
+    if (global) {
+      hipLaunchKernelGGL(all_ids_kernel, dim3(unsigned(ceil_div(tiles, 256))), dim3(256), 0, stream,
+                         int32_t(tiles), ws->score, ws->counts + kFixedIdx);
+      a.w.fixed = ws->score;
+    } else {
+      const uint32_t* trig = io->trigger ? io->trigger : io->seeds;
+      hipLaunchKernelGGL(ks.init_ids, dim3(unsigned(ceil_div(tiles, 64))), dim3(64), 0, stream, trig,
+                         int32_t(tiles), d->activate_all, ws->stamp, ws->lists, ws->counts,
+                         a.w.shard_cap);
+    }
+    return AGD_OK;
+  };
+  auto launch = [&](int32_t koff, hipStream_t s) {
+    a.w.koff = koff;
+    hipLaunchKernelGGL(ks.update_batch, dim3(blocks), dim3(ni), 0, s, a);
+  };
+  WorklistResult r;
+  int st = run_worklist(int32_t(tiles), blocks, d->nitermax_o, d->check_interval, stream, init,
+                        launch, r, d->time_launches != 0, d->use_graph != 0);
+  if (st) return st;
+  if (stats) {
+    stats->kernel_ms = r.kernel_ms;
+    stats->niter_o = r.niter;
+    stats->tile_updates = r.visits;
+    stats->cell_updates = r.visits * ni;
+    stats->launches = r.passes;
+    stats->gpu_ms = r.ms;
+    stats->converged = r.converged ? 1 : 0;
+    stats->grid_blocks = blocks;
+  }
+  if (!r.converged) {
+    set_error("batched eikonal solve did not converge within nitermax_o passes");
+    return AGD_NOCONV;
+  }
+  return AGD_OK;
+}
+
 template <class S>
 static int scheme_impl(const agd_eikonal_desc* d, const agd_eikonal_io* io, hipStream_t stream) {
   KernelSet<S> ks;
@@ -893,6 +1128,16 @@ static int scheme_impl(const agd_eikonal_desc* d, const agd_eikonal_io* io, hipS
 using namespace agd;
 
 extern "C" {
+
+int agd_eikonal_solve_batched(const agd_eikonal_desc* desc, int32_t ninst,
+                              const agd_batch_instance* instances, const agd_eikonal_io* io,
+                              agd_eikonal_stats* stats, void* stream) {
+  int st = check_desc(desc);
+  if (st) return st;
+  if (desc->scalar == AGD_F32)
+    return solve_batched_impl<float>(desc, ninst, instances, io, stats, as_stream(stream));
+  return solve_batched_impl<double>(desc, ninst, instances, io, stats, as_stream(stream));
+}
 
 int agd_scheme_supported(int32_t model, int32_t ndim, const int32_t* shape_i) {
   KernelSet<float> ks;

@@ -82,6 +82,35 @@ def isotropic2_bumps(n=512, seed=0, nbumps=32):
                     np.zeros((1, 2), dtype=np.float32), cost=cost)
 
 
+def isotropic2_batch(ninst=256, seed=3, nmin=64, nmax=1024, nbumps=32, device=None):
+    """Batch of 2D isotropic problems on [-1,1]^2 with independent sides
+    uniform in {nmin..nmax} per axis (so h differs per axis and problem),
+    costs c = 1 + 0.5 tanh(g) as isotropic2_bumps with per-problem bumps,
+    and 1..8 seeds at random points.  Costs are built with torch on
+    `device` (the grids total ~76 M cells for 256 problems)."""
+    import torch
+    rng = np.random.default_rng(seed)
+    dev = torch.device(device) if device is not None else torch.device('cpu')
+    out = []
+    for i in range(ninst):
+        n0, n1 = (int(v) for v in rng.integers(nmin, nmax + 1, 2))
+        centers = torch.tensor(rng.uniform(-1, 1, (nbumps, 2)), device=dev)
+        widths = torch.tensor(rng.uniform(0.05, 0.2, nbumps), device=dev)
+        amps = torch.tensor(rng.normal(size=nbumps), device=dev)
+        x0 = -1 + (torch.arange(n0, device=dev, dtype=torch.float64) + 0.5) * (2. / n0)
+        x1 = -1 + (torch.arange(n1, device=dev, dtype=torch.float64) + 0.5) * (2. / n1)
+        d0 = (x0[None, :] - centers[:, 0:1]) ** 2          # (nbumps, n0)
+        d1 = (x1[None, :] - centers[:, 1:2]) ** 2          # (nbumps, n1)
+        w = 1. / (2 * widths ** 2)
+        g = torch.einsum('b,bi,bj->ij', amps, torch.exp(-d0 * w[:, None]), torch.exp(-d1 * w[:, None]))
+        cost = (1 + 0.5 * torch.tanh(g)).to(torch.float32)
+        nseeds = int(rng.integers(1, 9))
+        seeds = rng.uniform(-1, 1, (nseeds, 2)).astype(np.float32)
+        out.append(Workload(f'isotropic2_batch[{i}]', 'Isotropic2', [[-1, 1], [-1, 1]], (n0, n1),
+                            seeds, cost=cost))
+    return out
+
+
 def riemann2_aniso(n=2000, seed=1, nseeds=4, sigma=20., amax=10.):
     """2D Riemannian eikonal on [-1,1]^2: M = R(theta)^T diag(1, a^2) R(theta),
     theta smooth in [0, pi), a log-uniform in [1, amax] and smoothed (sigma in

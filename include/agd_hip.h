@@ -131,6 +131,26 @@ int agd_eikonal_solve(const agd_eikonal_desc* desc, const agd_eikonal_io* io,
 int agd_eikonal_flow(const agd_eikonal_desc* desc, const agd_eikonal_io* io,
                      const agd_flow_io* flow, void* stream);
 
+/* One instance of a batched solve (no counterpart in the reference, whose
+ * solver runs one grid per call).  Instances share the descriptor's model,
+ * tile shape, scalar type and solver; each has its own shape, isotropic
+ * weights and tolerances. */
+typedef struct agd_batch_instance {
+  int64_t shape[3];        /* instance grid (unused axes = 1)            */
+  double iso_weights[3];   /* AGD_ISOTROPIC: per-axis weights (h^-2)     */
+  double atol, rtol;       /* convergence tolerances of the instance     */
+} agd_batch_instance;
+
+/* Solve several instances in one worklist (one pass launch per grid pass for
+ * the whole batch).  io buffers are the concatenation, in instance order, of
+ * each instance's tiled arrays (values, rhs, geom component-major over the
+ * concatenated cells, seed and trigger bits).  desc->shape is ignored; no
+ * walls, no periodic axes, no precomputed scheme; FP32 only.  Converges when
+ * every instance has converged. */
+int agd_eikonal_solve_batched(const agd_eikonal_desc* desc, int32_t ninst,
+                              const agd_batch_instance* instances, const agd_eikonal_io* io,
+                              agd_eikonal_stats* stats, void* stream);
+
 /* Precompute the scheme of every cell into io->scheme_weights /
  * io->scheme_codes (replaces the precompute_scheme export kernel,
  * _Kernel.py:254-268,346-359, Update.h:76-80 import_scheme_macro).  Needs

@@ -40,6 +40,7 @@ def test_struct_layouts_match_header():
     assert _lib.EikonalDesc.nitermax_o.offset == 88
     assert ctypes.sizeof(_lib.EikonalIO) == 64
     assert ctypes.sizeof(_lib.EikonalStats) == 56
+    assert ctypes.sizeof(_lib.BatchInstance) == 64
 
 
 def test_error_paths_without_device():
