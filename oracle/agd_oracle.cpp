@@ -11,7 +11,10 @@
 //       Geometry_.h:237-243.
 //   * Scheme per model: Isotropic_.h:20-41 (isotropic / diagonal),
 //       Riemann_.h:16-29 (Riemann, Rander = Riemann + drift),
-//       drift shift FiniteDifferences.h:41-45 (fact[0]+=s, fact[1]-=s).
+//       drift shift FiniteDifferences.h:41-45 (fact[0]+=s, fact[1]-=s),
+//       TTI_.h:41-134 (7 tangent ellipses of the dual unit ball, one
+//       Selling decomposition each, min or max of the 7 updates as
+//       HFMIter.h:29-56 with mix_is_min from diags()).
 //   * Neighbour values (min over -e/+e, ties keep -e), network sort and the
 //       upwind quadratic update: HFM.h:28-207, NetworkSort.h:17-60.
 //   * Convergence test |u_old-u_new| <= atol + rtol|u_new|: Update.h:211-219.
@@ -142,7 +145,8 @@ static void selling3(const S* m, S* w, int e[][3]) {
 }
 
 // ---------------------------------------------------------------- models
-enum Model { ISOTROPIC = 0, DIAGONAL = 1, RIEMANN = 2, RANDER = 3 };
+enum Model { ISOTROPIC = 0, DIAGONAL = 1, RIEMANN = 2, RANDER = 3, TTI = 4 };
+static const int kNMix = 7;  // TTI: nmix_macro (kernel_traits.py:47-52)
 
 static int symdim(int d) { return d * (d + 1) / 2; }
 static int nsym_of(int model, int d) {
@@ -154,27 +158,130 @@ static int ngeom_of(int model, int d) {
     case DIAGONAL: return d;
     case RIEMANN: return symdim(d);
     case RANDER: return symdim(d) + d;
+    case TTI: return 5 + d * d;
   }
   return 0;
 }
+static int nmix_of(int model) { return model == TTI ? kNMix : 1; }
 
+// nmix sub-stencils of nsym weights/offsets each (sub-stencil m at
+// [m*nsym, (m+1)*nsym)); the update is the min (mix_is_min) or the max of the
+// sub-stencils' updates.
 template <class S>
 struct Stencil {
-  int nsym;
-  S w[6];
-  int e[6][3];
+  int nsym, nmix;
+  bool mix_is_min;
+  S w[6 * kNMix];
+  int e[6 * kNMix][3];
   S drift[3];
   bool has_drift;
 };
 
-// Isotropic_.h:20-41 and Riemann_.h:16-29.
+// ---------------------------------------------------------------- TTI
+// TTI_.h:41-49: both roots of a + 2 b t + c t^2 (non-negative discriminant).
+template <class S>
+static void solve2(S a, S b, S c, S r[2]) {
+  const S sdelta = std::sqrt(b * b - a * c);
+  const S u = -b + sdelta, v = -sdelta - b;
+  if (std::fabs(c) > std::fabs(a)) { r[0] = u / c; r[1] = v / c; return; }
+  if (a != S(0)) { r[0] = a / u; r[1] = a / v; return; }
+  r[0] = 0; r[1] = Inf<S>();
+}
+
+// TTI_.h:53-58: smallest root above the threshold.
+template <class S>
+static S solve2_above(S a, S b, S c, S above) {
+  S r[2];
+  solve2<S>(a, b, c, r);
+  const bool ordered = r[0] < r[1];
+  const S rmin = ordered ? r[0] : r[1], rmax = ordered ? r[1] : r[0];
+  return rmin >= above ? rmin : rmax;
+}
+
+// TTI_.h:67-72: g = df(x) / <x, df(x)> with f(x) = C + 2<l,x> + <qx,x>,
+// q = (q00, q10, q11).
+template <class S>
+static void grad_ratio(const S* l, const S* q, const S* x, S* g) {
+  S h0 = q[0] * x[0] + q[1] * x[1];
+  S h1 = q[1] * x[0] + q[2] * x[1];
+  h0 = l[0] + h0;
+  h1 = l[1] + h1;
+  const S xhg = x[0] * h0 + x[1] * h1;
+  g[0] = h0 / xhg;
+  g[1] = h1 / xhg;
+}
+
+// x^T q y for a symmetric 2x2 q = (q00, q10, q11) (Geometry_.h scal_vmv order).
+template <class S>
+static S scal_vmv2f(const S* x, const S* q, const S* y) {
+  return x[0] * q[0] * y[0] + (x[0] * y[1] + x[1] * y[0]) * q[1] + x[1] * q[2] * y[1];
+}
+
+// TTI_.h:76-107: samples the curve f(x) = 0, x >= 0 at nmix points from
+// (a,0) to (0,b) and returns diag_s[i] = grad f / <x, grad f> there;
+// mix_is_min = det(diag_s[0], diag_s[nmix-1]) > 0.
+template <class S>
+static bool tti_diags(const S* l, const S* q, S diag_s[kNMix][2]) {
+  const S a = solve2_above<S>(S(-2), l[0], q[0], S(0));
+  const S b = solve2_above<S>(S(-2), l[1], q[2], S(0));
+  const S e0[2] = {S(0.5), S(0.5)}, e1[2] = {S(0.5), S(-0.5)};
+  const S L0 = l[0] * e0[0] + l[1] * e0[1], L1 = l[0] * e1[0] + l[1] * e1[1];
+  const S Q0 = scal_vmv2f<S>(e0, q, e0), Q1 = scal_vmv2f<S>(e0, q, e1), Q2 = scal_vmv2f<S>(e1, q, e1);
+  S x_s[kNMix][2];
+  x_s[0][0] = a; x_s[0][1] = 0;
+  x_s[kNMix - 1][0] = 0; x_s[kNMix - 1][1] = b;
+  for (int i = 1; i < kNMix - 1; ++i) {
+    const S t = S(i) / S(kNMix - 1);
+    const S v = (1 - t) * a - t * b;
+    const S u = solve2_above<S>(S(-2) + S(2) * L1 * v + Q2 * v * v, L0 + Q1 * v, Q0, std::fabs(v));
+    x_s[i][0] = (u + v) / 2;
+    x_s[i][1] = (u - v) / 2;
+  }
+  for (int i = 0; i < kNMix; ++i) grad_ratio<S>(l, q, x_s[i], diag_s[i]);
+  return diag_s[0][0] * diag_s[kNMix - 1][1] - diag_s[0][1] * diag_s[kNMix - 1][0] > 0;
+}
+
+// TTI_.h:112-134: geometry [l0, l1, q00, q10, q11, T (d x d, row-major)];
+// D0 = sum of the outer products of T's first d-1 rows, D1 = that of its
+// last row, sub-stencil i = Selling(diag_i[0] D0 + diag_i[1] D1).
+template <class S>
+static void tti_scheme(int d, const S* g, Stencil<S>& st) {
+  const S* l = g;
+  const S* q = g + 2;
+  const S* T = g + 5;
+  S diag_s[kNMix][2];
+  st.mix_is_min = tti_diags<S>(l, q, diag_s);
+  const int sd = symdim(d);
+  S D0[6], D1[6];
+  for (int i = 0, k = 0; i < d; ++i)
+    for (int j = 0; j <= i; ++j, ++k) {
+      D0[k] = T[i] * T[j];
+      D1[k] = T[(d - 1) * d + i] * T[(d - 1) * d + j];
+      if (d == 3) D0[k] += T[d + i] * T[d + j];
+    }
+  for (int m = 0; m < kNMix; ++m) {
+    S D[6];
+    for (int k = 0; k < sd; ++k) D[k] = diag_s[m][0] * D0[k] + diag_s[m][1] * D1[k];
+    if (d == 2) selling2<S>(D, st.w + m * sd, st.e + m * sd);
+    else selling3<S>(D, st.w + m * sd, st.e + m * sd);
+  }
+}
+
+// Isotropic_.h:20-41, Riemann_.h:16-29 and TTI_.h:112-134.
 template <class S>
 static void make_stencil(int model, int d, const S* g, const S* iso_w, Stencil<S>& st) {
   st.has_drift = false;
   st.nsym = nsym_of(model, d);
-  for (int k = 0; k < 6; ++k) {
+  st.nmix = nmix_of(model);
+  st.mix_is_min = true;
+  for (int k = 0; k < 6 * kNMix; ++k) {
     st.w[k] = 0;
     st.e[k][0] = st.e[k][1] = st.e[k][2] = 0;
+  }
+  if (model == TTI) {
+    st.drift[0] = st.drift[1] = st.drift[2] = 0;
+    tti_scheme<S>(d, g, st);
+    return;
   }
   st.drift[0] = st.drift[1] = st.drift[2] = 0;
   if (model == ISOTROPIC || model == DIAGONAL) {
@@ -329,23 +436,25 @@ struct Problem {
   double iso_w[3];
 };
 
-// Gather neighbour values for cell n (HFM.h:39-77, FiniteDifferences.h:20-106).
+// Gather neighbour values for cell n over the sub-stencil starting at
+// `base` (HFM.h:39-77, FiniteDifferences.h:20-106).
 template <class S, class Get>
-static void neighbours(const Problem& P, const Stencil<S>& st, int64_t n, Get get,
+static void neighbours(const Problem& P, const Stencil<S>& st, int base, int64_t n, Get get,
                        S* v, int* side, int64_t* nidx) {
   int64_t x[3];
   P.G.pos(n, x);
   for (int k = 0; k < st.nsym; ++k) {
+    const int* ek = st.e[base + k];
     S shift = 0;
     if (st.has_drift) {
-      for (int c = 0; c < P.d; ++c) shift += st.drift[c] * S(st.e[k][c]);
+      for (int c = 0; c < P.d; ++c) shift += st.drift[c] * S(ek[c]);
     }
     S best = Inf<S>();
     int bs = 0;
     int64_t bi = -1;
     for (int s = 0; s < 2; ++s) {
       const int eps = 2 * s - 1;
-      int ee[3] = {eps * st.e[k][0], eps * st.e[k][1], eps * st.e[k][2]};
+      int ee[3] = {eps * ek[0], eps * ek[1], eps * ek[2]};
       int64_t y[3];
       for (int c = 0; c < P.d; ++c) y[c] = x[c] + ee[c];
       S val = Inf<S>();
@@ -363,10 +472,50 @@ static void neighbours(const Problem& P, const Stencil<S>& st, int64_t n, Get ge
   }
 }
 
+// The update of cell n (HFMIter.h:29-56): the upwind update of each
+// sub-stencil, keeping the smaller (mix_is_min) or the larger value, with the
+// reference's comparison `mix_is_min == (u_new > u_mix)`.  fw/side/nidx (each
+// nullable) receive the chosen sub-stencil's flow weights, sides and
+// neighbour indices, *kmix its index.
+template <class S, class Get>
+static S cell_update(const Problem& P, const Stencil<S>& st, int64_t n, Get get, S rhs, S* fw,
+                     int* side, int64_t* nidx, int* kmix) {
+  const int ns = st.nsym;
+  for (int k = 0; k < ns; ++k) {
+    if (fw) fw[k] = 0;
+    if (side) side[k] = 0;
+    if (nidx) nidx[k] = -1;
+  }
+  if (kmix) *kmix = 0;
+  if (st.nmix == 1) {
+    S v[6];
+    neighbours<S>(P, st, 0, n, get, v, side, nidx);
+    return hfm_solve<S>(ns, st.w, v, rhs, fw);
+  }
+  S best = st.mix_is_min ? Inf<S>() : -Inf<S>();
+  for (int m = 0; m < st.nmix; ++m) {
+    S v[6], f[6];
+    int sd[6];
+    int64_t ni[6];
+    neighbours<S>(P, st, m * ns, n, get, v, sd, ni);
+    const S val = hfm_solve<S>(ns, st.w + m * ns, v, rhs, f);
+    if (st.mix_is_min == (best > val)) {
+      best = val;
+      for (int k = 0; k < ns; ++k) {
+        if (fw) fw[k] = f[k];
+        if (side) side[k] = sd[k];
+        if (nidx) nidx[k] = ni[k];
+      }
+      if (kmix) *kmix = m;
+    }
+  }
+  return best;
+}
+
 template <class S>
 static void stencil_at(const Problem& P, int64_t n, Stencil<S>& st) {
   const int ng = ngeom_of(P.model, P.d);
-  S g[9];
+  S g[16];
   for (int k = 0; k < ng; ++k) g[k] = S(P.geom[k * P.G.N + n]);
   S iw[3] = {S(P.iso_w[0]), S(P.iso_w[1]), S(P.iso_w[2])};
   make_stencil<S>(P.model, P.d, g, iw, st);
@@ -397,7 +546,6 @@ static void fill_problem(Problem& P, int model, int d, const int64_t* shape,
 template <class S>
 static int64_t jacobi(const Problem& P, double atol, double rtol, int64_t nitermax, double* uio) {
   const int64_t N = P.G.N;
-  const int ns = nsym_of(P.model, P.d);
   std::vector<Stencil<S>> st(N);
   for (int64_t n = 0; n < N; ++n) stencil_at<S>(P, n, st[n]);
   std::vector<S> u(N), un(N);
@@ -407,9 +555,8 @@ static int64_t jacobi(const Problem& P, double atol, double rtol, int64_t niterm
     bool changed = false;
     for (int64_t n = 0; n < N; ++n) {
       if (P.seeds[n]) { un[n] = u[n]; continue; }
-      S v[6];
-      neighbours<S>(P, st[n], n, [&](int64_t m) { return u[m]; }, v, nullptr, nullptr);
-      const S val = hfm_solve<S>(ns, st[n].w, v, S(P.rhs[n]), nullptr);
+      const S val = cell_update<S>(P, st[n], n, [&](int64_t m) { return u[m]; }, S(P.rhs[n]),
+                                   nullptr, nullptr, nullptr, nullptr);
       const S nv = (val < u[n]) ? val : u[n];
       const S diff = std::fabs(u[n] - nv);
       const S tol = S(atol) + S(rtol) * std::fabs(nv);
@@ -465,25 +612,50 @@ void orc_hfm_update(int nact, int64_t n, const double* w, const double* v,
   }
 }
 
-// Per-cell stencil export (weights, offsets) for a whole problem, FP32 or FP64.
+// Per-cell stencil export for a whole problem, FP32 or FP64: weights
+// w[N][nmix*nsym], offsets e[N][nmix*nsym][d], mix flag mix[N] (nullable;
+// 1 = min of the sub-stencil updates).
 void orc_stencils(int model, int d, const int64_t* shape, const double* geom,
-                  const double* iso_w, int use_float, double* w, int32_t* e) {
+                  const double* iso_w, int use_float, double* w, int32_t* e, int32_t* mix) {
   Problem P;
   fill_problem(P, model, d, shape, nullptr, geom, nullptr, nullptr, nullptr, iso_w);
-  const int ns = nsym_of(model, d);
+  const int ns = nsym_of(model, d) * nmix_of(model);
   for (int64_t n = 0; n < P.G.N; ++n) {
-    int ee[6][3];
+    static thread_local int ee[6 * kNMix][3];
+    bool mn = true;
     if (use_float) {
       Stencil<float> st;
       stencil_at<float>(P, n, st);
+      mn = st.mix_is_min;
       for (int k = 0; k < ns; ++k) { w[n * ns + k] = st.w[k]; for (int l = 0; l < d; ++l) ee[k][l] = st.e[k][l]; }
     } else {
       Stencil<double> st;
       stencil_at<double>(P, n, st);
+      mn = st.mix_is_min;
       for (int k = 0; k < ns; ++k) { w[n * ns + k] = st.w[k]; for (int l = 0; l < d; ++l) ee[k][l] = st.e[k][l]; }
     }
     for (int k = 0; k < ns; ++k)
       for (int l = 0; l < d; ++l) e[(n * ns + k) * d + l] = ee[k][l];
+    if (mix) mix[n] = mn ? 1 : 0;
+  }
+}
+
+// TTI_.h:76-107 for n cells: l[n][2], q[n][3] -> diag[n][nmix][2], returns
+// mix_is_min per cell in mix[n].  FP32 or FP64.
+void orc_tti_diags(int64_t n, const double* l, const double* q, double* diag, int32_t* mix,
+                   int use_float) {
+  for (int64_t c = 0; c < n; ++c) {
+    if (use_float) {
+      float lf[2] = {float(l[2 * c]), float(l[2 * c + 1])};
+      float qf[3] = {float(q[3 * c]), float(q[3 * c + 1]), float(q[3 * c + 2])};
+      float ds[kNMix][2];
+      mix[c] = tti_diags<float>(lf, qf, ds);
+      for (int i = 0; i < kNMix; ++i) { diag[(c * kNMix + i) * 2] = ds[i][0]; diag[(c * kNMix + i) * 2 + 1] = ds[i][1]; }
+    } else {
+      double ds[kNMix][2];
+      mix[c] = tti_diags<double>(l + 2 * c, q + 3 * c, ds);
+      for (int i = 0; i < kNMix; ++i) { diag[(c * kNMix + i) * 2] = ds[i][0]; diag[(c * kNMix + i) * 2 + 1] = ds[i][1]; }
+    }
   }
 }
 
@@ -495,7 +667,7 @@ int64_t orc_solve_fm(int model, int d, const int64_t* shape, const int* periodic
   Problem P;
   fill_problem(P, model, d, shape, periodic, geom, rhs, seeds, wallDist, iso_w);
   const int64_t N = P.G.N;
-  const int ns = nsym_of(model, d);
+  const int ns = nsym_of(model, d) * nmix_of(model);  // offsets of all sub-stencils
   // Stencils and reverse adjacency (cells whose update reads a given cell).
   std::vector<Stencil<double>> st(N);
   std::vector<int64_t> deg(N + 1, 0);
@@ -547,9 +719,8 @@ int64_t orc_solve_fm(int model, int d, const int64_t* shape, const int* periodic
     for (int64_t r = deg[n]; r < deg[n + 1]; ++r) {
       const int64_t m = rev[r];
       if (acc[m] || seeds[m]) continue;
-      double v[6];
-      neighbours<double>(P, st[m], m, get_acc, v, nullptr, nullptr);
-      const double val = hfm_solve<double>(ns, st[m].w, v, rhs[m], nullptr);
+      const double val = cell_update<double>(P, st[m], m, get_acc, rhs[m], nullptr, nullptr,
+                                             nullptr, nullptr);
       if (val < u[m]) {
         u[m] = val;
         heap.push(QE(val, m));
@@ -589,22 +760,25 @@ int orc_flow(int model, int d, const int64_t* shape, const int* periodic, const 
     int64_t ni[6] = {-1, -1, -1, -1, -1, -1};
     int e[6][3];
     if (!seeds[n]) {
+      int km = 0;
       if (use_float) {
         Stencil<float> st;
         stencil_at<float>(P, n, st);
-        float v[6], ff[6];
-        neighbours<float>(P, st, n, [&](int64_t m) { return float(u0[m]); }, v, side, ni);
-        const float val = hfm_solve<float>(ns, st.w, v, float(rhs[n]), ff);
+        float ff[6];
+        const float val = cell_update<float>(P, st, n, [&](int64_t m) { return float(u0[m]); },
+                                             float(rhs[n]), ff, side, ni, &km);
         if (val < float(u0[n])) u[n] = val;
-        for (int k = 0; k < ns; ++k) { f[k] = ff[k]; for (int c = 0; c < 3; ++c) e[k][c] = st.e[k][c]; }
+        for (int k = 0; k < ns; ++k) {
+          f[k] = ff[k];
+          for (int c = 0; c < 3; ++c) e[k][c] = st.e[km * ns + k][c];
+        }
       } else {
         Stencil<double> st;
         stencil_at<double>(P, n, st);
-        double v[6];
-        neighbours<double>(P, st, n, [&](int64_t m) { return u0[m]; }, v, side, ni);
-        const double val = hfm_solve<double>(ns, st.w, v, rhs[n], f);
+        const double val = cell_update<double>(P, st, n, [&](int64_t m) { return u0[m]; }, rhs[n],
+                                               f, side, ni, &km);
         if (val < u0[n]) u[n] = val;
-        for (int k = 0; k < ns; ++k) for (int c = 0; c < 3; ++c) e[k][c] = st.e[k][c];
+        for (int k = 0; k < ns; ++k) for (int c = 0; c < 3; ++c) e[k][c] = st.e[km * ns + k][c];
       }
     } else {
       for (int k = 0; k < ns; ++k) for (int c = 0; c < 3; ++c) e[k][c] = 0;

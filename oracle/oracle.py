@@ -25,8 +25,10 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 
-ISOTROPIC, DIAGONAL, RIEMANN, RANDER = 0, 1, 2, 3
-MODEL_IDS = {'Isotropic': ISOTROPIC, 'Diagonal': DIAGONAL, 'Riemann': RIEMANN, 'Rander': RANDER}
+ISOTROPIC, DIAGONAL, RIEMANN, RANDER, TTI = 0, 1, 2, 3, 4
+MODEL_IDS = {'Isotropic': ISOTROPIC, 'Diagonal': DIAGONAL, 'Riemann': RIEMANN, 'Rander': RANDER,
+             'TTI': TTI}
+NMIX = 7
 
 
 def build():
@@ -43,7 +45,8 @@ def _load():
     lib.orc_version.restype = c_int
     lib.orc_selling.argtypes = [c_int, c_int64, P, P, P, c_int]
     lib.orc_hfm_update.argtypes = [c_int, c_int64, P, P, P, P, P, c_int]
-    lib.orc_stencils.argtypes = [c_int, c_int, P, P, P, c_int, P, P]
+    lib.orc_stencils.argtypes = [c_int, c_int, P, P, P, c_int, P, P, P]
+    lib.orc_tti_diags.argtypes = [c_int64, P, P, P, P, c_int]
     lib.orc_solve_fm.argtypes = [c_int, c_int, P, P, P, P, P, P, P, P]
     lib.orc_solve_fm.restype = c_int64
     lib.orc_solve_jacobi.argtypes = [c_int, c_int, P, P, P, P, P, P, P, c_double, c_double,
@@ -119,6 +122,10 @@ class Problem:
     def nsym(self):
         return self.d if self.model in (ISOTROPIC, DIAGONAL) else self.d * (self.d + 1) // 2
 
+    @property
+    def nmix(self):
+        return NMIX if self.model == TTI else 1
+
     def cast(self, dtype):
         """Same problem with geometry and rhs rounded to dtype (the FP32
         inputs the GPU path sees)."""
@@ -142,13 +149,16 @@ class Problem:
                                     int(use_float), _p(u))
         return u.reshape(self.shape), int(it)
 
-    def stencils(self, use_float=False):
-        ns = self.nsym
+    def stencils(self, use_float=False, with_mix=False):
+        """Per-cell weights [N, nmix*nsym], offsets [N, nmix*nsym, d] (and the
+        min/max mix flag [N] with with_mix)."""
+        ns = self.nsym * self.nmix
         w = np.zeros((self.N, ns))
         e = np.zeros((self.N, ns, self.d), dtype=np.int32)
+        mix = np.zeros(self.N, dtype=np.int32)
         lib().orc_stencils(self.model, self.d, _p(self._shape), _p(self.geom), _p(self.iso_w),
-                           int(use_float), _p(w), _p(e))
-        return w, e
+                           int(use_float), _p(w), _p(e), _p(mix))
+        return (w, e, mix) if with_mix else (w, e)
 
     def flow(self, u, use_float=False):
         """One flow pass from values u: returns dict(values, weights [N,ns],
@@ -200,6 +210,38 @@ def wall_distance(walls, bound=10, periodic=None):
 
 
 # ---------------------------------------------------------------- preprocessing
+def tti_diags(l, q, use_float=False):
+    """TTI_.h:76-107 per cell: l[n, 2], q[n, 3] -> (diag[n, 7, 2], mix_is_min[n])."""
+    l = _c(np.asarray(l, dtype=np.float64).reshape(-1, 2), np.float64)
+    q = _c(np.asarray(q, dtype=np.float64).reshape(-1, 3), np.float64)
+    n = len(l)
+    diag = np.zeros((n, NMIX, 2))
+    mix = np.zeros(n, dtype=np.int32)
+    lib().orc_tti_diags(n, _p(l), _p(q), _p(diag), _p(mix), int(use_float))
+    return diag, mix.astype(bool)
+
+
+def tti_geometry(metric, h, shape):
+    """TTI.with_costs(h).flatten(transposed_transformation=True)
+    (Metrics/Seismic/tti.py:113-125, Metrics/base.py:175-185,
+    Seismic/implicit_base.py:57-63): [l0, l1, q00, q10, q11, T] with
+    T = (A diag(h))^{-T} row-major, A the inverse transformation (identity
+    when absent).  metric = (linear[2, ...], quadratic[2, 2, ...], A or None)."""
+    linear, quadratic, A = metric
+    d = len(shape)
+    full = lambda x, depth: np.broadcast_to(np.asarray(x, dtype=np.float64).reshape(
+        np.shape(x)[:depth] + (1,) * (d + depth - np.ndim(x))), np.shape(x)[:depth] + shape)
+    l = full(linear, 1)
+    q = full(quadratic, 2)
+    if A is None:
+        A = np.eye(d)
+    A = full(A, 2)
+    Ah = A * h.reshape((1, d) + (1,) * d)            # A diag(h): column j scaled by h_j
+    inv = np.moveaxis(np.linalg.inv(np.moveaxis(Ah, (0, 1), (-2, -1))), (-2, -1), (0, 1))
+    T = np.swapaxes(inv, 0, 1)                         # transpose of the inverse
+    return np.concatenate([l, flatten_lower(q), T.reshape((d * d,) + shape)], axis=0)
+
+
 def flatten_lower(m):
     d = m.shape[0]
     return np.stack([m[i, j] for i in range(d) for j in range(i + 1)], axis=0)
@@ -244,6 +286,8 @@ def make_problem(model, shape, h, metric=None, drift=None, cost=1., seeds_idx=()
             c = c.reshape((d,) + (1,) * d)
         c = np.broadcast_to(c, (d,) + shape)
         geom = (c * h.reshape((d,) + (1,) * d)) ** -2
+    elif mid == TTI:
+        geom = tti_geometry(metric, h, shape)
     else:
         m = np.asarray(metric, dtype=np.float64)
         if m.ndim == 2:
