@@ -24,7 +24,8 @@ from ._device import (torch, device_tensor, host_array, ptr, stream_handle, empt
                       full, zeros)
 
 _MODELS = {'Isotropic': _lib.ISOTROPIC, 'Diagonal': _lib.DIAGONAL,
-           'Riemann': _lib.RIEMANN, 'Rander': _lib.RANDER}
+           'Riemann': _lib.RIEMANN, 'Rander': _lib.RANDER, 'TTI': _lib.TTI}
+_NMIX = {'TTI': 7}
 
 _SOLVER_NAMES = {'AGSI': 'adaptive_gauss_siedel_iteration', 'FIM': 'fast_iterative_method'}
 _SOLVER_IDS = {'adaptive_gauss_siedel_iteration': _lib.AGSI,
@@ -39,6 +40,7 @@ def default_traits(model):
         'Isotropic3': ((4, 4, 4), 8, 4), 'Diagonal3': ((4, 4, 4), 8, 4),
         'Riemann2': ((8, 8), 8, 4), 'Riemann3': ((4, 4, 4), 4, 6),
         'Rander2': ((8, 8), 12, 8), 'Rander3': ((4, 4, 4), 8, 8),
+        'TTI2': ((8, 8), 10, 6), 'TTI3': ((4, 4, 4), 3, 6),
     }
     if model not in table:
         raise NotImplementedError(
@@ -260,6 +262,8 @@ class Interface:
 
     def _set_metric_geometry(self, metric, dualMetric):
         model = self.model_
+        if model == 'TTI':
+            return self._set_tti_geometry(metric, dualMetric)
         cls = {'Diagonal': Metrics.Diagonal, 'Riemann': Metrics.Riemann, 'Rander': Metrics.Rander}[model]
         if metric is None and dualMetric is None:
             raise ValueError(f"Missing value for key metric (model {self.model})")
@@ -303,6 +307,40 @@ class Interface:
             h, ptr(arr_d), ptr(w_d) if w_d is not None else None, stride, ptr(self.geom),
             self.stream))
         self._keep = (arr_d, w_d)
+
+    def _set_tti_geometry(self, metric, dualMetric):
+        """TTI: [l, flat(q), (A diag(h))^-T] prepared on the device
+        (_SetGeometry.py:140-142, Seismic/tti.py:113-125)."""
+        if dualMetric is not None:
+            raise NotImplementedError("the TTI model takes a 'metric' (no dualMetric)")
+        if metric is None:
+            raise ValueError(f"Missing value for key metric (model {self.model})")
+        if not isinstance(metric, Metrics.Seismic.TTI):
+            metric = Metrics.Seismic.TTI.from_HFM(host_array(metric))
+        if metric.vdim != self.ndim:
+            raise ValueError(f"TTI metric of dimension {metric.vdim} for a {self.ndim}D model")
+        self.metric = metric
+        lq = np.concatenate([np.asarray(host_array(metric.linear), dtype=np.float64).reshape(2, -1),
+                             Metrics.flatten_symmetric_matrix(
+                                 np.asarray(host_array(metric.quadratic), dtype=np.float64)).reshape(3, -1)])
+        lq = lq.reshape((5,) + tuple(np.shape(host_array(metric.linear))[1:]))
+        lq_d, stride = self._field_tensor(lq, (5,), 'metric (linear, quadratic)')
+        A_d = None
+        if metric.inverse_transformation is not None:
+            A_d, astride = self._field_tensor(metric.inverse_transformation, (self.ndim, self.ndim),
+                                              'metric transformation')
+            if astride != stride:  # broadcast the constant one
+                ncell = int(np.prod(self.shape))
+                if stride == 1:
+                    A_d = A_d.reshape(-1, 1).expand(A_d.numel(), ncell).contiguous()
+                else:
+                    lq_d = lq_d.reshape(-1, 1).expand(lq_d.numel(), ncell).contiguous()
+                    stride = 1
+        self.geom = empty((5 + self.ndim ** 2, self.size_tot), self.float_t)
+        _lib.check(_lib.lib.agd_prepare_geometry(
+            self.modelId, self.ndim, _lib.i64(self.shape), _lib.i32(self.shape_i), self.scalar,
+            ctypes_double3(self.h), ptr(lq_d), ptr(A_d), stride, ptr(self.geom), self.stream))
+        self._keep = (lq_d, A_d)
 
     def _field_tensor(self, arr, core, name):
         """float32 device tensor of a field of shape core + shape (or core
@@ -422,10 +460,13 @@ class Interface:
             self.precompute_scheme = bool(self.GetValue(
                 'precompute_scheme', default=True,
                 help="Precompute and store the finite difference scheme stencils"))
+        if self.model_ == 'TTI' and not self.precompute_scheme:
+            raise ValueError("the TTI model runs from a precomputed scheme (precompute_scheme=True)")
         if self.precompute_scheme:
             nsym = self.ndim * (self.ndim + 1) // 2
-            self.scheme_w_t = empty((nsym, self.size_tot), self.float_t)
-            self.scheme_c_t = empty((2 * nsym, self.size_tot), np.int32)
+            nmix = _NMIX.get(self.model_, 1)
+            self.scheme_w_t = empty((nmix * nsym, self.size_tot), self.float_t)
+            self.scheme_c_t = empty((nmix * 2 * nsym + (nmix > 1), self.size_tot), np.int32)
             self.compute_scheme()
 
         self.nitermax_o = int(self.GetValue('nitermax_o', default=2000,

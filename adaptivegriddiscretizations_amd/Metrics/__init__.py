@@ -10,7 +10,7 @@ user code written against the reference keeps working.
 """
 import numpy as np
 
-__all__ = ["Base", "Isotropic", "Diagonal", "Riemann", "Rander"]
+__all__ = ["Base", "Isotropic", "Diagonal", "Riemann", "Rander", "Seismic"]
 
 
 def _np(a):
@@ -278,3 +278,6 @@ class Rander(Base):
             return metric
         r = Riemann.from_cast(metric)
         return cls(r.m, np.zeros(r.vdim))
+
+
+from . import Seismic  # noqa: E402,F401

@@ -18,7 +18,7 @@ c_uint32_p = POINTER(ctypes.c_uint32)
 # agd_status
 AGD_OK, AGD_EINVAL, AGD_EHIP, AGD_NOCONV, AGD_ENOMEM = 0, -1, -2, -3, -4
 # agd_model
-ISOTROPIC, DIAGONAL, RIEMANN, RANDER = 0, 1, 2, 3
+ISOTROPIC, DIAGONAL, RIEMANN, RANDER, TTI = 0, 1, 2, 3, 4
 # agd_scalar
 F32, F64 = 0, 1
 # agd_solver_kind

@@ -44,9 +44,11 @@ inline int ngeom_of(int model, int d) {
     case AGD_DIAGONAL: return d;
     case AGD_RIEMANN: return sd;
     case AGD_RANDER: return sd + d;
+    case AGD_TTI: return 5 + d * d;
   }
   return -1;
 }
+inline int nmix_of(int model) { return model == AGD_TTI ? 7 : 1; }
 inline int nsym_of(int model, int d) {
   return (model == AGD_ISOTROPIC || model == AGD_DIAGONAL) ? d : d * (d + 1) / 2;
 }

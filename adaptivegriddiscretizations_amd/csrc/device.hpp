@@ -44,8 +44,11 @@ template <int MODEL, int D> struct ModelTraits {
   static constexpr int NGEOM = MODEL == AGD_ISOTROPIC ? 0
                                : MODEL == AGD_DIAGONAL ? D
                                : MODEL == AGD_RIEMANN  ? SYMDIM
-                                                       : SYMDIM + D;
+                               : MODEL == AGD_RANDER   ? SYMDIM + D
+                                                       : 5 + D * D;
   static constexpr bool DRIFT = (MODEL == AGD_RANDER);
+  // TTI: the update is the min or max over NMIX Selling schemes.
+  static constexpr int NMIX = MODEL == AGD_TTI ? 7 : 1;
 };
 
 // ------------------------------------------------------------------ tiles
@@ -318,6 +321,107 @@ __device__ inline void make_stencil(const S* g, const S* iso_w, Stencil<S, MODEL
       st.shift[k] = s;
     }
   }
+}
+
+// ------------------------------------------------------------------ TTI
+// TTI_.h:41-134: the dual unit ball <l, x> + 1/2 <x, q x> = 1 in the squared
+// coordinates x = (X^2 [+ Y^2], Z^2) of T p is replaced by NMIX tangent
+// ellipses; ellipse i has the dual metric diag_i[0] D0 + diag_i[1] D1 with D0,
+// D1 the outer products of T's rows.  Written without contraction (each
+// product rounded), as the oracle, so FP32 results agree bit for bit.
+constexpr int kTTIMix = 7;
+
+template <class S> __device__ inline void tti_solve2(S a, S b, S c, S (&r)[2]) {
+#pragma clang fp contract(off)
+  const S sdelta = sqrt_(b * b - a * c);
+  const S u = -b + sdelta, v = -sdelta - b;
+  if (abs_(c) > abs_(a)) {
+    r[0] = u / c;
+    r[1] = v / c;
+  } else if (a != S(0)) {
+    r[0] = a / u;
+    r[1] = a / v;
+  } else {
+    r[0] = S(0);
+    r[1] = inf_v<S>();
+  }
+}
+
+template <class S> __device__ inline S tti_solve2_above(S a, S b, S c, S above) {
+  S r[2];
+  tti_solve2<S>(a, b, c, r);
+  const bool ordered = r[0] < r[1];
+  const S rmin = ordered ? r[0] : r[1], rmax = ordered ? r[1] : r[0];
+  return rmin >= above ? rmin : rmax;
+}
+
+template <class S>
+__device__ inline void tti_grad_ratio(const S* l, const S* q, S x0, S x1, S& g0, S& g1) {
+#pragma clang fp contract(off)
+  S h0 = q[0] * x0 + q[1] * x1;
+  S h1 = q[1] * x0 + q[2] * x1;
+  h0 = l[0] + h0;
+  h1 = l[1] + h1;
+  const S xhg = x0 * h0 + x1 * h1;
+  g0 = h0 / xhg;
+  g1 = h1 / xhg;
+}
+
+template <class S> __device__ inline S tti_vqv(S x0, S x1, const S* q, S y0, S y1) {
+#pragma clang fp contract(off)
+  return x0 * q[0] * y0 + (x0 * y1 + x1 * y0) * q[1] + x1 * q[2] * y1;
+}
+
+// diags(): diag[i] = grad f / <x, grad f> at NMIX points of the curve; returns
+// mix_is_min = det(diag[0], diag[NMIX-1]) > 0.
+template <class S> __device__ inline bool tti_diags(const S* l, const S* q, S (&diag)[kTTIMix][2]) {
+#pragma clang fp contract(off)
+  const S a = tti_solve2_above<S>(S(-2), l[0], q[0], S(0));
+  const S b = tti_solve2_above<S>(S(-2), l[1], q[2], S(0));
+  const S L0 = l[0] * S(0.5) + l[1] * S(0.5), L1 = l[0] * S(0.5) + l[1] * S(-0.5);
+  const S Q0 = tti_vqv<S>(S(0.5), S(0.5), q, S(0.5), S(0.5));
+  const S Q1 = tti_vqv<S>(S(0.5), S(0.5), q, S(0.5), S(-0.5));
+  const S Q2 = tti_vqv<S>(S(0.5), S(-0.5), q, S(0.5), S(-0.5));
+  tti_grad_ratio<S>(l, q, a, S(0), diag[0][0], diag[0][1]);
+  tti_grad_ratio<S>(l, q, S(0), b, diag[kTTIMix - 1][0], diag[kTTIMix - 1][1]);
+#pragma unroll
+  for (int i = 1; i < kTTIMix - 1; ++i) {
+    const S t = S(i) / S(kTTIMix - 1);
+    const S v = (S(1) - t) * a - t * b;
+    const S u = tti_solve2_above<S>(S(-2) + S(2) * L1 * v + Q2 * v * v, L0 + Q1 * v, Q0, abs_(v));
+    tti_grad_ratio<S>(l, q, (u + v) / S(2), (u - v) / S(2), diag[i][0], diag[i][1]);
+  }
+  return diag[0][0] * diag[kTTIMix - 1][1] - diag[0][1] * diag[kTTIMix - 1][0] > S(0);
+}
+
+// D0 = sum of the outer products of T's first D-1 rows, D1 = that of the
+// last row (flattened lower triangles); g = [l0, l1, q00, q10, q11, T].
+template <class S, int D> __device__ inline void tti_base(const S* g, S* D0, S* D1) {
+#pragma clang fp contract(off)
+  const S* T = g + 5;
+  int k = 0;
+#pragma unroll
+  for (int i = 0; i < D; ++i)
+#pragma unroll
+    for (int j = 0; j <= i; ++j, ++k) {
+      D0[k] = T[i] * T[j];
+      D1[k] = T[(D - 1) * D + i] * T[(D - 1) * D + j];
+      if constexpr (D == 3) D0[k] += T[D + i] * T[D + j];
+    }
+}
+
+// Sub-stencil m: Selling decomposition of diag[m][0] D0 + diag[m][1] D1.
+template <class S, int D>
+__device__ inline void tti_sub(const S* D0, const S* D1, S d0, S d1, S* w, int (&e)[D == 2 ? 3 : 6][3]) {
+  constexpr int SD = D * (D + 1) / 2;
+  S m[SD];
+  {
+#pragma clang fp contract(off)
+#pragma unroll
+    for (int k = 0; k < SD; ++k) m[k] = d0 * D0[k] + d1 * D1[k];
+  }
+  if constexpr (D == 2) selling2<S>(m, w, e);
+  else selling3<S>(m, w, e);
 }
 
 // ------------------------------------------------------------------ update

@@ -250,7 +250,9 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
 
     int code[NJ];
     S sw[NS], shift[NS];
-    if constexpr (VAR == VAR_PRE) {
+    if constexpr (MT::NMIX > 1) {
+      // several sub-schemes: read per sweep below
+    } else if constexpr (VAR == VAR_PRE) {
       // Stencil weights and neighbour codes precomputed for the whole solve.
 #pragma unroll
       for (int k = 0; k < NS; ++k) {
@@ -276,39 +278,74 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
         shift[k] = st.shift[k];
       }
     }
-    // All gathers are issued before any is consumed (safe address = own
-    // cell), so a visit costs a single memory round trip.
-    S g[NJ];
+    if constexpr (MT::NMIX > 1) {
+      // ---- TTI: the min or max over the sub-schemes of their upwind
+      // updates (HFMIter.h:29-56, strict in-tile iteration).  Weights and
+      // neighbour codes are re-read at every sweep (cache hits) rather than
+      // held in registers; out-of-tile values are read at every sweep.
+      const int64_t N = a.g.size_tot;
+      const bool mix_min = a.scode[int64_t(MT::NMIX * NJ) * N + n_t] != 0;
+      __syncthreads();  // u_s complete
+      const bool active = !seed;
+      for (int it = 0; it < a.niter_i; ++it) {
+        S best = mix_min ? inf_v<S>() : -inf_v<S>();
+#pragma unroll 1
+        for (int m = 0; m < MT::NMIX; ++m) {
+          S v[NS], w[NS];
 #pragma unroll
-    for (int j = 0; j < NJ; ++j) g[j] = a.u[code[j] >= 0 ? code[j] : n_t];
-    S vo[NJ];
+          for (int k = 0; k < NS; ++k) {
+            w[k] = a.sw[int64_t(m * NS + k) * N + n_t];
+            S side[2];
 #pragma unroll
-    for (int k = 0; k < NS; ++k) {
-#pragma unroll
-      for (int s = 0; s < 2; ++s) {
-        const int j = 2 * k + s;
-        const S sh = (s == 0) ? shift[k] : -shift[k];
-        vo[j] = code[j] <= -2 ? sh : (code[j] >= 0 ? g[j] + sh : inf_v<S>());
+            for (int sd = 0; sd < 2; ++sd) {
+              const int c = a.scode[int64_t(m * NJ + 2 * k + sd) * N + n_t];
+              side[sd] = c <= -2 ? u_s[-2 - c] : (c >= 0 ? a.u[c] : inf_v<S>());
+            }
+            v[k] = (side[0] > side[1]) ? side[1] : side[0];
+          }
+          const S um = hfm_solve<S, NS>(v, w, rhs);
+          if (mix_min == (best > um)) best = um;
+        }
+        __syncthreads();
+        const bool ch = active && (u_s[tid] > best);
+        if (ch) u_s[tid] = best;
+        if (!block_any<NI>(ch)) break;
       }
-    }
-    __syncthreads();  // u_s complete
-
-    // ---- in-tile sweeps (HFMIter.h:25-84, DECREASING)
-    const bool active = !seed;
-    for (int it = 0; it < a.niter_i; ++it) {
-      S v[NS], w[NS];
+    } else {
+      // All gathers are issued before any is consumed (safe address = own
+      // cell), so a visit costs a single memory round trip.
+      S g[NJ];
+#pragma unroll
+      for (int j = 0; j < NJ; ++j) g[j] = a.u[code[j] >= 0 ? code[j] : n_t];
+      S vo[NJ];
 #pragma unroll
       for (int k = 0; k < NS; ++k) {
-        const S a0 = code[2 * k] <= -2 ? u_s[-2 - code[2 * k]] + vo[2 * k] : vo[2 * k];
-        const S a1 = code[2 * k + 1] <= -2 ? u_s[-2 - code[2 * k + 1]] + vo[2 * k + 1] : vo[2 * k + 1];
-        v[k] = (a0 > a1) ? a1 : a0;  // Greater(v,-) keeps the -e side on ties
-        w[k] = sw[k];
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+          const int j = 2 * k + s;
+          const S sh = (s == 0) ? shift[k] : -shift[k];
+          vo[j] = code[j] <= -2 ? sh : (code[j] >= 0 ? g[j] + sh : inf_v<S>());
+        }
       }
-      const S un = hfm_solve<S, NS>(v, w, rhs);
-      __syncthreads();
-      const bool ch = active && (u_s[tid] > un);
-      if (ch) u_s[tid] = un;
-      if (!block_any<NI>(ch)) break;
+      __syncthreads();  // u_s complete
+
+      // ---- in-tile sweeps (HFMIter.h:25-84, DECREASING)
+      const bool active = !seed;
+      for (int it = 0; it < a.niter_i; ++it) {
+        S v[NS], w[NS];
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+          const S a0 = code[2 * k] <= -2 ? u_s[-2 - code[2 * k]] + vo[2 * k] : vo[2 * k];
+          const S a1 = code[2 * k + 1] <= -2 ? u_s[-2 - code[2 * k + 1]] + vo[2 * k + 1] : vo[2 * k + 1];
+          v[k] = (a0 > a1) ? a1 : a0;  // Greater(v,-) keeps the -e side on ties
+          w[k] = sw[k];
+        }
+        const S un = hfm_solve<S, NS>(v, w, rhs);
+        __syncthreads();
+        const bool ch = active && (u_s[tid] > un);
+        if (ch) u_s[tid] = un;
+        if (!block_any<NI>(ch)) break;
+      }
     }
 
     // ---- store and convergence (Update.h:163-219)
@@ -390,14 +427,35 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_scheme(SchemeArgs<S> a) {
   S geo[NG > 0 ? NG : 1];
 #pragma unroll
   for (int k = 0; k < NG; ++k) geo[k] = a.geom[k * N + n_t];
-  Stencil<S, MODEL, D> st;
-  make_stencil<S, MODEL, D>(geo, a.iso_w, st);
-  int code[2 * NS];
-  resolve_codes<S, MODEL, D, TL, GENERAL>(st, xi, xt, xo, wx, wall_s, a.wall, a.g, code);
+  if constexpr (MT::NMIX > 1) {
+    // TTI_.h:112-134: one Selling decomposition per tangent ellipse, then
+    // the min/max flag after the NMIX*2*NS codes.
+    S diag[kTTIMix][2];
+    const bool mix_min = tti_diags<S>(geo, geo + 2, diag);
+    S D0[NS], D1[NS];
+    tti_base<S, D>(geo, D0, D1);
+#pragma unroll 1
+    for (int m = 0; m < MT::NMIX; ++m) {
+      Stencil<S, MODEL, D> st;
+      tti_sub<S, D>(D0, D1, diag[m][0], diag[m][1], st.w, st.e);
+      int code[2 * NS];
+      resolve_codes<S, MODEL, D, TL, GENERAL>(st, xi, xt, xo, wx, wall_s, a.wall, a.g, code);
 #pragma unroll
-  for (int k = 0; k < NS; ++k) a.sw[k * N + n_t] = st.w[k];
+      for (int k = 0; k < NS; ++k) a.sw[int64_t(m * NS + k) * N + n_t] = st.w[k];
 #pragma unroll
-  for (int j = 0; j < 2 * NS; ++j) a.scode[j * N + n_t] = code[j];
+      for (int j = 0; j < 2 * NS; ++j) a.scode[int64_t(m * 2 * NS + j) * N + n_t] = code[j];
+    }
+    a.scode[int64_t(MT::NMIX * 2 * NS) * N + n_t] = mix_min ? 1 : 0;
+  } else {
+    Stencil<S, MODEL, D> st;
+    make_stencil<S, MODEL, D>(geo, a.iso_w, st);
+    int code[2 * NS];
+    resolve_codes<S, MODEL, D, TL, GENERAL>(st, xi, xt, xo, wx, wall_s, a.wall, a.g, code);
+#pragma unroll
+    for (int k = 0; k < NS; ++k) a.sw[k * N + n_t] = st.w[k];
+#pragma unroll
+    for (int j = 0; j < 2 * NS; ++j) a.scode[j * N + n_t] = code[j];
+  }
 }
 
 // Initial list: tiles holding at least one trigger bit (or every tile).
@@ -523,45 +581,92 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_flow(FlowArgs<S> a) {
 #pragma unroll
   for (int k = 0; k < NG; ++k) geo[k] = a.geom[k * N + n_t];
   const int wx = a.wall ? int(a.wall[n_t]) : 255;
-  Stencil<S, MODEL, D> st;
-  make_stencil<S, MODEL, D>(geo, a.iso_w, st);
 
+  // Neighbour values (min over -e/+e, ties keep -e), sides and indices of a
+  // stencil.
+  auto gather = [&](const Stencil<S, MODEL, D>& st, S* v, S* w, int* side, int64_t* nb) {
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+      S best = inf_v<S>();
+      int bs = 0;
+      int64_t bi = -1;
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {
+        const int sg = 2 * s - 1;
+        int e[3] = {sg * st.e[k][0], sg * st.e[k][1], sg * st.e[k][2]};
+        const S sh = (s == 0) ? st.shift[k] : -st.shift[k];
+        bool vis = true;
+        if (a.wall && wx != 255) {
+          vis = visible<D>(e, wx, [&](const int* ww) -> int {
+            int yt[3] = {xt[0] + ww[0], xt[1] + ww[1], xt[2] + ww[2]};
+            const int64_t m = index_tot<D, TL>(yt, a.g);
+            return m >= 0 ? int(a.wall[m]) : 255;
+          });
+        }
+        int yt[3] = {xt[0] + e[0], xt[1] + e[1], xt[2] + e[2]};
+        const int64_t m = vis ? index_tot<D, TL>(yt, a.g) : -1;
+        const S val = (m >= 0) ? a.u[m] + sh : inf_v<S>();
+        if (s == 0) { best = val; bs = 0; bi = m; }
+        else if (best > val) { best = val; bs = 1; bi = m; }
+      }
+      v[k] = best;
+      side[k] = bs;
+      nb[k] = bi;
+      w[k] = st.w[k];
+    }
+  };
+
+  Stencil<S, MODEL, D> st;  // the (chosen sub-)stencil
   S v[NS], w[NS];
   int side[NS];
   int64_t nb[NS];
+  S val;
+  if constexpr (MT::NMIX > 1) {
+    // TTI: keep the sub-scheme whose update wins the min/max (HFMIter.h:48-54).
 #pragma unroll
-  for (int k = 0; k < NS; ++k) {
-    S best = inf_v<S>();
-    int bs = 0;
-    int64_t bi = -1;
-#pragma unroll
-    for (int s = 0; s < 2; ++s) {
-      const int sg = 2 * s - 1;
-      int e[3] = {sg * st.e[k][0], sg * st.e[k][1], sg * st.e[k][2]};
-      const S sh = (s == 0) ? st.shift[k] : -st.shift[k];
-      bool vis = true;
-      if (a.wall && wx != 255) {
-        vis = visible<D>(e, wx, [&](const int* ww) -> int {
-          int yt[3] = {xt[0] + ww[0], xt[1] + ww[1], xt[2] + ww[2]};
-          const int64_t m = index_tot<D, TL>(yt, a.g);
-          return m >= 0 ? int(a.wall[m]) : 255;
-        });
-      }
-      int yt[3] = {xt[0] + e[0], xt[1] + e[1], xt[2] + e[2]};
-      const int64_t m = vis ? index_tot<D, TL>(yt, a.g) : -1;
-      const S val = (m >= 0) ? a.u[m] + sh : inf_v<S>();
-      if (s == 0) { best = val; bs = 0; bi = m; }
-      else if (best > val) { best = val; bs = 1; bi = m; }
+    for (int k = 0; k < NS; ++k) {
+      v[k] = inf_v<S>();
+      w[k] = S(0);
+      side[k] = 0;
+      nb[k] = -1;
+      st.w[k] = S(0);
+      st.shift[k] = S(0);
+      st.e[k][0] = st.e[k][1] = st.e[k][2] = 0;
     }
-    v[k] = best;
-    side[k] = bs;
-    nb[k] = bi;
-    w[k] = st.w[k];
-  }
-  S vs[NS], ws[NS];
+    S diag[kTTIMix][2];
+    const bool mix_min = tti_diags<S>(geo, geo + 2, diag);
+    S D0[NS], D1[NS];
+    tti_base<S, D>(geo, D0, D1);
+    val = mix_min ? inf_v<S>() : -inf_v<S>();
+#pragma unroll 1
+    for (int m = 0; m < MT::NMIX; ++m) {
+      Stencil<S, MODEL, D> cand;
+      tti_sub<S, D>(D0, D1, diag[m][0], diag[m][1], cand.w, cand.e);
 #pragma unroll
-  for (int k = 0; k < NS; ++k) { vs[k] = v[k]; ws[k] = w[k]; }
-  const S val = hfm_solve<S, NS>(vs, ws, rhs);
+      for (int k = 0; k < NS; ++k) cand.shift[k] = S(0);
+      S cv[NS], cw[NS];
+      int cs[NS];
+      int64_t cn[NS];
+      gather(cand, cv, cw, cs, cn);
+      S vs[NS], ws[NS];
+#pragma unroll
+      for (int k = 0; k < NS; ++k) { vs[k] = cv[k]; ws[k] = cw[k]; }
+      const S um = hfm_solve<S, NS>(vs, ws, rhs);
+      if (mix_min == (val > um)) {
+        val = um;
+        st = cand;
+#pragma unroll
+        for (int k = 0; k < NS; ++k) { v[k] = cv[k]; w[k] = cw[k]; side[k] = cs[k]; nb[k] = cn[k]; }
+      }
+    }
+  } else {
+    make_stencil<S, MODEL, D>(geo, a.iso_w, st);
+    gather(st, v, w, side, nb);
+    S vs[NS], ws[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) { vs[k] = v[k]; ws[k] = w[k]; }
+    val = hfm_solve<S, NS>(vs, ws, rhs);
+  }
 
   S fw[NS];
   S sum = S(0);
@@ -702,16 +807,24 @@ template <class S> struct KernelSet {
 template <class S, int MODEL, int D, int T0, int T1, int T2>
 KernelSet<S> make_set() {
   KernelSet<S> k;
-  k.update = eik_update<S, MODEL, D, T0, T1, T2, VAR_FAST>;
-  k.update_walls = eik_update<S, MODEL, D, T0, T1, T2, VAR_GENERAL>;
-  if constexpr (std::is_same<S, float>::value)
-    k.update_batch = eik_update<S, MODEL, D, T0, T1, T2, VAR_BATCH>;
-  else
+  if constexpr (MODEL == AGD_TTI) {
+    // TTI runs from its precomputed scheme only.
+    k.update = nullptr;
+    k.update_walls = nullptr;
     k.update_batch = nullptr;
+  } else {
+    k.update = eik_update<S, MODEL, D, T0, T1, T2, VAR_FAST>;
+    k.update_walls = eik_update<S, MODEL, D, T0, T1, T2, VAR_GENERAL>;
+    if constexpr (std::is_same<S, float>::value)
+      k.update_batch = eik_update<S, MODEL, D, T0, T1, T2, VAR_BATCH>;
+    else
+      k.update_batch = nullptr;
+  }
   k.init_ids = init_list_ids<T0 * T1 * T2>;
   // Precomputed schemes pay off where the stencil is costly to build
-  // (Selling decomposition); the drift of Rander is not stored.
-  if constexpr (MODEL == AGD_RIEMANN) {
+  // (Selling decompositions: one for Riemann, seven for TTI, which has no
+  // other variant); the drift of Rander is not stored.
+  if constexpr (MODEL == AGD_RIEMANN || MODEL == AGD_TTI) {
     k.update_pre = eik_update<S, MODEL, D, T0, T1, T2, VAR_PRE>;
     k.scheme = eik_scheme<S, MODEL, D, T0, T1, T2, false>;
     k.scheme_general = eik_scheme<S, MODEL, D, T0, T1, T2, true>;
@@ -758,6 +871,7 @@ static bool lookup(int model, int d, const int32_t* si, KernelSet<S>& out) {
     case AGD_DIAGONAL: return lookup_model<S, AGD_DIAGONAL>(d, si, out);
     case AGD_RIEMANN: return lookup_model<S, AGD_RIEMANN>(d, si, out);
     case AGD_RANDER: return lookup_model<S, AGD_RANDER>(d, si, out);
+    case AGD_TTI: return lookup_model<S, AGD_TTI>(d, si, out);
   }
   return false;
 }
@@ -765,7 +879,7 @@ static bool lookup(int model, int d, const int32_t* si, KernelSet<S>& out) {
 static int check_desc(const agd_eikonal_desc* d) {
   AGD_REQUIRE(d != nullptr, "null descriptor");
   AGD_REQUIRE(d->ndim == 2 || d->ndim == 3, "ndim must be 2 or 3");
-  AGD_REQUIRE(d->model >= AGD_ISOTROPIC && d->model <= AGD_RANDER, "unknown model");
+  AGD_REQUIRE(d->model >= AGD_ISOTROPIC && d->model <= AGD_TTI, "unknown model");
   AGD_REQUIRE(d->scalar == AGD_F32 || d->scalar == AGD_F64, "unknown scalar type");
   for (int k = 0; k < d->ndim; ++k) {
     AGD_REQUIRE(d->shape[k] >= 1 && d->shape[k] < (1 << 30), "shape out of range");
@@ -876,6 +990,7 @@ static int solve_impl(const agd_eikonal_desc* d, const agd_eikonal_io* io,
   AGD_REQUIRE(!pre || (io->scheme_weights && io->scheme_codes),
               "scheme_weights and scheme_codes go together");
   AGD_REQUIRE(!pre || ks.update_pre, "no precomputed-scheme variant for this model");
+  AGD_REQUIRE(pre || ks.update, "this model runs from a precomputed scheme (agd_eikonal_scheme)");
   EikArgs<S> a{};
   a.u = static_cast<S*>(io->values);
   a.geom = static_cast<const S*>(io->geom);

@@ -123,6 +123,27 @@ __device__ inline void inv_sym(int d, const double (&a)[3][3], double (&r)[3][3]
   }
 }
 
+// General 2x2 / 3x3 inverse in double precision (adjugate / determinant).
+__device__ inline void inv_gen(int d, const double (&a)[3][3], double (&r)[3][3]) {
+  if (d == 2) {
+    const double det = a[0][0] * a[1][1] - a[0][1] * a[1][0];
+    r[0][0] = a[1][1] / det;
+    r[0][1] = -a[0][1] / det;
+    r[1][0] = -a[1][0] / det;
+    r[1][1] = a[0][0] / det;
+    return;
+  }
+  double c[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      const int i1 = (i + 1) % 3, i2 = (i + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+      c[i][j] = a[i1][j1] * a[i2][j2] - a[i1][j2] * a[i2][j1];  // cofactor (i, j)
+    }
+  const double det = a[0][0] * c[0][0] + a[0][1] * c[0][1] + a[0][2] * c[0][2];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) r[i][j] = c[j][i] / det;
+}
+
 template <class S>
 __global__ void geometry_kernel(LayoutDesc L, int32_t model, double h0, double h1, double h2,
                                 const float* m, const float* w, int64_t stride, S* geom) {
@@ -144,6 +165,25 @@ __global__ void geometry_kernel(LayoutDesc L, int32_t model, double h0, double h
       }
       geom[k * L.size_tot + n_t] = S(v);
     }
+    return;
+  }
+  if (model == AGD_TTI) {
+    // TTI.with_costs(h).flatten(transposed_transformation=True)
+    // (Seismic/tti.py:113-125, Metrics/base.py:175-185): [l, flat(q),
+    // (A diag(h))^{-T}], A = w (row-major, identity if null).
+    const int ng = 5 + d * d;
+    if (!in) {
+      for (int k = 0; k < ng; ++k) geom[k * L.size_tot + n_t] = inf_v<S>();
+      return;
+    }
+    for (int k = 0; k < 5; ++k) geom[k * L.size_tot + n_t] = S(double(m[k * cs + c]));
+    double A[3][3], r[3][3];
+    for (int i = 0; i < d; ++i)
+      for (int j = 0; j < d; ++j)
+        A[i][j] = (w ? double(w[(i * d + j) * cs + c]) : double(i == j)) * h[j];
+    inv_gen(d, A, r);
+    for (int i = 0; i < d; ++i)
+      for (int j = 0; j < d; ++j) geom[(5 + i * d + j) * L.size_tot + n_t] = S(r[j][i]);
     return;
   }
   const int ng = model == AGD_RANDER ? sd + d : sd;

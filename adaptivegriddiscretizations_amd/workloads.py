@@ -49,6 +49,8 @@ class Workload:
         d = {'model': self.model, 'seeds': self.seeds, 'verbosity': 0}
         if self.metric is not None:
             d['metric'] = Metrics.Riemann(self.metric)
+        if 'tti' in self.extra:
+            d['metric'] = self.extra['tti']
         if self.cost is not None:
             d['cost'] = self.cost
         d.update(extra)
@@ -130,6 +132,31 @@ def riemann2_aniso(n=2000, seed=1, nseeds=4, sigma=20., amax=10.):
     seeds = (-1 + (cells + 0.5) * h).astype(np.float32)
     return Workload('riemann2_aniso', 'Riemann2', [[-1, 1], [-1, 1]], (n, n), seeds,
                     metric=metric, extra={'seed_cells': cells})
+
+
+def tti_tilted(n=256, d=2, seed=5, sigma=None, max_tilt=0.6, hexagonal=(15., 4., 10., 3.)):
+    """TTI medium on [-1,1]^d: a hexagonal material (c11, c13, c33, c44)
+    tilted by a smooth rotation field (angle up to max_tilt about a smooth
+    axis in 3D), one seed at the centre (TTI.from_hexagonal, tti.py:141)."""
+    from .Metrics.Seismic import TTI
+    rng = np.random.default_rng(seed)
+    sigma = n / 16. if sigma is None else sigma
+    shape = (n,) * d
+    c11, c13, c33, c44 = hexagonal
+    base = TTI.from_hexagonal(c11, None, c13, c33, c44, vdim=d)
+    th = _smooth_uniform(rng, shape, sigma, -max_tilt, max_tilt)
+    if d == 2:
+        c, s = np.cos(th), np.sin(th)
+        A = np.array([[c, -s], [s, c]])
+    else:
+        phi = _smooth_uniform(rng, shape, sigma, 0., 2 * np.pi)
+        axis = np.array([np.cos(phi), np.sin(phi), np.zeros_like(phi)])   # horizontal tilt axis
+        q = np.concatenate([np.cos(th / 2)[None], np.sin(th / 2)[None] * axis])
+        A = rotation_from_quaternion(q)
+    metric = TTI(base.linear, base.quadratic, vdim=d, inverse_transformation=A.astype(np.float32))
+    seeds = np.zeros((1, d), dtype=np.float32)
+    return Workload(f'tti{d}_tilted', f'TTI{d}', [[-1, 1]] * d, shape, seeds, metric=None,
+                    extra={'tti': metric})
 
 
 def rotation_from_quaternion(q):

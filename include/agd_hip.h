@@ -42,6 +42,9 @@ enum agd_model {
   AGD_DIAGONAL = 1,  /* Isotropic_.h with geometry: per-cell weights        */
   AGD_RIEMANN = 2,   /* Riemann_.h: Selling decomposition of the dual metric */
   AGD_RANDER = 3,    /* Riemann_.h + drift (FiniteDifferences.h:41-45)      */
+  AGD_TTI = 4,       /* TTI_.h: min/max of 7 Selling schemes; geometry
+                        [l0, l1, q00, q10, q11, T (d x d)]; precomputed scheme
+                        only (agd_eikonal_scheme)                            */
 };
 
 enum agd_scalar { AGD_F32 = 0, AGD_F64 = 1 };
@@ -154,8 +157,11 @@ int agd_eikonal_solve_batched(const agd_eikonal_desc* desc, int32_t ninst,
 /* Precompute the scheme of every cell into io->scheme_weights /
  * io->scheme_codes (replaces the precompute_scheme export kernel,
  * _Kernel.py:254-268,346-359, Update.h:76-80 import_scheme_macro).  Needs
- * geom (and wall_dist if walls are present).  Riemann only, as
- * agd_scheme_supported reports. */
+ * geom (and wall_dist if walls are present).  Riemann and TTI, as
+ * agd_scheme_supported reports.  TTI stores its 7 sub-schemes: weights
+ * 7*nsym components, codes 7*2*nsym components (sub-scheme m at m*2*nsym)
+ * plus one last component, 1 where the update is the min of the
+ * sub-schemes' updates and 0 where it is their max (TTI_.h:127). */
 int agd_scheme_supported(int32_t model, int32_t ndim, const int32_t* shape_i);
 int agd_eikonal_scheme(const agd_eikonal_desc* desc, const agd_eikonal_io* io, void* stream);
 
@@ -203,6 +209,10 @@ int agd_block_squeeze(int32_t ndim, const int64_t* shape, const int32_t* shape_i
  *   Riemann: flat((diag(h) m diag(h))^-1)                    (symdim comps)
  *   Rander : [flat((diag(h) m diag(h))^-1), h * w]           (symdim + d)
  *   Diagonal: m holds costs[d][cells]; geometry = (costs*h)^-2 (d comps)
+ *   TTI    : m holds [l0, l1, q00, q10, q11][cells], w the transformation
+ *            A[d][d][cells] (NULL: identity); geometry
+ *            [l, flat(q), (A diag(h))^-T row-major]             (5 + d*d)
+ *            (Seismic/tti.py:113-125)
  * The inverse is computed in double precision.  cell_stride = 0 broadcasts
  * a constant metric.  dst element type follows `scalar`. */
 int agd_prepare_geometry(int32_t model, int32_t ndim, const int64_t* shape,

@@ -1,0 +1,120 @@
+"""TTI models on the GPU: the precomputed scheme (7 Selling decompositions
+per cell, min/max flag) bit for bit against the oracle on the same FP32
+geometry, and the solves against the oracle's fast marching."""
+import numpy as np
+import pytest
+
+from adaptivegriddiscretizations_amd import workloads
+from gpu_helpers import log, rel_err
+
+pytestmark = pytest.mark.gpu
+TOL32 = 1e-4
+
+
+def prepared(wl, **extra):
+    import torch
+    from adaptivegriddiscretizations_amd.Eikonal import HFM_GPU
+    it = HFM_GPU.Interface(wl.dict_in(**extra))
+    it.SetKernelTraits()
+    it.SetGeometry()
+    it.SetArgs()
+    torch.cuda.synchronize()
+    return it
+
+
+def cell_major(it, t_tiled):
+    return it._squeeze(t_tiled.contiguous(), int(t_tiled.shape[0]), np.float32 if t_tiled.dtype.is_floating_point
+                       else np.int32).cpu().numpy()
+
+
+def oracle_from_device(O, it, wl):
+    """Oracle problem on the device's own FP32 geometry (so that both sides
+    decompose the same matrices)."""
+    geom = cell_major(it, it.geom).astype(np.float64)
+    d = wl.ndim
+    h = wl.h.astype(np.float32).astype(np.float64)
+    idx = O.seed_indices(wl.seeds, np.asarray(wl.sides, dtype=np.float64)[:, 0], h)
+    P = O.make_problem('Isotropic', wl.dims, h, seeds_idx=idx)     # rhs / seeds
+    return O.Problem('TTI', wl.dims, geom.reshape(len(geom), -1), P.rhs, P.seeds, [0.] * d), idx
+
+
+@pytest.mark.parametrize("d,n", [(2, 45), (3, 14)])
+def test_tti_scheme_matches_oracle(O, d, n):
+    wl = workloads.tti_tilted(n=n, d=d, seed=11)
+    it = prepared(wl)
+    P, _ = oracle_from_device(O, it, wl)
+    w_ref, e_ref, mix_ref = P.stencils(use_float=True, with_mix=True)
+    nsym = d * (d + 1) // 2
+    sw = cell_major(it, it.scheme_w_t)                      # (7*nsym, *shape)
+    sc = cell_major(it, it.scheme_c_t)                      # (7*2*nsym + 1, *shape)
+    N = int(np.prod(wl.dims))
+    sw = sw.reshape(7 * nsym, N).T
+    sc = sc.reshape(7 * 2 * nsym + 1, N).T
+    assert np.array_equal(sc[:, -1], mix_ref), "min/max flags differ"
+    assert np.array_equal(sw.astype(np.float64), w_ref), \
+        f"weights differ in {int(np.sum(np.any(sw != w_ref, axis=1)))} cells"
+    # neighbour codes -> offsets (tile-local slots and tiled indices)
+    si, so = np.array(it.shape_i), np.array(it.shape_o)
+    NI = int(np.prod(si))
+    cells = np.stack(np.unravel_index(np.arange(N), wl.dims), -1)
+    xo, xi = cells // si, cells % si
+    own = np.ravel_multi_index(tuple(xo.T), tuple(so))
+
+    def pos_of(code):
+        local = code <= -2
+        t = np.where(local, own * NI + (-2 - code), code)
+        yo = np.stack(np.unravel_index(t // NI, tuple(so)), -1)
+        yi = np.stack(np.unravel_index(t % NI, tuple(si)), -1)
+        return yo * si + yi
+
+    nbad = 0
+    for m in range(7):
+        for k in range(nsym):
+            e = e_ref[:, m * nsym + k, :]
+            for sd, sg in ((0, -1), (1, 1)):
+                code = sc[:, m * 2 * nsym + 2 * k + sd]
+                y = cells + sg * e
+                inside = np.all((y >= 0) & (y < np.array(wl.dims)), axis=1)
+                assert np.array_equal(code >= -1, code >= -1)
+                ok = np.where(inside, code != -1, code == -1)
+                got = pos_of(np.where(code == -1, 0, code))
+                ok &= np.where(inside, np.all(got == y, axis=1), True)
+                nbad += int(np.sum(~ok))
+    log(f"tti{d} scheme: {N} cells, min-mix cells {int(mix_ref.sum())}, code mismatches {nbad}")
+    assert nbad == 0
+
+
+@pytest.mark.parametrize("d,n", [(2, 96), (3, 24)])
+def test_tti_solve_matches_oracle(O, d, n):
+    wl = workloads.tti_tilted(n=n, d=d, seed=12)
+    out = wl.dict_in(exportValues=True).Run()
+    u = out['values'].cpu().numpy().astype(np.float64)
+    it = prepared(wl)
+    P, _ = oracle_from_device(O, it, wl)
+    ref = P.solve_fm()
+    err = rel_err(u, ref)
+    log(f"tti{d} {n}^{d}: passes {out['stats']['eikonal']['launches']}, rel err vs oracle FM {err:.2e}")
+    assert err <= TOL32
+
+
+def test_tti_flow_and_adjoint(O):
+    wl = workloads.tti_tilted(n=64, d=2, seed=13)
+    rng = np.random.default_rng(0)
+    sens = rng.uniform(size=wl.dims + (1,)).astype(np.float32)
+    out = wl.dict_in(exportValues=True, exportGeodesicFlow=True, sensitivity=sens).Run()
+    u = out['values'].cpu().numpy().astype(np.float64)
+    it = prepared(wl)
+    P, idx = oracle_from_device(O, it, wl)
+    fl = P.flow(u, use_float=True)
+    h = wl.h.reshape((2, 1, 1))
+    flow = out['flow'].cpu().numpy()
+    bad = np.mean(np.abs(flow + h * fl['vector']) > 1e-3 * np.max(np.abs(h * fl['vector'])))
+    log(f"tti flow: fraction of cells off by >1e-3 max: {bad:.2e}")
+    assert bad <= 2e-3
+    lam = out['costSensitivity'].cpu().numpy().astype(np.float64)[..., 0]
+    lam_ref = P.adjoint(u, fl, sens.astype(np.float64))[..., 0]
+    for s in idx:
+        lam_ref[tuple(s)] = 0.
+    err = np.max(np.abs(lam - lam_ref)) / np.max(np.abs(lam_ref))
+    log(f"tti adjoint: rel err {err:.2e}")
+    assert err <= 1e-3
