@@ -8,6 +8,7 @@
 //            _SetGeometry.py:127-137, riemann.py:38-40,68-73, misc.py:8-15
 //   Rander:  [flat(inv(h m h)), h w]  _SetGeometry.py:139, rander.py:61-68
 //   Diagonal: (costs h)^-2           _SetGeometry.py:134-135, diagonal.py:32-35
+//   TTI:     [l, flat(q), (A h)^-T]  _SetGeometry.py:140-142, Seismic/tti.py:113-125
 //   Padding with +inf                _SetGeometry.py:189-190
 #include <hip/hip_runtime.h>
 
@@ -285,8 +286,9 @@ int agd_prepare_geometry(int32_t model, int32_t ndim, const int64_t* shape, cons
   int st = make_layout(ndim, shape, shape_i, L);
   if (st) return st;
   AGD_REQUIRE(ndim == 2 || ndim == 3, "ndim must be 2 or 3");
-  AGD_REQUIRE(model == AGD_RIEMANN || model == AGD_RANDER || model == AGD_DIAGONAL,
-              "geometry preparation applies to Diagonal, Riemann and Rander models");
+  AGD_REQUIRE(model == AGD_RIEMANN || model == AGD_RANDER || model == AGD_DIAGONAL ||
+                  model == AGD_TTI,
+              "geometry preparation applies to Diagonal, Riemann, Rander and TTI models");
   AGD_REQUIRE(h && m && geom, "null buffer");
   AGD_REQUIRE(model != AGD_RANDER || w, "Rander model needs the drift w");
   AGD_REQUIRE(cell_stride == 0 || cell_stride == 1, "cell stride must be 0 or 1");
