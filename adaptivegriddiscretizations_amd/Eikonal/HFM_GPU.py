@@ -140,9 +140,16 @@ class Interface:
                              help="Optional trait parameters for the eikonal kernel.")
         traits = {**traits, **(user or {})}
         self.traits = traits
-        for key, name in (('multiprecision', 'multiprecision'), ('values_float64', 'values_float64')):
-            if self.GetValue(key, default=False, help=f"{name} arithmetic (not provided)"):
-                raise NotImplementedError(f"'{key}' is not provided by the MI355X solver")
+        # Multiprecision (_Kernel.py:36-44): the reference splits values into
+        # float32 + integer multiples of a step; here the solve runs in FP64
+        # arithmetic (same rate as FP32 on the vector units), with the
+        # reference's float32-based default tolerances and rtol = 0.
+        self.values_float64 = bool(self.GetValue(
+            'values_float64', default=False,
+            help="Export values in float64 (implies multiprecision)"))
+        self.multiprecision = self.values_float64 or bool(self.GetValue(
+            'multiprecision', default=False,
+            help="Use multiprecision arithmetic, to improve accuracy (FP64 arithmetic here)"))
         if self.GetValue('factoringRadius', default=0, help="Source factorization (not provided)"):
             raise NotImplementedError("source factorization is not provided by the MI355X solver")
         self.order = self.GetValue('order', default=1, help="Scheme order (1 only)")
@@ -155,6 +162,10 @@ class Interface:
         self.float_t = np.dtype(traits['Scalar']).type
         if self.float_t not in (np.float32, np.float64):
             raise ValueError(f"Unsupported Scalar trait {traits['Scalar']}")
+        self.tol_float_t = self.float_t      # resolution behind the default tolerances
+        self.export_t = np.float64 if self.values_float64 else self.float_t
+        if self.multiprecision:
+            self.float_t = np.float64
         self.scalar = _lib.F32 if self.float_t == np.float32 else _lib.F64
         self.shape_i = tuple(int(s) for s in traits['shape_i'])
         self.niter_i = int(traits['niter_i'])
@@ -237,7 +248,7 @@ class Interface:
                                            help="First order variation of the cost function")
 
         # Tolerances (_SetGeometry.py:237-253)
-        res = np.finfo(self.float_t).resolution
+        res = np.finfo(self.tol_float_t).resolution
         if self.HasValue('atol') and self.HasValue('rtol'):
             tol = None
         else:
@@ -248,10 +259,10 @@ class Interface:
                 mean_cost = float(torch.nanmean(c.float()).item()) if hasattr(c, 'detach') \
                     else float(np.nanmean(np.asarray(c, dtype=np.float64)))
                 tol = mean_cost * res * 5.
-                self.hfmOut['keys']['default']['tol'] = self.float_t(tol)
+                self.hfmOut['keys']['default']['tol'] = self.tol_float_t(tol)
         self.atol = float(self.GetValue('atol', default=tol,
                                         help="Absolute convergence tolerance for the fixed point solver"))
-        self.rtol = float(self.GetValue('rtol', default=float(res * 5.),
+        self.rtol = float(self.GetValue('rtol', default=0. if self.multiprecision else float(res * 5.),
                                         help="Relative convergence tolerance for the fixed point solver"))
 
         # Walls
@@ -547,6 +558,9 @@ class Interface:
             _lib.check(_lib.lib.agd_block_squeeze(self.ndim, _lib.i64(self.shape), _lib.i32(self.shape_i),
                                                   1, np.dtype(self.float_t).itemsize,
                                                   ptr(self.values_t), ptr(out), self.stream))
+            if self.export_t != self.float_t:
+                from ._device import tdtype
+                out = out.to(tdtype(self.export_t))
             self._values = out
         if self.values_coef is not None:
             return ad.denseAD(self._output(self._values), self._output(self.values_coef))

@@ -13,7 +13,8 @@ PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(PKG_DIR, "csrc")
 LIB_NAME = "libagd_hip.so"
 LIB_PATH = os.path.join(PKG_DIR, LIB_NAME)
-SOURCES = ["layout.hip", "eikonal.hip", "adjoint.hip"]
+SOURCES = ["layout.hip", "eikonal.hip", "adjoint.hip", "eik_isotropic.hip", "eik_diagonal.hip",
+           "eik_riemann.hip", "eik_rander.hip", "eik_tti.hip"]
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 CFLAGS = [

@@ -1,31 +1,17 @@
-// eikonal.hip -- the eikonal tile solver for gfx950: grid-level worklist
-// driver, the fused tile-update kernel, the flow export kernel and the
-// seed / wall / stencil helpers.
-//
-// Tile update (reference: Update.h:20-227, HFMIter.h:10-86,
-// Propagation.h:18-167, FiniteDifferences.h:17-106):
-//   one workgroup per active tile, one lane per cell.  The lane loads its
-//   value, rhs, seed bit and geometry (coalesced: the tiled layout makes a
-//   tile contiguous), builds its stencil in registers (Selling for
-//   Riemann/Rander), then resolves each of its 2*nsym neighbours once: an LDS
-//   slot when the neighbour lies in the tile, otherwise the neighbour's value
-//   gathered from HBM/L2 and held in a register for the whole visit (values
-//   outside the tile do not change during the in-tile sweeps).  The tile's
-//   own values live in LDS and are relaxed niter_i times (Jacobi inside a
-//   sweep, early exit once a sweep changes nothing -- which cannot alter the
-//   result since the next sweep would read identical inputs).  The change
-//   test |u_old-u_new| <= atol + rtol|u_new| is reduced with a block-wide
-//   OR (a single wave64 ballot for 64-cell tiles) and drives tile activation.
+// eikonal.hip -- host side of the eikonal tile solver for gfx950: argument
+// checks, the per-device workspace, the grid-level worklist drivers (single,
+// batched), the flow and precomputed-scheme entry points, and the
+// model-independent kernels (initial lists, seeds, walls, stencil export).
+// The model kernels are in eikonal_kernels.hpp.
 //
 // Worklist (replaces the host round trip of _solvers.py:122-193): the tile
-// list of pass k+1 is built on the device while pass k runs.  A tile is
-// appended at most once per pass thanks to a per-tile stamp
+// list of pass k+1 is built on the device while pass k runs (worklist.hpp).
+// A tile is appended at most once per pass thanks to a per-tile stamp
 // (atomicMax(stamp, k+1) < k+1 <=> first to tag it), so neither the list
-// nor the flags need clearing.  Three rotating counters: pass k reads
-// count[k%3], appends to count[(k+1)%3] and zeroes count[(k+2)%3], which no
-// later pass of this launch touches before pass k+1.  Every launch uses the
-// same grid; workgroups stride over the list.  The host reads the next
-// count every `check_interval` passes (one small copy + sync).
+// nor the flags need clearing.  Every launch uses the same grid; workgroups
+// stride over the list.  The host reads the next list length every
+// `check_interval` passes (one small copy + sync), or after each replay of
+// a captured chunk of passes.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -36,427 +22,10 @@
 
 #include "common.hpp"
 #include "device.hpp"
+#include "eikonal_kernels.hpp"
 #include "worklist.hpp"
 
 namespace agd {
-
-// One instance of a batched solve: its own grid and constants; its tiles
-// occupy [tile_base, tile_base + g.ntiles) of the batch's tile space.
-struct InstDesc {
-  GridDesc g;
-  int32_t tile_base;
-  int32_t pad;
-  double iso_w[3];
-  double atol, rtol;
-};
-
-template <class S> struct EikArgs {
-  S* u;
-  const S* geom;
-  const S* sw;          // precomputed stencil weights (NS components) or null
-  const int32_t* scode; // precomputed neighbour codes (2*NS components) or null
-  const S* rhs;
-  const uint32_t* seeds;
-  const uint8_t* wall;
-  GridDesc g;
-  S atol, rtol;
-  S iso_w[3];
-  int32_t niter_i;
-  int32_t solver;
-  int32_t fim_front;
-  // worklist; FIM scores are packed (pass << 8) | score, two arrays by parity
-  WorkCtx w;
-  // batched solves: instance table and tile -> instance map
-  const InstDesc* inst;
-  const int32_t* tile_inst;
-};
-
-__device__ inline int32_t read_score(const int32_t* arr, int32_t tile, int32_t iter) {
-  const int32_t v = arr[tile];
-  return ((v >> 8) == iter) ? (v & 255) : 0;
-}
-
-// ------------------------------------------------------------------ update
-// Block-wide "any": one ballot for single-wave tiles, an LDS reduction
-// otherwise.
-template <int NI> __device__ inline bool block_any(bool p) {
-  if constexpr (NI <= 64) {
-    __syncthreads();
-    return __ballot(p) != 0;
-  } else {
-    return __syncthreads_or(p) != 0;
-  }
-}
-
-// Resolves the 2*nsym neighbours x +- e_k of a cell into one code each:
-//   code >= 0 : tiled index of a neighbour outside the tile (gathered),
-//   code <= -2: LDS slot (-2 - code) inside the tile,
-//   code == -1: no neighbour (outside the domain or hidden by a wall).
-template <class S, int MODEL, int D, class TL, bool GENERAL>
-__device__ inline void resolve_codes(const Stencil<S, MODEL, D>& st, const int* xi, const int* xt,
-                                     const int* xo, int wx, const uint8_t* wall_s,
-                                     const uint8_t* wall, const GridDesc& g, int* code,
-                                     int tile_base = 0) {
-  constexpr int NS = ModelTraits<MODEL, D>::NSYM;
-  constexpr int NI = TL::NI;
-#pragma unroll
-  for (int k = 0; k < NS; ++k) {
-#pragma unroll
-    for (int s = 0; s < 2; ++s) {
-      const int j = 2 * k + s;
-      const int sg = 2 * s - 1;
-      int e[3] = {sg * st.e[k][0], sg * st.e[k][1], sg * st.e[k][2]};
-      if constexpr (!GENERAL) {
-        // No walls, no periodic axes: unsigned range checks, tile
-        // coordinates by shifts, tile-local test against the tile's own
-        // (uniform) coordinates.
-        int yt[3] = {xt[0] + e[0], xt[1] + e[1], xt[2] + e[2]};
-        bool dom = true;
-#pragma unroll
-        for (int c = 0; c < D; ++c) dom = dom && unsigned(yt[c]) < unsigned(g.shape_tot[c]);
-        int yo[3] = {0, 0, 0}, yi[3] = {0, 0, 0};
-        bool local = true;
-#pragma unroll
-        for (int c = 0; c < D; ++c) {
-          const unsigned u = unsigned(yt[c]);
-          yo[c] = int(u / unsigned(TL::T[c]));
-          yi[c] = int(u % unsigned(TL::T[c]));
-          local = local && yo[c] == xo[c];
-        }
-        const int li = TL::index(yi);
-        int to = yo[0];
-#pragma unroll
-        for (int c = 1; c < D; ++c) to = to * g.shape_o[c] + yo[c];
-        code[j] = !dom ? -1 : (local ? -2 - li : (tile_base + to) * NI + li);
-      } else {
-        bool vis = true;
-        if (wall && wx != 255) {
-          vis = visible<D>(e, wx, [&](const int* w) -> int {
-            int yi[3] = {xi[0] + w[0], xi[1] + w[1], xi[2] + w[2]};
-            int yt[3] = {xt[0] + w[0], xt[1] + w[1], xt[2] + w[2]};
-            bool dom = true;
-#pragma unroll
-            for (int c = 0; c < D; ++c)
-              dom = dom && (((g.periodic >> c) & 1) || (yt[c] >= 0 && yt[c] < g.shape_tot[c]));
-            if (TL::inside(yi) && dom) return wall_s[TL::index(yi)];
-            const int m = index_tot<D, TL>(yt, g);
-            return m >= 0 ? int(wall[m]) : 255;
-          });
-        }
-        int yi[3] = {xi[0] + e[0], xi[1] + e[1], xi[2] + e[2]};
-        int yt[3] = {xt[0] + e[0], xt[1] + e[1], xt[2] + e[2]};
-        bool in_dom = true;
-#pragma unroll
-        for (int c = 0; c < D; ++c) in_dom = in_dom && (yt[c] >= 0 && yt[c] < g.shape_tot[c]);
-        const bool local = vis && TL::inside(yi) && in_dom;
-        const int m = (vis && !local) ? index_tot<D, TL>(yt, g) : -1;
-        code[j] = local ? -2 - TL::index(yi) : m;
-      }
-    }
-  }
-}
-
-// Kernel variants: VAR_FAST assumes no walls and no periodic axes, so its
-// neighbour indexing carries no wrap-around code; VAR_GENERAL handles both;
-// VAR_PRE reads stencil weights and neighbour codes precomputed once per
-// solve (any walls / periodicity are already resolved in the codes);
-// VAR_BATCH is VAR_FAST over the concatenated tiles of several instances,
-// each tile looking up its instance's grid and constants.
-enum { VAR_FAST = 0, VAR_GENERAL = 1, VAR_PRE = 2, VAR_BATCH = 3 };
-
-template <class S, int MODEL, int D, int T0, int T1, int T2, int VAR>
-__global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
-  constexpr bool GENERAL = VAR == VAR_GENERAL;
-  constexpr bool WALLS = GENERAL;
-  constexpr bool BATCH = VAR == VAR_BATCH;
-  static_assert((T0 * T1 * T2) % 64 == 0, "tiles are whole waves (pass_begin scans per wave)");
-  using TL = Tile<D, T0, T1, T2>;
-  using MT = ModelTraits<MODEL, D>;
-  constexpr int NI = TL::NI;
-  constexpr int NS = MT::NSYM;
-  constexpr int NG = MT::NGEOM;
-  constexpr int NJ = 2 * NS;
-  __shared__ S u_s[NI];
-  __shared__ uint8_t wall_s[WALLS ? NI : 1];
-
-  const int tid = threadIdx.x;
-  const PassView pv = pass_begin(a.w);
-  const int count = pv.count;
-  int xi[3];
-  TL::pos(tid, xi);
-  const int32_t tag = pv.iter + 1;
-  const bool fim = a.solver == AGD_FIM;
-
-  for (int li = blockIdx.x; li < count; li += gridDim.x) {
-    const int pk = pass_entry(pv, li);
-    int xo[3];
-    // Grid and constants of the tile's instance (the solve's own when not
-    // batched).  Batched list entries are tile ids, else packed coordinates.
-    GridDesc gi;
-    int tbase = 0;
-    S atol, rtol, iso_w[3];
-    int n_o;
-    if constexpr (BATCH) {
-      n_o = __builtin_amdgcn_readfirstlane(pk);
-      const int ii = __builtin_amdgcn_readfirstlane(a.tile_inst[n_o]);
-      const InstDesc* I = a.inst + ii;
-      gi = I->g;
-      tbase = I->tile_base;
-      atol = S(I->atol);
-      rtol = S(I->rtol);
-#pragma unroll
-      for (int k = 0; k < 3; ++k) iso_w[k] = S(I->iso_w[k]);
-      tile_pos<D>(n_o - tbase, gi, xo);
-    } else {
-      gi = a.g;
-      atol = a.atol;
-      rtol = a.rtol;
-#pragma unroll
-      for (int k = 0; k < 3; ++k) iso_w[k] = a.iso_w[k];
-      unpack_tile<D>(pk, xo);
-      n_o = tile_linear<D>(xo, a.g);
-    }
-    int xt[3] = {0, 0, 0};
-#pragma unroll
-    for (int k = 0; k < D; ++k) xt[k] = xo[k] * TL::T[k] + xi[k];
-    const int n_t = n_o * NI + tid;
-
-    // ---- activation inputs first: the tile and its face neighbours
-    // (lanes 0..2D of wave 0), and their FIM scores of the previous pass.
-    int nb = -1, nbp = pk, sc = 0;
-    if (tid < 2 * D + 1) {
-      if (tid == 2 * D) {
-        nb = n_o;
-      } else {
-        nb = tile_neighbor<D>(xo, tid >> 1, 2 * (tid & 1) - 1, gi, &nbp);
-        if (BATCH && nb >= 0) {
-          nb += tbase;
-          nbp = nb;
-        }
-      }
-      if (fim && nb >= 0) sc = read_score(pv.score_prev, nb, pv.iter - 1);
-    }
-
-    // ---- cell loads (coalesced: a tile is contiguous)
-    const S u_old = a.u[n_t];
-    const S rhs = a.rhs[n_t];
-    const bool seed = (a.seeds[n_t >> 5] >> (n_t & 31)) & 1u;
-    int wx = 255;
-    if constexpr (WALLS) {
-      if (a.wall) wx = a.wall[n_t];
-      wall_s[tid] = uint8_t(wx);
-    }
-    u_s[tid] = seed ? rhs : u_old;
-
-    int code[NJ];
-    S sw[NS], shift[NS];
-    if constexpr (MT::NMIX > 1) {
-      // several sub-schemes: read per sweep below
-    } else if constexpr (VAR == VAR_PRE) {
-      // Stencil weights and neighbour codes precomputed for the whole solve.
-#pragma unroll
-      for (int k = 0; k < NS; ++k) {
-        sw[k] = a.sw[k * a.g.size_tot + n_t];
-        shift[k] = S(0);
-      }
-#pragma unroll
-      for (int j = 0; j < NJ; ++j) code[j] = a.scode[j * a.g.size_tot + n_t];
-    } else {
-      S geo[NG > 0 ? NG : 1];
-#pragma unroll
-      for (int k = 0; k < NG; ++k) {
-        const S* __restrict__ gk = a.geom + k * a.g.size_tot;  // uniform base
-        geo[k] = gk[n_t];
-      }
-      Stencil<S, MODEL, D> st;
-      make_stencil<S, MODEL, D>(geo, iso_w, st);
-      if constexpr (WALLS) __syncthreads();  // wall_s complete before the visibility walks
-      resolve_codes<S, MODEL, D, TL, GENERAL>(st, xi, xt, xo, wx, wall_s, a.wall, gi, code, tbase);
-#pragma unroll
-      for (int k = 0; k < NS; ++k) {
-        sw[k] = st.w[k];
-        shift[k] = st.shift[k];
-      }
-    }
-    if constexpr (MT::NMIX > 1) {
-      // ---- TTI: the min or max over the sub-schemes of their upwind
-      // updates (HFMIter.h:29-56, strict in-tile iteration).  Weights and
-      // neighbour codes are re-read at every sweep (cache hits) rather than
-      // held in registers; out-of-tile values are read at every sweep.
-      const int64_t N = a.g.size_tot;
-      const bool mix_min = a.scode[int64_t(MT::NMIX * NJ) * N + n_t] != 0;
-      __syncthreads();  // u_s complete
-      const bool active = !seed;
-      for (int it = 0; it < a.niter_i; ++it) {
-        S best = mix_min ? inf_v<S>() : -inf_v<S>();
-#pragma unroll 1
-        for (int m = 0; m < MT::NMIX; ++m) {
-          S v[NS], w[NS];
-#pragma unroll
-          for (int k = 0; k < NS; ++k) {
-            w[k] = a.sw[int64_t(m * NS + k) * N + n_t];
-            S side[2];
-#pragma unroll
-            for (int sd = 0; sd < 2; ++sd) {
-              const int c = a.scode[int64_t(m * NJ + 2 * k + sd) * N + n_t];
-              side[sd] = c <= -2 ? u_s[-2 - c] : (c >= 0 ? a.u[c] : inf_v<S>());
-            }
-            v[k] = (side[0] > side[1]) ? side[1] : side[0];
-          }
-          const S um = hfm_solve<S, NS>(v, w, rhs);
-          if (mix_min == (best > um)) best = um;
-        }
-        __syncthreads();
-        const bool ch = active && (u_s[tid] > best);
-        if (ch) u_s[tid] = best;
-        if (!block_any<NI>(ch)) break;
-      }
-    } else {
-      // All gathers are issued before any is consumed (safe address = own
-      // cell), so a visit costs a single memory round trip.
-      S g[NJ];
-#pragma unroll
-      for (int j = 0; j < NJ; ++j) g[j] = a.u[code[j] >= 0 ? code[j] : n_t];
-      S vo[NJ];
-#pragma unroll
-      for (int k = 0; k < NS; ++k) {
-#pragma unroll
-        for (int s = 0; s < 2; ++s) {
-          const int j = 2 * k + s;
-          const S sh = (s == 0) ? shift[k] : -shift[k];
-          vo[j] = code[j] <= -2 ? sh : (code[j] >= 0 ? g[j] + sh : inf_v<S>());
-        }
-      }
-      __syncthreads();  // u_s complete
-
-      // ---- in-tile sweeps (HFMIter.h:25-84, DECREASING)
-      const bool active = !seed;
-      for (int it = 0; it < a.niter_i; ++it) {
-        S v[NS], w[NS];
-#pragma unroll
-        for (int k = 0; k < NS; ++k) {
-          const S a0 = code[2 * k] <= -2 ? u_s[-2 - code[2 * k]] + vo[2 * k] : vo[2 * k];
-          const S a1 = code[2 * k + 1] <= -2 ? u_s[-2 - code[2 * k + 1]] + vo[2 * k + 1] : vo[2 * k + 1];
-          v[k] = (a0 > a1) ? a1 : a0;  // Greater(v,-) keeps the -e side on ties
-          w[k] = sw[k];
-        }
-        const S un = hfm_solve<S, NS>(v, w, rhs);
-        __syncthreads();
-        const bool ch = active && (u_s[tid] > un);
-        if (ch) u_s[tid] = un;
-        if (!block_any<NI>(ch)) break;
-      }
-    }
-
-    // ---- store and convergence (Update.h:163-219)
-    const S un = u_s[tid];
-    a.u[n_t] = un;
-    const S diff = abs_(u_old - un);
-    const S tol = atol + rtol * abs_(un);
-    const bool changed = !(isnan_(diff) || diff <= tol);
-    const bool any = block_any<NI>(changed);
-
-    // ---- activation (Propagation.h:103-167), wave 0 only.  The
-    // global-iteration solver runs with AGSI tags: they only decide when to
-    // stop, every pass visits the full list.
-    if (tid < 64) {
-      if (!fim) {
-        enqueue_wave(any && nb >= 0, nb, nbp, tag, a.w.stamp, pv.list_out, pv.count_out);
-      } else {
-        const int front = a.fim_front;
-        const int prev = __shfl(sc, 2 * D);
-        int mx = 0;
-#pragma unroll
-        for (int i = 0; i < 2 * D; ++i) mx = max(mx, __shfl(sc, i));
-        int next;
-        bool prop;
-        if (!any) {
-          if (prev == 0 || prev == front) { next = 0; prop = false; }
-          else { next = front; prop = true; }
-        } else {
-          next = 1;
-          if (front > 2) next = max(next, mx - 1);
-          prop = next > 1;
-        }
-        if (tid == 2 * D) pv.score_next[n_o] = (pv.iter << 8) | next;
-        const bool want = (tid == 2 * D) ? any : (prop && tid < 2 * D && nb >= 0);
-        enqueue_wave(want, nb, nbp, tag, a.w.stamp, pv.list_out, pv.count_out);
-      }
-    }
-    __syncthreads();  // LDS reuse by the next tile of this workgroup
-  }
-}
-
-// ------------------------------------------------------------------ scheme
-// Stencil weights and neighbour codes of every cell, computed once per solve
-// for the VAR_PRE update (the counterpart of the reference's
-// precompute_scheme export kernel, _Kernel.py:254-268 and 346-359, with
-// offsets stored already resolved against the tiled grid).
-template <class S> struct SchemeArgs {
-  const S* geom;
-  const uint8_t* wall;
-  GridDesc g;
-  S iso_w[3];
-  S* sw;
-  int32_t* scode;
-};
-
-template <class S, int MODEL, int D, int T0, int T1, int T2, bool GENERAL>
-__global__ void __launch_bounds__(T0* T1* T2) eik_scheme(SchemeArgs<S> a) {
-  using TL = Tile<D, T0, T1, T2>;
-  using MT = ModelTraits<MODEL, D>;
-  constexpr int NI = TL::NI;
-  constexpr int NS = MT::NSYM;
-  constexpr int NG = MT::NGEOM;
-  __shared__ uint8_t wall_s[GENERAL ? NI : 1];
-  const int tid = threadIdx.x;
-  const int n_o = blockIdx.x;
-  int xi[3], xo[3], xt[3] = {0, 0, 0};
-  TL::pos(tid, xi);
-  tile_pos<D>(n_o, a.g, xo);
-#pragma unroll
-  for (int k = 0; k < D; ++k) xt[k] = xo[k] * TL::T[k] + xi[k];
-  const int64_t N = a.g.size_tot;
-  const int64_t n_t = int64_t(n_o) * NI + tid;
-  int wx = 255;
-  if constexpr (GENERAL) {
-    if (a.wall) wx = a.wall[n_t];
-    wall_s[tid] = uint8_t(wx);
-    __syncthreads();
-  }
-  S geo[NG > 0 ? NG : 1];
-#pragma unroll
-  for (int k = 0; k < NG; ++k) geo[k] = a.geom[k * N + n_t];
-  if constexpr (MT::NMIX > 1) {
-    // TTI_.h:112-134: one Selling decomposition per tangent ellipse, then
-    // the min/max flag after the NMIX*2*NS codes.
-    S diag[kTTIMix][2];
-    const bool mix_min = tti_diags<S>(geo, geo + 2, diag);
-    S D0[NS], D1[NS];
-    tti_base<S, D>(geo, D0, D1);
-#pragma unroll 1
-    for (int m = 0; m < MT::NMIX; ++m) {
-      Stencil<S, MODEL, D> st;
-      tti_sub<S, D>(D0, D1, diag[m][0], diag[m][1], st.w, st.e);
-      int code[2 * NS];
-      resolve_codes<S, MODEL, D, TL, GENERAL>(st, xi, xt, xo, wx, wall_s, a.wall, a.g, code);
-#pragma unroll
-      for (int k = 0; k < NS; ++k) a.sw[int64_t(m * NS + k) * N + n_t] = st.w[k];
-#pragma unroll
-      for (int j = 0; j < 2 * NS; ++j) a.scode[int64_t(m * 2 * NS + j) * N + n_t] = code[j];
-    }
-    a.scode[int64_t(MT::NMIX * 2 * NS) * N + n_t] = mix_min ? 1 : 0;
-  } else {
-    Stencil<S, MODEL, D> st;
-    make_stencil<S, MODEL, D>(geo, a.iso_w, st);
-    int code[2 * NS];
-    resolve_codes<S, MODEL, D, TL, GENERAL>(st, xi, xt, xo, wx, wall_s, a.wall, a.g, code);
-#pragma unroll
-    for (int k = 0; k < NS; ++k) a.sw[k * N + n_t] = st.w[k];
-#pragma unroll
-    for (int j = 0; j < 2 * NS; ++j) a.scode[j * N + n_t] = code[j];
-  }
-}
 
 // Initial list: tiles holding at least one trigger bit (or every tile).
 // 64-lane workgroups, one tile per lane, wave-aggregated appends into
@@ -539,164 +108,6 @@ __global__ void all_tiles_kernel(GridDesc g, int32_t* list, int32_t* count) {
     list[t] = pack_tile<D>(xo);
   }
   if (t == 0) *count = g.ntiles;
-}
-
-// ------------------------------------------------------------------ flow
-// One global pass with niter_i = 1 and flow export (Update.h:149-209).
-template <class S> struct FlowArgs {
-  S* u;
-  const S* geom;
-  const S* rhs;
-  const uint32_t* seeds;
-  const uint8_t* wall;
-  GridDesc g;
-  S iso_w[3];
-  S* fw;
-  S* wsum;
-  int32_t* fidx;
-  S* fvec;
-};
-
-template <class S, int MODEL, int D, int T0, int T1, int T2>
-__global__ void __launch_bounds__(T0* T1* T2) eik_flow(FlowArgs<S> a) {
-  using TL = Tile<D, T0, T1, T2>;
-  using MT = ModelTraits<MODEL, D>;
-  constexpr int NI = TL::NI;
-  constexpr int NS = MT::NSYM;
-  constexpr int NG = MT::NGEOM;
-  const int tid = threadIdx.x;
-  const int n_o = blockIdx.x;
-  int xi[3], xo[3], xt[3] = {0, 0, 0};
-  TL::pos(tid, xi);
-  tile_pos<D>(n_o, a.g, xo);
-#pragma unroll
-  for (int k = 0; k < D; ++k) xt[k] = xo[k] * TL::T[k] + xi[k];
-  const int64_t n_t = int64_t(n_o) * NI + tid;
-  const int64_t N = a.g.size_tot;
-
-  const S u_old = a.u[n_t];
-  const S rhs = a.rhs[n_t];
-  const bool seed = (a.seeds[n_t >> 5] >> (n_t & 31)) & 1u;
-  S geo[NG > 0 ? NG : 1];
-#pragma unroll
-  for (int k = 0; k < NG; ++k) geo[k] = a.geom[k * N + n_t];
-  const int wx = a.wall ? int(a.wall[n_t]) : 255;
-
-  // Neighbour values (min over -e/+e, ties keep -e), sides and indices of a
-  // stencil.
-  auto gather = [&](const Stencil<S, MODEL, D>& st, S* v, S* w, int* side, int64_t* nb) {
-#pragma unroll
-    for (int k = 0; k < NS; ++k) {
-      S best = inf_v<S>();
-      int bs = 0;
-      int64_t bi = -1;
-#pragma unroll
-      for (int s = 0; s < 2; ++s) {
-        const int sg = 2 * s - 1;
-        int e[3] = {sg * st.e[k][0], sg * st.e[k][1], sg * st.e[k][2]};
-        const S sh = (s == 0) ? st.shift[k] : -st.shift[k];
-        bool vis = true;
-        if (a.wall && wx != 255) {
-          vis = visible<D>(e, wx, [&](const int* ww) -> int {
-            int yt[3] = {xt[0] + ww[0], xt[1] + ww[1], xt[2] + ww[2]};
-            const int64_t m = index_tot<D, TL>(yt, a.g);
-            return m >= 0 ? int(a.wall[m]) : 255;
-          });
-        }
-        int yt[3] = {xt[0] + e[0], xt[1] + e[1], xt[2] + e[2]};
-        const int64_t m = vis ? index_tot<D, TL>(yt, a.g) : -1;
-        const S val = (m >= 0) ? a.u[m] + sh : inf_v<S>();
-        if (s == 0) { best = val; bs = 0; bi = m; }
-        else if (best > val) { best = val; bs = 1; bi = m; }
-      }
-      v[k] = best;
-      side[k] = bs;
-      nb[k] = bi;
-      w[k] = st.w[k];
-    }
-  };
-
-  Stencil<S, MODEL, D> st;  // the (chosen sub-)stencil
-  S v[NS], w[NS];
-  int side[NS];
-  int64_t nb[NS];
-  S val;
-  if constexpr (MT::NMIX > 1) {
-    // TTI: keep the sub-scheme whose update wins the min/max (HFMIter.h:48-54).
-#pragma unroll
-    for (int k = 0; k < NS; ++k) {
-      v[k] = inf_v<S>();
-      w[k] = S(0);
-      side[k] = 0;
-      nb[k] = -1;
-      st.w[k] = S(0);
-      st.shift[k] = S(0);
-      st.e[k][0] = st.e[k][1] = st.e[k][2] = 0;
-    }
-    S diag[kTTIMix][2];
-    const bool mix_min = tti_diags<S>(geo, geo + 2, diag);
-    S D0[NS], D1[NS];
-    tti_base<S, D>(geo, D0, D1);
-    val = mix_min ? inf_v<S>() : -inf_v<S>();
-#pragma unroll 1
-    for (int m = 0; m < MT::NMIX; ++m) {
-      Stencil<S, MODEL, D> cand;
-      tti_sub<S, D>(D0, D1, diag[m][0], diag[m][1], cand.w, cand.e);
-#pragma unroll
-      for (int k = 0; k < NS; ++k) cand.shift[k] = S(0);
-      S cv[NS], cw[NS];
-      int cs[NS];
-      int64_t cn[NS];
-      gather(cand, cv, cw, cs, cn);
-      S vs[NS], ws[NS];
-#pragma unroll
-      for (int k = 0; k < NS; ++k) { vs[k] = cv[k]; ws[k] = cw[k]; }
-      const S um = hfm_solve<S, NS>(vs, ws, rhs);
-      if (mix_min == (val > um)) {
-        val = um;
-        st = cand;
-#pragma unroll
-        for (int k = 0; k < NS; ++k) { v[k] = cv[k]; w[k] = cw[k]; side[k] = cs[k]; nb[k] = cn[k]; }
-      }
-    }
-  } else {
-    make_stencil<S, MODEL, D>(geo, a.iso_w, st);
-    gather(st, v, w, side, nb);
-    S vs[NS], ws[NS];
-#pragma unroll
-    for (int k = 0; k < NS; ++k) { vs[k] = v[k]; ws[k] = w[k]; }
-    val = hfm_solve<S, NS>(vs, ws, rhs);
-  }
-
-  S fw[NS];
-  S sum = S(0);
-  S vec[3] = {S(0), S(0), S(0)};
-  const bool live = !seed && val != inf_v<S>();
-#pragma unroll
-  for (int k = 0; k < NS; ++k) {
-    const S diff = val - v[k];
-    fw[k] = live ? w[k] * (diff > S(0) ? diff : S(0)) : S(0);
-    sum += fw[k];
-    const int sg = 2 * side[k] - 1;
-#pragma unroll
-    for (int c = 0; c < D; ++c) vec[c] += fw[k] * S(sg * st.e[k][c]);
-  }
-  __syncthreads();  // every lane has read its neighbours before any store
-  if (!seed && val < u_old) a.u[n_t] = val;
-  if (a.fw) {
-#pragma unroll
-    for (int k = 0; k < NS; ++k) a.fw[k * N + n_t] = fw[k];
-  }
-  if (a.fidx) {
-#pragma unroll
-    for (int k = 0; k < NS; ++k)
-      a.fidx[k * N + n_t] = (fw[k] != S(0) && nb[k] >= 0) ? int32_t(nb[k]) : INT32_MAX;
-  }
-  if (a.wsum) a.wsum[n_t] = sum;
-  if (a.fvec) {
-#pragma unroll
-    for (int c = 0; c < D; ++c) a.fvec[c * N + n_t] = vec[c];
-  }
 }
 
 // ------------------------------------------------------------------ stencil export
@@ -789,91 +200,55 @@ __global__ void wall_dist_cap(int64_t N, uint8_t* d, int bound) {
 }
 
 // ------------------------------------------------------------------ dispatch
-template <class S> struct KernelSet {
-  void (*update)(EikArgs<S>);
-  void (*update_walls)(EikArgs<S>);
-  void (*update_pre)(EikArgs<S>);          // null: no precomputed-scheme variant
-  void (*update_batch)(EikArgs<S>);        // null: no batched variant (FP64)
-  void (*init_ids)(const uint32_t*, int32_t, int32_t, int32_t*, int32_t*, int32_t*, int32_t);
-  void (*scheme)(SchemeArgs<S>);
-  void (*scheme_general)(SchemeArgs<S>);
-  void (*flow)(FlowArgs<S>);
+// Model-independent kernels of one tile shape.
+struct TileKernels {
   void (*init)(const uint32_t*, GridDesc, int32_t, int32_t*, int32_t*, int32_t*, int32_t);
+  void (*init_ids)(const uint32_t*, int32_t, int32_t, int32_t*, int32_t*, int32_t*, int32_t);
   void (*all_tiles)(GridDesc, int32_t*, int32_t*);
   void (*wallpass)(GridDesc, const uint8_t*, uint8_t*);
-  int ni;
 };
 
-template <class S, int MODEL, int D, int T0, int T1, int T2>
-KernelSet<S> make_set() {
-  KernelSet<S> k;
-  if constexpr (MODEL == AGD_TTI) {
-    // TTI runs from its precomputed scheme only.
-    k.update = nullptr;
-    k.update_walls = nullptr;
-    k.update_batch = nullptr;
-  } else {
-    k.update = eik_update<S, MODEL, D, T0, T1, T2, VAR_FAST>;
-    k.update_walls = eik_update<S, MODEL, D, T0, T1, T2, VAR_GENERAL>;
-    if constexpr (std::is_same<S, float>::value)
-      k.update_batch = eik_update<S, MODEL, D, T0, T1, T2, VAR_BATCH>;
-    else
-      k.update_batch = nullptr;
-  }
-  k.init_ids = init_list_ids<T0 * T1 * T2>;
-  // Precomputed schemes pay off where the stencil is costly to build
-  // (Selling decompositions: one for Riemann, seven for TTI, which has no
-  // other variant); the drift of Rander is not stored.
-  if constexpr (MODEL == AGD_RIEMANN || MODEL == AGD_TTI) {
-    k.update_pre = eik_update<S, MODEL, D, T0, T1, T2, VAR_PRE>;
-    k.scheme = eik_scheme<S, MODEL, D, T0, T1, T2, false>;
-    k.scheme_general = eik_scheme<S, MODEL, D, T0, T1, T2, true>;
-  } else {
-    k.update_pre = nullptr;
-    k.scheme = nullptr;
-    k.scheme_general = nullptr;
-  }
-  k.flow = eik_flow<S, MODEL, D, T0, T1, T2>;
+template <int D, int T0, int T1, int T2> TileKernels make_tile_kernels() {
+  TileKernels k;
   k.init = eik_init_list<D, T0 * T1 * T2>;
+  k.init_ids = init_list_ids<T0 * T1 * T2>;
   k.all_tiles = all_tiles_kernel<D>;
   k.wallpass = wall_dist_pass<D, T0, T1, T2>;
-  k.ni = T0 * T1 * T2;
   return k;
 }
 
-static int tile_key(int d, const int32_t* si) {
-  return d == 2 ? (si[0] * 100 + si[1]) : (si[0] * 10000 + si[1] * 100 + si[2]);
-}
-
-template <class S, int MODEL>
-static bool lookup_model(int d, const int32_t* si, KernelSet<S>& out) {
+static bool lookup_tile_kernels(int d, const int32_t* si, TileKernels& out) {
   const int key = tile_key(d, si);
   if (d == 2) {
     switch (key) {
-      case 808: out = make_set<S, MODEL, 2, 8, 8, 1>(); return true;
-      case 1616: out = make_set<S, MODEL, 2, 16, 16, 1>(); return true;
-      case 816: out = make_set<S, MODEL, 2, 8, 16, 1>(); return true;
+      case 808: out = make_tile_kernels<2, 8, 8, 1>(); return true;
+      case 1616: out = make_tile_kernels<2, 16, 16, 1>(); return true;
+      case 816: out = make_tile_kernels<2, 8, 16, 1>(); return true;
     }
   } else if (d == 3) {
     switch (key) {
-      case 40404: out = make_set<S, MODEL, 3, 4, 4, 4>(); return true;
-      case 80804: out = make_set<S, MODEL, 3, 8, 8, 4>(); return true;
-      case 40408: out = make_set<S, MODEL, 3, 4, 4, 8>(); return true;
+      case 40404: out = make_tile_kernels<3, 4, 4, 4>(); return true;
+      case 80804: out = make_tile_kernels<3, 8, 8, 4>(); return true;
+      case 40408: out = make_tile_kernels<3, 4, 4, 8>(); return true;
     }
   }
   return false;
 }
 
+template <class S> struct KernelSet : ModelKernels<S>, TileKernels {};
+
 template <class S>
 static bool lookup(int model, int d, const int32_t* si, KernelSet<S>& out) {
+  ModelKernels<S>& mk = out;
+  bool ok = false;
   switch (model) {
-    case AGD_ISOTROPIC: return lookup_model<S, AGD_ISOTROPIC>(d, si, out);
-    case AGD_DIAGONAL: return lookup_model<S, AGD_DIAGONAL>(d, si, out);
-    case AGD_RIEMANN: return lookup_model<S, AGD_RIEMANN>(d, si, out);
-    case AGD_RANDER: return lookup_model<S, AGD_RANDER>(d, si, out);
-    case AGD_TTI: return lookup_model<S, AGD_TTI>(d, si, out);
+    case AGD_ISOTROPIC: ok = lookup_isotropic<S>(d, si, mk); break;
+    case AGD_DIAGONAL: ok = lookup_diagonal<S>(d, si, mk); break;
+    case AGD_RIEMANN: ok = lookup_riemann<S>(d, si, mk); break;
+    case AGD_RANDER: ok = lookup_rander<S>(d, si, mk); break;
+    case AGD_TTI: ok = lookup_tti<S>(d, si, mk); break;
   }
-  return false;
+  return ok && lookup_tile_kernels(d, si, out);
 }
 
 static int check_desc(const agd_eikonal_desc* d) {
