@@ -152,9 +152,16 @@ class Interface:
             help="Use multiprecision arithmetic, to improve accuracy (FP64 arithmetic here)"))
         if self.GetValue('factoringRadius', default=0, help="Source factorization (not provided)"):
             raise NotImplementedError("source factorization is not provided by the MI355X solver")
-        self.order = self.GetValue('order', default=1, help="Scheme order (1 only)")
-        if self.order != 1:
-            raise NotImplementedError("only the first order scheme is provided")
+        self.order = int(self.GetValue('order', default=1,
+                                       help="Scheme order (1, or 2: second order where the solution is smooth)"))
+        if self.order not in (1, 2):
+            raise ValueError("order must be 1 or 2")
+        if self.order == 2:
+            if self.model_ == 'TTI':
+                raise NotImplementedError("the second-order scheme of the TTI model is not provided")
+            self.order2_threshold = float(self.GetValue(
+                'order2_threshold', default=0.3,
+                help="Relative threshold on second order differences, for the second-order scheme"))
         if self.GetValue('bound_active_blocks', default=False,
                          help="Limit the number of active blocks (not provided)"):
             raise NotImplementedError("bound_active_blocks is not provided by the MI355X solver")
@@ -467,7 +474,8 @@ class Interface:
         # and the default is on wherever a precomputed variant exists.
         self.scheme_w_t = self.scheme_c_t = None
         self.precompute_scheme = False
-        if _lib.lib.agd_scheme_supported(self.modelId, self.ndim, _lib.i32(self.shape_i)):
+        if self.order == 1 and _lib.lib.agd_scheme_supported(self.modelId, self.ndim,
+                                                             _lib.i32(self.shape_i)):
             self.precompute_scheme = bool(self.GetValue(
                 'precompute_scheme', default=True,
                 help="Precompute and store the finite difference scheme stencils"))
@@ -510,6 +518,8 @@ class Interface:
         d.check_interval = getattr(self, 'check_interval', 4)
         d.activate_all = int(getattr(self, 'warmStartAllTiles', False))
         d.use_graph = int(getattr(self, 'use_graph', True))
+        d.order = int(getattr(self, 'order', 1))
+        d.order2_threshold = float(getattr(self, 'order2_threshold', 0.3))
         return d
 
     def compute_scheme(self):

@@ -33,6 +33,7 @@ class EikonalDesc(ctypes.Structure):
         ("atol", c_double), ("rtol", c_double), ("nitermax_o", c_int64),
         ("iso_weights", c_double * 3), ("check_interval", c_int32), ("activate_all", c_int32),
         ("time_launches", c_int32), ("use_graph", c_int32),
+        ("order", c_int32), ("reserved", c_int32), ("order2_threshold", c_double),
     ]
 
 

@@ -366,6 +366,10 @@ static int solve_impl(const agd_eikonal_desc* d, const agd_eikonal_io* io,
               "scheme_weights and scheme_codes go together");
   AGD_REQUIRE(!pre || ks.update_pre, "no precomputed-scheme variant for this model");
   AGD_REQUIRE(pre || ks.update, "this model runs from a precomputed scheme (agd_eikonal_scheme)");
+  const bool order2 = d->order == 2;
+  AGD_REQUIRE(d->order >= 0 && d->order <= 2, "order must be 1 or 2 (0 means 1)");
+  AGD_REQUIRE(!order2 || (!pre && ks.update_order2),
+              "the second-order scheme has no precomputed-scheme variant (nor a TTI one)");
   EikArgs<S> a{};
   a.u = static_cast<S*>(io->values);
   a.geom = static_cast<const S*>(io->geom);
@@ -381,6 +385,7 @@ static int solve_impl(const agd_eikonal_desc* d, const agd_eikonal_io* io,
   a.niter_i = d->niter_i;
   a.solver = global ? AGD_AGSI : d->solver;
   a.fim_front = d->fim_front_width;
+  a.order2_thr = S(d->order2_threshold);
 
   auto init = [&](Workspace* ws) -> int {
     a.w.lists = ws->lists;
@@ -407,7 +412,9 @@ static int solve_impl(const agd_eikonal_desc* d, const agd_eikonal_io* io,
   };
   auto launch = [&](int32_t koff, hipStream_t s) {
     a.w.koff = koff;
-    auto kern = pre ? ks.update_pre : ((io->wall_dist || g.periodic) ? ks.update_walls : ks.update);
+    auto kern = pre ? ks.update_pre
+                    : order2 ? ks.update_order2
+                             : ((io->wall_dist || g.periodic) ? ks.update_walls : ks.update);
     hipLaunchKernelGGL(kern, dim3(blocks), dim3(ni), 0, s, a);
   };
   WorklistResult r;
@@ -450,6 +457,10 @@ static int flow_impl(const agd_eikonal_desc* d, const agd_eikonal_io* io, const 
   a.wsum = static_cast<S*>(f->flow_weightsum);
   a.fidx = f->flow_indices;
   a.fvec = static_cast<S*>(f->flow_vector);
+  AGD_REQUIRE(d->order >= 0 && d->order <= 2 && (d->order < 2 || d->model != AGD_TTI),
+              "order must be 1 or 2 (1 for TTI; 0 means 1)");
+  a.order2 = d->order == 2 ? 1 : 0;
+  a.order2_thr = S(d->order2_threshold);
   hipLaunchKernelGGL(ks.flow, dim3(g.ntiles), dim3(ks.ni), 0, stream, a);
   AGD_HIP_TRY(hipGetLastError());
   return AGD_OK;
@@ -478,6 +489,7 @@ static int solve_batched_impl(const agd_eikonal_desc* d, int32_t ninst,
               "batched solves take no walls and no precomputed scheme");
   AGD_REQUIRE(d->solver != AGD_FIM || d->fim_front_width >= 2, "fim_front_width must be >= 2");
   for (int k = 0; k < d->ndim; ++k) AGD_REQUIRE(!d->periodic[k], "batched solves are not periodic");
+  AGD_REQUIRE(d->order <= 1, "batched solves use the first-order scheme");
   const int ni = ks.ni;
   std::vector<InstDesc> h(static_cast<size_t>(ninst));
   int64_t tiles = 0;

@@ -60,7 +60,29 @@ template <class S> struct EikArgs {
   // batched solves: instance table and tile -> instance map
   const InstDesc* inst;
   const int32_t* tile_inst;
+  S order2_thr;  // VAR_ORDER2: threshold of the second-order criterion
 };
+
+// HFM.h:91-134: where |u0 - 2v + v2| < thr * max_k(u0 - v_k)_+ and u0 > v,
+// replace v by v + (v - v2)/3 and its weight w by 9/4 w.  Returns the mask
+// of corrected neighbours (their flow weights are scaled by 3/2).
+template <class S, int NS>
+__device__ inline unsigned order2_correct(S u0, S thr, S (&v)[NS], const S (&v2)[NS], S (&w)[NS]) {
+  S bound = S(0);
+#pragma unroll
+  for (int k = 0; k < NS; ++k) bound = max(bound, u0 - v[k]);
+  unsigned mask = 0;
+#pragma unroll
+  for (int k = 0; k < NS; ++k) {
+    const S diff2 = abs_(u0 - S(2) * v[k] + v2[k]);
+    if (diff2 < thr * bound && u0 > v[k]) {
+      v[k] += (v[k] - v2[k]) / S(3);
+      w[k] *= S(2.25);
+      mask |= 1u << k;
+    }
+  }
+  return mask;
+}
 
 __device__ inline int32_t read_score(const int32_t* arr, int32_t tile, int32_t iter) {
   const int32_t v = arr[tile];
@@ -87,7 +109,7 @@ template <class S, int MODEL, int D, class TL, bool GENERAL>
 __device__ inline void resolve_codes(const Stencil<S, MODEL, D>& st, const int* xi, const int* xt,
                                      const int* xo, int wx, const uint8_t* wall_s,
                                      const uint8_t* wall, const GridDesc& g, int* code,
-                                     int tile_base = 0) {
+                                     int tile_base = 0, int* code2 = nullptr) {
   constexpr int NS = ModelTraits<MODEL, D>::NSYM;
   constexpr int NI = TL::NI;
 #pragma unroll
@@ -142,6 +164,18 @@ __device__ inline void resolve_codes(const Stencil<S, MODEL, D>& st, const int* 
         const bool local = vis && TL::inside(yi) && in_dom;
         const int m = (vis && !local) ? index_tot<D, TL>(yt, g) : -1;
         code[j] = local ? -2 - TL::index(yi) : m;
+        if (code2) {
+          // further neighbour x + 2 sg e on the same side, behind the same
+          // visibility test (FiniteDifferences.h:75-90)
+          int y2i[3] = {yi[0] + e[0], yi[1] + e[1], yi[2] + e[2]};
+          int y2t[3] = {yt[0] + e[0], yt[1] + e[1], yt[2] + e[2]};
+          bool in2 = true;
+#pragma unroll
+          for (int c = 0; c < D; ++c) in2 = in2 && (y2t[c] >= 0 && y2t[c] < g.shape_tot[c]);
+          const bool local2 = vis && TL::inside(y2i) && in2;
+          const int m2 = (vis && !local2) ? index_tot<D, TL>(y2t, g) : -1;
+          code2[j] = local2 ? -2 - TL::index(y2i) : m2;
+        }
       }
     }
   }
@@ -152,12 +186,14 @@ __device__ inline void resolve_codes(const Stencil<S, MODEL, D>& st, const int* 
 // VAR_PRE reads stencil weights and neighbour codes precomputed once per
 // solve (any walls / periodicity are already resolved in the codes);
 // VAR_BATCH is VAR_FAST over the concatenated tiles of several instances,
-// each tile looking up its instance's grid and constants.
-enum { VAR_FAST = 0, VAR_GENERAL = 1, VAR_PRE = 2, VAR_BATCH = 3 };
+// each tile looking up its instance's grid and constants; VAR_ORDER2 is
+// VAR_GENERAL with the second-order correction (HFM.h:91-134).
+enum { VAR_FAST = 0, VAR_GENERAL = 1, VAR_PRE = 2, VAR_BATCH = 3, VAR_ORDER2 = 4 };
 
 template <class S, int MODEL, int D, int T0, int T1, int T2, int VAR>
 __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
-  constexpr bool GENERAL = VAR == VAR_GENERAL;
+  constexpr bool ORDER2 = VAR == VAR_ORDER2;
+  constexpr bool GENERAL = VAR == VAR_GENERAL || ORDER2;
   constexpr bool WALLS = GENERAL;
   constexpr bool BATCH = VAR == VAR_BATCH;
   static_assert((T0 * T1 * T2) % 64 == 0, "tiles are whole waves (pass_begin scans per wave)");
@@ -240,6 +276,7 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
     u_s[tid] = seed ? rhs : u_old;
 
     int code[NJ];
+    int code2[ORDER2 ? NJ : 1];
     S sw[NS], shift[NS];
     if constexpr (MT::NMIX > 1) {
       // several sub-schemes: read per sweep below
@@ -262,7 +299,8 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
       Stencil<S, MODEL, D> st;
       make_stencil<S, MODEL, D>(geo, iso_w, st);
       if constexpr (WALLS) __syncthreads();  // wall_s complete before the visibility walks
-      resolve_codes<S, MODEL, D, TL, GENERAL>(st, xi, xt, xo, wx, wall_s, a.wall, gi, code, tbase);
+      resolve_codes<S, MODEL, D, TL, GENERAL>(st, xi, xt, xo, wx, wall_s, a.wall, gi, code, tbase,
+                                              ORDER2 ? code2 : nullptr);
 #pragma unroll
       for (int k = 0; k < NS; ++k) {
         sw[k] = st.w[k];
@@ -308,6 +346,11 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
       S g[NJ];
 #pragma unroll
       for (int j = 0; j < NJ; ++j) g[j] = a.u[code[j] >= 0 ? code[j] : n_t];
+      S g2[ORDER2 ? NJ : 1];
+      if constexpr (ORDER2) {
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) g2[j] = a.u[code2[j] >= 0 ? code2[j] : n_t];
+      }
       S vo[NJ];
 #pragma unroll
       for (int k = 0; k < NS; ++k) {
@@ -324,13 +367,21 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
       const bool active = !seed;
       for (int it = 0; it < a.niter_i; ++it) {
         S v[NS], w[NS];
+        S w2v[ORDER2 ? NS : 1];  // further-neighbour values (order 2)
 #pragma unroll
         for (int k = 0; k < NS; ++k) {
           const S a0 = code[2 * k] <= -2 ? u_s[-2 - code[2 * k]] + vo[2 * k] : vo[2 * k];
           const S a1 = code[2 * k + 1] <= -2 ? u_s[-2 - code[2 * k + 1]] + vo[2 * k + 1] : vo[2 * k + 1];
           v[k] = (a0 > a1) ? a1 : a0;  // Greater(v,-) keeps the -e side on ties
           w[k] = sw[k];
+          if constexpr (ORDER2) {
+            // the further neighbour on the chosen side
+            const int j2 = 2 * k + ((a0 > a1) ? 1 : 0);
+            const int c2 = code2[j2];
+            w2v[k] = c2 <= -2 ? u_s[-2 - c2] : (c2 >= 0 ? g2[j2] : inf_v<S>());
+          }
         }
+        if constexpr (ORDER2) order2_correct<S, NS>(u_s[tid], a.order2_thr, v, w2v, w);
         const S un = hfm_solve<S, NS>(v, w, rhs);
         __syncthreads();
         const bool ch = active && (u_s[tid] > un);
@@ -463,6 +514,8 @@ template <class S> struct FlowArgs {
   S* wsum;
   int32_t* fidx;
   S* fvec;
+  int32_t order2;  // second-order correction (not with TTI)
+  S order2_thr;
 };
 
 template <class S, int MODEL, int D, int T0, int T1, int T2>
@@ -492,10 +545,11 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_flow(FlowArgs<S> a) {
 
   // Neighbour values (min over -e/+e, ties keep -e), sides and indices of a
   // stencil.
-  auto gather = [&](const Stencil<S, MODEL, D>& st, S* v, S* w, int* side, int64_t* nb) {
+  auto gather = [&](const Stencil<S, MODEL, D>& st, S* v, S* w, int* side, int64_t* nb,
+                    S* v2) {
 #pragma unroll
     for (int k = 0; k < NS; ++k) {
-      S best = inf_v<S>();
+      S best = inf_v<S>(), best2 = inf_v<S>();
       int bs = 0;
       int64_t bi = -1;
 #pragma unroll
@@ -514,10 +568,17 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_flow(FlowArgs<S> a) {
         int yt[3] = {xt[0] + e[0], xt[1] + e[1], xt[2] + e[2]};
         const int64_t m = vis ? index_tot<D, TL>(yt, a.g) : -1;
         const S val = (m >= 0) ? a.u[m] + sh : inf_v<S>();
-        if (s == 0) { best = val; bs = 0; bi = m; }
-        else if (best > val) { best = val; bs = 1; bi = m; }
+        S val2 = inf_v<S>();
+        if (v2 && vis) {
+          int y2[3] = {yt[0] + e[0], yt[1] + e[1], yt[2] + e[2]};
+          const int64_t m2 = index_tot<D, TL>(y2, a.g);
+          if (m2 >= 0) val2 = a.u[m2];
+        }
+        if (s == 0) { best = val; best2 = val2; bs = 0; bi = m; }
+        else if (best > val) { best = val; best2 = val2; bs = 1; bi = m; }
       }
       v[k] = best;
+      if (v2) v2[k] = best2;
       side[k] = bs;
       nb[k] = bi;
       w[k] = st.w[k];
@@ -529,6 +590,7 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_flow(FlowArgs<S> a) {
   int side[NS];
   int64_t nb[NS];
   S val;
+  unsigned o2mask = 0;  // order 2: neighbours with the second-order correction
   if constexpr (MT::NMIX > 1) {
     // TTI: keep the sub-scheme whose update wins the min/max (HFMIter.h:48-54).
 #pragma unroll
@@ -555,7 +617,7 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_flow(FlowArgs<S> a) {
       S cv[NS], cw[NS];
       int cs[NS];
       int64_t cn[NS];
-      gather(cand, cv, cw, cs, cn);
+      gather(cand, cv, cw, cs, cn, nullptr);
       S vs[NS], ws[NS];
 #pragma unroll
       for (int k = 0; k < NS; ++k) { vs[k] = cv[k]; ws[k] = cw[k]; }
@@ -569,10 +631,16 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_flow(FlowArgs<S> a) {
     }
   } else {
     make_stencil<S, MODEL, D>(geo, a.iso_w, st);
-    gather(st, v, w, side, nb);
+    S v2[NS];
+    gather(st, v, w, side, nb, a.order2 ? v2 : nullptr);
     S vs[NS], ws[NS];
 #pragma unroll
     for (int k = 0; k < NS; ++k) { vs[k] = v[k]; ws[k] = w[k]; }
+    if (a.order2) {
+      o2mask = order2_correct<S, NS>(u_old, a.order2_thr, vs, v2, ws);
+#pragma unroll
+      for (int k = 0; k < NS; ++k) v[k] = vs[k];  // corrected values (unsorted copy)
+    }
     val = hfm_solve<S, NS>(vs, ws, rhs);
   }
 
@@ -584,6 +652,7 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_flow(FlowArgs<S> a) {
   for (int k = 0; k < NS; ++k) {
     const S diff = val - v[k];
     fw[k] = live ? w[k] * (diff > S(0) ? diff : S(0)) : S(0);
+    if ((o2mask >> k) & 1u) fw[k] *= S(1.5);
     sum += fw[k];
     const int sg = 2 * side[k] - 1;
 #pragma unroll
@@ -615,6 +684,7 @@ template <class S> struct ModelKernels {
   void (*update_walls)(EikArgs<S>);
   void (*update_pre)(EikArgs<S>);          // null: no precomputed-scheme variant
   void (*update_batch)(EikArgs<S>);        // null: no batched variant (FP64)
+  void (*update_order2)(EikArgs<S>);       // null: no second-order variant (TTI)
   void (*scheme)(SchemeArgs<S>);
   void (*scheme_general)(SchemeArgs<S>);
   void (*flow)(FlowArgs<S>);
@@ -629,9 +699,11 @@ ModelKernels<S> make_model_kernels() {
     k.update = nullptr;
     k.update_walls = nullptr;
     k.update_batch = nullptr;
+    k.update_order2 = nullptr;
   } else {
     k.update = eik_update<S, MODEL, D, T0, T1, T2, VAR_FAST>;
     k.update_walls = eik_update<S, MODEL, D, T0, T1, T2, VAR_GENERAL>;
+    k.update_order2 = eik_update<S, MODEL, D, T0, T1, T2, VAR_ORDER2>;
     if constexpr (std::is_same<S, float>::value)
       k.update_batch = eik_update<S, MODEL, D, T0, T1, T2, VAR_BATCH>;
     else

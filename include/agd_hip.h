@@ -75,6 +75,10 @@ typedef struct agd_eikonal_desc {
   int32_t activate_all;   /* 1: start with every tile active (warm start) */
   int32_t time_launches;  /* 1: time every pass with an event pair (kernel_ms) */
   int32_t use_graph;      /* 1: capture check_interval passes once as a hipGraph and replay it */
+  int32_t order;          /* 1 (or 0) or 2: second-order upwind differences
+                             where the solution is smooth (HFM.h:91-134) */
+  int32_t reserved;       /* 0                                           */
+  double order2_threshold;/* order 2: |u-2v+v2| < threshold * max(u-v) (0.3) */
 } agd_eikonal_desc;
 
 /* Device pointers, tiled layout.  Scalars are float (AGD_F32) or double. */

@@ -426,8 +426,15 @@ static bool visible(const Grid& G, const int64_t* x, const int* e,
   return true;
 }
 
+// Scheme order (HFM.h:91-134): 2 enables the second-order correction where
+// the solution is smooth; set for the whole library by orc_set_order.
+static int g_order = 1;
+static double g_order2_threshold = 0.3;
+
 struct Problem {
   int model, d;
+  int order;
+  double order2_threshold;
   Grid G;
   const double* geom;       // ngeom * N, component-major
   const double* rhs;        // N: cost, seed value at seeds, +inf in walls
@@ -440,7 +447,7 @@ struct Problem {
 // `base` (HFM.h:39-77, FiniteDifferences.h:20-106).
 template <class S, class Get>
 static void neighbours(const Problem& P, const Stencil<S>& st, int base, int64_t n, Get get,
-                       S* v, int* side, int64_t* nidx) {
+                       S* v, int* side, int64_t* nidx, S* v2 = nullptr) {
   int64_t x[3];
   P.G.pos(n, x);
   for (int k = 0; k < st.nsym; ++k) {
@@ -449,24 +456,28 @@ static void neighbours(const Problem& P, const Stencil<S>& st, int base, int64_t
     if (st.has_drift) {
       for (int c = 0; c < P.d; ++c) shift += st.drift[c] * S(ek[c]);
     }
-    S best = Inf<S>();
+    S best = Inf<S>(), best2 = Inf<S>();
     int bs = 0;
     int64_t bi = -1;
     for (int s = 0; s < 2; ++s) {
       const int eps = 2 * s - 1;
       int ee[3] = {eps * ek[0], eps * ek[1], eps * ek[2]};
-      int64_t y[3];
-      for (int c = 0; c < P.d; ++c) y[c] = x[c] + ee[c];
-      S val = Inf<S>();
+      int64_t y[3], y2[3];
+      for (int c = 0; c < P.d; ++c) { y[c] = x[c] + ee[c]; y2[c] = y[c] + ee[c]; }
+      S val = Inf<S>(), val2 = Inf<S>();
       int64_t ny = -1;
       if (visible(P.G, x, ee, P.wallDist)) {
         ny = P.G.index(y);
         if (ny >= 0) val = get(ny) + (s == 0 ? shift : -shift);
+        // further neighbour x + 2 eps e, no shift (FiniteDifferences.h:75-90)
+        const int64_t ny2 = P.G.index(y2);
+        if (ny2 >= 0) val2 = get(ny2);
       }
-      if (s == 0) { best = val; bs = 0; bi = ny; }
-      else if (best > val) { best = val; bs = 1; bi = ny; }
+      if (s == 0) { best = val; best2 = val2; bs = 0; bi = ny; }
+      else if (best > val) { best = val; best2 = val2; bs = 1; bi = ny; }
     }
     v[k] = best;
+    if (v2) v2[k] = best2;
     if (side) side[k] = bs;
     if (nidx) nidx[k] = bi;
   }
@@ -479,7 +490,7 @@ static void neighbours(const Problem& P, const Stencil<S>& st, int base, int64_t
 // neighbour indices, *kmix its index.
 template <class S, class Get>
 static S cell_update(const Problem& P, const Stencil<S>& st, int64_t n, Get get, S rhs, S* fw,
-                     int* side, int64_t* nidx, int* kmix) {
+                     int* side, int64_t* nidx, int* kmix, S u0 = Inf<S>()) {
   const int ns = st.nsym;
   for (int k = 0; k < ns; ++k) {
     if (fw) fw[k] = 0;
@@ -487,6 +498,33 @@ static S cell_update(const Problem& P, const Stencil<S>& st, int64_t n, Get get,
     if (nidx) nidx[k] = -1;
   }
   if (kmix) *kmix = 0;
+  if (st.nmix == 1 && P.order == 2) {
+    // HFM.h:91-134: where |u0 - 2v + v2| < threshold * max_k(u0 - v_k)_+ and
+    // u0 > v, use v + (v - v2)/3 with weight 9/4 w (flow weight 3/2 w (u-v)+).
+    S v[6], v2[6], w[6];
+    bool o2[6];
+    neighbours<S>(P, st, 0, n, get, v, side, nidx, v2);
+    S bound = 0;
+    for (int k = 0; k < ns; ++k) bound = std::max(bound, u0 - v[k]);
+    const S thr = S(P.order2_threshold);
+    for (int k = 0; k < ns; ++k) {
+      const S diff2 = std::fabs(u0 - 2 * v[k] + v2[k]);
+      o2[k] = diff2 < thr * bound && u0 > v[k];
+      w[k] = st.w[k];
+      if (o2[k]) {
+        v[k] += (v[k] - v2[k]) / 3;
+        w[k] *= S(9) / S(4);
+      }
+    }
+    const S val = hfm_solve<S>(ns, w, v, rhs, nullptr);
+    if (fw && val != Inf<S>()) {
+      for (int k = 0; k < ns; ++k) {
+        const S diff = val - v[k];
+        fw[k] = st.w[k] * (diff > S(0) ? diff : S(0)) * (o2[k] ? S(1.5) : S(1));
+      }
+    }
+    return val;
+  }
   if (st.nmix == 1) {
     S v[6];
     neighbours<S>(P, st, 0, n, get, v, side, nidx);
@@ -531,6 +569,8 @@ static void fill_problem(Problem& P, int model, int d, const int64_t* shape,
                          const double* iso_w) {
   P.model = model;
   P.d = d;
+  P.order = g_order;
+  P.order2_threshold = g_order2_threshold;
   P.G.init(d, shape, periodic);
   P.geom = geom;
   P.rhs = rhs;
@@ -556,7 +596,7 @@ static int64_t jacobi(const Problem& P, double atol, double rtol, int64_t niterm
     for (int64_t n = 0; n < N; ++n) {
       if (P.seeds[n]) { un[n] = u[n]; continue; }
       const S val = cell_update<S>(P, st[n], n, [&](int64_t m) { return u[m]; }, S(P.rhs[n]),
-                                   nullptr, nullptr, nullptr, nullptr);
+                                   nullptr, nullptr, nullptr, nullptr, u[n]);
       const S nv = (val < u[n]) ? val : u[n];
       const S diff = std::fabs(u[n] - nv);
       const S tol = S(atol) + S(rtol) * std::fabs(nv);
@@ -573,6 +613,11 @@ static int64_t jacobi(const Problem& P, double atol, double rtol, int64_t niterm
 extern "C" {
 
 int orc_version() { return 3; }
+
+void orc_set_order(int order, double threshold) {
+  g_order = order;
+  g_order2_threshold = threshold;
+}
 
 // Selling decomposition of n symmetric matrices (row n: symdim coefficients,
 // lower triangle (d00, d10, d11, d20, d21, d22)).  Outputs weights[n][symdim]
@@ -664,6 +709,7 @@ void orc_tti_diags(int64_t n, const double* l, const double* q, double* diag, in
 int64_t orc_solve_fm(int model, int d, const int64_t* shape, const int* periodic,
                      const double* geom, const double* rhs, const uint8_t* seeds,
                      const uint8_t* wallDist, const double* iso_w, double* u) {
+  if (g_order != 1) return -1;  // the second-order scheme is solved by iteration
   Problem P;
   fill_problem(P, model, d, shape, periodic, geom, rhs, seeds, wallDist, iso_w);
   const int64_t N = P.G.N;
@@ -766,7 +812,7 @@ int orc_flow(int model, int d, const int64_t* shape, const int* periodic, const 
         stencil_at<float>(P, n, st);
         float ff[6];
         const float val = cell_update<float>(P, st, n, [&](int64_t m) { return float(u0[m]); },
-                                             float(rhs[n]), ff, side, ni, &km);
+                                             float(rhs[n]), ff, side, ni, &km, float(u0[n]));
         if (val < float(u0[n])) u[n] = val;
         for (int k = 0; k < ns; ++k) {
           f[k] = ff[k];
@@ -776,7 +822,7 @@ int orc_flow(int model, int d, const int64_t* shape, const int* periodic, const 
         Stencil<double> st;
         stencil_at<double>(P, n, st);
         const double val = cell_update<double>(P, st, n, [&](int64_t m) { return u0[m]; }, rhs[n],
-                                               f, side, ni, &km);
+                                               f, side, ni, &km, u0[n]);
         if (val < u0[n]) u[n] = val;
         for (int k = 0; k < ns; ++k) for (int c = 0; c < 3; ++c) e[k][c] = st.e[km * ns + k][c];
       }

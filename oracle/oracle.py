@@ -55,6 +55,7 @@ def _load():
     lib.orc_flow.argtypes = [c_int, c_int, P, P, P, P, P, P, P, c_int, P, P, P, P, P]
     lib.orc_adjoint.argtypes = [c_int64, c_int, P, P, P, c_int, P, P]
     lib.orc_forward_ad.argtypes = [c_int64, c_int, P, P, P, c_int, P, P]
+    lib.orc_set_order.argtypes = [c_int, c_double]
     lib.orc_wall_distance.argtypes = [c_int, P, P, P, c_int, P]
     return lib
 
@@ -137,8 +138,10 @@ class Problem:
 
     def solve_fm(self):
         u = np.zeros(self.N)
-        lib().orc_solve_fm(self.model, self.d, _p(self._shape), _p(self.periodic), _p(self.geom),
-                           _p(self.rhs), _p(self.seeds), _p(self.wall_dist), _p(self.iso_w), _p(u))
+        n = lib().orc_solve_fm(self.model, self.d, _p(self._shape), _p(self.periodic), _p(self.geom),
+                               _p(self.rhs), _p(self.seeds), _p(self.wall_dist), _p(self.iso_w), _p(u))
+        if n < 0:
+            raise ValueError("fast marching solves the first-order scheme only (use solve_jacobi)")
         return u.reshape(self.shape)
 
     def solve_jacobi(self, atol, rtol, nitermax=100000, use_float=False, u0=None):
@@ -210,6 +213,21 @@ def wall_distance(walls, bound=10, periodic=None):
 
 
 # ---------------------------------------------------------------- preprocessing
+class order:
+    """Context manager selecting the scheme order of the oracle (2 = the
+    second-order correction of HFM.h:91-134; solve with solve_jacobi)."""
+
+    def __init__(self, k, threshold=0.3):
+        self.k, self.threshold = int(k), float(threshold)
+
+    def __enter__(self):
+        lib().orc_set_order(self.k, self.threshold)
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_set_order(1, 0.3)
+
+
 def tti_diags(l, q, use_float=False):
     """TTI_.h:76-107 per cell: l[n, 2], q[n, 3] -> (diag[n, 7, 2], mix_is_min[n])."""
     l = _c(np.asarray(l, dtype=np.float64).reshape(-1, 2), np.float64)

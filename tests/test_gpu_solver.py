@@ -8,7 +8,7 @@ finite/infinite masks identical.
 import numpy as np
 import pytest
 
-from adaptivegriddiscretizations_amd import workloads, Metrics
+from adaptivegriddiscretizations_amd import Eikonal, Metrics, workloads
 from gpu_helpers import log, oracle_problem, rel_err, run_gpu
 
 pytestmark = pytest.mark.gpu
@@ -155,3 +155,41 @@ def test_graph_replay_matches_eager_launches(O, graph):
         u, out = run_gpu(wl, hipGraph=graph, check_interval=3)
         assert out['stats']['eikonal']['launches'] >= 6
         assert rel_err(u, ref) <= TOL32
+
+
+@pytest.mark.parametrize("d", [2, 3])
+def test_diagonal_matches_oracle(O, d):
+    """Diagonal metrics (per-axis costs): Isotropic_.h with geometry."""
+    from adaptivegriddiscretizations_amd import Metrics
+    n = 80 if d == 2 else 20
+    rng = np.random.default_rng(21 + d)
+    costs = (1 + 0.5 * rng.uniform(size=(d,) + (n,) * d)).astype(np.float32)
+    hfm = Eikonal.dictIn({'model': f'Diagonal{d}', 'metric': Metrics.Diagonal(costs),
+                          'seeds': [[0.1] * d], 'exportValues': True, 'verbosity': 0})
+    hfm.SetRect([[-1, 1]] * d, dims=[n] * d)
+    u = hfm.Run()['values'].cpu().numpy().astype(np.float64)
+    h = np.full(d, np.float32(2. / n)).astype(np.float64)
+    idx = O.seed_indices([[0.1] * d], np.full(d, -1.), h)
+    P = O.make_problem('Diagonal', (n,) * d, h, metric=costs.astype(np.float64), seeds_idx=idx)
+    err = rel_err(u, P.cast(np.float32).solve_fm())
+    log(f"diagonal{d}: rel err {err:.2e}")
+    assert err <= TOL32
+
+
+@pytest.mark.parametrize("key", ["multiprecision", "values_float64"])
+def test_multiprecision_reaches_fp64_accuracy(O, key):
+    """multiprecision / values_float64 run in FP64 arithmetic: with a tight
+    tolerance the values match the exact fixed point far below FP32
+    resolution (values_float64 also exports float64)."""
+    wl = workloads.riemann2_aniso(n=96, seed=6, sigma=4., amax=6.)
+    u, out = run_gpu(wl, atol=1e-11, **{key: True})
+    assert out['values'].dtype == (torch_dtype('float64') if key == 'values_float64' else torch_dtype('float32'))
+    P, _ = oracle_problem(O, wl, cast=False)
+    err = rel_err(u, P.solve_fm())
+    log(f"{key}: rel err {err:.2e}")
+    assert err <= (1e-9 if key == 'values_float64' else 1e-7)
+
+
+def torch_dtype(name):
+    import torch
+    return getattr(torch, name)
