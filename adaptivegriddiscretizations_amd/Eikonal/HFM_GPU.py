@@ -469,15 +469,15 @@ class Interface:
 
         # Precomputed scheme (_SetGeometry.py:198-207): stencil weights and
         # resolved neighbour indices stored once, instead of a Selling
-        # decomposition at every tile visit.  The reference stores them only
-        # when the geometry is shared along some axes; here HBM is plentiful
-        # and the default is on wherever a precomputed variant exists.
+        # decomposition at every tile visit.  Off by default, as in the
+        # reference: the stencils are rebuilt in registers.  TTI always uses
+        # it (seven sub-schemes per cell exceed the register budget).
         self.scheme_w_t = self.scheme_c_t = None
         self.precompute_scheme = False
         if self.order == 1 and _lib.lib.agd_scheme_supported(self.modelId, self.ndim,
                                                              _lib.i32(self.shape_i)):
             self.precompute_scheme = bool(self.GetValue(
-                'precompute_scheme', default=True,
+                'precompute_scheme', default=self.model_ == 'TTI',
                 help="Precompute and store the finite difference scheme stencils"))
         if self.model_ == 'TTI' and not self.precompute_scheme:
             raise ValueError("the TTI model runs from a precomputed scheme (precompute_scheme=True)")

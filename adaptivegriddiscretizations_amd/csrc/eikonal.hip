@@ -302,7 +302,9 @@ int get_workspace(int64_t ntiles, int64_t shard_cap, int64_t nhist, Workspace*& 
   if (!w.ev0) {
     AGD_HIP_TRY(hipEventCreate(&w.ev0));
     AGD_HIP_TRY(hipEventCreate(&w.ev1));
-    AGD_HIP_TRY(hipHostMalloc((void**)&w.host_counts, kRow * sizeof(int32_t), hipHostMallocDefault));
+    AGD_HIP_TRY(hipHostMalloc((void**)&w.host_counts, 2 * kRow * sizeof(int32_t), hipHostMallocDefault));
+    AGD_HIP_TRY(hipEventCreateWithFlags(&w.chunk_ev[0], hipEventDisableTiming));
+    AGD_HIP_TRY(hipEventCreateWithFlags(&w.chunk_ev[1], hipEventDisableTiming));
     AGD_HIP_TRY(hipStreamCreateWithFlags(&w.capture_stream, hipStreamNonBlocking));
     AGD_HIP_TRY(hipMalloc((void**)&w.counts, kCountsLen * sizeof(int32_t)));
   }

@@ -37,9 +37,10 @@ struct Workspace {
   int32_t* counts = nullptr;  // kCountsLen
   int32_t* hist = nullptr;    // hist_cap: list length of every pass
   int32_t* score = nullptr;   // 2 * tile_cap (FIM scores; fixed list of global iteration)
-  int32_t* host_counts = nullptr;  // pinned, kRow
+  int32_t* host_counts = nullptr;  // pinned, 2 * kRow (two chunks in flight)
   int64_t tile_cap = 0, shard_cap = 0, hist_cap = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t chunk_ev[2] = {nullptr, nullptr};  // list length of a chunk copied
   // Private stream used only to record pass chunks into a graph (the legacy
   // null stream, torch's default, cannot be captured).  Nothing executes on
   // it: the instantiated graph is launched on the caller's stream.
@@ -254,32 +255,59 @@ int run_worklist(int32_t ntiles, int32_t grid, int64_t nitermax, int check_inter
   AGD_HIP_TRY(hipEventRecord(ws->ev0, stream));
   int64_t k = 0;
   bool converged = false;
-  while (k < nitermax) {
-    if (graph) {
-      AGD_HIP_TRY(hipGraphLaunch(exec, stream));
-      pending = ci;
-      k += ci;
-    } else {
+  auto list_length = [&](const int32_t* row) {
+    int64_t n = 0;
+    for (int sh = 0; sh < kShards; ++sh) n += row[sh * kCountStride];
+    return n;
+  };
+  if (graph) {
+    // Pipelined chunks: chunk c+1 is queued before the host looks at the
+    // list length left by chunk c (copied right behind chunk c, ahead of
+    // chunk c+1 in stream order), so the device never idles on the host's
+    // check.  The chunk queued after convergence runs empty passes.
+    int slot = 0;
+    AGD_HIP_TRY(hipGraphLaunch(exec, stream));
+    k += ci;
+    AGD_HIP_TRY(hipMemcpyAsync(ws->host_counts, counts + (k % 3) * kRow, kRow * sizeof(int32_t),
+                               hipMemcpyDeviceToHost, stream));
+    AGD_HIP_TRY(hipEventRecord(ws->chunk_ev[0], stream));
+    while (true) {
+      const bool more = k < nitermax;
+      if (more) {
+        AGD_HIP_TRY(hipGraphLaunch(exec, stream));
+        k += ci;
+        AGD_HIP_TRY(hipMemcpyAsync(ws->host_counts + (1 - slot) * kRow, counts + (k % 3) * kRow,
+                                   kRow * sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+        AGD_HIP_TRY(hipEventRecord(ws->chunk_ev[1 - slot], stream));
+      }
+      AGD_HIP_TRY(hipEventSynchronize(ws->chunk_ev[slot]));
+      if (list_length(ws->host_counts + slot * kRow) == 0) {
+        converged = true;
+        break;
+      }
+      if (!more) break;
+      slot = 1 - slot;
+    }
+  } else {
+    while (k < nitermax) {
       if (time_launches) AGD_HIP_TRY(hipEventRecord(evs[2 * pending], stream));
       launch(int32_t(k), stream);
       if (time_launches) AGD_HIP_TRY(hipEventRecord(evs[2 * pending + 1], stream));
       ++pending;
       ++k;
-    }
-    if (graph || k % ci == 0 || k >= nitermax) {
-      AGD_HIP_TRY(hipMemcpyAsync(ws->host_counts, counts + (k % 3) * kRow, kRow * sizeof(int32_t),
-                                 hipMemcpyDeviceToHost, stream));
-      AGD_HIP_TRY(hipStreamSynchronize(stream));
-      int64_t next = 0;
-      for (int sh = 0; sh < kShards; ++sh) next += ws->host_counts[sh * kCountStride];
-      if (time_launches) {
-        const int st2 = drain();
-        if (st2) return st2;
-      }
-      pending = 0;
-      if (next == 0) {
-        converged = true;
-        break;
+      if (k % ci == 0 || k >= nitermax) {
+        AGD_HIP_TRY(hipMemcpyAsync(ws->host_counts, counts + (k % 3) * kRow,
+                                   kRow * sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+        AGD_HIP_TRY(hipStreamSynchronize(stream));
+        if (time_launches) {
+          const int st2 = drain();
+          if (st2) return st2;
+        }
+        pending = 0;
+        if (list_length(ws->host_counts) == 0) {
+          converged = true;
+          break;
+        }
       }
     }
   }

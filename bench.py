@@ -107,8 +107,9 @@ def main():
                     choices=['riemann3', 'riemann2', 'isotropic2', 'batch256'])
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--no-fp64', action='store_true')
-    ap.add_argument('--no-precompute', action='store_true',
-                    help='rebuild the stencil at every tile visit instead of storing it once per solve')
+    ap.add_argument('--precompute', action='store_true',
+                    help='store the stencils once per solve (precompute_scheme) instead of rebuilding '
+                         'them in registers at every tile visit')
     ap.add_argument('--no-graph', action='store_true',
                     help='launch every pass eagerly instead of replaying captured pass chunks')
     ap.add_argument('--copy-calibration', type=int, default=0, metavar='MIB',
@@ -139,7 +140,7 @@ def main():
     t0 = time.time()
     wl = make_workload(args.config)
     log(f"workload {wl.name} {wl.dims} generated in {time.time() - t0:.1f} s")
-    iface = prepare(wl, **({'precompute_scheme': False} if args.no_precompute else {}))
+    iface = prepare(wl, precompute_scheme=bool(args.precompute))
     desc = iface._desc()
     desc.use_graph = 0 if args.no_graph else 1
     desc.time_launches = 0
