@@ -1,0 +1,407 @@
+"""Learned warm start of the eikonal iteration (no counterpart in agd).
+
+A 4-level convolutional encoder-decoder (32-64-128-128 channels, 3x3
+convolutions, BF16 weights and activations, FP32 accumulation on the matrix
+cores, additive skip connections) maps features of a 2D Riemannian problem
+on a coarse grid to the ratio r = u / d_s, where u is the arrival time and
+d_s(x) = sqrt((x-s)^T M(s) (x-s)) the distance in the metric frozen at the
+seed.  The warm start is the inflated prediction
+
+    values = 1.05 * r_hat * d_s        (bilinear upsampling to the fine grid)
+
+with every tile active: the decreasing iteration then converges to the same
+discrete fixed point as a cold start whenever the start is a supersolution.
+The solver judges that: after the warm solve, agd_eikonal_check counts the
+cells left below their scheme update; any such cell triggers a cold solve.
+
+Every network kernel (forward, both backward passes, pooling, loss, Adam) is
+a HIP kernel of libagd_hip.so (csrc/surrogate.hip); torch only holds the
+buffers and builds the synthetic training fields.
+"""
+import time
+
+import numpy as np
+
+from .. import _lib
+from ._device import torch, ptr, stream_handle
+
+CP = 32  # channel padding (MFMA reduction depth)
+NFEAT = 6
+
+
+def _pad(c):
+    return (c + CP - 1) // CP * CP
+
+
+def _check(st):
+    _lib.check(st)
+
+
+class Conv3x3:
+    """One 3x3 convolution (no bias) with FP32 master weights, a BF16 copy
+    and Adam state.  Weights [9][cout_p][cin_p]; padded entries stay 0."""
+
+    def __init__(self, cin, cout, rng, gain=2.0):
+        self.cin, self.cout = cin, cout
+        self.cin_p, self.cout_p = _pad(cin), _pad(cout)
+        w = np.zeros((9, self.cout_p, self.cin_p), dtype=np.float32)
+        std = np.sqrt(gain / (9 * cin))
+        w[:, :cout, :cin] = rng.normal(scale=std, size=(9, cout, cin))
+        dev = torch.device('cuda')
+        self.w32 = torch.from_numpy(w).to(dev)
+        self.wb = self.w32.to(torch.bfloat16)
+        self.g = torch.zeros_like(self.w32)
+        self.m = torch.zeros_like(self.w32)
+        self.v = torch.zeros_like(self.w32)
+        self.wt = torch.empty((9, self.cin_p, self.cout_p), dtype=torch.bfloat16, device=dev)
+
+    def forward(self, x, relu, stream):
+        N, H, W, C = x.shape
+        assert C == self.cin_p
+        y = torch.empty((N, H, W, self.cout_p), dtype=torch.bfloat16, device=x.device)
+        _check(_lib.lib.agd_nn_conv3x3(ptr(x), ptr(self.wb), ptr(y), N, H, W, self.cin_p,
+                                       self.cout_p, int(relu), stream))
+        return y
+
+    def backward(self, x, gy, stream, need_gx=True):
+        """Accumulates the weight gradient into self.g; returns the input
+        gradient (before the previous layer's ReLU mask)."""
+        N, H, W, _ = x.shape
+        _check(_lib.lib.agd_nn_conv3x3_wgrad(ptr(x), ptr(gy), ptr(self.g), N, H, W, self.cin_p,
+                                             self.cout_p, stream))
+        if not need_gx:
+            return None
+        _check(_lib.lib.agd_nn_conv3x3_flip(ptr(self.wb), ptr(self.wt), self.cin_p, self.cout_p,
+                                            stream))
+        gx = torch.empty_like(x)
+        _check(_lib.lib.agd_nn_conv3x3(ptr(gy), ptr(self.wt), ptr(gx), N, H, W, self.cout_p,
+                                       self.cin_p, 0, stream))
+        return gx
+
+    def step(self, lr, t, stream, b1=0.9, b2=0.999, eps=1e-8):
+        _check(_lib.lib.agd_nn_adam(ptr(self.w32), ptr(self.g), ptr(self.m), ptr(self.v),
+                                    ptr(self.wb), self.w32.numel(), lr, b1, b2, eps, t, stream))
+        self.g.zero_()
+
+
+def _relu_mask(g, y, stream):
+    _check(_lib.lib.agd_nn_relu_mask(ptr(g), ptr(y), g.numel(), stream))
+
+
+def _add(a, b, stream):
+    out = torch.empty_like(a)
+    _check(_lib.lib.agd_nn_add(ptr(a), ptr(b), ptr(out), a.numel(), stream))
+    return out
+
+
+def _pool(x, stream, backward=False, fine_shape=None):
+    if not backward:
+        N, H, W, C = x.shape
+        y = torch.empty((N, H // 2, W // 2, C), dtype=x.dtype, device=x.device)
+        _check(_lib.lib.agd_nn_pool2(ptr(x), ptr(y), N, H // 2, W // 2, C, 0, stream))
+        return y
+    N, H, W, C = x.shape  # gradient on the coarse grid
+    g = torch.empty((N, 2 * H, 2 * W, C), dtype=x.dtype, device=x.device)
+    _check(_lib.lib.agd_nn_pool2(ptr(x), ptr(g), N, H, W, C, 1, stream))
+    return g
+
+
+def _up(x, stream, backward=False):
+    if not backward:
+        N, H, W, C = x.shape
+        y = torch.empty((N, 2 * H, 2 * W, C), dtype=x.dtype, device=x.device)
+        _check(_lib.lib.agd_nn_up2(ptr(x), ptr(y), N, H, W, C, 0, stream))
+        return y
+    N, H, W, C = x.shape  # gradient on the fine grid
+    g = torch.empty((N, H // 2, W // 2, C), dtype=x.dtype, device=x.device)
+    _check(_lib.lib.agd_nn_up2(ptr(x), ptr(g), N, H // 2, W // 2, C, 1, stream))
+    return g
+
+
+class Surrogate:
+    """4-level encoder-decoder; input (N, H, W, 32) BF16 features (6 used),
+    output channel 0 of (N, H, W, 32).  H, W multiples of 128."""
+
+    def __init__(self, widths=(32, 64, 128, 128), seed=0):
+        rng = np.random.default_rng(seed)
+        w = widths
+        self.enc = [(Conv3x3(NFEAT, w[0], rng), Conv3x3(w[0], w[0], rng))]
+        for i in range(1, 4):
+            self.enc.append((Conv3x3(w[i - 1], w[i], rng), Conv3x3(w[i], w[i], rng)))
+        self.dec = []
+        for i in (2, 1, 0):
+            self.dec.append((Conv3x3(w[i + 1], w[i], rng), Conv3x3(w[i], w[i], rng)))
+        self.head = Conv3x3(w[0], 1, rng, gain=1.0)
+        self.layers = [c for pair in self.enc + self.dec for c in pair] + [self.head]
+        self.t = 0
+
+    def forward(self, x, stream=None, cache=None):
+        s = stream_handle() if stream is None else stream
+        c = {} if cache is None else cache
+        h = x
+        skips = []
+        for lvl, (c1, c2) in enumerate(self.enc):
+            if lvl > 0:
+                c[f'p{lvl}_in'] = h
+                h = _pool(h, s)
+            c[f'e{lvl}_x'] = h
+            a = c1.forward(h, True, s)
+            c[f'e{lvl}_a'] = a
+            h = c2.forward(a, True, s)
+            c[f'e{lvl}_b'] = h
+            skips.append(h)
+        for j, (c1, c2) in enumerate(self.dec):
+            lvl = 2 - j
+            u = _up(h, s)
+            c[f'd{lvl}_u'] = u
+            a = c1.forward(u, True, s)
+            c[f'd{lvl}_a'] = a
+            sm = _add(a, skips[lvl], s)
+            c[f'd{lvl}_s'] = sm
+            h = c2.forward(sm, True, s)
+            c[f'd{lvl}_b'] = h
+        c['h_x'] = h
+        return self.head.forward(h, False, s)
+
+    def backward(self, gy, c, stream=None):
+        s = stream_handle() if stream is None else stream
+        g = self.head.backward(c['h_x'], gy, s)
+        gskip = [None] * 4
+        # decoder in reverse order: level 0, 1, 2 (self.dec[2 - level])
+        for lvl in (0, 1, 2):
+            c1, c2 = self.dec[2 - lvl]
+            _relu_mask(g, c[f'd{lvl}_b'], s)
+            g = c2.backward(c[f'd{lvl}_s'], g, s)       # gradient of the sum a + skip
+            gskip[lvl] = g.clone()
+            _relu_mask(g, c[f'd{lvl}_a'], s)
+            g = c1.backward(c[f'd{lvl}_u'], g, s)
+            g = _up(g, s, backward=True)
+        # encoder in reverse order: level 3 .. 0 (levels 0..2 also receive
+        # their skip gradient)
+        for lvl in (3, 2, 1, 0):
+            c1, c2 = self.enc[lvl]
+            if gskip[lvl] is not None:
+                g = _add(g, gskip[lvl], s)
+            _relu_mask(g, c[f'e{lvl}_b'], s)
+            g = c2.backward(c[f'e{lvl}_a'], g, s)
+            _relu_mask(g, c[f'e{lvl}_a'], s)
+            g = c1.backward(c[f'e{lvl}_x'], g, s, need_gx=lvl > 0)
+            if lvl > 0:
+                g = _pool(g, s, backward=True)
+        return g
+
+    def step(self, lr, stream=None):
+        s = stream_handle() if stream is None else stream
+        self.t += 1
+        for layer in self.layers:
+            layer.step(lr, self.t, s)
+
+    def loss_and_grad(self, pred, target, stream=None, mask=None):
+        s = stream_handle() if stream is None else stream
+        N, H, W, C = pred.shape
+        gy = torch.empty_like(pred)
+        loss = torch.zeros(1, dtype=torch.float32, device=pred.device)
+        _check(_lib.lib.agd_nn_mse(ptr(pred), ptr(target), ptr(mask), ptr(gy), N * H * W, C, 1.0,
+                                   float(N * H * W), ptr(loss), s))
+        return loss, gy
+
+    def train(self, feats, targets, epochs=10, batch=16, lr=2e-3, log=None, seed=0):
+        """Adam on (feats (N, H, W, 32) BF16, targets (N, H, W) FP32), cosine
+        learning-rate decay.  Returns the per-epoch mean losses."""
+        n = feats.shape[0]
+        rng = np.random.default_rng(seed)
+        s = stream_handle()
+        history = []
+        nsteps = epochs * max(1, n // batch)
+        step = 0
+        for ep in range(epochs):
+            perm = torch.as_tensor(rng.permutation(n), device=feats.device)
+            tot = torch.zeros((), dtype=torch.float32, device=feats.device)
+            nb = 0
+            for b0 in range(0, n - batch + 1, batch):
+                idx = perm[b0:b0 + batch]
+                x = feats[idx].contiguous()
+                t = targets[idx].contiguous()
+                cache = {}
+                pred = self.forward(x, s, cache)
+                loss, gy = self.loss_and_grad(pred, t, s)
+                self.backward(gy, cache, s)
+                lr_t = lr * 0.5 * (1 + np.cos(np.pi * step / nsteps))
+                self.step(lr_t, s)
+                tot += loss[0]
+                nb += 1
+                step += 1
+            history.append(float(tot.item()) / max(nb, 1))
+            if log:
+                log(f"epoch {ep}: loss {history[-1]:.3e}")
+        return history
+
+    def predict(self, feats, batch=16):
+        outs = []
+        s = stream_handle()
+        for b0 in range(0, feats.shape[0], batch):
+            outs.append(self.forward(feats[b0:b0 + batch].contiguous(), s))
+        return torch.cat(outs)
+
+
+# ---------------------------------------------------------------- problems
+def smooth_fields(count, n, sigma, lo, hi, gen, device):
+    """Gaussian-filtered (periodic) uniform noise, min-max mapped to [lo, hi)
+    per field, built with FFTs on the device."""
+    noise = torch.rand((count, n, n), generator=gen, device=device, dtype=torch.float32)
+    k = torch.fft.fftfreq(n, device=device)
+    g1 = torch.exp(-2 * (np.pi * sigma * k) ** 2)
+    f = torch.fft.ifft2(torch.fft.fft2(noise) * (g1[:, None] * g1[None, :])).real
+    fmin = f.amin(dim=(1, 2), keepdim=True)
+    fmax = f.amax(dim=(1, 2), keepdim=True)
+    f = (f - fmin) / (fmax - fmin).clamp_min(1e-30)
+    return lo + (hi - lo) * f * (1 - 1e-6)
+
+
+def riemann_fields(count, n=1024, seed=0, sigma=20., amax=10., device='cuda'):
+    """Metric fields of the 2D Riemannian workload (M = R(t)^T diag(1, a^2)
+    R(t), t and log a smoothed) and one random seed cell each."""
+    gen = torch.Generator(device=device)
+    gen.manual_seed(int(seed))
+    theta = smooth_fields(count, n, sigma, 0., np.pi, gen, device)
+    loga = smooth_fields(count, n, sigma, 0., float(np.log(amax)), gen, device)
+    a2 = torch.exp(2 * loga)
+    c, s = torch.cos(theta), torch.sin(theta)
+    m00 = c * c + a2 * s * s
+    m01 = (a2 - 1) * s * c
+    m11 = s * s + a2 * c * c
+    seeds = torch.randint(n // 16, n - n // 16, (count, 2), generator=gen, device=device)
+    return torch.stack([m00, m01, m11], dim=1), seeds
+
+
+def seed_distance(metric, seeds, h):
+    """d_s on the fine grid: distance in the metric frozen at the seed,
+    clamped below at h/2 (count, n, n)."""
+    count, _, n, _ = metric.shape
+    dev = metric.device
+    idx = torch.arange(n, device=dev, dtype=torch.float32)
+    ii = torch.arange(count, device=dev)
+    ms = metric[ii, :, seeds[:, 0], seeds[:, 1]]                       # (count, 3)
+    dy = (idx[None, :] - seeds[:, 0:1].float()) * h                      # (count, n)
+    dx = (idx[None, :] - seeds[:, 1:2].float()) * h
+    d2 = (ms[:, 0, None, None] * dy[:, :, None] ** 2 + 2 * ms[:, 1, None, None] * dy[:, :, None]
+          * dx[:, None, :] + ms[:, 2, None, None] * dx[:, None, :] ** 2)
+    return torch.sqrt(d2).clamp_min(0.5 * h)
+
+
+def features(metric, ds, factor=4):
+    """(count, n/f, n/f, 32) BF16 coarse features: log m00,
+    m01/sqrt(m00 m11), log m11, d_s, d_s^2 (pooled) and a constant."""
+    count, _, n, _ = metric.shape
+    pool = torch.nn.functional.avg_pool2d
+    m = pool(metric, factor)
+    dsc = pool(ds[:, None], factor)[:, 0]
+    f = torch.zeros((count, n // factor, n // factor, CP), dtype=torch.float32, device=metric.device)
+    f[..., 0] = torch.log(m[:, 0])
+    f[..., 1] = m[:, 1] / torch.sqrt(m[:, 0] * m[:, 2])
+    f[..., 2] = torch.log(m[:, 2])
+    f[..., 3] = dsc
+    f[..., 4] = 1.
+    f[..., 5] = dsc * dsc
+    return f.to(torch.bfloat16)
+
+
+def dict_in(metric_i, seed_i, n, **extra):
+    """dictIn of one problem ([-1,1]^2, n^2 cells, seed at a cell centre)."""
+    from . import dictIn
+    from .. import Metrics
+    h = 2. / n
+    m = torch.stack([torch.stack([metric_i[0], metric_i[1]]), torch.stack([metric_i[1], metric_i[2]])])
+    seed_pt = (-1 + (seed_i.double().cpu().numpy() + 0.5) * h)[None].astype(np.float32)
+    hfm = dictIn({'model': 'Riemann2', 'metric': Metrics.Riemann(m), 'seeds': seed_pt,
+                  'verbosity': 0, 'exportValues': True, **extra})
+    hfm.SetRect([[-1, 1], [-1, 1]], dims=[n, n])
+    return hfm
+
+
+def solve_batched(metric, seeds, atol, rtol, values=None, batch=16, check=False):
+    """Solves of every field in batches of `batch` problems (warm when
+    `values` is given: every tile starts active).  Returns the values
+    (count, n, n), the summed batch GPU time (ms), the passes and, with
+    `check`, the number of cells per problem left below their scheme update
+    (agd_eikonal_check)."""
+    import ctypes
+    from . import batch as B
+    count, _, n, _ = metric.shape
+    out = torch.empty((count, n, n), dtype=torch.float32, device=metric.device)
+    ms, passes = 0., 0
+    bad = []
+    for b0 in range(0, count, batch):
+        hfms = []
+        for i in range(b0, min(count, b0 + batch)):
+            extra = {'atol': atol, 'rtol': rtol}
+            if values is not None:
+                extra.update(values=values[i], warmStartAllTiles=True)
+            hfms.append(dict_in(metric[i], seeds[i], n, **extra))
+        b = B.Batch(hfms)
+        st = b.solve(reset=False)
+        for j, it in enumerate(b.ifaces):
+            out[b0 + j] = b.values(j)
+            if check:
+                nv = ctypes.c_int64(0)
+                _check(_lib.lib.agd_eikonal_check(it._desc(), it._io(), atol, rtol,
+                                                  ctypes.byref(nv), it.stream))
+                bad.append(int(nv.value))
+        ms += float(st.gpu_ms)
+        passes += int(st.launches)
+    return (out, ms, passes, bad) if check else (out, ms, passes)
+
+
+def solve_warm(net, metric, seeds, atol, rtol, batch=16, inflate=1.05):
+    """Warm-started solves judged by the solver: problems with cells left
+    below their scheme update are solved again from a cold start.  Returns
+    values, total GPU ms (prediction excluded), passes, fallback count."""
+    warm = warm_values(net, metric, seeds, inflate)
+    u, ms, passes, bad = solve_batched(metric, seeds, atol, rtol, values=warm, batch=batch,
+                                       check=True)
+    redo = [i for i, v in enumerate(bad) if v > 0]
+    if redo:
+        idx = torch.as_tensor(redo, device=metric.device)
+        u2, ms2, p2 = solve_batched(metric[idx], seeds[idx], atol, rtol, batch=batch)
+        u[idx] = u2
+        ms += ms2
+        passes += p2
+    return u, ms, passes, len(redo)
+
+
+def build_dataset(count, n=1024, factor=4, seed=0, chunk=256, atol=1e-6, rtol=1e-6, log=None):
+    """Coarse features (count, n/f, n/f, 32) BF16 and targets r = u / d_s
+    pooled (count, n/f, n/f) FP32 of `count` random problems, labelled by
+    cold solves of the solver itself; built `chunk` fields at a time."""
+    h = 2. / n
+    feats, targs = [], []
+    t0 = time.time()
+    for c0 in range(0, count, chunk):
+        k = min(chunk, count - c0)
+        metric, seeds = riemann_fields(k, n, seed=seed * 100003 + c0)
+        ds = seed_distance(metric, seeds, h)
+        u, _, _ = solve_batched(metric, seeds, atol, rtol)
+        feats.append(features(metric, ds, factor))
+        r = torch.nn.functional.avg_pool2d((u / ds)[:, None], factor)[:, 0]
+        targs.append(r.contiguous())
+        del metric, ds, u
+        if log:
+            log(f"dataset: {c0 + k}/{count} fields ({time.time() - t0:.1f} s)")
+    return torch.cat(feats), torch.cat(targs)
+
+
+def warm_values(net, metric, seeds, inflate=1.05, factor=4):
+    """Inflated predictions on the fine grid (count, n, n): inflate *
+    upsampled r_hat * d_s (d_s = 0 at the seed cell itself is replaced by
+    the seed value 0 through the seed's fixed value)."""
+    count, _, n, _ = metric.shape
+    h = 2. / n
+    ds = seed_distance(metric, seeds, h)
+    pred = net.predict(features(metric, ds, factor))
+    out = torch.empty((count, n, n), dtype=torch.float32, device=metric.device)
+    nc = n // factor
+    s = stream_handle()
+    for i in range(count):
+        _check(_lib.lib.agd_nn_warm_values(ptr(pred[i].contiguous()), nc, nc, CP, factor, inflate,
+                                           ptr(ds[i].contiguous()), ptr(out[i]), n, n, s))
+    return out

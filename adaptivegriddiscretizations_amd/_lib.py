@@ -80,6 +80,8 @@ SIGNATURES = {
                                    c_void_p]),
     "agd_eikonal_solve_batched": (c_int32, [POINTER(EikonalDesc), c_int32, POINTER(BatchInstance),
                                             POINTER(EikonalIO), POINTER(EikonalStats), c_void_p]),
+    "agd_eikonal_check": (c_int32, [POINTER(EikonalDesc), POINTER(EikonalIO), c_double, c_double,
+                                    POINTER(c_int64), c_void_p]),
     "agd_scheme_supported": (c_int32, [c_int32, c_int32, c_int32_p]),
     "agd_eikonal_scheme": (c_int32, [POINTER(EikonalDesc), POINTER(EikonalIO), c_void_p]),
     "agd_eikonal_adjoint": (c_int32, [POINTER(EikonalDesc), c_void_p, c_void_p, c_void_p,
@@ -101,6 +103,23 @@ SIGNATURES = {
                                     c_void_p, c_void_p, c_void_p]),
     "agd_stencil_export": (c_int32, [c_int32, c_int32, c_int32, c_int64, c_void_p, c_double_p,
                                      c_void_p, c_void_p, c_void_p]),
+    "agd_nn_conv3x3": (c_int32, [c_void_p, c_void_p, c_void_p, c_int32, c_int32, c_int32, c_int32,
+                                 c_int32, c_int32, c_void_p]),
+    "agd_nn_conv3x3_wgrad": (c_int32, [c_void_p, c_void_p, c_void_p, c_int32, c_int32, c_int32,
+                                       c_int32, c_int32, c_void_p]),
+    "agd_nn_conv3x3_flip": (c_int32, [c_void_p, c_void_p, c_int32, c_int32, c_void_p]),
+    "agd_nn_relu_mask": (c_int32, [c_void_p, c_void_p, c_int64, c_void_p]),
+    "agd_nn_pool2": (c_int32, [c_void_p, c_void_p, c_int32, c_int32, c_int32, c_int32, c_int32,
+                               c_void_p]),
+    "agd_nn_up2": (c_int32, [c_void_p, c_void_p, c_int32, c_int32, c_int32, c_int32, c_int32,
+                             c_void_p]),
+    "agd_nn_add": (c_int32, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p]),
+    "agd_nn_mse": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_int32, c_double,
+                             c_double, c_void_p, c_void_p]),
+    "agd_nn_adam": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_double,
+                              c_double, c_double, c_double, c_int32, c_void_p]),
+    "agd_nn_warm_values": (c_int32, [c_void_p, c_int32, c_int32, c_int32, c_int32, c_double,
+                                     c_void_p, c_void_p, c_int32, c_int32, c_void_p]),
     "agd_malloc": (c_int32, [POINTER(c_void_p), c_int64]),
     "agd_free": (c_int32, [c_void_p]),
     "agd_memcpy_htod": (c_int32, [c_void_p, c_void_p, c_int64, c_void_p]),

@@ -307,6 +307,7 @@ int get_workspace(int64_t ntiles, int64_t shard_cap, int64_t nhist, Workspace*& 
     AGD_HIP_TRY(hipEventCreateWithFlags(&w.chunk_ev[1], hipEventDisableTiming));
     AGD_HIP_TRY(hipStreamCreateWithFlags(&w.capture_stream, hipStreamNonBlocking));
     AGD_HIP_TRY(hipMalloc((void**)&w.counts, kCountsLen * sizeof(int32_t)));
+    AGD_HIP_TRY(hipMalloc((void**)&w.check_count, 64));
   }
   if (w.tile_cap < ntiles) {
     if (w.stamp) {
@@ -442,10 +443,11 @@ static int solve_impl(const agd_eikonal_desc* d, const agd_eikonal_io* io,
 
 template <class S>
 static int flow_impl(const agd_eikonal_desc* d, const agd_eikonal_io* io, const agd_flow_io* f,
-                     hipStream_t stream) {
+                     hipStream_t stream, int32_t* violations = nullptr, double chk_atol = 0.,
+                     double chk_rtol = 0.) {
   KernelSet<S> ks;
   AGD_REQUIRE(lookup<S>(d->model, d->ndim, d->shape_i, ks), "unsupported tile shape for this model");
-  AGD_REQUIRE(io && io->values && io->rhs && io->seeds && f, "missing buffers");
+  AGD_REQUIRE(io && io->values && io->rhs && io->seeds && (f || violations), "missing buffers");
   const GridDesc g = make_grid(d);
   FlowArgs<S> a{};
   a.u = static_cast<S*>(io->values);
@@ -455,10 +457,13 @@ static int flow_impl(const agd_eikonal_desc* d, const agd_eikonal_io* io, const 
   a.wall = io->wall_dist;
   a.g = g;
   for (int k = 0; k < 3; ++k) a.iso_w[k] = S(d->iso_weights[k]);
-  a.fw = static_cast<S*>(f->flow_weights);
-  a.wsum = static_cast<S*>(f->flow_weightsum);
-  a.fidx = f->flow_indices;
-  a.fvec = static_cast<S*>(f->flow_vector);
+  a.fw = f ? static_cast<S*>(f->flow_weights) : nullptr;
+  a.wsum = f ? static_cast<S*>(f->flow_weightsum) : nullptr;
+  a.fidx = f ? f->flow_indices : nullptr;
+  a.fvec = f ? static_cast<S*>(f->flow_vector) : nullptr;
+  a.violations = violations;
+  a.chk_atol = S(chk_atol);
+  a.chk_rtol = S(chk_rtol);
   AGD_REQUIRE(d->order >= 0 && d->order <= 2 && (d->order < 2 || d->model != AGD_TTI),
               "order must be 1 or 2 (1 for TTI; 0 means 1)");
   a.order2 = d->order == 2 ? 1 : 0;
@@ -674,6 +679,27 @@ int agd_eikonal_flow(const agd_eikonal_desc* desc, const agd_eikonal_io* io,
   if (st) return st;
   if (desc->scalar == AGD_F32) return flow_impl<float>(desc, io, flow, as_stream(stream));
   return flow_impl<double>(desc, io, flow, as_stream(stream));
+}
+
+int agd_eikonal_check(const agd_eikonal_desc* desc, const agd_eikonal_io* io, double atol,
+                      double rtol, int64_t* violations, void* stream) {
+  int st = check_desc(desc);
+  if (st) return st;
+  AGD_REQUIRE(violations, "null output");
+  hipStream_t s = as_stream(stream);
+  Workspace* ws = nullptr;
+  st = get_workspace(1, 1, 1, ws);
+  if (st) return st;
+  AGD_HIP_TRY(hipMemsetAsync(ws->check_count, 0, sizeof(int32_t), s));
+  st = desc->scalar == AGD_F32
+           ? flow_impl<float>(desc, io, nullptr, s, ws->check_count, atol, rtol)
+           : flow_impl<double>(desc, io, nullptr, s, ws->check_count, atol, rtol);
+  if (st) return st;
+  int32_t n = 0;
+  AGD_HIP_TRY(hipMemcpyAsync(&n, ws->check_count, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  AGD_HIP_TRY(hipStreamSynchronize(s));
+  *violations = n;
+  return AGD_OK;
 }
 
 int agd_stencil_export(int32_t model, int32_t ndim, int32_t scalar, int64_t n, const void* geom,

@@ -516,6 +516,11 @@ template <class S> struct FlowArgs {
   S* fvec;
   int32_t order2;  // second-order correction (not with TTI)
   S order2_thr;
+  // Residual check instead of export: count the non-seed cells whose update
+  // exceeds their value by more than chk_atol + chk_rtol |u| (a value below
+  // the fixed point, which the decreasing iteration cannot raise).
+  int32_t* violations;
+  S chk_atol, chk_rtol;
 };
 
 template <class S, int MODEL, int D, int T0, int T1, int T2>
@@ -657,6 +662,12 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_flow(FlowArgs<S> a) {
     const int sg = 2 * side[k] - 1;
 #pragma unroll
     for (int c = 0; c < D; ++c) vec[c] += fw[k] * S(sg * st.e[k][c]);
+  }
+  if (a.violations) {
+    const bool bad = !seed && val > u_old + a.chk_atol + a.chk_rtol * abs_(u_old);
+    const uint64_t m = __ballot(bad);
+    if ((threadIdx.x & 63) == 0 && m) atomicAdd(a.violations, int(__popcll(m)));
+    return;
   }
   __syncthreads();  // every lane has read its neighbours before any store
   if (!seed && val < u_old) a.u[n_t] = val;

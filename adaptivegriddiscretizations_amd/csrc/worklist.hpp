@@ -37,6 +37,7 @@ struct Workspace {
   int32_t* counts = nullptr;  // kCountsLen
   int32_t* hist = nullptr;    // hist_cap: list length of every pass
   int32_t* score = nullptr;   // 2 * tile_cap (FIM scores; fixed list of global iteration)
+  int32_t* check_count = nullptr;  // residual-check counter
   int32_t* host_counts = nullptr;  // pinned, 2 * kRow (two chunks in flight)
   int64_t tile_cap = 0, shard_cap = 0, hist_cap = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;

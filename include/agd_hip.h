@@ -158,6 +158,14 @@ int agd_eikonal_solve_batched(const agd_eikonal_desc* desc, int32_t ninst,
                               const agd_batch_instance* instances, const agd_eikonal_io* io,
                               agd_eikonal_stats* stats, void* stream);
 
+/* Residual check ("the solver as judge" of a warm start; no counterpart in
+ * the reference): counts the non-seed cells whose scheme update exceeds their
+ * value by more than atol + rtol |u|.  After a solve started from a
+ * supersolution the count is 0; a positive count means the start was below
+ * the fixed point somewhere (the decreasing iteration cannot recover). */
+int agd_eikonal_check(const agd_eikonal_desc* desc, const agd_eikonal_io* io, double atol,
+                      double rtol, int64_t* violations, void* stream);
+
 /* Precompute the scheme of every cell into io->scheme_weights /
  * io->scheme_codes (replaces the precompute_scheme export kernel,
  * _Kernel.py:254-268,346-359, Update.h:76-80 import_scheme_macro).  Needs
@@ -254,6 +262,41 @@ int agd_memcpy_htod(void* dst, const void* src, int64_t bytes, void* stream);
 int agd_memcpy_dtoh(void* dst, const void* src, int64_t bytes, void* stream);
 int agd_memset(void* dst, int32_t value, int64_t bytes, void* stream);
 int agd_stream_synchronize(void* stream);
+
+/* ------------------------------------------------------------ surrogate
+ * Kernels of the convolutional surrogate that warm-starts the iteration (no
+ * counterpart in the reference).  Activations are BF16 NHWC with channels
+ * padded to multiples of 32 (W a multiple of 16 for the forward
+ * convolution); weights BF16 [9][cout][cin], tap = 3*(dy+1)+(dx+1);
+ * accumulation in FP32 on the matrix cores. */
+int agd_nn_conv3x3(const void* in, const void* w, void* out, int32_t N, int32_t H, int32_t W,
+                   int32_t Cin, int32_t Cout, int32_t relu, void* stream);
+/* dw[9][cout][cin] (FP32) += gradient of the convolution w.r.t. its weights. */
+int agd_nn_conv3x3_wgrad(const void* in, const void* gout, float* dw, int32_t N, int32_t H,
+                         int32_t W, int32_t Cin, int32_t Cout, void* stream);
+/* Weights of the data gradient: gin = agd_nn_conv3x3(gout, wt) with wt from
+ * this call (taps reversed, cin and cout exchanged). */
+int agd_nn_conv3x3_flip(const void* w, void* wt, int32_t Cin, int32_t Cout, void* stream);
+int agd_nn_relu_mask(void* g, const void* x, int64_t n, void* stream);
+/* 2x2 average pooling / nearest 2x upsampling (H, W: the coarse grid) and
+ * their adjoints (backward = 1). */
+int agd_nn_pool2(const void* in, void* out, int32_t N, int32_t H, int32_t W, int32_t C,
+                 int32_t backward, void* stream);
+int agd_nn_up2(const void* in, void* out, int32_t N, int32_t H, int32_t W, int32_t C,
+               int32_t backward, void* stream);
+int agd_nn_add(const void* a, const void* b, void* out, int64_t n, void* stream);
+/* Masked mean square error on channel 0 of pred, scaled by 1/scale^2 and
+ * divided by count; writes the gradient and adds the loss to *loss. */
+int agd_nn_mse(const void* pred, const float* target, const float* mask, void* gpred,
+               int64_t npix, int32_t C, double scale, double count, float* loss, void* stream);
+int agd_nn_adam(float* w, const float* g, float* m, float* v, void* wb, int64_t n, double lr,
+                double b1, double b2, double eps, int32_t step, void* stream);
+/* Warm-start values on the fine H x W grid: inflate * scale * (bilinear
+ * upsampling by `factor` of channel 0 of the coarse prediction, clamped below
+ * at 0); scale (H x W, FP32) may be NULL. */
+int agd_nn_warm_values(const void* pred, int32_t Hc, int32_t Wc, int32_t C, int32_t factor,
+                       double inflate, const float* scale, float* out, int32_t H, int32_t W,
+                       void* stream);
 
 #ifdef __cplusplus
 }
