@@ -355,10 +355,17 @@ def solve_batched(metric, seeds, atol, rtol, values=None, batch=16, check=False)
 def solve_warm(net, metric, seeds, atol, rtol, batch=16, inflate=1.05):
     """Warm-started solves judged by the solver: problems with cells left
     below their scheme update are solved again from a cold start.  Returns
-    values, total GPU ms (prediction excluded), passes, fallback count."""
+    values, total GPU ms (network prediction included), passes, fallback
+    count."""
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
     warm = warm_values(net, metric, seeds, inflate)
+    e1.record()
+    e1.synchronize()
+    ms_pred = e0.elapsed_time(e1)
     u, ms, passes, bad = solve_batched(metric, seeds, atol, rtol, values=warm, batch=batch,
                                        check=True)
+    ms += ms_pred
     redo = [i for i, v in enumerate(bad) if v > 0]
     if redo:
         idx = torch.as_tensor(redo, device=metric.device)

@@ -104,9 +104,9 @@ def test_network_forward_and_weight_gradients():
     from adaptivegriddiscretizations_amd.Eikonal import surrogate as S
     rng = np.random.default_rng(2)
     net = S.Surrogate(seed=3)
-    x = bf(np.concatenate([rng.normal(size=(2, 128, 128, S.NFEAT)),
-                           np.zeros((2, 128, 128, S.CP - S.NFEAT))], axis=-1))
-    t = torch.as_tensor(rng.normal(size=(2, 128, 128)).astype(np.float32), device='cuda')
+    x = bf(np.concatenate([rng.normal(size=(1, 128, 128, S.NFEAT)),
+                           np.zeros((1, 128, 128, S.CP - S.NFEAT))], axis=-1))
+    t = torch.as_tensor(rng.normal(size=(1, 128, 128)).astype(np.float32), device='cuda')
     cache = {}
     pred = net.forward(x, cache=cache)
     _, gy = net.loss_and_grad(pred, t)
