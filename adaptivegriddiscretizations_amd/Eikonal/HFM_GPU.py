@@ -20,6 +20,7 @@ from .. import AutomaticDifferentiation as ad
 from .. import FiniteDifferences as fd
 from .. import Metrics
 from .. import _lib
+from . import _factor
 from ._device import (torch, device_tensor, host_array, ptr, stream_handle, empty,
                       full, zeros)
 
@@ -150,8 +151,21 @@ class Interface:
         self.multiprecision = self.values_float64 or bool(self.GetValue(
             'multiprecision', default=False,
             help="Use multiprecision arithmetic, to improve accuracy (FP64 arithmetic here)"))
-        if self.GetValue('factoringRadius', default=0, help="Source factorization (not provided)"):
-            raise NotImplementedError("source factorization is not provided by the MI355X solver")
+        # Source factorization (_Kernel.py:47-49, 301-308): around a single
+        # seed, neighbour values are shifted by the finite-difference error of
+        # the distance for the metric frozen at the seed.
+        self.factoringRadius = float(self.GetValue(
+            'factoringRadius', default=0,
+            help="Use source factorization, to improve accuracy"))
+        if self.factoringRadius < 0:
+            raise ValueError("factoringRadius must be >= 0")
+        if self.factoringRadius and self.model_ == 'TTI':
+            raise NotImplementedError("source factorization of the TTI model is not provided")
+        if self.factoringRadius:
+            # The point choice concerns the CPU solver's factorization; the
+            # GPU kernel always freezes the metric at the seed.
+            self.GetValue('factoringPointChoice', default='Seed',
+                          help="Where the factorization metric is taken (the seed, on the GPU)")
         self.order = int(self.GetValue('order', default=1,
                                        help="Scheme order (1, or 2: second order where the solution is smooth)"))
         if self.order not in (1, 2):
@@ -360,6 +374,18 @@ class Interface:
             ctypes_double3(self.h), ptr(lq_d), ptr(A_d), stride, ptr(self.geom), self.stream))
         self._keep = (lq_d, A_d)
 
+    def _cost_metric(self):
+        """The metric with the grid steps and the cost folded in, evaluated
+        at a few points (_Interface.py:202-211)."""
+        metric = drift = None
+        if self.model_ == 'Diagonal':
+            metric = self.metric.costs
+        elif self.model_ in ('Riemann', 'Rander'):
+            metric = self.metric.m
+            if self.model_ == 'Rander':
+                drift = self.metric.w
+        return _factor.CostMetric(self.model_, self.h, self.cost_t, metric, drift)
+
     def _field_tensor(self, arr, core, name):
         """float32 device tensor of a field of shape core + shape (or core
         alone, broadcast), and its cell stride (1 field, 0 constant)."""
@@ -413,6 +439,8 @@ class Interface:
         self.reverseAD = self.HasValue('sensitivity')
         self.seedIndices = None
         self.seed_coef = None
+        self.seed_rows = np.zeros(0, dtype=np.int64)
+        self.factor = None
         if seeds is not None:
             seeds = np.asarray(host_array(seeds), dtype=self.hfmIn.float_t)
             if seeds.ndim != 2 or seeds.shape[1] != self.ndim:
@@ -430,15 +458,34 @@ class Interface:
             seedValues = np.asarray(host_array(seedValues), dtype=np.float64).reshape(-1)
             if len(seedValues) != len(seeds):
                 raise ValueError("seedValues must have one entry per seed")
-            if self.GetValue('seedRadius', default=0.,
-                             help="Spread the seeds over a radius given in pixels (0 only)") != 0.:
-                raise NotImplementedError("seedRadius>0 is not provided yet")
-            idx = np.round(cont).astype(np.int64)  # round half to even (_SetArgs.py:44-45)
-            for k in range(self.ndim):
-                if self.periodic[k]:
-                    idx[:, k] %= self.shape[k]
+            seedRadius = float(self.GetValue(
+                'seedRadius', default=2. if self.factoringRadius > 0 else 0.,
+                help="Spread the seeds over a radius given in pixels, so as to improve accuracy."))
+            if (seedRadius > 0 or self.factoringRadius) and len(seeds) != 1:
+                raise ValueError("seedRadius > 0 and factoringRadius apply to a single seed")
+            cm = self._cost_metric() if (seedRadius > 0 or self.factoringRadius) else None
+            if seedRadius <= 0:
+                idx = np.round(cont).astype(np.int64)  # round half to even (_SetArgs.py:44-45)
+                for k in range(self.ndim):
+                    if self.periodic[k]:
+                        idx[:, k] %= self.shape[k]
+                self.seed_rows = np.arange(len(seeds))
+            else:
+                # _SetArgs.py:46-75: the seed becomes the cells within
+                # seedRadius, valued by the mean of the cost metric norms at
+                # both ends.  Each cell keeps the seed's AD row.
+                idx, seedValues = _factor.spread_seed(cm, cont[0], seedRadius, self.shape,
+                                                      self.periodic, seedValues[0])
+                if len(idx) == 0:
+                    raise ValueError("seedRadius selects no cell of the domain")
+                self.seed_rows = np.zeros(len(idx), dtype=np.int64)
             if np.any(idx < 0) or np.any(idx >= np.asarray(self.shape)):
                 raise ValueError("Seed outside the domain")
+            if self.factoringRadius:
+                a0, _ = cm.at(np.asarray(cont[:1], dtype=np.float64), self.shape, self.periodic)
+                self.factor = {'radius2': self.factoringRadius ** 2,
+                               'origin': np.asarray(cont[0], dtype=np.float64),
+                               'metric': cm.factor_metric(a0)}
             self.seedIndices = idx
             self.seedValues = seedValues
             tiled = fd.tiled_index(idx, self.shape_i, self.shape_o)
@@ -474,8 +521,8 @@ class Interface:
         # it (seven sub-schemes per cell exceed the register budget).
         self.scheme_w_t = self.scheme_c_t = None
         self.precompute_scheme = False
-        if self.order == 1 and _lib.lib.agd_scheme_supported(self.modelId, self.ndim,
-                                                             _lib.i32(self.shape_i)):
+        if self.order == 1 and not self.factoringRadius and _lib.lib.agd_scheme_supported(
+                self.modelId, self.ndim, _lib.i32(self.shape_i)):
             self.precompute_scheme = bool(self.GetValue(
                 'precompute_scheme', default=self.model_ == 'TTI',
                 help="Precompute and store the finite difference scheme stencils"))
@@ -520,6 +567,13 @@ class Interface:
         d.use_graph = int(getattr(self, 'use_graph', True))
         d.order = int(getattr(self, 'order', 1))
         d.order2_threshold = float(getattr(self, 'order2_threshold', 0.3))
+        fac = getattr(self, 'factor', None)
+        if fac is not None:
+            d.factor_radius2 = fac['radius2']
+            for k in range(self.ndim):
+                d.factor_origin[k] = float(fac['origin'][k])
+            for k, v in enumerate(fac['metric']):
+                d.factor_metric[k] = float(v)
         return d
 
     def compute_scheme(self):
@@ -625,6 +679,9 @@ class Interface:
             return
         if self.model_ == 'Rander':
             raise NotImplementedError("automatic differentiation of the Rander model is not provided yet")
+        if self.order == 2 or self.factoringRadius:
+            # _PostProcess.py:165-166: the linearised scheme is the flow's.
+            self.Warn("Eikonal AD ignores order2 and source factorization")
         if self.forwardAD:
             self.values_coef = self.forward_variation()
         if self.reverseAD:
@@ -632,7 +689,8 @@ class Interface:
             lam, seedSens = self.reverse(sens)
             self.costSensitivity = lam
             self.hfmOut['costSensitivity'] = self._output(lam)
-            # seedValues_rev is the identity (seedRadius = 0): one row per seed.
+            # _PostProcess.py:217-219: each seed collects the sensitivities of
+            # the cells it was spread to (one cell per seed without spreading).
             self.hfmOut['seedValueSensitivity'] = self._output(seedSens)
 
     def _expand_components(self, x_c, pad):
@@ -701,7 +759,8 @@ class Interface:
         else:
             rhs = zeros(tuple(self.shape) + (nf,), self.float_t)
         if self.seedIndices is not None:
-            rhs[self._seed_pos(rhs.device)] = sc if sc is not None else 0.
+            rows = torch.as_tensor(self.seed_rows, device=rhs.device)
+            rhs[self._seed_pos(rhs.device)] = sc[rows] if sc is not None else 0.
         if self.walls is not None:
             rhs[device_tensor(self.walls, np.bool_)] = 0.
         rhs_c = torch.movedim(rhs, -1, 0).contiguous()
@@ -719,8 +778,11 @@ class Interface:
         if self.seedIndices is None:
             return lam, lam.new_zeros((0, int(sens.shape[-1])))
         pos = self._seed_pos(lam.device)
-        seedSens = lam[pos].clone()
+        cellSens = lam[pos].clone()
         lam[pos] = 0
+        nseeds = int(self.seed_rows.max()) + 1 if len(self.seed_rows) else 0
+        seedSens = cellSens.new_zeros((nseeds, int(cellSens.shape[-1])))
+        seedSens.index_add_(0, torch.as_tensor(self.seed_rows, device=lam.device), cellSens)
         return lam, seedSens
 
     # ------------------------------------------------------------ outputs

@@ -25,6 +25,9 @@
 #include "eikonal_kernels.hpp"
 #include "worklist.hpp"
 
+static_assert(sizeof(agd_eikonal_desc) == 232, "agd_eikonal_desc layout changed");
+static_assert(sizeof(agd_eikonal_io) == 64, "agd_eikonal_io layout changed");
+
 namespace agd {
 
 // Initial list: tiles holding at least one trigger bit (or every tile).
@@ -268,7 +271,20 @@ static int check_desc(const agd_eikonal_desc* d) {
   for (int k = 0; k < d->ndim; ++k)
     AGD_REQUIRE(ceil_div(d->shape[k], d->shape_i[k]) <= (d->ndim == 2 ? 32768 : 1024),
                 "too many tiles along an axis (2D: 32768, 3D: 1024)");
+  AGD_REQUIRE(d->factor_radius2 >= 0., "factor_radius2 must be >= 0 (0 disables factorization)");
+  AGD_REQUIRE(d->factor_radius2 == 0. || d->model != AGD_TTI,
+              "source factorization is not provided for the TTI model");
   return AGD_OK;
+}
+
+// Source factorization parameters of a solve (disabled when factor_radius2
+// is 0).
+template <class S> static FactorDesc<S> make_factor(const agd_eikonal_desc* d) {
+  FactorDesc<S> f{};
+  f.r2 = S(d->factor_radius2);
+  for (int k = 0; k < 3; ++k) f.origin[k] = S(d->factor_origin[k]);
+  for (int k = 0; k < 6; ++k) f.m[k] = S(d->factor_metric[k]);
+  return f;
 }
 
 static GridDesc make_grid(const agd_eikonal_desc* d) {
@@ -373,6 +389,8 @@ static int solve_impl(const agd_eikonal_desc* d, const agd_eikonal_io* io,
   AGD_REQUIRE(d->order >= 0 && d->order <= 2, "order must be 1 or 2 (0 means 1)");
   AGD_REQUIRE(!order2 || (!pre && ks.update_order2),
               "the second-order scheme has no precomputed-scheme variant (nor a TTI one)");
+  const bool factor = d->factor_radius2 > 0.;
+  AGD_REQUIRE(!factor || !pre, "source factorization has no precomputed-scheme variant");
   EikArgs<S> a{};
   a.u = static_cast<S*>(io->values);
   a.geom = static_cast<const S*>(io->geom);
@@ -389,6 +407,7 @@ static int solve_impl(const agd_eikonal_desc* d, const agd_eikonal_io* io,
   a.solver = global ? AGD_AGSI : d->solver;
   a.fim_front = d->fim_front_width;
   a.order2_thr = S(d->order2_threshold);
+  a.fac = make_factor<S>(d);
 
   auto init = [&](Workspace* ws) -> int {
     a.w.lists = ws->lists;
@@ -415,9 +434,10 @@ static int solve_impl(const agd_eikonal_desc* d, const agd_eikonal_io* io,
   };
   auto launch = [&](int32_t koff, hipStream_t s) {
     a.w.koff = koff;
+    // walls, periodic axes and source factorization run in the general variant
     auto kern = pre ? ks.update_pre
                     : order2 ? ks.update_order2
-                             : ((io->wall_dist || g.periodic) ? ks.update_walls : ks.update);
+                             : ((io->wall_dist || g.periodic || factor) ? ks.update_walls : ks.update);
     hipLaunchKernelGGL(kern, dim3(blocks), dim3(ni), 0, s, a);
   };
   WorklistResult r;
@@ -468,6 +488,7 @@ static int flow_impl(const agd_eikonal_desc* d, const agd_eikonal_io* io, const 
               "order must be 1 or 2 (1 for TTI; 0 means 1)");
   a.order2 = d->order == 2 ? 1 : 0;
   a.order2_thr = S(d->order2_threshold);
+  a.fac = make_factor<S>(d);
   hipLaunchKernelGGL(ks.flow, dim3(g.ntiles), dim3(ks.ni), 0, stream, a);
   AGD_HIP_TRY(hipGetLastError());
   return AGD_OK;
@@ -497,6 +518,7 @@ static int solve_batched_impl(const agd_eikonal_desc* d, int32_t ninst,
   AGD_REQUIRE(d->solver != AGD_FIM || d->fim_front_width >= 2, "fim_front_width must be >= 2");
   for (int k = 0; k < d->ndim; ++k) AGD_REQUIRE(!d->periodic[k], "batched solves are not periodic");
   AGD_REQUIRE(d->order <= 1, "batched solves use the first-order scheme");
+  AGD_REQUIRE(d->factor_radius2 == 0., "batched solves take no source factorization");
   const int ni = ks.ni;
   std::vector<InstDesc> h(static_cast<size_t>(ninst));
   int64_t tiles = 0;

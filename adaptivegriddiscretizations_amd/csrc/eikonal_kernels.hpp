@@ -31,6 +31,13 @@
 
 namespace agd {
 
+// Source factorization parameters (see factor_shifts below).
+template <class S> struct FactorDesc {
+  S origin[3];
+  S r2;    // squared radius; 0 disables
+  S m[6];  // M: diagonal (Isotropic, Diagonal) or lower triangle
+};
+
 // One instance of a batched solve: its own grid and constants; its tiles
 // occupy [tile_base, tile_base + g.ntiles) of the batch's tile space.
 struct InstDesc {
@@ -61,6 +68,7 @@ template <class S> struct EikArgs {
   const InstDesc* inst;
   const int32_t* tile_inst;
   S order2_thr;  // VAR_ORDER2: threshold of the second-order criterion
+  FactorDesc<S> fac;  // VAR_GENERAL, VAR_ORDER2: source factorization
 };
 
 // HFM.h:91-134: where |u0 - 2v + v2| < thr * max_k(u0 - v_k)_+ and u0 > v,
@@ -82,6 +90,95 @@ __device__ inline unsigned order2_correct(S u0, S thr, S (&v)[NS], const S (&v2)
     }
   }
   return mask;
+}
+
+// ------------------------------------------------------------------ factor
+// Source factorization (EuclideanFactor.h:8-30, Constants.h:50-61,
+// FiniteDifferences.h:30-45).  Inside the ball |x - origin|^2 < r2 (grid
+// units) the value of the neighbour x + s e is shifted by
+//   fact_s = T(x) - T(x + s e) + s <e, grad T(x)>,   T(y) = |y - origin|_M,
+// so that T solves the scheme exactly when the metric is the constant M.
+// The second-order correction reads its further neighbour x + 2 s e shifted
+// by fact2_s = 2 fact_s - (T(x) - 2 T(x + s e) + T(x + 2 s e)).
+
+// <x, M y>: one fused multiply-add per coefficient of M after the (rounded)
+// coordinate products; off-diagonal pairs as fma(x_i, y_j, x_j y_i).
+template <class S, int D, bool DIAG>
+__device__ inline S factor_dot(const S* m, const S* x, const S* y) {
+#pragma clang fp contract(off)
+  S r = S(0);
+  if constexpr (DIAG) {
+#pragma unroll
+    for (int i = 0; i < D; ++i) r = fma_(x[i] * y[i], m[i], r);
+  } else {
+    int k = 0;
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+#pragma unroll
+      for (int j = 0; j <= i; ++j, ++k) {
+        const S p = (i == j) ? x[i] * y[i] : fma_(x[i], y[j], x[j] * y[i]);
+        r = fma_(p, m[k], r);
+      }
+    }
+  }
+  return r;
+}
+
+// fact[0], fact[1]: shifts of x - e and x + e; fact2 (nullable) likewise
+// for x - 2e and x + 2e.  Differences of norms in cancellation-free form.
+template <class S>
+__device__ inline void euclid_factor(S xx, S xe, S ee, S* fact, S* fact2) {
+#pragma clang fp contract(off)
+  const S nx = sqrt_(xx);
+  const S nxm = sqrt_(xx - S(2) * xe + ee);
+  const S nxp = sqrt_(xx + S(2) * xe + ee);
+  const S dm = (S(2) * xe - ee) / (nx + nxm);   // |x| - |x - e|
+  const S dp = (-S(2) * xe - ee) / (nx + nxp);  // |x| - |x + e|
+  const S ge = xe / nx;                         // <e, grad T(x)>
+  fact[0] = dm - ge;
+  fact[1] = ge + dp;
+  if (fact2) {
+    const S nxm2 = sqrt_(xx - S(4) * xe + S(4) * ee);
+    const S nxp2 = sqrt_(xx + S(4) * xe + S(4) * ee);
+    const S am = (-S(2) * xe + S(3) * ee) / (nxm + nxm2);  // |x - 2e| - |x - e|
+    const S ap = (S(2) * xe + S(3) * ee) / (nxp + nxp2);   // |x + 2e| - |x + e|
+    fact2[0] = S(2) * fact[0] - (dm + am);
+    fact2[1] = S(2) * fact[1] - (dp + ap);
+  }
+}
+
+// Adds the factorization shifts of the NS offsets e[k] of the cell at grid
+// position xt to sh[2k + s] (and sets sh2, when given): the drift shift
+// already in sh is kept (fact_0 + drift, fact_1 - drift as
+// FiniteDifferences.h:41-45).  No-op outside the ball.
+template <class S, int MODEL, int D, int NS>
+__device__ inline void factor_shifts(const FactorDesc<S>& f, const int* xt, const int (&e)[NS][3],
+                                     S* sh, S* sh2) {
+#pragma clang fp contract(off)
+  S x[3] = {S(0), S(0), S(0)};
+  S n2 = S(0);
+#pragma unroll
+  for (int c = 0; c < D; ++c) {
+    x[c] = S(xt[c]) - f.origin[c];
+    n2 = fma_(x[c], x[c], n2);
+  }
+  if (!(n2 < f.r2)) return;
+  constexpr bool DIAG = MODEL == AGD_ISOTROPIC || MODEL == AGD_DIAGONAL;
+  const S xx = factor_dot<S, D, DIAG>(f.m, x, x);
+#pragma unroll
+  for (int k = 0; k < NS; ++k) {
+    S ef[3] = {S(e[k][0]), S(e[k][1]), S(e[k][2])};
+    const S xe = factor_dot<S, D, DIAG>(f.m, x, ef);
+    const S ee = factor_dot<S, D, DIAG>(f.m, ef, ef);
+    S fact[2], fact2[2];
+    euclid_factor<S>(xx, xe, ee, fact, sh2 ? fact2 : nullptr);
+    sh[2 * k] = fact[0] + sh[2 * k];
+    sh[2 * k + 1] = fact[1] + sh[2 * k + 1];
+    if (sh2) {
+      sh2[2 * k] = fact2[0];
+      sh2[2 * k + 1] = fact2[1];
+    }
+  }
 }
 
 __device__ inline int32_t read_score(const int32_t* arr, int32_t tile, int32_t iter) {
@@ -277,7 +374,14 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
 
     int code[NJ];
     int code2[ORDER2 ? NJ : 1];
-    S sw[NS], shift[NS];
+    // sh[2k + s]: shift of the value of neighbour x -+ e_k (drift, source
+    // factorization); sh2 likewise for the further neighbours (order 2).
+    S sw[NS], sh[NJ];
+    S sh2[ORDER2 ? NJ : 1];
+    if constexpr (ORDER2) {
+#pragma unroll
+      for (int j = 0; j < NJ; ++j) sh2[j] = S(0);
+    }
     if constexpr (MT::NMIX > 1) {
       // several sub-schemes: read per sweep below
     } else if constexpr (VAR == VAR_PRE) {
@@ -285,7 +389,8 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
 #pragma unroll
       for (int k = 0; k < NS; ++k) {
         sw[k] = a.sw[k * a.g.size_tot + n_t];
-        shift[k] = S(0);
+        sh[2 * k] = S(0);
+        sh[2 * k + 1] = S(0);
       }
 #pragma unroll
       for (int j = 0; j < NJ; ++j) code[j] = a.scode[j * a.g.size_tot + n_t];
@@ -304,7 +409,11 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
 #pragma unroll
       for (int k = 0; k < NS; ++k) {
         sw[k] = st.w[k];
-        shift[k] = st.shift[k];
+        sh[2 * k] = st.shift[k];
+        sh[2 * k + 1] = -st.shift[k];
+      }
+      if constexpr (GENERAL) {
+        if (a.fac.r2 > S(0)) factor_shifts<S, MODEL, D, NS>(a.fac, xt, st.e, sh, ORDER2 ? sh2 : nullptr);
       }
     }
     if constexpr (MT::NMIX > 1) {
@@ -353,14 +462,7 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
       }
       S vo[NJ];
 #pragma unroll
-      for (int k = 0; k < NS; ++k) {
-#pragma unroll
-        for (int s = 0; s < 2; ++s) {
-          const int j = 2 * k + s;
-          const S sh = (s == 0) ? shift[k] : -shift[k];
-          vo[j] = code[j] <= -2 ? sh : (code[j] >= 0 ? g[j] + sh : inf_v<S>());
-        }
-      }
+      for (int j = 0; j < NJ; ++j) vo[j] = code[j] <= -2 ? sh[j] : (code[j] >= 0 ? g[j] + sh[j] : inf_v<S>());
       __syncthreads();  // u_s complete
 
       // ---- in-tile sweeps (HFMIter.h:25-84, DECREASING)
@@ -378,7 +480,7 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
             // the further neighbour on the chosen side
             const int j2 = 2 * k + ((a0 > a1) ? 1 : 0);
             const int c2 = code2[j2];
-            w2v[k] = c2 <= -2 ? u_s[-2 - c2] : (c2 >= 0 ? g2[j2] : inf_v<S>());
+            w2v[k] = c2 <= -2 ? u_s[-2 - c2] + sh2[j2] : (c2 >= 0 ? g2[j2] + sh2[j2] : inf_v<S>());
           }
         }
         if constexpr (ORDER2) order2_correct<S, NS>(u_s[tid], a.order2_thr, v, w2v, w);
@@ -521,6 +623,7 @@ template <class S> struct FlowArgs {
   // the fixed point, which the decreasing iteration cannot raise).
   int32_t* violations;
   S chk_atol, chk_rtol;
+  FactorDesc<S> fac;
 };
 
 template <class S, int MODEL, int D, int T0, int T1, int T2>
@@ -552,6 +655,16 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_flow(FlowArgs<S> a) {
   // stencil.
   auto gather = [&](const Stencil<S, MODEL, D>& st, S* v, S* w, int* side, int64_t* nb,
                     S* v2) {
+    S sh[2 * NS], sh2[2 * NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+      sh[2 * k] = st.shift[k];
+      sh[2 * k + 1] = -st.shift[k];
+      sh2[2 * k] = sh2[2 * k + 1] = S(0);
+    }
+    if constexpr (MT::NMIX == 1) {
+      if (a.fac.r2 > S(0)) factor_shifts<S, MODEL, D, NS>(a.fac, xt, st.e, sh, v2 ? sh2 : nullptr);
+    }
 #pragma unroll
     for (int k = 0; k < NS; ++k) {
       S best = inf_v<S>(), best2 = inf_v<S>();
@@ -561,7 +674,6 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_flow(FlowArgs<S> a) {
       for (int s = 0; s < 2; ++s) {
         const int sg = 2 * s - 1;
         int e[3] = {sg * st.e[k][0], sg * st.e[k][1], sg * st.e[k][2]};
-        const S sh = (s == 0) ? st.shift[k] : -st.shift[k];
         bool vis = true;
         if (a.wall && wx != 255) {
           vis = visible<D>(e, wx, [&](const int* ww) -> int {
@@ -572,12 +684,12 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_flow(FlowArgs<S> a) {
         }
         int yt[3] = {xt[0] + e[0], xt[1] + e[1], xt[2] + e[2]};
         const int64_t m = vis ? index_tot<D, TL>(yt, a.g) : -1;
-        const S val = (m >= 0) ? a.u[m] + sh : inf_v<S>();
+        const S val = (m >= 0) ? a.u[m] + sh[2 * k + s] : inf_v<S>();
         S val2 = inf_v<S>();
         if (v2 && vis) {
           int y2[3] = {yt[0] + e[0], yt[1] + e[1], yt[2] + e[2]};
           const int64_t m2 = index_tot<D, TL>(y2, a.g);
-          if (m2 >= 0) val2 = a.u[m2];
+          if (m2 >= 0) val2 = a.u[m2] + sh2[2 * k + s];
         }
         if (s == 0) { best = val; best2 = val2; bs = 0; bi = m; }
         else if (best > val) { best = val; best2 = val2; bs = 1; bi = m; }

@@ -79,6 +79,16 @@ typedef struct agd_eikonal_desc {
                              where the solution is smooth (HFM.h:91-134) */
   int32_t reserved;       /* 0                                           */
   double order2_threshold;/* order 2: |u-2v+v2| < threshold * max(u-v) (0.3) */
+  /* Source factorization (EuclideanFactor.h:8-30, Constants.h:50-61): the
+   * cells with |x - factor_origin|^2 < factor_radius2 (grid units, Euclidean)
+   * shift their neighbour values by the finite-difference error of
+   * T(x) = |x - factor_origin|_M, M = factor_metric.  0 disables it.  Not with
+   * TTI, a precomputed scheme or batched solves. */
+  double factor_radius2;
+  double factor_origin[3];/* seed position in grid units                 */
+  double factor_metric[6];/* M in grid units: squared costs (Isotropic, Diagonal:
+                             ndim entries) or lower triangle (d00, d10, d11,
+                             d20, d21, d22) for Riemann and Rander */
 } agd_eikonal_desc;
 
 /* Device pointers, tiled layout.  Scalars are float (AGD_F32) or double. */

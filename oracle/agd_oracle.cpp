@@ -431,10 +431,59 @@ static bool visible(const Grid& G, const int64_t* x, const int* e,
 static int g_order = 1;
 static double g_order2_threshold = 0.3;
 
+// Source factorization (EuclideanFactor.h:8-30, Constants.h:50-61,
+// FiniteDifferences.h:30-45), set for the whole library by orc_set_factor:
+// inside |x - origin|^2 < r2 the value of neighbour x + s e is shifted by
+// fact_s = T(x) - T(x + s e) + s <e, grad T(x)>, T(y) = |y - origin|_M, and
+// the further neighbour x + 2 s e (order 2) by
+// fact2_s = 2 fact_s - (T(x) - 2 T(x + s e) + T(x + 2 s e)).
+struct Factor {
+  double r2 = 0.;       // 0: off
+  double origin[3] = {0., 0., 0.};
+  double m[6] = {0., 0., 0., 0., 0., 0.};  // diagonal, or lower triangle
+};
+static Factor g_factor;
+
+// <x, M y> with one fma per coefficient after the rounded coordinate
+// products (off-diagonal pairs fma(x_i, y_j, x_j y_i)).
+template <class S>
+static S factor_dot(int d, bool diag, const S* m, const S* x, const S* y) {
+  S r = 0;
+  if (diag) {
+    for (int i = 0; i < d; ++i) r = fmaT(x[i] * y[i], m[i], r);
+    return r;
+  }
+  int k = 0;
+  for (int i = 0; i < d; ++i)
+    for (int j = 0; j <= i; ++j, ++k) {
+      const S p = (i == j) ? x[i] * y[i] : fmaT(x[i], y[j], x[j] * y[i]);
+      r = fmaT(p, m[k], r);
+    }
+  return r;
+}
+
+// EuclideanFactor.h:8-30: norm differences in cancellation-free form.
+template <class S>
+static void euclid_factor(S xx, S xe, S ee, S* f, S* f2) {
+  const S nx = std::sqrt(xx);
+  const S nxm = std::sqrt(xx - S(2) * xe + ee), nxp = std::sqrt(xx + S(2) * xe + ee);
+  const S dm = (S(2) * xe - ee) / (nx + nxm);
+  const S dp = (-S(2) * xe - ee) / (nx + nxp);
+  const S ge = xe / nx;
+  f[0] = dm - ge;
+  f[1] = ge + dp;
+  const S nxm2 = std::sqrt(xx - S(4) * xe + S(4) * ee), nxp2 = std::sqrt(xx + S(4) * xe + S(4) * ee);
+  const S am = (-S(2) * xe + S(3) * ee) / (nxm + nxm2);
+  const S ap = (S(2) * xe + S(3) * ee) / (nxp + nxp2);
+  f2[0] = S(2) * f[0] - (dm + am);
+  f2[1] = S(2) * f[1] - (dp + ap);
+}
+
 struct Problem {
   int model, d;
   int order;
   double order2_threshold;
+  Factor factor;
   Grid G;
   const double* geom;       // ngeom * N, component-major
   const double* rhs;        // N: cost, seed value at seeds, +inf in walls
@@ -450,11 +499,36 @@ static void neighbours(const Problem& P, const Stencil<S>& st, int base, int64_t
                        S* v, int* side, int64_t* nidx, S* v2 = nullptr) {
   int64_t x[3];
   P.G.pos(n, x);
+  // factorization: position relative to the origin, inside the ball?
+  S xr[3] = {0, 0, 0};
+  bool fac = false;
+  const bool diag = P.model == ISOTROPIC || P.model == DIAGONAL;
+  S fm[6];
+  if (P.factor.r2 > 0) {
+    S n2 = 0;
+    for (int c = 0; c < P.d; ++c) {
+      xr[c] = S(x[c]) - S(P.factor.origin[c]);
+      n2 = fmaT(xr[c], xr[c], n2);
+    }
+    fac = n2 < S(P.factor.r2);
+    for (int c = 0; c < 6; ++c) fm[c] = S(P.factor.m[c]);
+  }
   for (int k = 0; k < st.nsym; ++k) {
     const int* ek = st.e[base + k];
     S shift = 0;
     if (st.has_drift) {
       for (int c = 0; c < P.d; ++c) shift += st.drift[c] * S(ek[c]);
+    }
+    S sh[2] = {shift, -shift}, sh2[2] = {0, 0};
+    if (fac) {
+      const S ef[3] = {S(ek[0]), S(ek[1]), S(ek[2])};
+      S f[2], f2[2];
+      euclid_factor<S>(factor_dot<S>(P.d, diag, fm, xr, xr), factor_dot<S>(P.d, diag, fm, xr, ef),
+                       factor_dot<S>(P.d, diag, fm, ef, ef), f, f2);
+      sh[0] = f[0] + sh[0];
+      sh[1] = f[1] + sh[1];
+      sh2[0] = f2[0];
+      sh2[1] = f2[1];
     }
     S best = Inf<S>(), best2 = Inf<S>();
     int bs = 0;
@@ -468,10 +542,11 @@ static void neighbours(const Problem& P, const Stencil<S>& st, int base, int64_t
       int64_t ny = -1;
       if (visible(P.G, x, ee, P.wallDist)) {
         ny = P.G.index(y);
-        if (ny >= 0) val = get(ny) + (s == 0 ? shift : -shift);
-        // further neighbour x + 2 eps e, no shift (FiniteDifferences.h:75-90)
+        if (ny >= 0) val = get(ny) + sh[s];
+        // further neighbour x + 2 eps e: factorization shift only
+        // (FiniteDifferences.h:75-90, HFM.h:116)
         const int64_t ny2 = P.G.index(y2);
-        if (ny2 >= 0) val2 = get(ny2);
+        if (ny2 >= 0) val2 = get(ny2) + sh2[s];
       }
       if (s == 0) { best = val; best2 = val2; bs = 0; bi = ny; }
       else if (best > val) { best = val; best2 = val2; bs = 1; bi = ny; }
@@ -571,6 +646,7 @@ static void fill_problem(Problem& P, int model, int d, const int64_t* shape,
   P.d = d;
   P.order = g_order;
   P.order2_threshold = g_order2_threshold;
+  P.factor = g_factor;
   P.G.init(d, shape, periodic);
   P.geom = geom;
   P.rhs = rhs;
@@ -612,11 +688,22 @@ static int64_t jacobi(const Problem& P, double atol, double rtol, int64_t niterm
 
 extern "C" {
 
-int orc_version() { return 3; }
+int orc_version() { return 4; }
 
 void orc_set_order(int order, double threshold) {
   g_order = order;
   g_order2_threshold = threshold;
+}
+
+// Source factorization for the following solves (r2 = 0 switches it off):
+// origin[3] in grid units, m[6] the metric at the origin in grid units.
+void orc_set_factor(double r2, const double* origin, const double* m) {
+  g_factor = Factor{};
+  g_factor.r2 = r2;
+  if (r2 > 0) {
+    for (int k = 0; k < 3; ++k) g_factor.origin[k] = origin[k];
+    for (int k = 0; k < 6; ++k) g_factor.m[k] = m[k];
+  }
 }
 
 // Selling decomposition of n symmetric matrices (row n: symdim coefficients,
@@ -709,7 +796,8 @@ void orc_tti_diags(int64_t n, const double* l, const double* q, double* diag, in
 int64_t orc_solve_fm(int model, int d, const int64_t* shape, const int* periodic,
                      const double* geom, const double* rhs, const uint8_t* seeds,
                      const uint8_t* wallDist, const double* iso_w, double* u) {
-  if (g_order != 1) return -1;  // the second-order scheme is solved by iteration
+  // the second-order and factored schemes are solved by iteration
+  if (g_order != 1 || g_factor.r2 > 0) return -1;
   Problem P;
   fill_problem(P, model, d, shape, periodic, geom, rhs, seeds, wallDist, iso_w);
   const int64_t N = P.G.N;

@@ -12,7 +12,9 @@ GPU eikonal path:
     HFM_CUDA/_Kernel.py:316-317, Metrics/riemann.py:38-73,
     Metrics/rander.py:61-68, Metrics/misc.py:8-15);
   * rhs = cost with rhs[seed] = seed value, walls -> rhs = inf and fixed
-    (_SetArgs.py:116-126).
+    (_SetArgs.py:116-126);
+  * seed spreading over a radius and the source factorization metric
+    (_SetArgs.py:46-78, _Interface.py:202-211, _Kernel.py:301-308).
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it.
 """
 import ctypes
@@ -56,6 +58,7 @@ def _load():
     lib.orc_adjoint.argtypes = [c_int64, c_int, P, P, P, c_int, P, P]
     lib.orc_forward_ad.argtypes = [c_int64, c_int, P, P, P, c_int, P, P]
     lib.orc_set_order.argtypes = [c_int, c_double]
+    lib.orc_set_factor.argtypes = [c_double, P, P]
     lib.orc_wall_distance.argtypes = [c_int, P, P, P, c_int, P]
     return lib
 
@@ -333,3 +336,140 @@ def make_problem(model, shape, h, metric=None, drift=None, cost=1., seeds_idx=()
         wd = wall_distance(walls, wall_bound, periodic)
     return Problem(mid, shape, None if geom is None else geom.reshape(-1, N), rhs, seeds, iso_w,
                    wd, periodic)
+
+
+# ---------------------------------------------------------------- source factorization
+class factor:
+    """Context manager: source factorization (EuclideanFactor.h:8-30,
+    Constants.h:50-61) for the following solves and flows: radius in grid
+    units around origin (grid units), metric m6 at the origin in grid units
+    (squared costs for Isotropic/Diagonal, lower triangle otherwise; see
+    factor_metric)."""
+
+    def __init__(self, radius, origin, m6):
+        self.r2 = float(radius) ** 2
+        self.origin = _c(list(np.asarray(origin, dtype=np.float64).reshape(-1)) + [0.] * 3, np.float64)[:3].copy()
+        self.m6 = _c(list(np.asarray(m6, dtype=np.float64).reshape(-1)) + [0.] * 6, np.float64)[:6].copy()
+
+    def __enter__(self):
+        lib().orc_set_factor(self.r2, _p(self.origin), _p(self.m6))
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_set_factor(0., None, None)
+
+
+def _interp_corners(pts, shape, periodic):
+    """Multilinear (order 1) interpolation stencil at points pts[k, d] (grid
+    units): corner cells [2^d, k, d] and weights [2^d, k].  Non-periodic axes
+    clamp the point onto [0, n-1]."""
+    pts = np.asarray(pts, dtype=np.float64)
+    k, d = pts.shape
+    lo = np.zeros((k, d), dtype=np.int64)
+    t = np.zeros((k, d))
+    for c in range(d):
+        n = shape[c]
+        x = pts[:, c]
+        if periodic[c]:
+            b = np.floor(x)
+        else:
+            x = np.clip(x, 0, n - 1)
+            b = np.clip(np.floor(x), 0, max(n - 2, 0))
+        lo[:, c] = b.astype(np.int64)
+        t[:, c] = x - b
+    cells, weights = [], []
+    for bits in range(2 ** d):
+        off = np.array([(bits >> (d - 1 - c)) & 1 for c in range(d)])
+        idx = lo + off
+        w = np.prod(np.where(off == 1, t, 1 - t), axis=1)
+        for c in range(d):
+            idx[:, c] = idx[:, c] % shape[c] if periodic[c] else np.minimum(idx[:, c], shape[c] - 1)
+        cells.append(idx)
+        weights.append(w)
+    return np.stack(cells), np.stack(weights)
+
+
+def cost_metric_at(model, pts, h, shape, metric=None, drift=None, cost=1., periodic=None):
+    """The cost metric of _Interface.py:202-211 in grid units at points
+    pts[k, d] (grid units): metric.with_costs(h).with_cost(cost), i.e.
+    Isotropic c_i = h_i cost, Diagonal c_i = costs_i h_i cost, Riemann
+    m_ij h_i h_j cost^2, Rander (m_ij h_i h_j cost^2, w_i h_i cost),
+    multilinearly interpolated (Interpolation.py order 1).  Returns
+    ('diag', c[k, d]) or ('sym', m[k, d, d], w[k, d] or None)."""
+    mid = model if isinstance(model, int) else MODEL_IDS[model]
+    shape = tuple(int(s) for s in shape)
+    d = len(shape)
+    pts = np.asarray(pts, dtype=np.float64).reshape(-1, d)
+    h = np.asarray(h, dtype=np.float64)
+    periodic = list(periodic or [0] * d)
+    cells, wts = _interp_corners(pts, shape, periodic)
+
+    def field(a, core):
+        a = np.asarray(a, dtype=np.float64)
+        if a.shape == core:
+            a = a.reshape(core + (1,) * d)
+        return np.broadcast_to(a, core + shape)
+
+    def at(f, core):
+        # f[core..., *shape] at every corner -> [2^d, k, *core]
+        sel = f[(Ellipsis,) + tuple(np.moveaxis(cells, -1, 0))]
+        return np.moveaxis(sel, tuple(range(len(core))), tuple(range(2, 2 + len(core))))
+
+    c = at(field(cost, ()), ())
+    if mid in (ISOTROPIC, DIAGONAL):
+        costs = np.ones((d,)) if mid == ISOTROPIC else metric
+        cc = at(field(costs, (d,)), (d,)) * h * c[..., None]
+        return 'diag', np.einsum('ck,ckd->kd', wts, cc)
+    if mid not in (RIEMANN, RANDER):
+        raise NotImplementedError("cost metric of the TTI model")
+    m = at(field(metric, (d, d)), (d, d)) * np.outer(h, h) * (c ** 2)[..., None, None]
+    mi = np.einsum('ck,ckij->kij', wts, m)
+    wi = None
+    if mid == RANDER:
+        w = at(field(drift, (d,)), (d,)) * h * c[..., None]
+        wi = np.einsum('ck,ckd->kd', wts, w)
+    return 'sym', mi, wi
+
+
+def metric_norm(cm, diff):
+    """Norms of diff[k, d] for the cost metric cm (one point, broadcast, or
+    one per row)."""
+    diff = np.asarray(diff, dtype=np.float64)
+    if cm[0] == 'diag':
+        return np.sqrt(np.sum((cm[1] * diff) ** 2, axis=-1))
+    m, w = cm[1], cm[2]
+    n = np.sqrt(np.einsum('ki,kij,kj->k', diff, np.broadcast_to(m, (len(diff),) + m.shape[-2:]), diff))
+    if w is not None:
+        n = n + np.sum(np.broadcast_to(w, diff.shape) * diff, axis=-1)
+    return n
+
+
+def factor_metric(cm):
+    """The factorization metric of _Kernel.py:301-308 from the cost metric at
+    the seed: squared costs (diagonal models), else the lower triangle of m
+    (the drift of Rander is left out)."""
+    if cm[0] == 'diag':
+        return (cm[1].reshape(-1) ** 2)
+    m = cm[1].reshape(cm[1].shape[-2:])
+    return flatten_lower(m)
+
+
+def spread_seed(seed, radius, shape, cm_at, value=0., periodic=None):
+    """_SetArgs.py:46-75: the single seed (grid units) is replaced by the
+    cells at Euclidean distance < radius, inside the domain on non-periodic
+    axes, with values value + (|x - seed|_{M(seed)} + |x - seed|_{M(x)}) / 2
+    for the cost metric M = cm_at(points).  Returns (cells[k, d], values[k])."""
+    seed = np.asarray(seed, dtype=np.float64).reshape(-1)
+    d = len(seed)
+    periodic = np.asarray(list(periodic or [0] * d), dtype=bool)
+    axes = [np.arange(int(np.floor(c - radius)), int(np.ceil(c + radius)) + 1) for c in seed]
+    neigh = np.stack(np.meshgrid(*axes, indexing='ij'), -1).reshape(-1, d)
+    neigh = neigh[np.linalg.norm(neigh - seed, axis=1) < radius]
+    shp = np.asarray(shape)
+    nper = ~periodic
+    inside = np.all((neigh[:, nper] >= -0.5) & (neigh[:, nper] < shp[nper] - 0.5), axis=1)
+    neigh = neigh[inside]
+    diff = neigh - seed
+    vals = value + 0.5 * (metric_norm(cm_at(seed[None]), diff) + metric_norm(cm_at(neigh), diff))
+    cells = np.where(periodic, neigh % shp, neigh)
+    return cells.astype(np.int64), vals
