@@ -37,6 +37,14 @@ __device__ inline double sqrt_(double a) { return __builtin_sqrt(a); }
 __device__ inline bool isnan_(float a) { return a != a; }
 __device__ inline bool isnan_(double a) { return a != a; }
 
+// Product of two integers of magnitude below 2^23 whose product fits in 32
+// bits: a full-rate 24-bit multiply (v_mul_i32_i24) where a 32-bit one
+// (v_mul_lo_u32) issues at quarter rate.  Used for tile coordinates and
+// extents (below 2^15), and for superbase entries and stencil offsets,
+// which stay below sqrt(6 cond(D)) in magnitude (the reduction decreases
+// sum_i |b_i|_D^2), far below 2^23 for any FP32 metric.
+__device__ inline int imul24(int a, int b) { return __mul24(a, b); }
+
 // ------------------------------------------------------------------ models
 template <int MODEL, int D> struct ModelTraits {
   static constexpr int SYMDIM = D * (D + 1) / 2;
@@ -110,8 +118,8 @@ template <int D> __device__ inline void unpack_tile(int p, int* xo) {
   }
 }
 template <int D> __device__ inline int tile_linear(const int* xo, const GridDesc& g) {
-  if constexpr (D == 2) return xo[0] * g.shape_o[1] + xo[1];
-  else return (xo[0] * g.shape_o[1] + xo[1]) * g.shape_o[2] + xo[2];
+  if constexpr (D == 2) return imul24(xo[0], g.shape_o[1]) + xo[1];
+  else return imul24(imul24(xo[0], g.shape_o[1]) + xo[1], g.shape_o[2]) + xo[2];
 }
 
 // Position of tile n_o in the tile grid.
@@ -165,14 +173,14 @@ __device__ inline int index_tot(const int* y_in, const GridDesc& g) {
   if (!ok) return -1;
   int yo[3], yi[3];
 #pragma unroll
-  for (int k = 0; k < D; ++k) {
-    yo[k] = y[k] / TL::T[k];
-    yi[k] = y[k] % TL::T[k];
+  for (int k = 0; k < D; ++k) {  // y >= 0 here: unsigned division by a power of two
+    yo[k] = int(unsigned(y[k]) / unsigned(TL::T[k]));
+    yi[k] = int(unsigned(y[k]) % unsigned(TL::T[k]));
   }
   if constexpr (D == 2) {
-    return (yo[0] * g.shape_o[1] + yo[1]) * TL::NI + (yi[0] * TL::T[1] + yi[1]);
+    return (imul24(yo[0], g.shape_o[1]) + yo[1]) * TL::NI + (yi[0] * TL::T[1] + yi[1]);
   } else {
-    return ((yo[0] * g.shape_o[1] + yo[1]) * g.shape_o[2] + yo[2]) * TL::NI +
+    return (imul24(imul24(yo[0], g.shape_o[1]) + yo[1], g.shape_o[2]) + yo[2]) * TL::NI +
            ((yi[0] * TL::T[1] + yi[1]) * TL::T[2] + yi[2]);
   }
 }
@@ -182,19 +190,19 @@ __device__ inline int index_tot(const int* y_in, const GridDesc& g) {
 // per coefficient of the lower triangle (d00, d10, d11, d20, d21, d22).
 template <class S> __device__ inline S scal_vmv2(const int* x, const S* m, const int* y) {
   S r = S(0);
-  r = fma_(S(x[0] * y[0]), m[0], r);
-  r = fma_(S(x[1] * y[0] + x[0] * y[1]), m[1], r);
-  r = fma_(S(x[1] * y[1]), m[2], r);
+  r = fma_(S(imul24(x[0], y[0])), m[0], r);
+  r = fma_(S(imul24(x[1], y[0]) + imul24(x[0], y[1])), m[1], r);
+  r = fma_(S(imul24(x[1], y[1])), m[2], r);
   return r;
 }
 template <class S> __device__ inline S scal_vmv3(const int* x, const S* m, const int* y) {
   S r = S(0);
-  r = fma_(S(x[0] * y[0]), m[0], r);
-  r = fma_(S(x[1] * y[0] + x[0] * y[1]), m[1], r);
-  r = fma_(S(x[1] * y[1]), m[2], r);
-  r = fma_(S(x[2] * y[0] + x[0] * y[2]), m[3], r);
-  r = fma_(S(x[2] * y[1] + x[1] * y[2]), m[4], r);
-  r = fma_(S(x[2] * y[2]), m[5], r);
+  r = fma_(S(imul24(x[0], y[0])), m[0], r);
+  r = fma_(S(imul24(x[1], y[0]) + imul24(x[0], y[1])), m[1], r);
+  r = fma_(S(imul24(x[1], y[1])), m[2], r);
+  r = fma_(S(imul24(x[2], y[0]) + imul24(x[0], y[2])), m[3], r);
+  r = fma_(S(imul24(x[2], y[1]) + imul24(x[1], y[2])), m[4], r);
+  r = fma_(S(imul24(x[2], y[2])), m[5], r);
   return r;
 }
 
@@ -259,9 +267,9 @@ template <int I, int J, int K, int L, class S>
 __device__ inline void selling3_out(const int (&sb)[4][3], const S* m, S& w, int* e) {
   const S s = -scal_vmv3<S>(sb[I], m, sb[J]);
   w = s > S(0) ? s : S(0);
-  e[0] = sb[K][1] * sb[L][2] - sb[K][2] * sb[L][1];
-  e[1] = sb[K][2] * sb[L][0] - sb[K][0] * sb[L][2];
-  e[2] = sb[K][0] * sb[L][1] - sb[K][1] * sb[L][0];
+  e[0] = imul24(sb[K][1], sb[L][2]) - imul24(sb[K][2], sb[L][1]);
+  e[1] = imul24(sb[K][2], sb[L][0]) - imul24(sb[K][0], sb[L][2]);
+  e[2] = imul24(sb[K][0], sb[L][1]) - imul24(sb[K][1], sb[L][0]);
 }
 
 // Selling decomposition, 3D: weights w[6], offsets e[6][3].  Permutations

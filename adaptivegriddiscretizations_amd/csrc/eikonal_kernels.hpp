@@ -236,7 +236,7 @@ __device__ inline void resolve_codes(const Stencil<S, MODEL, D>& st, const int* 
         const int li = TL::index(yi);
         int to = yo[0];
 #pragma unroll
-        for (int c = 1; c < D; ++c) to = to * g.shape_o[c] + yo[c];
+        for (int c = 1; c < D; ++c) to = imul24(to, g.shape_o[c]) + yo[c];
         code[j] = !dom ? -1 : (local ? -2 - li : (tile_base + to) * NI + li);
       } else {
         bool vis = true;
@@ -463,6 +463,12 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
       S vo[NJ];
 #pragma unroll
       for (int j = 0; j < NJ; ++j) vo[j] = code[j] <= -2 ? sh[j] : (code[j] >= 0 ? g[j] + sh[j] : inf_v<S>());
+      // LDS slot read at every sweep: the neighbour's own slot inside the
+      // tile, the cell's slot otherwise (read and discarded), so that the
+      // sweep carries no branch per neighbour.
+      int slot[NJ];
+#pragma unroll
+      for (int j = 0; j < NJ; ++j) slot[j] = code[j] <= -2 ? -2 - code[j] : tid;
       __syncthreads();  // u_s complete
 
       // ---- in-tile sweeps (HFMIter.h:25-84, DECREASING)
@@ -470,10 +476,17 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
       for (int it = 0; it < a.niter_i; ++it) {
         S v[NS], w[NS];
         S w2v[ORDER2 ? NS : 1];  // further-neighbour values (order 2)
+        S x[NJ];
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) x[j] = u_s[slot[j]];
+        // Keeps every read unconditional (the compiler would otherwise sink
+        // each into a branch with its own wait on the LDS).
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) asm volatile("" ::"v"(x[j]));
 #pragma unroll
         for (int k = 0; k < NS; ++k) {
-          const S a0 = code[2 * k] <= -2 ? u_s[-2 - code[2 * k]] + vo[2 * k] : vo[2 * k];
-          const S a1 = code[2 * k + 1] <= -2 ? u_s[-2 - code[2 * k + 1]] + vo[2 * k + 1] : vo[2 * k + 1];
+          const S a0 = code[2 * k] <= -2 ? x[2 * k] + vo[2 * k] : vo[2 * k];
+          const S a1 = code[2 * k + 1] <= -2 ? x[2 * k + 1] + vo[2 * k + 1] : vo[2 * k + 1];
           v[k] = (a0 > a1) ? a1 : a0;  // Greater(v,-) keeps the -e side on ties
           w[k] = sw[k];
           if constexpr (ORDER2) {
