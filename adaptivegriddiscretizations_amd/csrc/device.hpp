@@ -45,6 +45,16 @@ __device__ inline bool isnan_(double a) { return a != a; }
 // sum_i |b_i|_D^2), far below 2^23 for any FP32 metric.
 __device__ inline int imul24(int a, int b) { return __mul24(a, b); }
 
+// base[idx] for an array spanning less than 4 GiB: the byte offset is formed
+// in 32 bits, so that the access takes the scalar-base + 32-bit-offset form
+// (one address VGPR per gather instead of a 64-bit pair).
+template <class T> __device__ inline T ld32(const T* base, uint32_t idx) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + idx * uint32_t(sizeof(T)));
+}
+template <class T> __device__ inline void st32(T* base, uint32_t idx, T v) {
+  *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + idx * uint32_t(sizeof(T))) = v;
+}
+
 // ------------------------------------------------------------------ models
 template <int MODEL, int D> struct ModelTraits {
   static constexpr int SYMDIM = D * (D + 1) / 2;

@@ -391,6 +391,9 @@ static int solve_impl(const agd_eikonal_desc* d, const agd_eikonal_io* io,
               "the second-order scheme has no precomputed-scheme variant (nor a TTI one)");
   const bool factor = d->factor_radius2 > 0.;
   AGD_REQUIRE(!factor || !pre, "source factorization has no precomputed-scheme variant");
+  // Fields of 4 GiB or more go to the general variant (64-bit addressing).
+  const bool big = uint64_t(g.size_tot) * sizeof(S) > uint64_t(0xffffffffu);
+  AGD_REQUIRE(!pre || !big, "a precomputed scheme needs fields below 4 GiB (2^30 FP32 cells)");
   EikArgs<S> a{};
   a.u = static_cast<S*>(io->values);
   a.geom = static_cast<const S*>(io->geom);
@@ -437,7 +440,8 @@ static int solve_impl(const agd_eikonal_desc* d, const agd_eikonal_io* io,
     // walls, periodic axes and source factorization run in the general variant
     auto kern = pre ? ks.update_pre
                     : order2 ? ks.update_order2
-                             : ((io->wall_dist || g.periodic || factor) ? ks.update_walls : ks.update);
+                             : ((io->wall_dist || g.periodic || factor || big) ? ks.update_walls
+                                                                               : ks.update);
     hipLaunchKernelGGL(kern, dim3(blocks), dim3(ni), 0, s, a);
   };
   WorklistResult r;
@@ -542,7 +546,8 @@ static int solve_batched_impl(const agd_eikonal_desc* d, int32_t ninst,
     I.atol = bi[i].atol;
     I.rtol = bi[i].rtol;
     tiles += I.g.ntiles;
-    AGD_REQUIRE(tiles * ni < (int64_t(1) << 31), "batches of 2^31 padded cells or more are not supported");
+    AGD_REQUIRE(uint64_t(tiles * ni) * sizeof(S) <= uint64_t(0xffffffffu),
+                "batches hold less than 4 GiB per field (2^30 FP32 cells)");
   }
   GridDesc gb{};
   gb.ntiles = int32_t(tiles);

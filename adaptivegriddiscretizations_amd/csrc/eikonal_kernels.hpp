@@ -293,6 +293,14 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
   constexpr bool GENERAL = VAR == VAR_GENERAL || ORDER2;
   constexpr bool WALLS = GENERAL;
   constexpr bool BATCH = VAR == VAR_BATCH;
+  // Every variant but the general ones addresses per-cell arrays with 32-bit
+  // byte offsets (the host sends grids of 4 GiB per field or more to the
+  // general variant).
+  constexpr bool OFF32 = !GENERAL;
+  auto ldu = [&](const S* base, int idx) -> S {
+    if constexpr (OFF32) return ld32(base, uint32_t(idx));
+    else return base[idx];
+  };
   static_assert((T0 * T1 * T2) % 64 == 0, "tiles are whole waves (pass_begin scans per wave)");
   using TL = Tile<D, T0, T1, T2>;
   using MT = ModelTraits<MODEL, D>;
@@ -300,7 +308,7 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
   constexpr int NS = MT::NSYM;
   constexpr int NG = MT::NGEOM;
   constexpr int NJ = 2 * NS;
-  __shared__ S u_s[NI];
+  __shared__ S u_s[NI + 1];  // slot NI holds 0: the read of neighbours outside the tile
   __shared__ uint8_t wall_s[WALLS ? NI : 1];
 
   const int tid = threadIdx.x;
@@ -362,8 +370,8 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
     }
 
     // ---- cell loads (coalesced: a tile is contiguous)
-    const S u_old = a.u[n_t];
-    const S rhs = a.rhs[n_t];
+    const S u_old = ldu(a.u, n_t);
+    const S rhs = ldu(a.rhs, n_t);
     const bool seed = (a.seeds[n_t >> 5] >> (n_t & 31)) & 1u;
     int wx = 255;
     if constexpr (WALLS) {
@@ -371,6 +379,7 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
       wall_s[tid] = uint8_t(wx);
     }
     u_s[tid] = seed ? rhs : u_old;
+    if (tid == 0) u_s[NI] = S(0);
 
     int code[NJ];
     int code2[ORDER2 ? NJ : 1];
@@ -388,18 +397,18 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
       // Stencil weights and neighbour codes precomputed for the whole solve.
 #pragma unroll
       for (int k = 0; k < NS; ++k) {
-        sw[k] = a.sw[k * a.g.size_tot + n_t];
+        sw[k] = ld32(a.sw + k * a.g.size_tot, uint32_t(n_t));
         sh[2 * k] = S(0);
         sh[2 * k + 1] = S(0);
       }
 #pragma unroll
-      for (int j = 0; j < NJ; ++j) code[j] = a.scode[j * a.g.size_tot + n_t];
+      for (int j = 0; j < NJ; ++j) code[j] = ld32(a.scode + j * a.g.size_tot, uint32_t(n_t));
     } else {
       S geo[NG > 0 ? NG : 1];
 #pragma unroll
       for (int k = 0; k < NG; ++k) {
         const S* __restrict__ gk = a.geom + k * a.g.size_tot;  // uniform base
-        geo[k] = gk[n_t];
+        geo[k] = ldu(gk, n_t);
       }
       Stencil<S, MODEL, D> st;
       make_stencil<S, MODEL, D>(geo, iso_w, st);
@@ -422,7 +431,7 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
       // neighbour codes are re-read at every sweep (cache hits) rather than
       // held in registers; out-of-tile values are read at every sweep.
       const int64_t N = a.g.size_tot;
-      const bool mix_min = a.scode[int64_t(MT::NMIX * NJ) * N + n_t] != 0;
+      const bool mix_min = ld32(a.scode + int64_t(MT::NMIX * NJ) * N, uint32_t(n_t)) != 0;
       __syncthreads();  // u_s complete
       const bool active = !seed;
       for (int it = 0; it < a.niter_i; ++it) {
@@ -432,12 +441,12 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
           S v[NS], w[NS];
 #pragma unroll
           for (int k = 0; k < NS; ++k) {
-            w[k] = a.sw[int64_t(m * NS + k) * N + n_t];
+            w[k] = ld32(a.sw + int64_t(m * NS + k) * N, uint32_t(n_t));
             S side[2];
 #pragma unroll
             for (int sd = 0; sd < 2; ++sd) {
-              const int c = a.scode[int64_t(m * NJ + 2 * k + sd) * N + n_t];
-              side[sd] = c <= -2 ? u_s[-2 - c] : (c >= 0 ? a.u[c] : inf_v<S>());
+              const int c = ld32(a.scode + int64_t(m * NJ + 2 * k + sd) * N, uint32_t(n_t));
+              side[sd] = c <= -2 ? u_s[-2 - c] : (c >= 0 ? ld32(a.u, uint32_t(c)) : inf_v<S>());
             }
             v[k] = (side[0] > side[1]) ? side[1] : side[0];
           }
@@ -454,7 +463,7 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
       // cell), so a visit costs a single memory round trip.
       S g[NJ];
 #pragma unroll
-      for (int j = 0; j < NJ; ++j) g[j] = a.u[code[j] >= 0 ? code[j] : n_t];
+      for (int j = 0; j < NJ; ++j) g[j] = ldu(a.u, code[j] >= 0 ? code[j] : n_t);
       S g2[ORDER2 ? NJ : 1];
       if constexpr (ORDER2) {
 #pragma unroll
@@ -464,11 +473,13 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
 #pragma unroll
       for (int j = 0; j < NJ; ++j) vo[j] = code[j] <= -2 ? sh[j] : (code[j] >= 0 ? g[j] + sh[j] : inf_v<S>());
       // LDS slot read at every sweep: the neighbour's own slot inside the
-      // tile, the cell's slot otherwise (read and discarded), so that the
-      // sweep carries no branch per neighbour.
+      // tile, slot NI (holding 0) otherwise, whose value vo then passes
+      // through unchanged (0 + vo == vo up to the sign of a zero, which no
+      // comparison or later sum distinguishes).  The sweep carries no
+      // branch per neighbour.
       int slot[NJ];
 #pragma unroll
-      for (int j = 0; j < NJ; ++j) slot[j] = code[j] <= -2 ? -2 - code[j] : tid;
+      for (int j = 0; j < NJ; ++j) slot[j] = code[j] <= -2 ? -2 - code[j] : NI;
       __syncthreads();  // u_s complete
 
       // ---- in-tile sweeps (HFMIter.h:25-84, DECREASING)
@@ -485,8 +496,8 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
         for (int j = 0; j < NJ; ++j) asm volatile("" ::"v"(x[j]));
 #pragma unroll
         for (int k = 0; k < NS; ++k) {
-          const S a0 = code[2 * k] <= -2 ? x[2 * k] + vo[2 * k] : vo[2 * k];
-          const S a1 = code[2 * k + 1] <= -2 ? x[2 * k + 1] + vo[2 * k + 1] : vo[2 * k + 1];
+          const S a0 = x[2 * k] + vo[2 * k];
+          const S a1 = x[2 * k + 1] + vo[2 * k + 1];
           v[k] = (a0 > a1) ? a1 : a0;  // Greater(v,-) keeps the -e side on ties
           w[k] = sw[k];
           if constexpr (ORDER2) {
@@ -507,7 +518,8 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
 
     // ---- store and convergence (Update.h:163-219)
     const S un = u_s[tid];
-    a.u[n_t] = un;
+    if constexpr (OFF32) st32(a.u, uint32_t(n_t), un);
+    else a.u[n_t] = un;
     const S diff = abs_(u_old - un);
     const S tol = atol + rtol * abs_(un);
     const bool changed = !(isnan_(diff) || diff <= tol);
