@@ -491,7 +491,7 @@ class Interface:
             tiled = fd.tiled_index(idx, self.shape_i, self.shape_o)
             tiled_d = device_tensor(tiled, np.int64)
             vals_d = device_tensor(seedValues, np.float64)
-            _lib.check(_lib.lib.agd_scatter_seeds(self.scalar, len(seeds), ptr(tiled_d), ptr(vals_d),
+            _lib.check(_lib.lib.agd_scatter_seeds(self.scalar, len(idx), ptr(tiled_d), ptr(vals_d),
                                                   ptr(self.rhs_t), ptr(self.seeds_t),
                                                   ptr(self.trigger_t), self.stream))
         elif self.solver != 'global_iteration' and not self.warmStartAllTiles:

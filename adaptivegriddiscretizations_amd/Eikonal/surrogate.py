@@ -206,13 +206,16 @@ class Surrogate:
         return loss, gy
 
     def train(self, feats, targets, epochs=10, batch=16, lr=2e-3, log=None, seed=0):
-        """Adam on (feats (N, H, W, 32) BF16, targets (N, H, W) FP32), cosine
-        learning-rate decay.  Returns the per-epoch mean losses."""
+        """Adam on (feats (N, H, W, 32) BF16, targets (N, H, W) FP32): linear
+        learning-rate warm-up over the first tenth of the steps (at most 50;
+        full-size first Adam steps throw the deep network far off), then
+        cosine decay.  Returns the per-epoch mean losses."""
         n = feats.shape[0]
         rng = np.random.default_rng(seed)
         s = stream_handle()
         history = []
         nsteps = epochs * max(1, n // batch)
+        warmup = max(1, min(50, nsteps // 10))
         step = 0
         for ep in range(epochs):
             perm = torch.as_tensor(rng.permutation(n), device=feats.device)
@@ -226,7 +229,7 @@ class Surrogate:
                 pred = self.forward(x, s, cache)
                 loss, gy = self.loss_and_grad(pred, t, s)
                 self.backward(gy, cache, s)
-                lr_t = lr * 0.5 * (1 + np.cos(np.pi * step / nsteps))
+                lr_t = lr * min(1., (step + 1) / warmup) * 0.5 * (1 + np.cos(np.pi * step / nsteps))
                 self.step(lr_t, s)
                 tot += loss[0]
                 nb += 1
@@ -291,11 +294,12 @@ def seed_distance(metric, seeds, h):
 
 def features(metric, ds, factor=4):
     """(count, n/f, n/f, 32) BF16 coarse features: log m00,
-    m01/sqrt(m00 m11), log m11, d_s, d_s^2 (pooled) and a constant."""
+    m01/sqrt(m00 m11), log m11, d_s/4, (d_s/4)^2 (pooled) and a constant.
+    The distance is scaled to order one on [-1, 1]^2."""
     count, _, n, _ = metric.shape
     pool = torch.nn.functional.avg_pool2d
     m = pool(metric, factor)
-    dsc = pool(ds[:, None], factor)[:, 0]
+    dsc = 0.25 * pool(ds[:, None], factor)[:, 0]
     f = torch.zeros((count, n // factor, n // factor, CP), dtype=torch.float32, device=metric.device)
     f[..., 0] = torch.log(m[:, 0])
     f[..., 1] = m[:, 1] / torch.sqrt(m[:, 0] * m[:, 2])
