@@ -393,7 +393,14 @@ def build_dataset(count, n=1024, factor=4, seed=0, chunk=256, atol=1e-6, rtol=1e
         ds = seed_distance(metric, seeds, h)
         u, _, _ = solve_batched(metric, seeds, atol, rtol)
         feats.append(features(metric, ds, factor))
-        r = torch.nn.functional.avg_pool2d((u / ds)[:, None], factor)[:, 0]
+        # Cells no stencil reaches keep u = +inf (a corner whose upwind
+        # offsets all leave the grid): pool the finite ratios only.
+        r = u / ds
+        fin = torch.isfinite(r)
+        pool = torch.nn.functional.avg_pool2d
+        num = pool(torch.where(fin, r, torch.zeros_like(r))[:, None], factor)[:, 0]
+        den = pool(fin.float()[:, None], factor)[:, 0]
+        r = torch.where(den > 0, num / den.clamp_min(1e-12), torch.ones_like(num))
         targs.append(r.contiguous())
         del metric, ds, u
         if log:

@@ -348,7 +348,12 @@ def bench_warmstart(args):
     warm1, ms_w1, p_w1, fb1 = S.solve_warm(net, metric, seeds, atol, rtol, batch=1)
     torch.cuda.synchronize()
     wall_warm1 = time.perf_counter() - t0
-    err = float(((warm - cold).abs().amax(dim=(1, 2)) / cold.abs().amax(dim=(1, 2))).max())
+    fin = torch.isfinite(cold)          # cells no stencil reaches stay +inf in both
+    same_inf = bool((torch.isfinite(warm) == fin).all())
+    dev = torch.where(fin, (warm - cold).abs(), torch.zeros_like(cold)).amax(dim=(1, 2))
+    err = float((dev / torch.where(fin, cold, torch.zeros_like(cold)).amax(dim=(1, 2))).max())
+    if not same_inf:
+        err = float('inf')
     log(f"cold: {ms_c:.1f} ms GPU ({wall_cold:.2f} s wall), {p_c / args.test:.1f} passes each; "
         f"warm+batched: {ms_w:.1f} ms GPU ({wall_warm:.2f} s wall), fallbacks {fb}; error vs cold {err:.2e}")
     cells = args.test * n * n
