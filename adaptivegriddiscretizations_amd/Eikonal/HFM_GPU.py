@@ -176,9 +176,12 @@ class Interface:
             self.order2_threshold = float(self.GetValue(
                 'order2_threshold', default=0.3,
                 help="Relative threshold on second order differences, for the second-order scheme"))
-        if self.GetValue('bound_active_blocks', default=False,
-                         help="Limit the number of active blocks (not provided)"):
-            raise NotImplementedError("bound_active_blocks is not provided by the MI355X solver")
+        # _Kernel.py:64-69, _SetGeometry.py:36-37: freeze the tiles whose
+        # smallest change lies above a moving threshold (AGSI only).
+        self.bound_active_blocks = self.GetValue(
+            'bound_active_blocks', default=False,
+            help="Limit the number of active blocks in the front. "
+                 "Admissible values : (False,True, or positive integer)")
 
         self.float_t = np.dtype(traits['Scalar']).type
         if self.float_t not in (np.float32, np.float64):
@@ -196,12 +199,16 @@ class Interface:
             raise ValueError(f"No kernel compiled for tile shape {self.shape_i}")
         self.size_i = int(np.prod(self.shape_i))
 
-        solver = 'FIM' if (self.HasValue('fim_front_width') or fim_default) else 'AGSI'
+        solver = 'FIM' if ((self.HasValue('fim_front_width') or fim_default)
+                           and not self.bound_active_blocks) else 'AGSI'
         solver = self.GetValue('solver', default=solver,
                                help="Choice of fixed point solver (AGSI, FIM, global_iteration)")
         self.solver = _SOLVER_NAMES.get(solver, solver)
         if self.solver not in _SOLVER_IDS:
             raise ValueError(f"Unrecognized solver : {solver}")
+        if self.bound_active_blocks and self.solver != 'adaptive_gauss_siedel_iteration':
+            # _Kernel.py:78-80 (the freezing runs in the AGSI loop only)
+            raise ValueError(f"Incompatible options found for {self.solver} solver (bound_active_blocks)")
         self.fim_front_width = fim_default
         if self.solver == 'fast_iterative_method':
             self.fim_front_width = self.GetValue(
@@ -221,6 +228,9 @@ class Interface:
             help="Apply periodic boundary conditions on some axes"))
         self.shape_o = tuple(int(s) for s in fd.round_up_ratio(self.shape, self.shape_i))
         self.ntiles = int(np.prod(self.shape_o))
+        if self.bound_active_blocks is True:   # _SetGeometry.py:36-37
+            self.bound_active_blocks = 12 * np.prod(self.shape_o) / np.max(self.shape_o)
+        self.bound_active_blocks = int(np.ceil(self.bound_active_blocks or 0))
         self.size_tot = self.ntiles * self.size_i
         if 'gridScale' in self.hfmIn:
             self.GetValue('gridScale', help="Scale of the computational grid")
@@ -291,6 +301,10 @@ class Interface:
         self.wallDistBound = int(self.GetValue(
             'wallDistBound', default=10,
             help="Bound on the computed distance to the obstacles.\n(Ideally a sharp upper bound on the stencil width.)"))
+        if self.bound_active_blocks:   # _SetGeometry.py:255-258
+            self.minChg_delta_min = float(self.GetValue(
+                'minChg_delta_min', default=float(np.min(self.h)) / 10.,
+                help="Minimal threshold increase with bound_active_blocks method"))
 
     def _set_metric_geometry(self, metric, dualMetric):
         model = self.model_
@@ -567,6 +581,9 @@ class Interface:
         d.use_graph = int(getattr(self, 'use_graph', True))
         d.order = int(getattr(self, 'order', 1))
         d.order2_threshold = float(getattr(self, 'order2_threshold', 0.3))
+        if getattr(self, 'bound_active_blocks', 0):
+            d.bound_active_blocks = int(self.bound_active_blocks)
+            d.minchg_delta_min = float(self.minChg_delta_min)
         fac = getattr(self, 'factor', None)
         if fac is not None:
             d.factor_radius2 = fac['radius2']

@@ -25,7 +25,7 @@
 #include "eikonal_kernels.hpp"
 #include "worklist.hpp"
 
-static_assert(sizeof(agd_eikonal_desc) == 232, "agd_eikonal_desc layout changed");
+static_assert(sizeof(agd_eikonal_desc) == 248, "agd_eikonal_desc layout changed");
 static_assert(sizeof(agd_eikonal_io) == 64, "agd_eikonal_io layout changed");
 
 namespace agd {
@@ -99,6 +99,22 @@ __global__ void tile_inst_kernel(int32_t ninst, const InstDesc* inst, int32_t nt
     else hi = mid - 1;
   }
   tile_inst[t] = lo;
+}
+
+// bound_active_blocks state at the start of a solve: no pending change,
+// thresholds +inf, counters cleared (update requests `needs` are zeroed by
+// the caller: every tile of the initial list is a real update).
+__global__ void bab_init_kernel(int32_t ntiles, double* pending, int32_t* ctl) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < ntiles) pending[t] = __builtin_huge_val();
+  if (blockIdx.x == 0 && threadIdx.x < kShards) {
+    const int sh = threadIdx.x * kCountStride;
+    ctl[kBabProp + sh] = 0;
+    ctl[kBabFrozen + sh] = 0;
+    ctl[kBabMin + sh] = ord_int(__builtin_huge_valf());
+    ctl[kBabMax + sh] = ord_int(-__builtin_huge_valf());
+    if (threadIdx.x < 2) reinterpret_cast<double*>(ctl + kBabThr)[threadIdx.x] = __builtin_huge_val();
+  }
 }
 
 // Every tile, as packed coordinates (global-iteration passes).
@@ -329,12 +345,17 @@ int get_workspace(int64_t ntiles, int64_t shard_cap, int64_t nhist, Workspace*& 
     if (w.stamp) {
       AGD_HIP_TRY(hipFree(w.stamp));
       AGD_HIP_TRY(hipFree(w.score));
+      AGD_HIP_TRY(hipFree(w.needs));
+      AGD_HIP_TRY(hipFree(w.pending));
     }
     const int64_t cap = std::max<int64_t>(ntiles, 1024);
     AGD_HIP_TRY(hipMalloc((void**)&w.stamp, cap * sizeof(int32_t)));
     AGD_HIP_TRY(hipMalloc((void**)&w.score, 2 * cap * sizeof(int32_t)));
+    AGD_HIP_TRY(hipMalloc((void**)&w.needs, cap * sizeof(int32_t)));
+    AGD_HIP_TRY(hipMalloc((void**)&w.pending, cap * sizeof(double)));
     w.tile_cap = cap;
   }
+  if (!w.bab) AGD_HIP_TRY(hipMalloc((void**)&w.bab, kBabLen * sizeof(int32_t)));
   if (w.shard_cap < shard_cap) {
     if (w.lists) AGD_HIP_TRY(hipFree(w.lists));
     const int64_t cap = std::max<int64_t>(shard_cap, 256);
@@ -391,6 +412,10 @@ static int solve_impl(const agd_eikonal_desc* d, const agd_eikonal_io* io,
               "the second-order scheme has no precomputed-scheme variant (nor a TTI one)");
   const bool factor = d->factor_radius2 > 0.;
   AGD_REQUIRE(!factor || !pre, "source factorization has no precomputed-scheme variant");
+  const bool bab = d->bound_active_blocks > 0;
+  AGD_REQUIRE(!bab || d->solver == AGD_AGSI, "bound_active_blocks runs with the AGSI solver");
+  AGD_REQUIRE(!bab || d->minchg_delta_min > 0., "bound_active_blocks needs minchg_delta_min > 0");
+  AGD_REQUIRE(!bab || !pre, "bound_active_blocks runs without a precomputed scheme");
   // Fields of 4 GiB or more go to the general variant (64-bit addressing).
   const bool big = uint64_t(g.size_tot) * sizeof(S) > uint64_t(0xffffffffu);
   AGD_REQUIRE(!pre || !big, "a precomputed scheme needs fields below 4 GiB (2^30 FP32 cells)");
@@ -433,16 +458,29 @@ static int solve_impl(const agd_eikonal_desc* d, const agd_eikonal_io* io,
       hipLaunchKernelGGL(ks.init, dim3(unsigned(ceil_div(g.ntiles, 64))), dim3(64), 0, stream, trig,
                          g, d->activate_all, ws->stamp, ws->lists, ws->counts, a.w.shard_cap);
     }
+    if (bab) {
+      a.bab.ctl = ws->bab;
+      a.bab.needs = ws->needs;
+      a.bab.pending = ws->pending;
+      AGD_HIP_TRY(hipMemsetAsync(ws->needs, 0, size_t(g.ntiles) * sizeof(int32_t), stream));
+      hipLaunchKernelGGL(bab_init_kernel, dim3(unsigned(ceil_div(g.ntiles, 256))), dim3(256), 0,
+                         stream, g.ntiles, ws->pending, ws->bab);
+    }
     return AGD_OK;
   };
   auto launch = [&](int32_t koff, hipStream_t s) {
     a.w.koff = koff;
-    // walls, periodic axes and source factorization run in the general variant
+    // walls, periodic axes, source factorization and bound_active_blocks run
+    // in the general variant
     auto kern = pre ? ks.update_pre
                     : order2 ? ks.update_order2
-                             : ((io->wall_dist || g.periodic || factor || big) ? ks.update_walls
-                                                                               : ks.update);
+                             : ((io->wall_dist || g.periodic || factor || big || bab)
+                                    ? ks.update_walls
+                                    : ks.update);
     hipLaunchKernelGGL(kern, dim3(blocks), dim3(ni), 0, s, a);
+    if (bab)
+      hipLaunchKernelGGL(bab_threshold_kernel, dim3(1), dim3(64), 0, s, a.bab.ctl,
+                         int64_t(d->bound_active_blocks), d->minchg_delta_min);
   };
   WorklistResult r;
   int st = run_worklist(g.ntiles, blocks, d->nitermax_o, d->check_interval, stream, init, launch,
@@ -523,6 +561,7 @@ static int solve_batched_impl(const agd_eikonal_desc* d, int32_t ninst,
   for (int k = 0; k < d->ndim; ++k) AGD_REQUIRE(!d->periodic[k], "batched solves are not periodic");
   AGD_REQUIRE(d->order <= 1, "batched solves use the first-order scheme");
   AGD_REQUIRE(d->factor_radius2 == 0., "batched solves take no source factorization");
+  AGD_REQUIRE(d->bound_active_blocks == 0, "batched solves take no bound_active_blocks");
   const int ni = ks.ni;
   std::vector<InstDesc> h(static_cast<size_t>(ninst));
   int64_t tiles = 0;

@@ -69,6 +69,7 @@ template <class S> struct EikArgs {
   const int32_t* tile_inst;
   S order2_thr;  // VAR_ORDER2: threshold of the second-order criterion
   FactorDesc<S> fac;  // VAR_GENERAL, VAR_ORDER2: source factorization
+  BabCtx bab;         // bound_active_blocks (AGSI), ctl null: off
 };
 
 // HFM.h:91-134: where |u0 - 2v + v2| < thr * max_k(u0 - v_k)_+ and u0 > v,
@@ -198,6 +199,25 @@ template <int NI> __device__ inline bool block_any(bool p) {
   }
 }
 
+// Block-wide minimum (the same value in every thread).
+template <int NI, class S> __device__ inline S block_min(S v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const S o = __shfl_xor(v, off);
+    v = o < v ? o : v;
+  }
+  if constexpr (NI > 64) {
+    __shared__ S part[NI / 64];
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    v = part[0];
+#pragma unroll
+    for (int w = 1; w < NI / 64; ++w) v = part[w] < v ? part[w] : v;
+  }
+  return v;
+}
+
 // Resolves the 2*nsym neighbours x +- e_k of a cell into one code each:
 //   code >= 0 : tiled index of a neighbour outside the tile (gathered),
 //   code <= -2: LDS slot (-2 - code) inside the tile,
@@ -318,6 +338,10 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
   TL::pos(tid, xi);
   const int32_t tag = pv.iter + 1;
   const bool fim = a.solver == AGD_FIM;
+  // bound_active_blocks (general variants only): the threshold of this pass
+  // (set after the previous one)
+  const bool bab = GENERAL && a.bab.ctl != nullptr;
+  const S bab_thr = bab ? S(reinterpret_cast<const double*>(a.bab.ctl + kBabThr)[1]) : inf_v<S>();
 
   for (int li = blockIdx.x; li < count; li += gridDim.x) {
     const int pk = pass_entry(pv, li);
@@ -369,6 +393,18 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
       if (fim && nb >= 0) sc = read_score(pv.score_prev, nb, pv.iter - 1);
     }
 
+    // ---- bound_active_blocks: a tile listed only because it is frozen
+    // (no update was requested for this pass) skips the update and only
+    // re-examines its pending change value against the threshold.
+    bool real = true;
+    S pend = inf_v<S>();
+    if (bab) {
+      real = __builtin_amdgcn_readfirstlane(a.bab.needs[n_o]) >= pv.iter;
+      pend = S(a.bab.pending[n_o]);
+    }
+    bool any = false;
+    S minchg = inf_v<S>();
+    if (real) {
     // ---- cell loads (coalesced: a tile is contiguous)
     const S u_old = ldu(a.u, n_t);
     const S rhs = ldu(a.rhs, n_t);
@@ -523,13 +559,35 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
     const S diff = abs_(u_old - un);
     const S tol = atol + rtol * abs_(un);
     const bool changed = !(isnan_(diff) || diff <= tol);
-    const bool any = block_any<NI>(changed);
+    any = block_any<NI>(changed);
+    if (bab) minchg = block_min<NI>(changed ? un : inf_v<S>());
+    }  // real
 
     // ---- activation (Propagation.h:103-167), wave 0 only.  The
     // global-iteration solver runs with AGSI tags: they only decide when to
     // stop, every pass visits the full list.
     if (tid < 64) {
-      if (!fim) {
+      if (bab) {
+        // Propagation.h:116-126 with the change still pending from a frozen
+        // visit: propagate (self and faces, with an update request) below
+        // the threshold, else stay frozen (self only) while a change waits.
+        const S m = minchg < pend ? minchg : pend;
+        const bool prop = m < bab_thr;
+        const bool frz = !prop && m != inf_v<S>();
+        if (prop && nb >= 0) atomicMax(a.bab.needs + nb, tag);
+        const bool want = prop ? nb >= 0 : (frz && tid == 2 * D);
+        enqueue_wave(want, nb, nbp, tag, a.w.stamp, pv.list_out, pv.count_out);
+        if (tid == 0) {
+          a.bab.pending[n_o] = frz ? double(m) : __builtin_huge_val();
+          const int sh = (blockIdx.x % kShards) * kCountStride;
+          if (prop) atomicAdd(a.bab.ctl + kBabProp + sh, 1);
+          if (frz) atomicAdd(a.bab.ctl + kBabFrozen + sh, 1);
+          if (m != inf_v<S>()) {
+            atomicMin(a.bab.ctl + kBabMin + sh, ord_int(float(m)));
+            atomicMax(a.bab.ctl + kBabMax + sh, ord_int(float(m)));
+          }
+        }
+      } else if (!fim) {
         enqueue_wave(any && nb >= 0, nb, nbp, tag, a.w.stamp, pv.list_out, pv.count_out);
       } else {
         const int front = a.fim_front;

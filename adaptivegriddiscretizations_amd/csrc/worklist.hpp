@@ -31,8 +31,22 @@ constexpr int kFixedIdx = 3 * kRow;                      // fixed-list length (g
 constexpr int kBaseIdx = 3 * kRow + kCountStride;        // pass base
 constexpr int kCountsLen = 3 * kRow + 2 * kCountStride;
 
+// bound_active_blocks control block (one per device workspace): per-shard
+// counters of the pass just run (tiles that propagated, tiles frozen, and
+// the order-preserving integer images of the smallest and largest pending
+// change value), then the two thresholds as doubles.
+constexpr int kBabProp = 0;
+constexpr int kBabFrozen = kRow;
+constexpr int kBabMin = 2 * kRow;
+constexpr int kBabMax = 3 * kRow;
+constexpr int kBabThr = 4 * kRow;                         // double[2]: prev, next
+constexpr int kBabLen = 4 * kRow + 2 * kCountStride;
+
 struct Workspace {
   int32_t* stamp = nullptr;
+  int32_t* needs = nullptr;   // bound_active_blocks: last pass a real update was requested for
+  double* pending = nullptr;  // bound_active_blocks: unpropagated smallest change (+inf: none)
+  int32_t* bab = nullptr;     // bound_active_blocks control block (kBabLen)
   int32_t* lists = nullptr;   // 2 * kShards * shard_cap
   int32_t* counts = nullptr;  // kCountsLen
   int32_t* hist = nullptr;    // hist_cap: list length of every pass
@@ -169,6 +183,74 @@ __device__ inline int32_t pass_entry(const PassView& v, int li) {
 }
 
 __global__ inline void advance_kernel(int32_t* counts, int32_t g) { counts[kBaseIdx] += g; }
+
+// ---- bound_active_blocks (Propagation.h:53-99,120-126, _solvers.py:195-224)
+// Order-preserving map of a float onto int32 (atomicMin/atomicMax on floats).
+__device__ inline int32_t ord_int(float f) {
+  const int32_t i = __float_as_int(f);
+  return i >= 0 ? i : int32_t(0x80000000u - uint32_t(i)) ;
+}
+__device__ inline float ord_float(int32_t i) {
+  return __int_as_float(i >= 0 ? i : int32_t(0x80000000u - uint32_t(i)));
+}
+
+struct BabCtx {
+  int32_t* ctl;            // control block (kBabLen), null: disabled
+  int32_t* needs;          // per tile
+  double* pending;         // per tile
+};
+
+// Threshold update after pass iter (one wave; the kernel of pass iter + 1
+// reads the new value).  nConsidered = tiles the pass propagated or froze;
+// below `bound` the threshold is lifted (+inf).  Otherwise the threshold
+// grows by max(delta * clamp(bound / nActive, 0.7, 2), delta_min), where
+// delta is the previous increase or, at the start, the spread of the
+// pending change values.  Counters are cleared for the next pass.
+__global__ inline void __launch_bounds__(64) bab_threshold_kernel(int32_t* ctl, int64_t bound,
+                                                                  double delta_min) {
+  const int lane = threadIdx.x;
+  int prop = ctl[kBabProp + lane * kCountStride];
+  int frz = ctl[kBabFrozen + lane * kCountStride];
+  int mn = ctl[kBabMin + lane * kCountStride];
+  int mx = ctl[kBabMax + lane * kCountStride];
+  for (int off = 32; off > 0; off >>= 1) {
+    prop += __shfl_xor(prop, off);
+    frz += __shfl_xor(frz, off);
+    mn = min(mn, __shfl_xor(mn, off));
+    mx = max(mx, __shfl_xor(mx, off));
+  }
+  ctl[kBabProp + lane * kCountStride] = 0;
+  ctl[kBabFrozen + lane * kCountStride] = 0;
+  ctl[kBabMin + lane * kCountStride] = ord_int(__builtin_huge_valf());
+  ctl[kBabMax + lane * kCountStride] = ord_int(-__builtin_huge_valf());
+  if (lane != 0) return;
+  double* thr = reinterpret_cast<double*>(ctl + kBabThr);
+  double prev = thr[0];
+  double next = thr[1];
+  const double prev_old = prev;
+  prev = next;
+  const int64_t considered = int64_t(prop) + frz;
+  if (considered < bound) {
+    next = __builtin_huge_val();
+  } else {
+    double lo = prev_old;
+    double delta = next - lo;
+    if (!__builtin_isfinite(delta)) {
+      lo = double(ord_float(mn));
+      next = double(ord_float(mx));
+      delta = next - lo;
+      if (!__builtin_isfinite(delta)) delta = 0.;
+    }
+    const double act = double(prop > 1 ? prop : 1);
+    double mult = double(bound) / act;
+    mult = mult < 2. ? mult : 2.;
+    mult = mult > 0.7 ? mult : 0.7;
+    const double inc = delta * mult;
+    next += inc > delta_min ? inc : delta_min;
+  }
+  thr[0] = prev;
+  thr[1] = next;
+}
 
 struct WorklistResult {
   int64_t passes = 0;      // launches issued

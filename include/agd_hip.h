@@ -89,6 +89,13 @@ typedef struct agd_eikonal_desc {
   double factor_metric[6];/* M in grid units: squared costs (Isotropic, Diagonal:
                              ndim entries) or lower triangle (d00, d10, d11,
                              d20, d21, d22) for Riemann and Rander */
+  /* bound_active_blocks (_solvers.py:145-181,195-224, Propagation.h:53-99):
+   * with the AGSI solver, a tile whose smallest changed value is at or above
+   * a moving threshold is frozen (kept in the list without propagating)
+   * until the threshold passes it; the threshold is advanced after every
+   * pass to keep about bound_active_blocks tiles active.  0 disables it. */
+  int64_t bound_active_blocks;
+  double minchg_delta_min; /* smallest threshold increase (> 0)           */
 } agd_eikonal_desc;
 
 /* Device pointers, tiled layout.  Scalars are float (AGD_F32) or double. */
