@@ -188,7 +188,7 @@ __global__ inline void advance_kernel(int32_t* counts, int32_t g) { counts[kBase
 // Order-preserving map of a float onto int32 (atomicMin/atomicMax on floats).
 __device__ inline int32_t ord_int(float f) {
   const int32_t i = __float_as_int(f);
-  return i >= 0 ? i : int32_t(0x80000000u - uint32_t(i)) ;
+  return i >= 0 ? i : int32_t(0x80000000u - uint32_t(i));
 }
 __device__ inline float ord_float(int32_t i) {
   return __int_as_float(i >= 0 ? i : int32_t(0x80000000u - uint32_t(i)));
