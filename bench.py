@@ -15,8 +15,8 @@ Reported:
   roofline      the tile-update kernel's algorithmic HBM bytes
                 (36.125 B per cell-update: u read+write 8, rhs 4, metric 24,
                 seed bit 1/8) over its summed per-launch HIP-event time;
-  cpu_baseline  the oracle's single-thread FP64 fast marching on a 100^3
-                sample of the same generator (solved cells/s).
+  cpu_baseline  the oracle's single-thread FP64 fast marching on a 150^3
+                sample of the same generator (solved cells/s, ~10 s).
 
 Usage: python bench.py [--steps K] [--warmup W] [--config riemann3|riemann2|isotropic2]
 """
@@ -45,7 +45,7 @@ def log(msg):
 def make_workload(name, small=False):
     from adaptivegriddiscretizations_amd import workloads
     if name == 'riemann3':
-        return workloads.riemann3_aniso(n=100 if small else 200)
+        return workloads.riemann3_aniso(n=150 if small else 200)
     if name == 'riemann2':
         return workloads.riemann2_aniso(n=1000 if small else 2000)
     if name == 'isotropic2':
