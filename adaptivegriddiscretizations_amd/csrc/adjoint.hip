@@ -13,11 +13,14 @@
 //     in-degree count (atomics), exclusive scan (hipCUB), scatter (atomic
 //     cursors), then each row sorted by source index so the summation order,
 //     hence the FP32 result, is deterministic;
-//  2. tile Gauss-Seidel on the transposed system with the same device
+//  2. each row stably partitioned into the sources inside the row's tile,
+//     then those outside (rowloc marks the split);
+//  3. tile Gauss-Seidel on the transposed system with the same device
 //     worklist as the eikonal solver (AGSI activation, as the reference's
 //     SolveLinear switches FIM to AGSI, _PostProcess.py:117-118).  The tile's
-//     lambda values sit in LDS; sources outside the tile are read through
-//     the cache at every sweep.
+//     lambda values sit in LDS; the sources outside the tile do not change
+//     during a visit, so their part of each row is summed once per visit and
+//     only the in-tile part is re-read at every sweep.
 #include <hip/hip_runtime.h>
 
 #include <hipcub/hipcub.hpp>
@@ -74,6 +77,32 @@ __global__ void adj_sort_rows(int64_t N, const int32_t* rowptr, int32_t* src, S*
   }
 }
 
+// Stable partition of every row: sources in the row's own tile (of NI
+// cells) first, then the others; rowloc[y] = end of the in-tile part.
+template <class S>
+__global__ void lin_partition_rows(int64_t N, int ni, const int32_t* rowptr, int32_t* src, S* wt,
+                                   int32_t* rowloc) {
+  const int64_t y = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (y >= N) return;
+  const int32_t r0 = rowptr[y], r1 = rowptr[y + 1];
+  const int64_t tile = y / ni;
+  auto out = [&](int32_t s) { return int64_t(s) / ni != tile; };
+  int32_t nloc = r0;
+  for (int32_t i = r0; i < r1; ++i) {
+    const int32_t s = src[i];
+    if (out(s)) continue;
+    const S w = wt[i];
+    for (int32_t j = i; j > nloc; --j) {  // shift the outside entries right
+      src[j] = src[j - 1];
+      wt[j] = wt[j - 1];
+    }
+    src[nloc] = s;
+    wt[nloc] = w;
+    ++nloc;
+  }
+  rowloc[y] = nloc;
+}
+
 // Forward system: row x holds x's own upwind neighbours in stencil order
 // (already deterministic, no sort needed).
 template <class S>
@@ -119,6 +148,7 @@ template <class S> struct LinArgs {
   const S* rhs;
   const S* diag;
   const int32_t* rowptr;
+  const int32_t* rowloc;  // end of each row's in-tile part
   const int32_t* src;
   const S* wt;
   GridDesc g;
@@ -139,6 +169,7 @@ __global__ void __launch_bounds__(T0* T1* T2) adj_update(LinArgs<S> a) {
   static_assert((T0 * T1 * T2) % 64 == 0, "tiles are whole waves (pass_begin scans per wave)");
   using TL = Tile<D, T0, T1, T2>;
   constexpr int NI = TL::NI;
+  static_assert((NI & (NI - 1)) == 0, "in-tile slots are taken as src & (NI - 1)");
   __shared__ S l_s[NI];
   const int tid = threadIdx.x;
   const PassView pv = pass_begin(a.w);
@@ -153,16 +184,15 @@ __global__ void __launch_bounds__(T0* T1* T2) adj_update(LinArgs<S> a) {
     const S rhs = a.rhs[n_t];
     const S dg = a.diag[n_t];
     const S diag = dg == S(0) ? S(1) : dg;
-    const int32_t r0 = a.rowptr[n_t], r1 = a.rowptr[n_t + 1];
+    const int32_t r0 = a.rowptr[n_t], rl = a.rowloc[n_t], r1 = a.rowptr[n_t + 1];
+    // sources outside the tile: fixed during the visit, summed once
+    S sum_out = rhs;
+    for (int32_t j = rl; j < r1; ++j) sum_out += a.wt[j] * init_val(a.lam[a.src[j]]);
     l_s[tid] = init_val(lold);
     __syncthreads();
     for (int it = 0; it < a.niter_i; ++it) {
-      S sum = rhs;
-      for (int32_t j = r0; j < r1; ++j) {
-        const int32_t s = a.src[j];
-        const S val = (s / NI == n_o) ? l_s[s % NI] : init_val(a.lam[s]);
-        sum += a.wt[j] * val;
-      }
+      S sum = sum_out;
+      for (int32_t j = r0; j < rl; ++j) sum += a.wt[j] * l_s[a.src[j] & (NI - 1)];
       const S nv = sum / diag;
       __syncthreads();
       const bool ch = !(nv == l_s[tid]);
@@ -253,13 +283,14 @@ static int linear_impl(bool forward, const agd_eikonal_desc* d, const S* values,
   const dim3 grid(unsigned(ceil_div(N, 256))), block(256);
 
   // ---- 1. the system's rows (reverse: transposed flow graph)
-  int32_t *deg = nullptr, *rowptr = nullptr, *src = nullptr;
+  int32_t *deg = nullptr, *rowptr = nullptr, *rowloc = nullptr, *src = nullptr;
   uint32_t* trig = nullptr;
   S* wt = nullptr;
   void* tmp = nullptr;
   size_t tmp_bytes = 0;
   AGD_HIP_TRY(hipMalloc((void**)&deg, (N + 1) * sizeof(int32_t)));
   AGD_HIP_TRY(hipMalloc((void**)&rowptr, (N + 1) * sizeof(int32_t)));
+  AGD_HIP_TRY(hipMalloc((void**)&rowloc, N * sizeof(int32_t)));
   AGD_HIP_TRY(hipMalloc((void**)&trig, ceil_div(N, 32) * sizeof(uint32_t)));
   AGD_HIP_TRY(hipMemsetAsync(deg, 0, (N + 1) * sizeof(int32_t), stream));
   AGD_HIP_TRY(hipMemsetAsync(trig, 0, ceil_div(N, 32) * sizeof(uint32_t), stream));
@@ -283,6 +314,7 @@ static int linear_impl(bool forward, const agd_eikonal_desc* d, const S* values,
     hipLaunchKernelGGL(adj_fill<S>, grid, block, 0, stream, N, nsym, fw, fidx, deg, src, wt);
     hipLaunchKernelGGL(adj_sort_rows<S>, grid, block, 0, stream, N, rowptr, src, wt);
   }
+  hipLaunchKernelGGL(lin_partition_rows<S>, grid, block, 0, stream, N, ls.ni, rowptr, src, wt, rowloc);
   hipLaunchKernelGGL(adj_trigger<S>, grid, block, 0, stream, N, values, rowptr, trig);
   AGD_HIP_TRY(hipGetLastError());
 
@@ -299,6 +331,7 @@ static int linear_impl(bool forward, const agd_eikonal_desc* d, const S* values,
     a.rhs = sens + int64_t(r) * N;
     a.diag = wsum;
     a.rowptr = rowptr;
+    a.rowloc = rowloc;
     a.src = src;
     a.wt = wt;
     a.g = g;
@@ -337,6 +370,7 @@ static int linear_impl(bool forward, const agd_eikonal_desc* d, const S* values,
   (void)hipStreamSynchronize(stream);
   (void)hipFree(deg);
   (void)hipFree(rowptr);
+  (void)hipFree(rowloc);
   (void)hipFree(trig);
   (void)hipFree(src);
   (void)hipFree(wt);
