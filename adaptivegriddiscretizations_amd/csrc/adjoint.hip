@@ -19,8 +19,8 @@
 //     worklist as the eikonal solver (AGSI activation, as the reference's
 //     SolveLinear switches FIM to AGSI, _PostProcess.py:117-118).  The tile's
 //     lambda values sit in LDS; the sources outside the tile do not change
-//     during a visit, so their part of each row is summed once per visit and
-//     only the in-tile part is re-read at every sweep.
+//     during a visit, so their part of each row is summed once per visit,
+//     and the in-tile part (slots and weights) is held in registers.
 #include <hip/hip_runtime.h>
 
 #include <hipcub/hipcub.hpp>
@@ -188,11 +188,24 @@ __global__ void __launch_bounds__(T0* T1* T2) adj_update(LinArgs<S> a) {
     // sources outside the tile: fixed during the visit, summed once
     S sum_out = rhs;
     for (int32_t j = rl; j < r1; ++j) sum_out += a.wt[j] * init_val(a.lam[a.src[j]]);
+    // in-tile sources: the first K held in registers (a zero weight pads
+    // shorter rows), any further ones re-read at every sweep
+    constexpr int K = 8;
+    int slot[K];
+    S wk[K];
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+      const bool in = r0 + q < rl;
+      slot[q] = in ? (a.src[r0 + q] & (NI - 1)) : tid;
+      wk[q] = in ? a.wt[r0 + q] : S(0);
+    }
     l_s[tid] = init_val(lold);
     __syncthreads();
     for (int it = 0; it < a.niter_i; ++it) {
       S sum = sum_out;
-      for (int32_t j = r0; j < rl; ++j) sum += a.wt[j] * l_s[a.src[j] & (NI - 1)];
+#pragma unroll
+      for (int q = 0; q < K; ++q) sum += wk[q] * l_s[slot[q]];
+      for (int32_t j = r0 + K; j < rl; ++j) sum += a.wt[j] * l_s[a.src[j] & (NI - 1)];
       const S nv = sum / diag;
       __syncthreads();
       const bool ch = !(nv == l_s[tid]);
