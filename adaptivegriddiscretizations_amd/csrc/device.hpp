@@ -34,6 +34,14 @@ __device__ inline float abs_(float a) { return __builtin_fabsf(a); }
 __device__ inline double abs_(double a) { return __builtin_fabs(a); }
 __device__ inline float sqrt_(float a) { return __builtin_sqrtf(a); }
 __device__ inline double sqrt_(double a) { return __builtin_sqrt(a); }
+// Square root and quotient of the update: the hardware instructions in FP32
+// (v_sqrt_f32, v_rcp_f32: about 1 ulp, against 17 and 11 instructions for
+// the correctly rounded sequences; the update's error budget is 1e-4 of the
+// values), correctly rounded in FP64.
+__device__ inline float sqrt_fast(float a) { return __builtin_amdgcn_sqrtf(a); }
+__device__ inline double sqrt_fast(double a) { return __builtin_sqrt(a); }
+__device__ inline float div_fast(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ inline double div_fast(double a, double b) { return a / b; }
 __device__ inline bool isnan_(float a) { return a != a; }
 __device__ inline bool isnan_(double a) { return a != a; }
 
@@ -282,10 +290,11 @@ __device__ inline void selling3_out(const int (&sb)[4][3], const S* m, S& w, int
   e[2] = imul24(sb[K][0], sb[L][1]) - imul24(sb[K][1], sb[L][0]);
 }
 
-// Selling decomposition, 3D: weights w[6], offsets e[6][3].  Permutations
-// {0,1,2,3},{0,2,1,3},{0,3,1,2},{1,2,0,3},{1,3,0,2},{2,3,0,1}.  The loop runs
-// in rounds of the 6 phases until no lane of the wave is still iterating.
-template <class S> __device__ inline void selling3(const S* m, S* w, int (&e)[6][3]) {
+// Selling decomposition, 3D, reference order of operations: weights w[6],
+// offsets e[6][3].  Permutations {0,1,2,3},{0,2,1,3},{0,3,1,2},{1,2,0,3},
+// {1,3,0,2},{2,3,0,1}.  The loop runs in rounds of the 6 phases until no lane
+// of the wave is still iterating.
+template <class S> __device__ inline void selling3_int(const S* m, S* w, int (&e)[6][3]) {
   int sb[4][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {-1, -1, -1}};
   int it = 0, red = 0;
 #pragma unroll 1
@@ -304,6 +313,104 @@ template <class S> __device__ inline void selling3(const S* m, S* w, int (&e)[6]
   selling3_out<1, 2, 0, 3>(sb, m, w[3], e[3]);
   selling3_out<1, 3, 0, 2>(sb, m, w[4], e[4]);
   selling3_out<2, 3, 0, 1>(sb, m, w[5], e[5]);
+}
+
+// <x, m y> with the superbase entries held as floats.  While every entry is
+// an integer of magnitude <= kSellingExactEntry = 2^11, every product and
+// sum of two products below is an integer below 2^24, exact in FP32: the
+// coefficient matches the integer one of scal_vmv3 and the fused
+// multiply-adds that follow are the same operations in the same order.  The
+// result is bit-identical, without the six integer-to-float conversions.
+constexpr float kSellingExactEntry = 2048.f;
+template <class S> __device__ inline S scal_vmv3f(const float* x, const S* m, const float* y) {
+  S r = S(0);
+  r = fma_(S(x[0] * y[0]), m[0], r);
+  r = fma_(S(fma_(x[1], y[0], x[0] * y[1])), m[1], r);
+  r = fma_(S(x[1] * y[1]), m[2], r);
+  r = fma_(S(fma_(x[2], y[0], x[0] * y[2])), m[3], r);
+  r = fma_(S(fma_(x[2], y[1], x[1] * y[2])), m[4], r);
+  r = fma_(S(x[2] * y[2]), m[5], r);
+  return r;
+}
+
+// One phase (i,j,k,l) of the 3D reduction (Geometry3.h:27-37).  A lane is
+// still iterating while red < 6 (phases since its last flip) and the phase
+// index is below kSellingMaxIter: lanes that stopped have the same count of
+// phases as every lane still iterating, so the iteration cap is one
+// wave-uniform flag.  The flip itself runs only when some lane of the wave
+// flips (most phases of a tile flip in no lane: neighbouring cells have
+// close metrics).  pr[P] keeps <b_i, m b_j> of the current superbase: a flip
+// negates b_i and leaves b_j, and rounding to nearest is symmetric, so the
+// product after the flip is exactly the negated one.
+template <class S, int I, int J, int K, int L, int P>
+__device__ inline void selling3f_step(float (&sb)[4][3], const S* m, S (&pr)[6], int& red,
+                                      float& emax, bool below_cap) {
+  const S r = scal_vmv3f<S>(sb[I], m, sb[J]);
+  const bool flip = below_cap && red < 6 && r > S(0);
+  red = flip ? 1 : red + 1;
+  pr[P] = flip ? -r : r;
+  if (__any(flip)) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const float bi = sb[I][c];
+      sb[K][c] = flip ? sb[K][c] + bi : sb[K][c];
+      sb[L][c] = flip ? sb[L][c] + bi : sb[L][c];
+      sb[I][c] = flip ? -bi : bi;
+    }
+    // a flip only enlarges b_k and b_l
+    const float ek = __builtin_fmaxf(__builtin_fmaxf(__builtin_fabsf(sb[K][0]), __builtin_fabsf(sb[K][1])),
+                                     __builtin_fabsf(sb[K][2]));
+    const float el = __builtin_fmaxf(__builtin_fmaxf(__builtin_fabsf(sb[L][0]), __builtin_fabsf(sb[L][1])),
+                                     __builtin_fabsf(sb[L][2]));
+    emax = __builtin_fmaxf(emax, __builtin_fmaxf(ek, el));
+  }
+}
+
+template <int I, int J, int K, int L, class S>
+__device__ inline void selling3f_out(const int (&sb)[4][3], S p, S& w, int* e) {
+  const S s = -p;
+  w = s > S(0) ? s : S(0);
+  e[0] = imul24(sb[K][1], sb[L][2]) - imul24(sb[K][2], sb[L][1]);
+  e[1] = imul24(sb[K][2], sb[L][0]) - imul24(sb[K][0], sb[L][2]);
+  e[2] = imul24(sb[K][0], sb[L][1]) - imul24(sb[K][1], sb[L][0]);
+}
+
+// Selling decomposition, 3D: weights w[6], offsets e[6][3], bit-identical to
+// selling3_int.  The weights are the products of the last phase of each pair
+// (after the last flip, the six phases that end the reduction see the final
+// superbase).  A wave in which some lane stopped at the iteration cap, or
+// had a superbase entry beyond kSellingExactEntry (metrics of condition
+// number above about 7e5), redoes the reduction in integers.
+template <class S> __device__ inline void selling3(const S* m, S* w, int (&e)[6][3]) {
+  float sb[4][3] = {{1.f, 0.f, 0.f}, {0.f, 1.f, 0.f}, {0.f, 0.f, 1.f}, {-1.f, -1.f, -1.f}};
+  S pr[6];
+  int red = 0;
+  float emax = 1.f;
+#pragma unroll 1
+  for (int step = 0; step < kSellingMaxIter; step += 6) {
+    selling3f_step<S, 0, 1, 2, 3, 0>(sb, m, pr, red, emax, step + 0 < kSellingMaxIter);
+    selling3f_step<S, 0, 2, 1, 3, 1>(sb, m, pr, red, emax, step + 1 < kSellingMaxIter);
+    selling3f_step<S, 0, 3, 1, 2, 2>(sb, m, pr, red, emax, step + 2 < kSellingMaxIter);
+    selling3f_step<S, 1, 2, 0, 3, 3>(sb, m, pr, red, emax, step + 3 < kSellingMaxIter);
+    selling3f_step<S, 1, 3, 0, 2, 4>(sb, m, pr, red, emax, step + 4 < kSellingMaxIter);
+    selling3f_step<S, 2, 3, 0, 1, 5>(sb, m, pr, red, emax, step + 5 < kSellingMaxIter);
+    if (!__any(red < 6)) break;
+  }
+  if (__any(red < 6 || emax > kSellingExactEntry)) {
+    selling3_int<S>(m, w, e);
+    return;
+  }
+  int ib[4][3];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) ib[i][c] = int(sb[i][c]);
+  selling3f_out<0, 1, 2, 3>(ib, pr[0], w[0], e[0]);
+  selling3f_out<0, 2, 1, 3>(ib, pr[1], w[1], e[1]);
+  selling3f_out<0, 3, 1, 2>(ib, pr[2], w[2], e[2]);
+  selling3f_out<1, 2, 0, 3>(ib, pr[3], w[3], e[3]);
+  selling3f_out<1, 3, 0, 2>(ib, pr[4], w[4], e[4]);
+  selling3f_out<2, 3, 0, 1>(ib, pr[5], w[5], e[5]);
 }
 
 // Per-cell stencil: NSYM symmetric offsets +-e_k with weights w_k, plus the
@@ -488,8 +595,8 @@ __device__ inline S hfm_solve(S (&v)[N], S (&w)[N], S rhs) {
     }
   }
   const S delta = b * b - a * c;
-  const S sdelta = sqrt_(delta > S(0) ? delta : S(0));
-  const S value = (a == S(0)) ? inf_v<S>() : (b + sdelta) / a;
+  const S sdelta = sqrt_fast(delta > S(0) ? delta : S(0));
+  const S value = (a == S(0)) ? inf_v<S>() : div_fast(b + sdelta, a);
   return (vmin == inf_v<S>()) ? vmin : vmin + value;
 }
 

@@ -31,8 +31,11 @@ def test_selling_stencils_bit_exact(O, d, dtype):
     import torch
     L = _lib()
     rng = np.random.default_rng(100 + d)
-    n = 20000
-    m = np.concatenate([random_spd_flat(rng, d, n // 2, 10.), random_spd_flat(rng, d, n // 2, 1e4)])
+    # moderate, strong and extreme anisotropy: the last group reaches the
+    # reduction's iteration cap and superbase entries beyond 2^11 (the
+    # integer path of the 3D reduction)
+    n = 21000
+    m = np.concatenate([random_spd_flat(rng, d, n // 3, c) for c in (10., 1e4, 1e7)])
     m = m.astype(dtype)
     geom = torch.from_numpy(np.ascontiguousarray(m.T)).cuda()  # component-major
     sd = d * (d + 1) // 2
