@@ -238,8 +238,9 @@ __device__ inline void selling2_step(int (&sb)[3][2], const S* m, int& red) {
   }
 }
 
-// Selling decomposition, 2D: weights w[3], offsets e[3][2].
-template <class S> __device__ inline void selling2(const S* m, S* w, int (&e)[3][3]) {
+// Selling decomposition, 2D, reference order of operations: weights w[3],
+// offsets e[3][2].
+template <class S> __device__ inline void selling2_int(const S* m, S* w, int (&e)[3][3]) {
   int sb[3][2] = {{1, 0}, {0, 1}, {-1, -1}};
   int it = 0, red = 0;
   bool done = false;
@@ -411,6 +412,68 @@ template <class S> __device__ inline void selling3(const S* m, S* w, int (&e)[6]
   selling3f_out<1, 2, 0, 3>(ib, pr[3], w[3], e[3]);
   selling3f_out<1, 3, 0, 2>(ib, pr[4], w[4], e[4]);
   selling3f_out<2, 3, 0, 1>(ib, pr[5], w[5], e[5]);
+}
+
+// 2D counterparts of the above: <x, m y> on float-held integer entries
+// (bit-identical to scal_vmv2 while entries stay within kSellingExactEntry)
+// and one cyclic phase (i,j,k) of Geometry2.h:25-33.  Pair (i, i+1 mod 3)
+// is the pair of weight i; a flip negates b_i and leaves b_j, so pr[I]
+// becomes exactly the negated product.
+template <class S> __device__ inline S scal_vmv2f(const float* x, const S* m, const float* y) {
+  S r = S(0);
+  r = fma_(S(x[0] * y[0]), m[0], r);
+  r = fma_(S(fma_(x[1], y[0], x[0] * y[1])), m[1], r);
+  r = fma_(S(x[1] * y[1]), m[2], r);
+  return r;
+}
+
+template <class S, int I, int J, int K>
+__device__ inline void selling2f_step(float (&sb)[3][2], const S* m, S (&pr)[3], int& red,
+                                      float& emax, bool below_cap) {
+  const S r = scal_vmv2f<S>(sb[I], m, sb[J]);
+  const bool flip = below_cap && red < 3 && r > S(0);
+  red = flip ? 1 : red + 1;
+  pr[I] = flip ? -r : r;
+  if (__any(flip)) {
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const float bi = sb[I][c];
+      sb[K][c] = flip ? bi - sb[J][c] : sb[K][c];
+      sb[I][c] = flip ? -bi : bi;
+    }
+    emax = __builtin_fmaxf(emax, __builtin_fmaxf(__builtin_fabsf(sb[K][0]), __builtin_fabsf(sb[K][1])));
+  }
+}
+
+// Selling decomposition, 2D: weights w[3], offsets e[3][2], bit-identical to
+// selling2_int (same scheme as selling3: weights from the last phase of each
+// pair, integer path for a wave with a lane at the iteration cap or beyond
+// kSellingExactEntry).
+template <class S> __device__ inline void selling2(const S* m, S* w, int (&e)[3][3]) {
+  float sb[3][2] = {{1.f, 0.f}, {0.f, 1.f}, {-1.f, -1.f}};
+  S pr[3];
+  int red = 0;
+  float emax = 1.f;
+#pragma unroll 1
+  for (int step = 0; step < kSellingMaxIter; step += 3) {
+    selling2f_step<S, 0, 1, 2>(sb, m, pr, red, emax, step + 0 < kSellingMaxIter);
+    selling2f_step<S, 1, 2, 0>(sb, m, pr, red, emax, step + 1 < kSellingMaxIter);
+    selling2f_step<S, 2, 0, 1>(sb, m, pr, red, emax, step + 2 < kSellingMaxIter);
+    if (!__any(red < 3)) break;
+  }
+  if (__any(red < 3 || emax > kSellingExactEntry)) {
+    selling2_int<S>(m, w, e);
+    return;
+  }
+  // Geometry2.h:40-48: w_r = max(0, -<b_r, D b_{r+1}>), e_r = perp(b_{r+2}).
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const S sr = -pr[r];
+    w[r] = sr > S(0) ? sr : S(0);
+  }
+  e[0][0] = -int(sb[2][1]); e[0][1] = int(sb[2][0]); e[0][2] = 0;
+  e[1][0] = -int(sb[0][1]); e[1][1] = int(sb[0][0]); e[1][2] = 0;
+  e[2][0] = -int(sb[1][1]); e[2][1] = int(sb[1][0]); e[2][2] = 0;
 }
 
 // Per-cell stencil: NSYM symmetric offsets +-e_k with weights w_k, plus the
