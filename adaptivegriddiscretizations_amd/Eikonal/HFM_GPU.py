@@ -119,6 +119,7 @@ class Interface:
         self.Solve()
         self.PostProcess()
         self.SolveAD()
+        self.GetGeodesics()
         self.FinalCheck()
         if self.extractValues or self.retself:
             retval = [self.hfmOut]
@@ -257,8 +258,12 @@ class Interface:
             'exportGeodesicFlow', default=False,
             help="Export the upwind geodesic flow (direction of the geodesics)")
         tips = self.GetValue('tips', default=None, help="Tips from which to compute the minimal geodesics")
-        if tips is not None:
-            raise NotImplementedError("geodesic backtracking ('tips') is not provided yet")
+        self.tips = None if tips is None else np.asarray(host_array(tips), dtype=self.float_t).reshape(-1, self.ndim)
+        if self.GetValue('tips_Unoriented', default=None, verbosity=3,
+                         help="Unoriented tips (curvature penalized models)") is not None:
+            raise NotImplementedError("'tips_Unoriented' belongs to the curvature penalized models, "
+                                      "which this solver does not provide")
+        self.hasTips = self.tips is not None
 
         # Cost function (_SetGeometry.py:221-233): a denseAD cost, or a
         # separate costVariation, requests forward AD.
@@ -661,7 +666,7 @@ class Interface:
     def PostProcess(self):
         """_PostProcess.py:46-95: one global pass exporting the upwind flow."""
         self.flow_needed = (self.exportGeodesicFlow or self.reverseAD or self.forwardAD
-                            or getattr(self, 'keep_flow', False))
+                            or self.hasTips or getattr(self, 'keep_flow', False))
         if not self.flow_needed:
             return
         if self.verbosity >= 1:
@@ -688,6 +693,99 @@ class Interface:
             fv = self._squeeze(self.flow_vector_t, self.ndim, self.float_t)
             h = torch.as_tensor(self.h, dtype=fv.dtype, device=fv.device).reshape((self.ndim,) + (1,) * self.ndim)
             self.hfmOut['flow'] = self._output(fv * (-h))
+
+    # ------------------------------------------------------------ geodesics
+    _GEODESIC_STOP = ('Continue', 'AtSeed', 'InWall', 'Stationnary', 'PastSeed', 'VanishingFlow')
+
+    def GetGeodesics(self):
+        """_GetGeodesics.py:9-147: backtrack the minimal geodesics from the
+        tips along the upwind flow (agd_geodesic_ode), in chunks of
+        geodesic_typical_length points per launch until every geodesic has
+        stopped or geodesic_max_length points are reached."""
+        if not self.hasTips:
+            return
+        if self.verbosity >= 1:
+            print("Computing the geodesics")
+        tips = self.hfmIn.PointFromIndex(self.tips, to=True).astype(self.float_t)
+        self.GetValue('geodesic_online_flow', default=False,
+                      help="Compute the flow online when extracting geodesics (here the flow of the "
+                           "post-processing pass is always used)")
+        step = float(self.GetValue('geodesic_step', default=0.25,
+                                   help='Step size, in pixels, for the geodesic ODE solver'))
+        eucl_delay = int(self.GetValue(
+            'geodesic_PastSeed_delay', default=np.sqrt(self.ndim) / step,
+            help="Delay, in iterations, for the 'PastSeed' stopping criterion of the geodesic ODE solver"))
+        nymin_delay = int(self.GetValue(
+            'geodesic_Stationnary_delay', default=8. * np.sqrt(self.ndim) / step,
+            help="Delay, in iterations, for the 'Stationnary' stopping criterion of the geodesic ODE solver"))
+        shape_max = int(np.max(self.shape))
+        max_len = int(max(40, 20 * shape_max / step))
+        max_len = self.GetValue('geodesic_max_length', default=max_len,
+                                help="Maximum allowed length of geodesics.")
+        typical_len = int(max(40, 0.5 * shape_max / step))
+        typical_len = int(self.GetValue('geodesic_typical_length', default=min(max_len, typical_len),
+                                        help="Typical expected length of geodesics (number of points)."))
+        causality = float(self.GetValue('geodesic_causalityTolerance', default=4.,
+                                        help="Used in criterion for rejecting points in flow interpolation"))
+        eucl_bound = self.GetValue('geodesic_targetTolerance', default=6,
+                                   help="Tolerance, in pixels, for declaring a seed as reached.")
+        # Distance to the seeds, in fifths of a pixel, up to eucl_bound pixels
+        # (the trigger bits: seeds without the walls).
+        desc = self._desc()
+        eucl = empty((self.size_tot,), np.uint8)
+        tmp = empty((self.size_tot,), np.uint8)
+        mult = 5
+        _lib.check(_lib.lib.agd_geodesic_eucl(desc, ptr(self.trigger_t), int(np.ceil(eucl_bound)), mult,
+                                              int(mult * eucl_bound), ptr(eucl), ptr(tmp), self.stream))
+        del tmp
+        pastseed_delay = int(self.GetValue(
+            'pastseed_delay', default=0,
+            help="Number of geodesic backtracking steps before checking the PastSeed criterion "
+                 "(use if seed and tip are close)"))
+        self.GetValue('geodesic_block_size', default=64,
+                      help="Block size for the GPU based geodesic solver (one wave64 here)")
+
+        ngeo = len(tips)
+        stopping = ["Stopping criterion"] * ngeo
+        geodesics = [[tip.reshape(1, -1)] for tip in tips]
+        corresp = list(range(ngeo))
+        geo_max_it = int(np.ceil(max_len / typical_len)) if np.isfinite(max_len) else None
+        it = 0
+        dev = self.values_t.device
+        t0 = time.time()
+        while corresp:
+            if geo_max_it is not None and it >= geo_max_it:
+                self.Warn("Geodesic solver failed to converge, or geodesic has too many points"
+                          " (in latter case, try setting 'geodesic_max_length':np.inf)")
+                break
+            it += 1
+            n = len(corresp)
+            starts = np.stack([geodesics[i][-1][-1, :] for i in corresp]).astype(self.float_t)
+            x_s = torch.full((n, typical_len, self.ndim), float('nan'), dtype=self.values_t.dtype, device=dev)
+            x_s[:, 0, :] = torch.as_tensor(starts, device=dev)
+            len_s = torch.full((n,), -1, dtype=torch.int32, device=dev)
+            stop_s = torch.full((n,), -1, dtype=torch.int8, device=dev)
+            gd = _lib.GeodesicDesc()
+            gd.ngeo, gd.max_len = n, typical_len
+            gd.eucl_delay, gd.nymin_delay, gd.pastseed_delay = eucl_delay, nymin_delay, pastseed_delay
+            gd.step, gd.causality_tolerance = step, causality
+            _lib.check(_lib.lib.agd_geodesic_ode(desc, gd, ptr(self.values_t), ptr(self.flow_vector_t),
+                                                 ptr(self.flow_weightsum_t), ptr(eucl), ptr(x_s),
+                                                 ptr(len_s), ptr(stop_s), self.stream))
+            x_h, len_h, stop_h = x_s.cpu().numpy(), len_s.cpu().numpy(), stop_s.cpu().numpy()
+            nxt = []
+            for i, x, ln, sp in zip(corresp, x_h, len_h, stop_h):
+                geodesics[i].append(x[1:int(ln)])
+                if sp != 0:
+                    stopping[i] = self._GEODESIC_STOP[int(sp)]
+                else:
+                    nxt.append(i)
+            corresp = nxt
+            pastseed_delay = 0
+        self.stats['geodesic'] = {'time': time.time() - t0, 'launches': it}
+        geos = [np.concatenate(g, axis=0) for g in geodesics]
+        self.hfmOut['geodesics'] = [self.hfmIn.PointFromIndex(g).T for g in geos]
+        self.hfmOut['geodesic_stopping_criteria'] = stopping
 
     # ------------------------------------------------------------ AD
     def SolveAD(self):

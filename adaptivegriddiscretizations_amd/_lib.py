@@ -63,6 +63,14 @@ class EikonalStats(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class GeodesicDesc(ctypes.Structure):
+    _fields_ = [
+        ("ngeo", c_int32), ("max_len", c_int32), ("eucl_delay", c_int32), ("nymin_delay", c_int32),
+        ("pastseed_delay", c_int32), ("reserved", c_int32), ("step", c_double),
+        ("causality_tolerance", c_double),
+    ]
+
+
 class FlowIO(ctypes.Structure):
     _fields_ = [
         ("flow_weights", c_void_p), ("flow_weightsum", c_void_p),
@@ -92,6 +100,10 @@ SIGNATURES = {
     "agd_eikonal_forward_ad": (c_int32, [POINTER(EikonalDesc), c_void_p, c_void_p, c_void_p,
                                          c_void_p, c_int32, c_void_p, c_void_p, c_double,
                                          c_double, POINTER(EikonalStats), c_void_p]),
+    "agd_geodesic_eucl": (c_int32, [POINTER(EikonalDesc), c_void_p, c_int32, c_int32, c_int32,
+                                    c_void_p, c_void_p, c_void_p]),
+    "agd_geodesic_ode": (c_int32, [POINTER(EikonalDesc), POINTER(GeodesicDesc), c_void_p, c_void_p,
+                                   c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "agd_block_expand": (c_int32, [c_int32, c_int64_p, c_int32_p, c_int32, c_int32, c_void_p,
                                    c_int64, c_void_p, c_double, c_void_p]),
     "agd_block_squeeze": (c_int32, [c_int32, c_int64_p, c_int32_p, c_int32, c_int32, c_void_p,

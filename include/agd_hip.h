@@ -218,6 +218,42 @@ int agd_eikonal_forward_ad(const agd_eikonal_desc* desc, const void* values,
                            void* dvalues, double atol, double rtol,
                            agd_eikonal_stats* stats, void* stream);
 
+/* ------------------------------------------------------------ geodesics */
+/* Geodesic backtracking from tips (Kernel_GeodesicODE.h:265-413 with the
+ * imported flow of GeodesicODE_Opt.h:222-238; driven by
+ * _GetGeodesics.py:86-131, which launches the reference's GeodesicODE
+ * kernel).  One geodesic per lane, midpoint steps along the normalised,
+ * causally filtered interpolation of the upwind flow vector. */
+typedef struct agd_geodesic_desc {
+  int32_t ngeo;            /* number of geodesics                          */
+  int32_t max_len;         /* points per geodesic row of x (>= 1)          */
+  int32_t eucl_delay;      /* PastSeed history delay, in steps             */
+  int32_t nymin_delay;     /* Stationnary history delay, in steps          */
+  int32_t pastseed_delay;  /* steps before the PastSeed test is enabled    */
+  int32_t reserved;        /* 0                                            */
+  double step;             /* ODE step, in grid units (pixels)             */
+  double causality_tolerance; /* corner exclusion: value >= min + tol / flow weight sum */
+} agd_geodesic_desc;
+
+/* Capped distance to the seeds read by the PastSeed criterion
+ * (_GetGeodesics.py:50-67, inf_convolution.py:24-93): eucl = 0 on the seed
+ * cells (bit array `seeds`, tiled), then `niter` min-plus passes over the 3^d
+ * neighbourhood with the integer kernel round(mult |offset|) saturating at
+ * 255, and values above `bound` set to 255.  eucl and tmp: tiled uint8
+ * arrays of the solver's padded size (padding = 255). */
+int agd_geodesic_eucl(const agd_eikonal_desc* desc, const uint32_t* seeds, int32_t niter,
+                      int32_t mult, int32_t bound, uint8_t* eucl, uint8_t* tmp, void* stream);
+/* Backtracks gd->ngeo geodesics.  values, flow_vector (ndim components,
+ * grid units, towards the upwind neighbours), flow_weightsum: tiled arrays of
+ * desc->scalar from agd_eikonal_flow; eucl from agd_geodesic_eucl.
+ * x: ngeo x max_len x ndim scalars, row start = starting point (continuous
+ * grid index); points 1 .. len-1 of each row are written.  stop: 0 continue
+ * (max_len reached), 1 AtSeed, 2 InWall, 3 Stationnary, 4 PastSeed,
+ * 5 VanishingFlow.  1 + max(eucl_delay, nymin_delay) must be <= 256. */
+int agd_geodesic_ode(const agd_eikonal_desc* desc, const agd_geodesic_desc* gd, const void* values,
+                     const void* flow_vector, const void* flow_weightsum, const uint8_t* eucl,
+                     void* x, int32_t* len, int8_t* stop, void* stream);
+
 /* ------------------------------------------------------------ layout */
 /* block_expand (FiniteDifferences.py:164-195): src cell-major
  * [ncomp][shape], dst tiled [ncomp][size_tot], padded with pad.

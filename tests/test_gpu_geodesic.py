@@ -1,0 +1,142 @@
+"""Geodesic backtracking on the GPU ('tips' -> 'geodesics',
+'geodesic_stopping_criteria'; _GetGeodesics.py, Kernel_GeodesicODE.h).
+
+* The reference's own assertion on its isotropic flow notebook
+  (Notebooks_Repro/Flow_GPU.ipynb, cells 10-12): 16 uniform tips on a 202^2
+  grid all stop AtSeed with at most 450 points.
+* The same on its walls notebook (Walls_GPU.ipynb, cells 11-16): every
+  geodesic goes around the wall to the seed.
+* The device distance to the seeds equals the oracle's (uint8, exact).
+* Backtracked geodesics equal the oracle's restatement run on the same
+  values and flow (FP32 device vs FP64 oracle: same stopping criteria,
+  lengths within 2 points, points within 2e-2 pixel over the common part).
+* Constant metrics have straight geodesics, up to the first-order scheme's
+  flow (3D isotropic: within a pixel of the segment; FP32 and FP64 paths).
+"""
+import numpy as np
+import pytest
+
+from adaptivegriddiscretizations_amd import Eikonal, Metrics, _lib
+from gpu_helpers import log
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_inputs(hfmIn, iface):
+    """Cell-major values, flow vector and flow weight sum of the solve, and
+    the seed mask."""
+    u = iface._squeeze(iface.values_t, 1, iface.float_t)[0].cpu().numpy().astype(np.float64)
+    fv = iface._squeeze(iface.flow_vector_t, iface.ndim, iface.float_t).cpu().numpy().astype(np.float64)
+    ws = iface._squeeze(iface.flow_weightsum_t, 1, iface.float_t)[0].cpu().numpy().astype(np.float64)
+    idx, _ = hfmIn.IndexFromPoint(np.asarray(hfmIn['seeds']).reshape(-1, iface.ndim))
+    seeds = np.zeros(u.shape, bool)
+    seeds[tuple(idx.T)] = True
+    return u, fv, ws, seeds
+
+
+def test_flow_notebook_isotropic_all_at_seed():
+    n = 201
+    hfmIn = Eikonal.dictIn({'model': 'Isotropic2', 'seed': [0., 0.], 'exportValues': True, 'cost': 1.,
+                            'exportGeodesicFlow': True, 'verbosity': 0})
+    hfmIn.SetRect([[-1, 1], [-1, 1]], dimx=n + 1, sampleBoundary=True)
+    hfmIn.SetUniformTips((4, 4))
+    out = hfmIn.Run()
+    stops = out['geodesic_stopping_criteria']
+    lens = [geo.shape[1] for geo in out['geodesics']]
+    log(f"isotropic2 202^2, 16 tips: stops {set(stops)}, lengths {min(lens)}..{max(lens)}")
+    assert all(s == 'AtSeed' for s in stops)
+    assert max(lens) <= 450
+    for geo in out['geodesics']:
+        assert geo.shape[0] == 2
+        assert np.linalg.norm(geo[:, -1]) < 3 * 2. / n   # ends next to the seed at the origin
+
+
+def test_walls_notebook_all_at_seed():
+    """Notebooks_Repro/Walls_GPU.ipynb cells 11-16: a wall segment on the
+    half-domain [-1,1] x [0,1]; every geodesic from the 4 x 4 tips goes
+    around it to the seed."""
+    n = 201
+    hfmIn = Eikonal.dictIn({'model': 'Isotropic2', 'seed': [0, 0], 'exportValues': 1, 'cost': 1,
+                            'verbosity': 0})
+    hfmIn.SetRect([[-1, 1], [0, 1]], dimx=n + 1, sampleBoundary=True)
+    hfmIn['tips'] = hfmIn.Grid(dims=(4, 4)).reshape(2, -1).T
+    X = hfmIn.Grid()
+    hfmIn['walls'] = np.logical_and(X[0] == X[0, int(2 * n / 3)], X[1] <= 2. / 3.)
+    out = hfmIn.Run()
+    stops = out['geodesic_stopping_criteria']
+    log(f"isotropic2 with a wall: stops {stops}")
+    assert all(s == 'AtSeed' for s in stops)
+
+
+@pytest.mark.parametrize("case", ["riemann2", "isotropic2_cost"])
+def test_geodesics_match_oracle(case):
+    import geodesic_oracle as G
+    if case == "riemann2":
+        n = 199                                       # Flow_GPU.ipynb cell 17
+        hfmIn = Eikonal.dictIn({'model': 'Riemann2', 'seed': [0., 0.], 'exportValues': 1,
+                                'metric': Metrics.Riemann(np.array([[1., 1.], [1., 2.]], np.float32)),
+                                'verbosity': 0})
+        hfmIn.SetRect([[-1, 1], [-1, 1]], dimx=n + 1, sampleBoundary=True)
+        hfmIn.SetUniformTips((4, 4))
+    else:
+        n = 200                                       # Flow_GPU.ipynb cell 33 (variable cost)
+        hfmIn = Eikonal.dictIn({'model': 'Isotropic2', 'exportValues': 1, 'seed': [0., 0.],
+                                'verbosity': 0})
+        hfmIn.SetRect([[-1, 1], [-1, 1]], dimx=n + 1, sampleBoundary=True)
+        X = hfmIn.Grid()
+        hfmIn.update({'cost': (np.prod(np.sin(2 * np.pi * X), axis=0) + 1.1).astype(np.float32),
+                      'tips': hfmIn.Grid(dims=(5, 4)).reshape(2, -1).T})
+    hfmIn['retself'] = True
+    out, iface = hfmIn.Run()
+    u, fv, ws, seeds = oracle_inputs(hfmIn, iface)
+    tips = hfmIn.PointFromIndex(np.asarray(hfmIn['tips']).reshape(-1, 2), to=True)
+    geos_o, stops_o = G.geodesics(u, fv, ws, seeds, tips)
+    # the device distance to the seeds
+    e = torch_uint8(iface.size_tot)
+    tmp = torch_uint8(iface.size_tot)
+    _lib.check(_lib.lib.agd_geodesic_eucl(iface._desc(), iface.trigger_t.data_ptr(), 6, 5, 30,
+                                          e.data_ptr(), tmp.data_ptr(), iface.stream))
+    e_cm = iface._squeeze(e, 1, np.uint8)[0].cpu().numpy()
+    assert np.array_equal(e_cm, G.eucl_distance(seeds, 6, 5, 30))
+    worst, dlen = 0., 0
+    for geo, go, so, sg in zip(out['geodesics'], geos_o, stops_o, out['geodesic_stopping_criteria']):
+        gi = hfmIn.PointFromIndex(geo.T, to=True)
+        assert sg == so
+        m = min(len(gi), len(go))
+        dlen = max(dlen, abs(len(gi) - len(go)))
+        worst = max(worst, float(np.abs(gi[:m] - go[:m]).max()))
+    log(f"{case}: {len(geos_o)} geodesics, stops {set(stops_o)}, max point difference {worst:.2e} px, "
+        f"max length difference {dlen}")
+    assert dlen <= 2
+    assert worst <= 2e-2
+
+
+def torch_uint8(n):
+    import torch
+    return torch.empty((n,), dtype=torch.uint8, device='cuda')
+
+
+@pytest.mark.parametrize("multiprecision", [False, True])
+def test_isotropic3_straight_geodesics(multiprecision):
+    n = 41
+    hfmIn = Eikonal.dictIn({'model': 'Isotropic3', 'seed': [0., 0., 0.], 'exportValues': 1,
+                            'multiprecision': multiprecision, 'verbosity': 0})
+    hfmIn.SetRect([[-1, 1]] * 3, dimx=n, sampleBoundary=True)
+    rng = np.random.default_rng(11)
+    tips = rng.uniform(-0.9, 0.9, (24, 3))
+    hfmIn['tips'] = tips
+    out = hfmIn.Run()
+    h = 2. / (n - 1)
+    worst = 0.
+    for tip, geo, stop in zip(tips, out['geodesics'], out['geodesic_stopping_criteria']):
+        assert stop == 'AtSeed'
+        seg = -tip
+        rel = geo.T - tip
+        t = rel @ seg / (seg @ seg)
+        perp = np.linalg.norm(rel - t[:, None] * seg, axis=1) / h
+        worst = max(worst, float(perp.max()))
+        assert np.linalg.norm(geo[:, -1]) < 2 * h
+    log(f"isotropic3 41^3 multiprecision={multiprecision}: max distance to the segment {worst:.3f} px")
+    # within a pixel of the segment: the first-order scheme's upwind flow is
+    # not exactly radial (0.76 pixel measured on this grid)
+    assert worst < 1.0
