@@ -687,12 +687,40 @@ class Interface:
         torch.cuda.synchronize()
         self.stats['flow'] = {'niter_o': 1, 'time': time.time() - t0}
         self._values = None
-        if self.exportGeodesicFlow:
-            if self.model_ == 'Rander':
-                raise NotImplementedError("flow normalisation of the Rander model is not provided yet")
+        # _PostProcess.py:80-91: with a drift, the flow is divided by its
+        # metric norm (the flow on the adimensionized grid), which the AD
+        # right-hand sides and sensitivities also carry.
+        self.flow_normalization = None
+        if self.exportGeodesicFlow or self.model_ == 'Rander':
             fv = self._squeeze(self.flow_vector_t, self.ndim, self.float_t)
-            h = torch.as_tensor(self.h, dtype=fv.dtype, device=fv.device).reshape((self.ndim,) + (1,) * self.ndim)
-            self.hfmOut['flow'] = self._output(fv * (-h))
+            if self.model_ == 'Rander':
+                self.flow_normalization = self._rander_flow_normalization(fv)
+                fv = fv / self.flow_normalization
+            if self.exportGeodesicFlow:
+                h = torch.as_tensor(self.h, dtype=fv.dtype, device=fv.device).reshape((self.ndim,) + (1,) * self.ndim)
+                self.hfmOut['flow'] = self._output(fv * (-h))
+
+    def _seed_mask(self):
+        """Cell-major bool mask of the seed bits (seeds and walls)."""
+        words = self.seeds_t.to(torch.int64) & 0xffffffff
+        bits = ((words.reshape(-1, 1) >> torch.arange(32, device=words.device)) & 1).to(torch.uint8)
+        bits = bits.reshape(-1)[:self.size_tot].contiguous()
+        return self._squeeze(bits, 1, np.uint8)[0].bool()
+
+    def _rander_flow_normalization(self, fv):
+        """metric.norm(-flow) with the metric in grid units (with_costs(h):
+        N_h(v) = sqrt(<h v, m h v>) + <w, h v>), 1 on the seeds and walls
+        (_PostProcess.py:86-87, Metrics/rander.py:30-32)."""
+        arr_d, w_d = self._keep
+        d, n = self.ndim, int(np.prod(self.shape))
+        m = arr_d.to(fv.dtype).reshape(d, d, -1)
+        w = w_d.to(fv.dtype).reshape(d, -1)
+        h = torch.as_tensor(self.h, dtype=fv.dtype, device=fv.device)
+        v = -fv.reshape(d, n) * h[:, None]
+        mv = (m * v[None, :, :]).sum(1)
+        norm = torch.sqrt((v * mv).sum(0)) + (w * v).sum(0)
+        norm = torch.where(self._seed_mask().reshape(-1), torch.ones_like(norm), norm)
+        return norm.reshape(tuple(self.shape))
 
     # ------------------------------------------------------------ geodesics
     _GEODESIC_STOP = ('Continue', 'AtSeed', 'InWall', 'Stationnary', 'PastSeed', 'VanishingFlow')
@@ -792,8 +820,6 @@ class Interface:
         """_PostProcess.py:155-219: forward then reverse differentiation."""
         if not (self.forwardAD or self.reverseAD):
             return
-        if self.model_ == 'Rander':
-            raise NotImplementedError("automatic differentiation of the Rander model is not provided yet")
         if self.order == 2 or self.factoringRadius:
             # _PostProcess.py:165-166: the linearised scheme is the flow's.
             self.Warn("Eikonal AD ignores order2 and source factorization")
@@ -878,6 +904,8 @@ class Interface:
             rhs[self._seed_pos(rhs.device)] = sc[rows] if sc is not None else 0.
         if self.walls is not None:
             rhs[device_tensor(self.walls, np.bool_)] = 0.
+        if getattr(self, "flow_normalization", None) is not None:      # _PostProcess.py:185
+            rhs = rhs * self.flow_normalization.reshape(self.flow_normalization.shape + (1,))
         rhs_c = torch.movedim(rhs, -1, 0).contiguous()
         return self._linear_solve(_lib.lib.agd_eikonal_forward_ad, 'forwardAD', rhs_c)
 
@@ -895,6 +923,8 @@ class Interface:
         pos = self._seed_pos(lam.device)
         cellSens = lam[pos].clone()
         lam[pos] = 0
+        if getattr(self, "flow_normalization", None) is not None:      # _PostProcess.py:213-214
+            lam = lam / self.flow_normalization.reshape(self.flow_normalization.shape + (1,))
         nseeds = int(self.seed_rows.max()) + 1 if len(self.seed_rows) else 0
         seedSens = cellSens.new_zeros((nseeds, int(cellSens.shape[-1])))
         seedSens.index_add_(0, torch.as_tensor(self.seed_rows, device=lam.device), cellSens)

@@ -26,6 +26,11 @@ class EikonalSolve(torch.autograd.Function):
             if key in hfm:
                 del hfm[key]
         hfm['cost'] = cost.detach()
+        if str(hfm['model']).startswith('Rander'):
+            # the reference's drift-model sensitivities divide by the flow's
+            # metric norm (_PostProcess.py:213-214), which is not the
+            # gradient of the solve: no gradient is offered for them here
+            raise NotImplementedError("EikonalSolve does not differentiate the Rander model")
         iface = HFM_GPU.Interface(hfm)
         iface.keep_flow = True
         iface.SetKernelTraits()
