@@ -559,7 +559,7 @@ class Interface:
         self.raiseOnNonConvergence = self.GetValue('raiseOnNonConvergence', default=True,
                                                    help="Raise an exception if a solver fails to converge")
         self.check_interval = int(self.GetValue(
-            'check_interval', default=8,
+            'check_interval', default=18,
             help="Grid passes between two convergence checks on the host"))
         self.use_graph = bool(self.GetValue(
             'hipGraph', default=True,
@@ -581,7 +581,7 @@ class Interface:
         d.atol = self.atol
         d.rtol = self.rtol
         d.nitermax_o = getattr(self, 'nitermax_o', 2000)
-        d.check_interval = getattr(self, 'check_interval', 4)
+        d.check_interval = getattr(self, 'check_interval', 18)
         d.activate_all = int(getattr(self, 'warmStartAllTiles', False))
         d.use_graph = int(getattr(self, 'use_graph', True))
         d.order = int(getattr(self, 'order', 1))

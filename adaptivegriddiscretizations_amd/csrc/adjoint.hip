@@ -366,8 +366,9 @@ static int linear_impl(bool forward, const agd_eikonal_desc* d, const S* values,
                          trig, g.ntiles, ws->stamp, ws->lists, ws->counts, a.w.shard_cap);
       return AGD_OK;
     };
-    auto launch = [&](int32_t koff, hipStream_t s) {
+    auto launch = [&](int32_t koff, hipStream_t s, bool phase) {
       a.w.koff = koff;
+    a.w.phase_from_koff = phase ? 1 : 0;
       hipLaunchKernelGGL(ls.update, dim3(blocks), dim3(ls.ni), 0, s, a);
     };
     WorklistResult res;

@@ -468,8 +468,9 @@ static int solve_impl(const agd_eikonal_desc* d, const agd_eikonal_io* io,
     }
     return AGD_OK;
   };
-  auto launch = [&](int32_t koff, hipStream_t s) {
+  auto launch = [&](int32_t koff, hipStream_t s, bool phase) {
     a.w.koff = koff;
+    a.w.phase_from_koff = phase ? 1 : 0;
     // walls, periodic axes, source factorization and bound_active_blocks run
     // in the general variant
     auto kern = pre ? ks.update_pre
@@ -652,8 +653,9 @@ static int solve_batched_impl(const agd_eikonal_desc* d, int32_t ninst,
     }
     return AGD_OK;
   };
-  auto launch = [&](int32_t koff, hipStream_t s) {
+  auto launch = [&](int32_t koff, hipStream_t s, bool phase) {
     a.w.koff = koff;
+    a.w.phase_from_koff = phase ? 1 : 0;
     hipLaunchKernelGGL(ks.update_batch, dim3(blocks), dim3(ni), 0, s, a);
   };
   WorklistResult r;

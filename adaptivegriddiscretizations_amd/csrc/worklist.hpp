@@ -120,6 +120,10 @@ struct WorkCtx {
   int32_t tile_cap;
   int32_t shard_cap;
   int32_t koff;
+  // 1 when the pass number is koff modulo 6 (eager passes: base 0; graph
+  // chunks of a multiple of 6 passes): the counter row and the list parity
+  // then follow from koff, and a wave loads only its row.
+  int32_t phase_from_koff;
 };
 
 struct PassView {
@@ -140,15 +144,22 @@ struct PassView {
 // list length.  Must be called by every thread of the block.
 __device__ inline PassView pass_begin(const WorkCtx& c) {
   const int lane = threadIdx.x & 63;
-  const int c0 = c.counts[0 * kRow + lane * kCountStride];
-  const int c1 = c.counts[1 * kRow + lane * kCountStride];
-  const int c2 = c.counts[2 * kRow + lane * kCountStride];
   const int base = c.counts[kBaseIdx];
   const int fixed_n = c.counts[kFixedIdx];
   PassView v;
   v.iter = base + c.koff;
-  const int r = v.iter % 3;
-  const int mine = r == 0 ? c0 : (r == 1 ? c1 : c2);
+  int mine, par;
+  if (c.phase_from_koff) {
+    mine = c.counts[(c.koff % 3) * kRow + lane * kCountStride];
+    par = c.koff & 1;
+  } else {
+    const int c0 = c.counts[0 * kRow + lane * kCountStride];
+    const int c1 = c.counts[1 * kRow + lane * kCountStride];
+    const int c2 = c.counts[2 * kRow + lane * kCountStride];
+    const int r = v.iter % 3;
+    mine = r == 0 ? c0 : (r == 1 ? c1 : c2);
+    par = v.iter & 1;
+  }
   int incl = mine;
 #pragma unroll
   for (int d = 1; d < 64; d <<= 1) {
@@ -159,7 +170,7 @@ __device__ inline PassView pass_begin(const WorkCtx& c) {
   v.fixed = c.fixed != nullptr;
   v.count = v.fixed ? fixed_n : __shfl(incl, 63);
   v.shard_cap = c.shard_cap;
-  v.list_in = v.fixed ? c.fixed : c.lists + int64_t(v.iter & 1) * kShards * c.shard_cap;
+  v.list_in = v.fixed ? c.fixed : c.lists + int64_t(par) * kShards * c.shard_cap;
   const int shard = blockIdx.x % kShards;
   v.list_out = c.lists + int64_t((v.iter + 1) & 1) * kShards * c.shard_cap +
                int64_t(shard) * c.shard_cap;
@@ -320,7 +331,7 @@ int run_worklist(int32_t ntiles, int32_t grid, int64_t nitermax, int check_inter
     hipGraph_t g = nullptr;
     hipStream_t cs = ws->capture_stream;
     AGD_HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-    for (int i = 0; i < ci; ++i) launch(i, cs);
+    for (int i = 0; i < ci; ++i) launch(i, cs, ci % 6 == 0);
     hipLaunchKernelGGL(advance_kernel, dim3(1), dim3(1), 0, cs, counts, ci);
     // End the capture whatever happened inside it, so the stream is usable again.
     const hipError_t cap_err = hipGetLastError();
@@ -374,7 +385,7 @@ int run_worklist(int32_t ntiles, int32_t grid, int64_t nitermax, int check_inter
   } else {
     while (k < nitermax) {
       if (time_launches) AGD_HIP_TRY(hipEventRecord(evs[2 * pending], stream));
-      launch(int32_t(k), stream);
+      launch(int32_t(k), stream, true);
       if (time_launches) AGD_HIP_TRY(hipEventRecord(evs[2 * pending + 1], stream));
       ++pending;
       ++k;

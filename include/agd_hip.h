@@ -71,7 +71,9 @@ typedef struct agd_eikonal_desc {
   double atol, rtol;      /* |u_old-u_new| <= atol + rtol |u_new|        */
   int64_t nitermax_o;     /* maximum number of grid-level passes         */
   double iso_weights[3];  /* AGD_ISOTROPIC: per-axis weights (h^-2)      */
-  int32_t check_interval; /* passes between host convergence checks (>=1) */
+  int32_t check_interval; /* passes between host convergence checks (>=1);
+                             with use_graph, a multiple of 6 lets every pass
+                             read only its own row of worklist counters */
   int32_t activate_all;   /* 1: start with every tile active (warm start) */
   int32_t time_launches;  /* 1: time every pass with an event pair (kernel_ms) */
   int32_t use_graph;      /* 1: capture check_interval passes once as a hipGraph and replay it */
