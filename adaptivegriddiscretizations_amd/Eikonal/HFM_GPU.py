@@ -268,10 +268,15 @@ class Interface:
         # Cost function (_SetGeometry.py:221-233): a denseAD cost, or a
         # separate costVariation, requests forward AD.
         if self.HasValue('speed'):
-            speed = self.GetValue('speed', help="speed = 1/cost (scales the metric)")
+            speed = self.GetValue('speed', help="speed = 1/cost (scales the metric, accepts AD)")
             if ad.is_ad(speed):
-                raise NotImplementedError("forward AD through 'speed' is not provided; use 'cost'")
-            self.cost = 1. / speed
+                # cost = 1/speed: d cost = -d speed / speed^2 (chain rule of the
+                # reference's denseAD division)
+                sv = speed.value
+                sq = sv * sv
+                self.cost = ad.Dense.denseAD(1. / sv, -speed.coef / sq.reshape(tuple(sq.shape) + (1,)))
+            else:
+                self.cost = 1. / speed
         else:
             self.cost = self.GetValue('cost', default=None, help="cost = 1/speed (scales the metric, accepts AD)")
             if self.cost is None:

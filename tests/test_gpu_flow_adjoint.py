@@ -195,3 +195,23 @@ def test_autograd_cost_gradient(O):
         ok += abs(fd - grad[c]) <= 2e-2 * max(abs(fd), 1e-2 * np.max(np.abs(g_ref)))
     log(f"autograd vs FP64 finite differences: {ok}/32 cells within 2e-2")
     assert ok >= 30
+
+
+def test_forward_ad_through_speed():
+    """A denseAD 'speed' gives the variation of the solve with cost = 1/speed:
+    the same coefficients as costVariation = -dspeed / speed^2."""
+    from adaptivegriddiscretizations_amd import AutomaticDifferentiation as ad
+    wl = workloads.riemann2_aniso(n=64, seed=47, sigma=4., amax=4., nseeds=1)
+    rng = np.random.default_rng(4)
+    speed = (1 + 0.5 * rng.uniform(size=wl.dims)).astype(np.float32)
+    dspeed = rng.normal(size=wl.dims + (2,)).astype(np.float32)
+    h1 = wl.dict_in(exportValues=True)
+    h1['speed'] = ad.Dense.denseAD(speed, dspeed)
+    du1 = h1.Run()['values'].coef.cpu().numpy().astype(np.float64)
+    cost = (1. / speed).astype(np.float32)
+    h2 = wl.dict_in(exportValues=True, cost=cost,
+                    costVariation=(-dspeed / (speed ** 2)[..., None]).astype(np.float32))
+    du2 = h2.Run()['values'].coef.cpu().numpy().astype(np.float64)
+    err = np.max(np.abs(du1 - du2)) / np.max(np.abs(du2))
+    log(f"forward AD through speed vs costVariation: rel diff {err:.2e}")
+    assert err <= 1e-4
