@@ -172,8 +172,6 @@ class Interface:
         if self.order not in (1, 2):
             raise ValueError("order must be 1 or 2")
         if self.order == 2:
-            if self.model_ == 'TTI':
-                raise NotImplementedError("the second-order scheme of the TTI model is not provided")
             self.order2_threshold = float(self.GetValue(
                 'order2_threshold', default=0.3,
                 help="Relative threshold on second order differences, for the second-order scheme"))
@@ -545,7 +543,7 @@ class Interface:
         # it (seven sub-schemes per cell exceed the register budget).
         self.scheme_w_t = self.scheme_c_t = None
         self.precompute_scheme = False
-        if self.order == 1 and not self.factoringRadius and _lib.lib.agd_scheme_supported(
+        if (self.order == 1 or self.model_ == 'TTI') and not self.factoringRadius and _lib.lib.agd_scheme_supported(
                 self.modelId, self.ndim, _lib.i32(self.shape_i)):
             self.precompute_scheme = bool(self.GetValue(
                 'precompute_scheme', default=self.model_ == 'TTI',
@@ -556,7 +554,10 @@ class Interface:
             nsym = self.ndim * (self.ndim + 1) // 2
             nmix = _NMIX.get(self.model_, 1)
             self.scheme_w_t = empty((nmix * nsym, self.size_tot), self.float_t)
-            self.scheme_c_t = empty((nmix * 2 * nsym + (nmix > 1), self.size_tot), np.int32)
+            # TTI: the mix flag after the codes, then (second order) the
+            # codes of the further neighbours x -+ 2 e_k
+            ncodes = nmix * 2 * nsym + (nmix > 1) + (nmix * 2 * nsym if nmix > 1 and self.order == 2 else 0)
+            self.scheme_c_t = empty((ncodes, self.size_tot), np.int32)
             self.compute_scheme()
 
         self.nitermax_o = int(self.GetValue('nitermax_o', default=2000,

@@ -303,6 +303,13 @@ template <class S> static FactorDesc<S> make_factor(const agd_eikonal_desc* d) {
   return f;
 }
 
+// TTI scheme codes: NMIX * 2 * nsym neighbour codes, the mix flag, then
+// (second order) NMIX * 2 * nsym further-neighbour codes.
+static int64_t tti_codes2_offset(const agd_eikonal_desc* d) {
+  const int nsym = d->ndim * (d->ndim + 1) / 2;
+  return int64_t(kTTIMix) * 2 * nsym + 1;
+}
+
 static GridDesc make_grid(const agd_eikonal_desc* d) {
   GridDesc g{};
   g.ntiles = 1;
@@ -408,8 +415,9 @@ static int solve_impl(const agd_eikonal_desc* d, const agd_eikonal_io* io,
   AGD_REQUIRE(pre || ks.update, "this model runs from a precomputed scheme (agd_eikonal_scheme)");
   const bool order2 = d->order == 2;
   AGD_REQUIRE(d->order >= 0 && d->order <= 2, "order must be 1 or 2 (0 means 1)");
-  AGD_REQUIRE(!order2 || (!pre && ks.update_order2),
-              "the second-order scheme has no precomputed-scheme variant (nor a TTI one)");
+  const bool tti = d->model == AGD_TTI;
+  AGD_REQUIRE(!order2 || (!pre && ks.update_order2) || (pre && tti),
+              "the second-order scheme has no precomputed-scheme variant (except TTI's)");
   const bool factor = d->factor_radius2 > 0.;
   AGD_REQUIRE(!factor || !pre, "source factorization has no precomputed-scheme variant");
   const bool bab = d->bound_active_blocks > 0;
@@ -424,6 +432,8 @@ static int solve_impl(const agd_eikonal_desc* d, const agd_eikonal_io* io,
   a.geom = static_cast<const S*>(io->geom);
   a.sw = static_cast<const S*>(io->scheme_weights);
   a.scode = io->scheme_codes;
+  // TTI, second order: the further-neighbour codes follow the mix flag
+  a.scode2 = (pre && tti && order2) ? io->scheme_codes + tti_codes2_offset(d) * g.size_tot : nullptr;
   a.rhs = static_cast<const S*>(io->rhs);
   a.seeds = io->seeds;
   a.wall = io->wall_dist;
@@ -527,8 +537,7 @@ static int flow_impl(const agd_eikonal_desc* d, const agd_eikonal_io* io, const 
   a.violations = violations;
   a.chk_atol = S(chk_atol);
   a.chk_rtol = S(chk_rtol);
-  AGD_REQUIRE(d->order >= 0 && d->order <= 2 && (d->order < 2 || d->model != AGD_TTI),
-              "order must be 1 or 2 (1 for TTI; 0 means 1)");
+  AGD_REQUIRE(d->order >= 0 && d->order <= 2, "order must be 1 or 2 (0 means 1)");
   a.order2 = d->order == 2 ? 1 : 0;
   a.order2_thr = S(d->order2_threshold);
   a.fac = make_factor<S>(d);
@@ -694,6 +703,8 @@ static int scheme_impl(const agd_eikonal_desc* d, const agd_eikonal_io* io, hipS
   for (int k = 0; k < 3; ++k) a.iso_w[k] = S(d->iso_weights[k]);
   a.sw = static_cast<S*>(io->scheme_weights);
   a.scode = io->scheme_codes;
+  a.scode2 = (d->model == AGD_TTI && d->order == 2) ? io->scheme_codes + tti_codes2_offset(d) * g.size_tot
+                                                     : nullptr;
   hipLaunchKernelGGL((io->wall_dist || g.periodic) ? ks.scheme_general : ks.scheme, dim3(g.ntiles),
                      dim3(ks.ni), 0, stream, a);
   AGD_HIP_TRY(hipGetLastError());

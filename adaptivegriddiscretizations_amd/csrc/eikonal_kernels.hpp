@@ -53,6 +53,7 @@ template <class S> struct EikArgs {
   const S* geom;
   const S* sw;          // precomputed stencil weights (NS components) or null
   const int32_t* scode; // precomputed neighbour codes (2*NS components) or null
+  const int32_t* scode2; // TTI, second order: further-neighbour codes, or null
   const S* rhs;
   const uint32_t* seeds;
   const uint8_t* wall;
@@ -474,7 +475,7 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
         S best = mix_min ? inf_v<S>() : -inf_v<S>();
 #pragma unroll 1
         for (int m = 0; m < MT::NMIX; ++m) {
-          S v[NS], w[NS];
+          S v[NS], w[NS], v2[NS];
 #pragma unroll
           for (int k = 0; k < NS; ++k) {
             w[k] = ld32(a.sw + int64_t(m * NS + k) * N, uint32_t(n_t));
@@ -484,8 +485,15 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
               const int c = ld32(a.scode + int64_t(m * NJ + 2 * k + sd) * N, uint32_t(n_t));
               side[sd] = c <= -2 ? u_s[-2 - c] : (c >= 0 ? ld32(a.u, uint32_t(c)) : inf_v<S>());
             }
-            v[k] = (side[0] > side[1]) ? side[1] : side[0];
+            const int sd = (side[0] > side[1]) ? 1 : 0;  // Greater(v,-) keeps the -e side on ties
+            v[k] = side[sd];
+            if (a.scode2) {
+              // the further neighbour on the chosen side (second order)
+              const int c2 = ld32(a.scode2 + int64_t(m * NJ + 2 * k + sd) * N, uint32_t(n_t));
+              v2[k] = c2 <= -2 ? u_s[-2 - c2] : (c2 >= 0 ? ld32(a.u, uint32_t(c2)) : inf_v<S>());
+            }
           }
+          if (a.scode2) order2_correct<S, NS>(u_s[tid], a.order2_thr, v, v2, w);
           const S um = hfm_solve<S, NS>(v, w, rhs);
           if (mix_min == (best > um)) best = um;
         }
@@ -626,6 +634,7 @@ template <class S> struct SchemeArgs {
   S iso_w[3];
   S* sw;
   int32_t* scode;
+  int32_t* scode2;  // TTI, second order: codes of the further neighbours x -+ 2 e_k, or null
 };
 
 template <class S, int MODEL, int D, int T0, int T1, int T2, bool GENERAL>
@@ -666,11 +675,26 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_scheme(SchemeArgs<S> a) {
       Stencil<S, MODEL, D> st;
       tti_sub<S, D>(D0, D1, diag[m][0], diag[m][1], st.w, st.e);
       int code[2 * NS];
-      resolve_codes<S, MODEL, D, TL, GENERAL>(st, xi, xt, xo, wx, wall_s, a.wall, a.g, code);
+      int code2[2 * NS];
+      resolve_codes<S, MODEL, D, TL, GENERAL>(st, xi, xt, xo, wx, wall_s, a.wall, a.g, code, 0,
+                                              (GENERAL && a.scode2) ? code2 : nullptr);
+      if (!GENERAL && a.scode2) {
+        // no walls: the further neighbours are the codes of the doubled offsets
+        Stencil<S, MODEL, D> st2 = st;
+#pragma unroll
+        for (int k = 0; k < NS; ++k)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) st2.e[k][c] = 2 * st.e[k][c];
+        resolve_codes<S, MODEL, D, TL, GENERAL>(st2, xi, xt, xo, wx, wall_s, a.wall, a.g, code2);
+      }
 #pragma unroll
       for (int k = 0; k < NS; ++k) a.sw[int64_t(m * NS + k) * N + n_t] = st.w[k];
 #pragma unroll
       for (int j = 0; j < 2 * NS; ++j) a.scode[int64_t(m * 2 * NS + j) * N + n_t] = code[j];
+      if (a.scode2) {
+#pragma unroll
+        for (int j = 0; j < 2 * NS; ++j) a.scode2[int64_t(m * 2 * NS + j) * N + n_t] = code2[j];
+      }
     }
     a.scode[int64_t(MT::NMIX * 2 * NS) * N + n_t] = mix_min ? 1 : 0;
   } else {
@@ -699,7 +723,7 @@ template <class S> struct FlowArgs {
   S* wsum;
   int32_t* fidx;
   S* fvec;
-  int32_t order2;  // second-order correction (not with TTI)
+  int32_t order2;  // second-order correction
   S order2_thr;
   // Residual check instead of export: count the non-seed cells whose update
   // exceeds their value by more than chk_atol + chk_rtol |u| (a value below
@@ -814,17 +838,24 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_flow(FlowArgs<S> a) {
       tti_sub<S, D>(D0, D1, diag[m][0], diag[m][1], cand.w, cand.e);
 #pragma unroll
       for (int k = 0; k < NS; ++k) cand.shift[k] = S(0);
-      S cv[NS], cw[NS];
+      S cv[NS], cw[NS], cv2[NS];
       int cs[NS];
       int64_t cn[NS];
-      gather(cand, cv, cw, cs, cn, nullptr);
+      gather(cand, cv, cw, cs, cn, a.order2 ? cv2 : nullptr);
       S vs[NS], ws[NS];
 #pragma unroll
       for (int k = 0; k < NS; ++k) { vs[k] = cv[k]; ws[k] = cw[k]; }
+      unsigned cmask = 0;
+      if (a.order2) {
+        cmask = order2_correct<S, NS>(u_old, a.order2_thr, vs, cv2, ws);
+#pragma unroll
+        for (int k = 0; k < NS; ++k) cv[k] = vs[k];  // corrected values (unsorted copy)
+      }
       const S um = hfm_solve<S, NS>(vs, ws, rhs);
       if (mix_min == (val > um)) {
         val = um;
         st = cand;
+        o2mask = cmask;
 #pragma unroll
         for (int k = 0; k < NS; ++k) { v[k] = cv[k]; w[k] = cw[k]; side[k] = cs[k]; nb[k] = cn[k]; }
       }

@@ -191,8 +191,11 @@ int agd_eikonal_check(const agd_eikonal_desc* desc, const agd_eikonal_io* io, do
  * geom (and wall_dist if walls are present).  Riemann and TTI, as
  * agd_scheme_supported reports.  TTI stores its 7 sub-schemes: weights
  * 7*nsym components, codes 7*2*nsym components (sub-scheme m at m*2*nsym)
- * plus one last component, 1 where the update is the min of the
- * sub-schemes' updates and 0 where it is their max (TTI_.h:127). */
+ * plus one component, 1 where the update is the min of the sub-schemes'
+ * updates and 0 where it is their max (TTI_.h:127); with desc->order 2,
+ * 7*2*nsym more components follow: the codes of the further neighbours
+ * x -+ 2 e_k of the second-order scheme (HFM.h:91-134), which
+ * agd_eikonal_solve then reads. */
 int agd_scheme_supported(int32_t model, int32_t ndim, const int32_t* shape_i);
 int agd_eikonal_scheme(const agd_eikonal_desc* desc, const agd_eikonal_io* io, void* stream);
 

@@ -607,11 +607,35 @@ static S cell_update(const Problem& P, const Stencil<S>& st, int64_t n, Get get,
   }
   S best = st.mix_is_min ? Inf<S>() : -Inf<S>();
   for (int m = 0; m < st.nmix; ++m) {
-    S v[6], f[6];
+    S v[6], f[6], v2[6], w[6];
     int sd[6];
     int64_t ni[6];
-    neighbours<S>(P, st, m * ns, n, get, v, sd, ni);
-    const S val = hfm_solve<S>(ns, st.w + m * ns, v, rhs, f);
+    neighbours<S>(P, st, m * ns, n, get, v, sd, ni, P.order == 2 ? v2 : nullptr);
+    for (int k = 0; k < ns; ++k) w[k] = st.w[m * ns + k];
+    bool o2[6] = {false, false, false, false, false, false};
+    if (P.order == 2) {
+      // the second-order correction of each sub-stencil (HFMIter.h:43 passes
+      // every sub-stencil's further neighbours to HFMUpdate, HFM.h:91-134)
+      S bound = 0;
+      for (int k = 0; k < ns; ++k) bound = std::max(bound, u0 - v[k]);
+      const S thr = S(P.order2_threshold);
+      for (int k = 0; k < ns; ++k) {
+        const S diff2 = std::fabs(u0 - 2 * v[k] + v2[k]);
+        o2[k] = diff2 < thr * bound && u0 > v[k];
+        if (o2[k]) {
+          v[k] += (v[k] - v2[k]) / 3;
+          w[k] *= S(9) / S(4);
+        }
+      }
+    }
+    S val = hfm_solve<S>(ns, w, v, rhs, f);
+    if (P.order == 2) {
+      for (int k = 0; k < ns; ++k) {
+        const S diff = val - v[k];
+        f[k] = (val == Inf<S>()) ? S(0)
+                                 : st.w[m * ns + k] * (diff > S(0) ? diff : S(0)) * (o2[k] ? S(1.5) : S(1));
+      }
+    }
     if (st.mix_is_min == (best > val)) {
       best = val;
       for (int k = 0; k < ns; ++k) {

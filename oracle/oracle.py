@@ -251,7 +251,7 @@ def tti_geometry(metric, h, shape):
     linear, quadratic, A = metric
     d = len(shape)
     full = lambda x, depth: np.broadcast_to(np.asarray(x, dtype=np.float64).reshape(
-        np.shape(x)[:depth] + (1,) * (d + depth - np.ndim(x))), np.shape(x)[:depth] + shape)
+        np.shape(x) + (1,) * (d + depth - np.ndim(x))), np.shape(x)[:depth] + shape)
     l = full(linear, 1)
     q = full(quadratic, 2)
     if A is None:

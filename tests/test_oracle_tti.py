@@ -112,3 +112,33 @@ def test_tti3_without_quadratic_part_is_riemannian(O):
     M = np.diag(1. / np.array([l[0], l[0], l[1]]))
     P_r = O.make_problem('Riemann', (n,) * 3, h, metric=M, seeds_idx=[[3, 4, 5]])
     np.testing.assert_allclose(P_tti.solve_fm(), P_r.solve_fm(), rtol=1e-10)
+
+
+@pytest.mark.parametrize("d", [2, 3])
+def test_tti_second_order_without_quadratic_part_is_riemannian(O, d):
+    """The second-order correction applied to every sub-stencil: with q = 0
+    the 7 sub-stencils are the Riemannian one, so the second-order TTI fixed
+    point equals the second-order Riemannian one (itself pinned by its
+    convergence rates), and differs from the first-order one."""
+    n = 25 if d == 2 else 11
+    h = np.array([1. / n] * d)
+    if d == 2:
+        l = np.array([2., 0.7])
+        th = 0.4
+        A = np.array([[np.cos(th), -np.sin(th)], [np.sin(th), np.cos(th)]]) @ np.diag([1., 1.5])
+    else:
+        l = np.array([2., 0.7])
+        A = None                      # (TTI3: transversely isotropic about the last axis)
+    seed = [[n // 3] * d]
+    P_tti = O.make_problem('TTI', (n,) * d, h, metric=(l, np.zeros((2, 2)), A), seeds_idx=seed)
+    if d == 2:
+        M = A.T @ np.diag(1. / l) @ A
+    else:
+        M = np.diag(1. / np.array([l[0], l[0], l[1]]))
+    P_r = O.make_problem('Riemann', (n,) * d, h, metric=M, seeds_idx=seed)
+    with O.order(2):
+        u_t, _ = P_tti.solve_jacobi(1e-13, 0.)
+        u_r, _ = P_r.solve_jacobi(1e-13, 0.)
+    np.testing.assert_allclose(u_t, u_r, rtol=1e-9)
+    u1, _ = P_tti.solve_jacobi(1e-13, 0.)
+    assert np.max(np.abs(u1 - u_t)) > 1e-4
