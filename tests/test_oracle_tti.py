@@ -142,3 +142,24 @@ def test_tti_second_order_without_quadratic_part_is_riemannian(O, d):
     np.testing.assert_allclose(u_t, u_r, rtol=1e-9)
     u1, _ = P_tti.solve_jacobi(1e-13, 0.)
     assert np.max(np.abs(u1 - u_t)) > 1e-4
+
+
+@pytest.mark.parametrize("d", [2, 3])
+def test_tti_geometry_of_field_inputs(O, d):
+    """tti_geometry with linear, quadratic and transformation given as
+    per-cell fields equals its constant form broadcast, cell by cell, and
+    its last d*d components are (A diag(h))^{-T} (row-major)."""
+    shape = (5, 4) if d == 2 else (3, 4, 2)
+    h = np.array([0.1, 0.2, 0.05][:d])
+    rng = np.random.default_rng(9)
+    l = np.array([2., 0.7])
+    q = np.array([[0.3, -0.1], [-0.1, 0.2]])
+    A = np.linalg.qr(rng.normal(size=(d, d)))[0] * 1.3
+    g_const = O.tti_geometry((l, q, A), h, shape)
+    ext = (1,) * d
+    field = lambda x: np.broadcast_to(x.reshape(x.shape + ext), x.shape + shape).copy()
+    g_field = O.tti_geometry((field(l), field(q), field(A)), h, shape)
+    assert g_field.shape == g_const.shape == (5 + d * d,) + shape
+    assert np.array_equal(g_field, g_const)
+    T = np.linalg.inv(A * h[None, :]).T
+    assert np.allclose(g_const[5:].reshape((d, d) + shape)[(...,) + (0,) * d], T, rtol=1e-13)
