@@ -1,17 +1,29 @@
 """dictIn / dictOut: the input and output containers of the eikonal solver.
 
-Drop-in counterpart of agd/Eikonal/DictIn.py for the GPU path: same keys,
-same grid helpers (SetRect, Grid, Axes, PointFromIndex, IndexFromPoint,
-SetUniformTips, gridScales, corners, shape), same dispatch on 'mode'.
-'gpu' returns device arrays (torch tensors on the current HIP device, the
-counterpart of the reference's cupy arrays); 'gpu_transfer' returns numpy.
-The CPU modes of the reference call the external HamiltonFastMarching
-library, which this package does not provide.
+Drop-in counterpart of agd/Eikonal/DictIn.py for the GPU path: the same keys
+and key aliases, the same grid helpers (SetRect, Grid, Axes, PointFromIndex,
+IndexFromPoint, SetUniformTips, gridScales, corners, shape) and the same
+dispatch on 'mode'.  'gpu' returns device arrays (torch tensors on the current
+HIP device, the counterpart of the reference's cupy arrays); 'gpu_transfer'
+returns numpy arrays.  The CPU modes of the reference call the external
+HamiltonFastMarching library, which this package does not provide.
+
+Design: both containers are thin mappings over a plain dict (`store`) that
+share one alias rule (a singular key such as 'seed' reads and writes a
+one-element list under its plural key).  The grid of a dictIn is described
+by three stored keys -- 'dims', 'origin' (lower corner of the box) and
+'gridScale' or 'gridScales' -- and every grid helper derives from them.  The
+float arithmetic of SetRect and PointFromIndex is carried out in the
+container's float type (float32 in the GPU modes) so that dims, origin and
+continuous seed indices come out exactly as the reference computes them
+(DictIn.py:404-453): the seed at the centre of an even grid lands exactly on
+a rounding tie.
 """
 from collections.abc import MutableMapping
 
 import numpy as np
 
+# Keys whose values are stored as arrays of the container's float type.
 _array_float_fields = {
     'origin', 'dims', 'gridScale', 'gridScales', 'values',
     'seeds', 'seeds_Unoriented', 'tips', 'tips_Unoriented',
@@ -23,6 +35,7 @@ _array_float_fields = {
     'control', 'controls', 'state_transition_costs',
 }
 
+# Singular input keys and the plural key that holds them as a list of one.
 _singleIn = {
     'seed': 'seeds', 'seedValue': 'seedValues',
     'seed_Unoriented': 'seeds_Unoriented', 'seedValue_Unoriented': 'seedValues_Unoriented',
@@ -32,10 +45,16 @@ _singleIn = {
 
 _readonlyIn = {'float_t'}
 
+# Where a mode keeps its arrays: two modes are interchangeable on a container
+# only if they agree here.
 _array_module = {
     'cpu': 'host', 'cpu_raw': 'host', 'gpu_transfer': 'host',
     'gpu': 'device', 'cpu_transfer': 'device',
 }
+
+# Modes whose default float type is float32 (the GPU arithmetic); the others
+# default to float64.
+_float32_modes = {'gpu', 'cpu_transfer', 'gpu_transfer'}
 
 _singleOut = {
     'geodesic': 'geodesics', 'geodesic_Unoriented': 'geodesics_Unoriented',
@@ -49,42 +68,43 @@ SEModels = {'ReedsShepp2', 'ReedsSheppForward2', 'Elastica2', 'Dubins2',
 
 dimModels = {'ElasticaExt2_5': 3, 'Riemann3_Periodic': 3, 'ReedsSheppGPU3': 5}
 
+# The keys that describe the grid: SetRect sets three of them at once.
+_DOMAIN_KEYS = ('gridScale', 'gridScales', 'dims', 'origin')
+
 
 def is_tensor(x):
     return type(x).__module__.startswith('torch')
 
 
-class dictOut(MutableMapping):
-    """Dictionary-like output of the eikonal solver."""
+def _host(x):
+    """numpy view of an array that may live on the device."""
+    return x.detach().cpu().numpy() if is_tensor(x) else np.asarray(x)
 
-    def __init__(self, store=None):
-        self.store = store if store is not None else {}
 
-    def __copy__(self):
-        return dictOut(self.store.copy())
+class _AliasedStore(MutableMapping):
+    """Mapping over `self.store` in which the keys of `_aliases` stand for a
+    one-element list kept under their plural key."""
 
-    def copy(self):
-        return self.__copy__()
+    _aliases = {}
 
-    def __repr__(self):
-        return f"dictOut({self.store})"
+    def _plural(self, key):
+        return self._aliases.get(key, key)
 
-    def __setitem__(self, key, value):
-        if key in _singleOut:
-            key = _singleOut[key]
-            value = [value]
-        self.store[key] = value
+    def _put(self, key, value):
+        plural = self._plural(key)
+        self.store[plural] = [value] if plural != key else value
 
     def __getitem__(self, key):
-        if key in _singleOut:
-            values = self.store[_singleOut[key]]
-            if len(values) != 1:
-                raise ValueError(f"Found {len(values)} values for key {key}")
-            return values[0]
-        return self.store[key]
+        plural = self._plural(key)
+        value = self.store[plural]
+        if plural == key:
+            return value
+        if len(value) != 1:
+            raise ValueError(f"Found {len(value)} values for key {key}")
+        return value[0]
 
     def __delitem__(self, key):
-        del self.store[_singleOut.get(key, key)]
+        del self.store[self._plural(key)]
 
     def __iter__(self):
         return iter(self.store)
@@ -95,61 +115,66 @@ class dictOut(MutableMapping):
     def keys(self):
         return self.store.keys()
 
+    def __repr__(self):
+        return f"{type(self).__name__}({self.store})"
+
+
+class dictOut(_AliasedStore):
+    """Dictionary-like output of the eikonal solver."""
+
+    _aliases = _singleOut
+
+    def __init__(self, store=None):
+        self.store = {} if store is None else store
+
+    def __setitem__(self, key, value):
+        self._put(key, value)
+
+    def copy(self):
+        return dictOut(dict(self.store))
+
+    __copy__ = copy
+
 
 def CenteredLinspace(a, b, n):
-    """linspace shifted by half a step (cell centres of [a,b] split in n)."""
-    n_ = int(n)
-    assert n == n_
-    r, dr = np.linspace(a, b, n_, endpoint=False, retstep=True)
-    if np.any(np.isnan(dr)):
-        assert n == 1
-        dr = b - a
-    return r + dr / 2
+    """The n cell centres of [a, b]: a + (i + 1/2) (b - a)/n.
+
+    Evaluated as (i h + a) + h/2, the order of numpy.linspace(a, b, n,
+    endpoint=False) shifted by half its step, so that the centres agree with
+    the reference's (DictIn.py:97-107) to the last bit.  n may be given as an
+    integral float."""
+    count = int(n)
+    if count != n or count < 1:
+        raise ValueError(f"CenteredLinspace needs a positive integer count, got {n}")
+    a = np.asarray(a)
+    step = (np.asarray(b) - a) / count
+    return np.arange(count, dtype=step.dtype) * step + a + step / 2
 
 
-class dictIn(MutableMapping):
+class dictIn(_AliasedStore):
     """Dictionary-like input of the eikonal solver (agd/Eikonal/DictIn.py:110).
 
     Keys in _array_float_fields are cast to arrays of float_t on assignment
-    (float32 in the GPU modes); torch tensors are kept as tensors.
+    (float32 in the GPU modes); torch tensors stay tensors.
     """
 
     default_mode = 'gpu'
+    _aliases = _singleIn
 
     def __init__(self, store=None):
-        if store is None:
-            store = dict()
-        self.store = {'arrayOrdering': 'RowMajor'}
-        if 'mode' in store:
-            mode = store['mode']
-            self.store['mode'] = mode
-        else:
-            mode = dictIn.default_mode
-        assert mode in ('cpu', 'cpu_raw', 'cpu_transfer', 'gpu', 'gpu_transfer')
+        init = dict(store) if store else {}
+        mode = init.get('mode', dictIn.default_mode)
+        if mode not in _array_module:
+            raise ValueError(f"Unknown mode {mode!r}; expected one of {sorted(_array_module)}")
         self._mode = mode
-        float_t = np.float32 if mode in ('gpu', 'cpu_transfer', 'gpu_transfer') else np.float64
-        if 'float_t' in store:
-            float_t = store['float_t']
-            self.store['float_t'] = float_t
-        self._float_t = float_t
-        if store:
-            self.update(store)
+        self._float_t = init.get('float_t', np.float32 if mode in _float32_modes else np.float64)
+        self.store = {'arrayOrdering': 'RowMajor'}
+        for key in ('mode', 'float_t'):
+            if key in init:
+                self.store[key] = init[key]
+        self.update(init)
 
-    def array_float_caster(self, x):
-        from ..AutomaticDifferentiation import Dense
-        if isinstance(x, Dense.denseAD):  # cast the value and the coefficients alike
-            return Dense.denseAD(self.array_float_caster(x.value), self.array_float_caster(x.coef))
-        if is_tensor(x):
-            import torch
-            return x.to(dtype=torch.float32 if self.float_t == np.float32 else torch.float64)
-        return np.asarray(x, dtype=self.float_t)
-
-    def __copy__(self):
-        return dictIn(self.store.copy())
-
-    def copy(self):
-        return self.__copy__()
-
+    # ------- storage and casting ------
     @property
     def mode(self):
         return self._mode
@@ -158,44 +183,39 @@ class dictIn(MutableMapping):
     def float_t(self):
         return self._float_t
 
-    def __repr__(self):
-        return f"dictIn({self.store})"
+    def array_float_caster(self, x):
+        """x as an array of float_t: torch tensors keep their device, denseAD
+        values cast their value and coefficients alike."""
+        from ..AutomaticDifferentiation import Dense
+        if isinstance(x, Dense.denseAD):
+            return Dense.denseAD(self.array_float_caster(x.value), self.array_float_caster(x.coef))
+        if is_tensor(x):
+            import torch
+            return x.to(dtype=torch.float32 if self.float_t == np.float32 else torch.float64)
+        return np.asarray(x, dtype=self.float_t)
+
+    def copy(self):
+        return dictIn(dict(self.store))
+
+    __copy__ = copy
 
     def __setitem__(self, key, value):
         if key == 'mode':
-            if _array_module[value] != _array_module[self.mode]:
+            if _array_module[value] != _array_module[self._mode]:
                 raise ValueError('Switching between modes with distinct array storage')
             self._mode = value
         if key in _readonlyIn and self.store[key] != value:
             raise ValueError(f"Key {key} is readonly (set at init)")
-        if key in _singleIn:
-            key = _singleIn[key]
+        plural = self._plural(key)
+        if plural != key:
             value = [value]
-        if key in _array_float_fields and not (isinstance(value, np.ndarray) and
-                                               value.dtype == self.float_t):
-            value = self.array_float_caster(value)
-        self.store[key] = value
+        if plural in _array_float_fields:
+            already = isinstance(value, np.ndarray) and value.dtype == self.float_t
+            if not already:
+                value = self.array_float_caster(value)
+        self.store[plural] = value
 
-    def __getitem__(self, key):
-        if key in _singleIn:
-            values = self.store[_singleIn[key]]
-            if len(values) != 1:
-                raise ValueError(f"Found {len(values)} values for key {key}")
-            return values[0]
-        return self.store[key]
-
-    def __delitem__(self, key):
-        del self.store[_singleIn.get(key, key)]
-
-    def __iter__(self):
-        return iter(self.store)
-
-    def __len__(self):
-        return len(self.store)
-
-    def keys(self):
-        return self.store.keys()
-
+    # ------- running the solver ------
     def RunHelp(self, mode=None):
         from . import HFM_GPU
         help(HFM_GPU.RunGPU)
@@ -203,36 +223,23 @@ class dictIn(MutableMapping):
     def Run(self, join=None, **kwargs):
         """Run the eikonal solver and return a dictOut (DictIn.py:248-285)."""
         if join is not None:
-            other = self.copy()
-            other.update(join)
-            return other.Run(**kwargs)
+            merged = self.copy()
+            merged.update(join)
+            return merged.Run(**kwargs)
         if self['arrayOrdering'] != 'RowMajor':
             raise ValueError("Unsupported array ordering")
-
-        def to_dictOut(out):
-            if isinstance(out, (tuple, list)) and len(out) and isinstance(out[0], dict):
-                return (dictOut(out[0]),) + tuple(out[1:])
-            if isinstance(out, (tuple, list)):
-                return type(out)(dictOut(o) if isinstance(o, dict) else o for o in out)
-            return dictOut(out)
-
-        if self.mode in ('cpu', 'cpu_raw', 'cpu_transfer'):
+        if self.mode not in ('gpu', 'gpu_transfer'):
             raise NotImplementedError(
                 f"mode={self.mode!r} calls the HamiltonFastMarching CPU library, which is not "
                 "part of this package; use mode='gpu' or 'gpu_transfer'")
         from . import HFM_GPU
-        if self.mode == 'gpu':
-            return to_dictOut(HFM_GPU.RunGPU(self, **kwargs))
-        out = HFM_GPU.RunGPU(self, host_outputs=True, **kwargs)
-        return to_dictOut(out)
+        extra = {'host_outputs': True} if self.mode == 'gpu_transfer' else {}
+        return _wrap_outputs(HFM_GPU.RunGPU(self, **extra, **kwargs))
 
-    # ------- Grid related functions ------
+    # ------- grid ------
     @property
     def shape(self):
-        dims = self['dims']
-        if is_tensor(dims):
-            dims = dims.cpu().numpy()
-        return tuple(np.asarray(dims).astype(int))
+        return tuple(int(n) for n in _host(self['dims']))
 
     @property
     def size(self):
@@ -247,11 +254,12 @@ class dictIn(MutableMapping):
         model = self['model']
         if model in dimModels:
             return dimModels[model]
-        dim = int(model[-1])
-        return (2 * dim - 1) if self.SE else dim
+        d = int(model[-1])
+        return 2 * d - 1 if self.SE else d
 
     @property
     def gridScales(self):
+        """Cell side along each axis."""
         if self.SE:
             raise NotImplementedError("Curvature penalized models are not provided")
         if 'gridScales' in self:
@@ -260,82 +268,104 @@ class dictIn(MutableMapping):
 
     @property
     def corners(self):
-        gridScales = _host(self.gridScales)
-        dims = _host(self['dims'])
-        origin = _host(self.get('origin', np.zeros_like(dims)))
-        return (origin, origin + gridScales * dims)
+        """Lower and upper corners of the box."""
+        h = _host(self.gridScales)
+        n = _host(self['dims'])
+        lo = _host(self['origin']) if 'origin' in self else np.zeros_like(n)
+        return lo, lo + h * n
 
     def Axes(self, dims=None):
-        bottom, top = self.corners
-        if dims is None:
-            dims = _host(self['dims'])
-        return [self.array_float_caster(CenteredLinspace(b, t, d)) for b, t, d in zip(bottom, top, dims)]
+        """Cell-centre coordinates along each axis (with `dims` cells per axis
+        instead of the grid's own when given)."""
+        lo, hi = self.corners
+        counts = _host(self['dims']) if dims is None else dims
+        return [self.array_float_caster(CenteredLinspace(a, b, n)) for a, b, n in zip(lo, hi, counts)]
 
     def Grid(self, dims=None):
-        axes = self.Axes(dims)
+        """All cell centres, coordinate first: shape (vdim, *dims)."""
         ordering = self['arrayOrdering']
-        if ordering == 'RowMajor':
-            return np.array(np.meshgrid(*axes, indexing='ij', copy=False))
-        if ordering == 'YXZ_RowMajor':
-            return np.array(np.meshgrid(*axes, copy=False))
-        raise ValueError('Unsupported arrayOrdering : ' + ordering)
+        if ordering not in ('RowMajor', 'YXZ_RowMajor'):
+            raise ValueError('Unsupported arrayOrdering : ' + ordering)
+        mesh = np.meshgrid(*self.Axes(dims), indexing='ij' if ordering == 'RowMajor' else 'xy',
+                           copy=False)
+        return np.stack(mesh)
 
     def SetUniformTips(self, dims):
+        """Geodesic tips at the cell centres of a coarser grid `dims`."""
         self['tips'] = self.Grid(dims).reshape(self.vdim, -1).T
 
     def SetRect(self, sides, sampleBoundary=False, gridScale=None, gridScales=None,
                 dimx=None, dims=None):
-        """Box domain (DictIn.py:404-440): cell-centred sampling unless
-        sampleBoundary, grid scale from gridScale(s), dimx or dims."""
-        domain_count = sum(e in self for e in ('gridScale', 'gridScales', 'dims', 'origin'))
-        if domain_count not in (0, 3):
+        """Box domain `sides` = [[lo_0, hi_0], ...] (DictIn.py:404-440).
+
+        The cell side comes from whichever of gridScale, gridScales (per
+        axis), dimx (cells along the first axis) or dims (cells per axis) is
+        given, in that order of precedence.  Points are cell centres, or lie
+        on the box faces with sampleBoundary.  The cell counts are the
+        rounded box widths over the cell side, and the box is re-centred on
+        the rounded grid."""
+        present = sum(key in self for key in _DOMAIN_KEYS)
+        if present not in (0, 3):
             raise ValueError("Domain already partially set")
-        caster = self.array_float_caster
-        corner0, corner1 = caster(sides).T
-        dim = len(corner0)
-        sb = float(sampleBoundary)
-        width = corner1 - corner0
+        box = self.array_float_caster(sides)
+        lo, hi = box[:, 0], box[:, 1]
+        faces = float(sampleBoundary)      # 1: points on the faces, 0: cell centres
+        h = self._cell_sides(hi - lo, faces, gridScale, gridScales, dimx, dims)
+        # Box width in cells, its rounding, and the half-leftover shift of
+        # the lower corner, in float_t and in the reference's order of
+        # operations.
+        cells = (hi - lo) / h + faces
+        counts = np.round(cells)
+        if not np.min(counts) > 0:
+            raise ValueError(f"SetRect gives an empty grid: {counts}")
+        self['dims'] = counts
+        self['origin'] = lo + (cells - counts - faces) * h / 2
+
+    def _cell_sides(self, width, faces, gridScale, gridScales, dimx, dims):
+        """Stores the grid-scale key SetRect derives from its arguments and
+        returns the per-axis cell side (float_t)."""
+        ndim = len(width)
+        cast = self.array_float_caster
         if gridScale is not None:
-            gridScales = [gridScale] * dim
             self['gridScale'] = gridScale
-        elif gridScales is not None:
+            return cast([gridScale] * ndim)
+        if gridScales is not None:
             self['gridScales'] = gridScales
-        elif dimx is not None:
-            gridScale = width[0] / (dimx - sb)
-            gridScales = [gridScale] * dim
-            self['gridScale'] = gridScale
-        elif dims is not None:
-            gridScales = width / (caster(dims) - sb)
-            self['gridScales'] = gridScales
-        else:
-            raise ValueError('Missing argument gridScale, gridScales, dimx, or dims')
-        h = caster(gridScales)
-        ratios = (corner1 - corner0) / h + sb
-        dims = np.round(ratios)
-        assert np.min(dims) > 0
-        origin = corner0 + (ratios - dims - sb) * h / 2
-        self['dims'] = dims
-        self['origin'] = origin
+            return cast(gridScales)
+        if dimx is not None:
+            side = width[0] / (dimx - faces)
+            self['gridScale'] = side
+            return cast([side] * ndim)
+        if dims is not None:
+            sides = width / (cast(dims) - faces)
+            self['gridScales'] = sides
+            return cast(sides)
+        raise ValueError('Missing argument gridScale, gridScales, dimx, or dims')
 
     def PointFromIndex(self, index, to=False):
-        """Index -> point; with to=True, point -> continuous index
-        (DictIn.py:442-453).  Arithmetic in float_t, as the reference."""
-        bottom, _ = self.corners
-        scale = _host(self.gridScales)
-        start = bottom + self.float_t(0.5) * scale
-        index = np.asarray(_host(index), dtype=self.float_t)
-        assert index.shape[-1] == self.vdim
-        if not to:
-            return start + scale * index
-        return (index - start) / scale
+        """Cell index -> point x = lo + h/2 + h index; with to=True, point ->
+        continuous index (x - (lo + h/2)) / h (DictIn.py:442-453).  Float
+        arithmetic in float_t."""
+        lo, _ = self.corners
+        h = _host(self.gridScales)
+        first = lo + self.float_t(0.5) * h      # centre of cell 0
+        x = np.asarray(_host(index), dtype=self.float_t)
+        if x.shape[-1] != self.vdim:
+            raise ValueError(f"points of dimension {self.vdim} expected, got shape {x.shape}")
+        return (x - first) / h if to else first + h * x
 
     def IndexFromPoint(self, point):
-        continuousIndex = self.PointFromIndex(point, to=True)
-        index = np.round(continuousIndex)
-        return index.astype(int), (continuousIndex - index)
+        """Nearest cell index of each point (ties to even) and the offset of
+        the point from that cell's centre, in cells."""
+        cont = self.PointFromIndex(point, to=True)
+        nearest = np.round(cont)
+        return nearest.astype(int), cont - nearest
 
 
-def _host(x):
-    if is_tensor(x):
-        return x.detach().cpu().numpy()
-    return np.asarray(x)
+def _wrap_outputs(out):
+    """The solver's dict outputs as dictOut, within a tuple/list return."""
+    if isinstance(out, dict):
+        return dictOut(out)
+    if isinstance(out, (tuple, list)):
+        return type(out)(dictOut(o) if isinstance(o, dict) else o for o in out)
+    return out

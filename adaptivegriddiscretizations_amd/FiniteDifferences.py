@@ -1,10 +1,13 @@
-"""Tiled (bi-level) layout helpers on host arrays.
+"""Index arithmetic of the tiled (bi-level) layout on the host.
 
-Same conventions as the reference (agd/FiniteDifferences.py:156-222): the
-inner (tile) axes are moved last, the array is padded up to a multiple of
-the tile shape.  The device-side equivalents are agd_block_expand /
-agd_block_squeeze in libagd_hip.so; these numpy versions serve host-side
-bookkeeping (seed indices, tile counts).
+The solver stores every per-cell field tile after tile: a grid of shape
+`shape` is padded up to whole tiles of shape `shape_i`, and cell x lives at
+n_o * prod(shape_i) + n_i, with n_o the row-major index of its tile
+x // shape_i in the tile grid shape_o and n_i the row-major index of x %
+shape_i inside the tile (the layout of agd/FiniteDifferences.py:156-222 and
+Update.h:60-66).  The data movement itself runs on the device
+(agd_block_expand / agd_block_squeeze in libagd_hip.so); the host only needs
+tile counts and the tiled index of a few cells (seeds, receivers).
 """
 import numpy as np
 
@@ -13,40 +16,6 @@ def round_up_ratio(num, den):
     """Smallest integer >= num/den, elementwise (num, den integers, den>0)."""
     num, den = np.asarray(num), np.asarray(den)
     return (num + den - 1) // den
-
-
-def block_expand(arr, shape_i, shape_o=None, **kwargs):
-    """Pad and reshape so that the tile axes come last; kwargs go to np.pad."""
-    ndim = len(shape_i)
-    if ndim == 0:
-        return arr
-    shape_pre = arr.shape[:-ndim]
-    ndim_pre = len(shape_pre)
-    shape_tot = np.array(arr.shape[-ndim:])
-    shape_i = np.array(shape_i)
-    if shape_o is None:
-        shape_o = round_up_ratio(shape_tot, shape_i)
-    shape_pad = (0,) * ndim_pre + tuple(shape_o * shape_i - shape_tot)
-    arr = np.pad(arr, tuple((0, s) for s in shape_pad), **kwargs)
-    shape_interleaved = tuple(np.stack((shape_o, shape_i), axis=1).flatten())
-    arr = arr.reshape(shape_pre + shape_interleaved)
-    rg = np.arange(ndim)
-    return np.moveaxis(arr, ndim_pre + 1 + 2 * rg, ndim_pre + ndim + rg)
-
-
-def block_squeeze(arr, shape):
-    """Inverse of block_expand."""
-    ndim = len(shape)
-    if ndim == 0:
-        return arr
-    shape_pre = arr.shape[:-2 * ndim]
-    ndim_pre = len(shape_pre)
-    shape_o = arr.shape[-2 * ndim:-ndim]
-    shape_i = arr.shape[-ndim:]
-    rg = np.arange(ndim)
-    arr = np.moveaxis(arr, ndim_pre + ndim + rg, ndim_pre + 1 + 2 * rg)
-    arr = arr.reshape(shape_pre + tuple(so * si for so, si in zip(shape_o, shape_i)))
-    return arr[tuple(slice(0, s) for s in shape_pre + tuple(shape))]
 
 
 def tiled_index(index, shape_i, shape_o):

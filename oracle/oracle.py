@@ -3,9 +3,12 @@
 Loads liboracle.so (agd_oracle.cpp, built by the Makefile next to this
 file) and restates, in numpy FP64, the reference's host preprocessing of the
 GPU eikonal path:
-  * grid geometry: cell centres origin + (i+1/2) h, continuous index
-    (x - origin)/h - 1/2, seeds rounded half to even
-    (agd/Eikonal/DictIn.py:404-453, HFM_CUDA/_SetArgs.py:44-45);
+  * grid geometry: the box arithmetic of SetRect in float32, cell centres
+    origin + (i+1/2) h, continuous index (x - origin)/h - 1/2, seeds rounded
+    half to even (agd/Eikonal/DictIn.py:97-107,404-453,
+    HFM_CUDA/_SetArgs.py:44-45);
+  * the tiled (bi-level) layout, as a checker of the device's
+    agd_block_expand / agd_block_squeeze (FiniteDifferences.py:164-222);
   * metric preparation: D = (diag(h) M diag(h))^-1 flattened lower-triangular
     for Riemann, [flat(inv(h m h)), h w] for Rander, (c h)^-2 for Diagonal,
     h^-2 for Isotropic (HFM_CUDA/_SetGeometry.py:127-142,
@@ -275,6 +278,86 @@ def grid_h(sides, dims):
     dims = np.asarray(dims, dtype=np.float64)
     h = (sides[:, 1] - sides[:, 0]) / dims
     return h, sides[:, 0]
+
+
+def set_rect(sides, float_t=np.float32, sample_boundary=False, gridScale=None, gridScales=None,
+             dimx=None, dims=None):
+    """Box domain of agd/Eikonal/DictIn.py:404-440, in the arithmetic type
+    float_t and in the reference's order of operations (so that dims and
+    origin are reproduced to the bit): the grid scale h comes from the one
+    argument given, then ratios = (hi - lo)/h + sb, dims = round(ratios),
+    origin = lo + ((ratios - dims) - sb) h / 2.  Returns a dict of the keys
+    SetRect stores."""
+    box = np.asarray(sides, dtype=float_t)
+    lo, hi = box[:, 0], box[:, 1]
+    sb = float(sample_boundary)
+    width = hi - lo
+    out = {}
+    if gridScale is not None:
+        out['gridScale'] = np.asarray(gridScale, dtype=float_t)
+        h = np.asarray([gridScale] * len(lo), dtype=float_t)
+    elif gridScales is not None:
+        out['gridScales'] = np.asarray(gridScales, dtype=float_t)
+        h = out['gridScales']
+    elif dimx is not None:
+        g = width[0] / (dimx - sb)
+        out['gridScale'] = np.asarray(g, dtype=float_t)
+        h = np.asarray([g] * len(lo), dtype=float_t)
+    elif dims is not None:
+        out['gridScales'] = width / (np.asarray(dims, dtype=float_t) - sb)
+        h = out['gridScales']
+    else:
+        raise ValueError("no grid scale")
+    ratios = (hi - lo) / h + sb
+    n = np.round(ratios)
+    out['dims'] = n
+    out['origin'] = lo + (ratios - n - sb) * h / 2
+    return out
+
+
+def point_from_index(origin, h, index, float_t=np.float32, to=False):
+    """DictIn.py:442-453 in float_t: x = (origin + h/2) + h i, and the
+    continuous index (x - (origin + h/2)) / h for to=True."""
+    origin = np.asarray(origin, dtype=float_t)
+    h = np.asarray(h, dtype=float_t)
+    start = origin + float_t(0.5) * h
+    index = np.asarray(index, dtype=float_t)
+    return (index - start) / h if to else start + h * index
+
+
+def centred_axis(a, b, n):
+    """The n cell centres of [a, b] as numpy.linspace(a, b, n, endpoint=False)
+    shifted by half its step (DictIn.py:97-107)."""
+    r, dr = np.linspace(a, b, n, endpoint=False, retstep=True)
+    return r + dr / 2
+
+
+def tile_expand(arr, shape_i, pad):
+    """Tiled layout of a cell-major array [..., *shape] -> [..., *shape_o,
+    *shape_i] (the bi-level layout of FiniteDifferences.py:164-195): pad the
+    grid up to whole tiles with `pad`, split every axis k into (tile index,
+    position in the tile), then gather the tile axes first.  Layout checker of
+    the device's agd_block_expand."""
+    d = len(shape_i)
+    lead, shape = arr.shape[:-d], arr.shape[-d:]
+    shape_o = [-(-s // t) for s, t in zip(shape, shape_i)]
+    full = np.full(lead + tuple(o * t for o, t in zip(shape_o, shape_i)), pad, dtype=arr.dtype)
+    full[(Ellipsis,) + tuple(slice(0, s) for s in shape)] = arr
+    split = full.reshape(lead + tuple(x for o, t in zip(shape_o, shape_i) for x in (o, t)))
+    nl = len(lead)
+    perm = list(range(nl)) + [nl + 2 * k for k in range(d)] + [nl + 2 * k + 1 for k in range(d)]
+    return split.transpose(perm)
+
+
+def tile_squeeze(arr, shape):
+    """Inverse of tile_expand: [..., *shape_o, *shape_i] -> [..., *shape]."""
+    d = len(shape)
+    nl = arr.ndim - 2 * d
+    perm = list(range(nl)) + [x for k in range(d) for x in (nl + k, nl + d + k)]
+    inter = arr.transpose(perm)
+    lead = arr.shape[:nl]
+    full = inter.reshape(lead + tuple(arr.shape[nl + k] * arr.shape[nl + d + k] for k in range(d)))
+    return full[(Ellipsis,) + tuple(slice(0, s) for s in shape)]
 
 
 def seed_indices(seeds, origin, h, dtype=np.float32):

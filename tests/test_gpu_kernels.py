@@ -55,14 +55,14 @@ def test_selling_stencils_bit_exact(O, d, dtype):
 
 @pytest.mark.parametrize("shape,shape_i", [((37, 50), (8, 8)), ((64, 64), (16, 16)),
                                            ((9, 13, 10), (4, 4, 4)), ((20, 18, 7), (8, 8, 4))])
-def test_block_expand_squeeze(shape, shape_i):
+def test_block_expand_squeeze(O, shape, shape_i):
     import torch
     from adaptivegriddiscretizations_amd import FiniteDifferences as fd
     L = _lib()
     rng = np.random.default_rng(0)
     a = rng.normal(size=(3,) + shape).astype(np.float32)
     src = torch.from_numpy(a).cuda()
-    ref = fd.block_expand(a, shape_i, mode='constant', constant_values=np.inf)
+    ref = O.tile_expand(a, shape_i, np.inf)
     dst = torch.empty(ref.size, dtype=torch.float32, device='cuda')
     L.check(L.lib.agd_block_expand(len(shape), L.i64(shape), L.i32(shape_i), 3, 4, src.data_ptr(), 1,
                                    dst.data_ptr(), float('inf'), None))
@@ -91,8 +91,7 @@ def test_prepare_geometry(O, d, model):
     w = (0.2 * rng.normal(size=(d,) + shape)).astype(np.float32)
     h = np.array([0.1, 0.2, 0.05][:d])
     P = O.make_problem(model, shape, h, metric=m.astype(np.float64), drift=w.astype(np.float64))
-    from adaptivegriddiscretizations_amd import FiniteDifferences as fd
-    ref = fd.block_expand(P.geom.reshape((-1,) + shape), shape_i, mode='constant', constant_values=np.inf)
+    ref = O.tile_expand(P.geom.reshape((-1,) + shape), shape_i, np.inf)
     ng = ref.shape[0]
     out = torch.empty(ref.size, dtype=torch.float32, device='cuda')
     md, wd = torch.from_numpy(m).cuda(), torch.from_numpy(w).cuda()

@@ -57,16 +57,104 @@ def test_cpu_modes_are_not_provided():
 
 
 @pytest.mark.parametrize("shape,shape_i", [((10, 13), (4, 4)), ((9, 5, 7), (4, 4, 4))])
-def test_block_expand_squeeze_and_tiled_index(shape, shape_i):
+def test_tiled_layout_and_tiled_index(O, shape, shape_i):
     rng = np.random.default_rng(0)
     a = rng.normal(size=(2,) + shape)
-    b = fd.block_expand(a, shape_i, mode='constant', constant_values=np.inf)
+    b = O.tile_expand(a, shape_i, np.inf)
     shape_o = tuple(fd.round_up_ratio(shape, shape_i))
     assert b.shape == (2,) + shape_o + tuple(shape_i)
-    np.testing.assert_array_equal(fd.block_squeeze(b, shape), a)
+    np.testing.assert_array_equal(O.tile_squeeze(b, shape), a)
+    # the padding holds the pad value and nothing else
+    assert np.isinf(b).sum() == 2 * (int(np.prod(b.shape[1:])) - int(np.prod(shape)))
     idx = np.stack(np.unravel_index(np.arange(int(np.prod(shape))), shape), axis=-1)
     t = fd.tiled_index(idx, shape_i, shape_o)
     np.testing.assert_array_equal(b[0].reshape(-1)[t], a[0].reshape(-1))
+    # closed form: cell x sits in tile x // shape_i at position x % shape_i
+    x = idx[len(idx) // 2]
+    tile, pos = tuple(x // np.array(shape_i)), tuple(x % np.array(shape_i))
+    assert b[(0,) + tile + pos] == a[(0,) + tuple(x)]
+
+
+def test_oracle_set_rect_closed_forms(O):
+    """The oracle's SetRect arithmetic on closed-form cases: [-1,1]^2 with
+    512 cells has h = 2^-8 and origin -1; [0,1] x [0,2] sampled with 51
+    points on the first axis, faces included, has 51 x 101 points, h = 1/50,
+    and the box of its cells starts half a cell below 0."""
+    r = O.set_rect([[-1, 1], [-1, 1]], dims=[512, 512])
+    assert r['gridScales'].tolist() == [2. ** -8] * 2
+    assert r['dims'].tolist() == [512, 512] and r['origin'].tolist() == [-1., -1.]
+    r = O.set_rect([[0, 1], [0, 2]], sample_boundary=True, dimx=51)
+    assert r['gridScale'] == np.float32(1 / 50) and r['dims'].tolist() == [51, 101]
+    assert np.all(np.abs(r['origin'] + np.float32(0.01)) <= 1e-8)
+    r = O.set_rect([[0, 3]], gridScale=0.5)
+    assert r['dims'].tolist() == [6] and r['origin'].tolist() == [0.]
+    # a width that is not a whole number of cells: the grid is re-centred
+    r = O.set_rect([[0, 1.3125]], gridScale=0.25)     # 5.25 cells
+    assert r['dims'].tolist() == [5] and r['origin'].tolist() == [0.03125]
+    r = O.set_rect([[0, 1.375]], gridScale=0.25)      # 5.5 cells: a tie, rounded to even
+    assert r['dims'].tolist() == [6] and r['origin'].tolist() == [-0.0625]
+    # cell centres of [a, b]
+    np.testing.assert_array_equal(O.centred_axis(0., 1., 4), [0.125, 0.375, 0.625, 0.875])
+    assert O.point_from_index([-1., -1.], [2. ** -8] * 2, [[255.5, 255.5]]).tolist() == [[0., 0.]]
+
+
+def test_set_rect_and_point_maps_match_oracle_bitwise(O):
+    """dictIn's SetRect / PointFromIndex / Axes reproduce the float32
+    arithmetic of DictIn.py:97-107,404-453 to the bit, on random boxes and
+    every way of giving the grid scale."""
+    rng = np.random.default_rng(5)
+    checked = 0
+    for trial in range(600):
+        d = int(rng.integers(1, 4))
+        lo = rng.uniform(-5, 5, d).round(int(rng.integers(0, 4)))
+        hi = lo + rng.uniform(0.1, 10, d).round(int(rng.integers(0, 4)))
+        sides = np.stack([lo, hi], 1).tolist()
+        sb = bool(rng.integers(0, 2))
+        kind = int(rng.integers(0, 4))
+        kw = [{'gridScale': float(rng.uniform(0.001, 0.5))},
+              {'gridScales': rng.uniform(0.001, 0.5, d).tolist()},
+              {'dimx': int(rng.integers(2, 3000))},
+              {'dims': rng.integers(2, 3000, d).tolist()}][kind]
+        hf = Eikonal.dictIn({'model': f'Isotropic{d}'})
+        with np.errstate(all='ignore'):
+            try:
+                hf.SetRect(sides, sampleBoundary=sb, **kw)
+            except ValueError:
+                continue
+        ref = O.set_rect(sides, np.float32, sb, **kw)
+        for k, v in ref.items():
+            got = np.asarray(hf[k])
+            assert got.dtype == v.dtype and np.array_equal(got, v), (k, got, v)
+        idx = rng.integers(0, 50, (4, d))
+        p = hf.PointFromIndex(idx)
+        np.testing.assert_array_equal(p, O.point_from_index(hf['origin'], hf.gridScales, idx))
+        np.testing.assert_array_equal(hf.PointFromIndex(p, to=True),
+                                      O.point_from_index(hf['origin'], hf.gridScales, p, to=True))
+        if trial % 10 == 0:
+            lo_c, hi_c = hf.corners
+            for ax, a, b, n in zip(hf.Axes(), lo_c, hi_c, hf.shape):
+                np.testing.assert_array_equal(ax, np.asarray(O.centred_axis(a, b, n), np.float32))
+        checked += 1
+    assert checked > 500
+
+
+def test_aliases_copy_and_run_join():
+    hfmIn = Eikonal.dictIn({'model': 'Isotropic2', 'tip': [0.1, 0.2]})
+    assert hfmIn['tips'].shape == (1, 2) and 'tip' not in hfmIn.store
+    del hfmIn['tip']
+    assert 'tips' not in hfmIn
+    hfmIn['seeds'] = [[0., 0.], [1., 1.]]
+    with pytest.raises(ValueError):
+        hfmIn['seed']
+    c = hfmIn.copy()
+    c['model'] = 'Isotropic3'
+    assert hfmIn['model'] == 'Isotropic2'
+    with pytest.raises(ValueError):
+        Eikonal.dictIn({'mode': 'gpu', 'float_t': np.float32})['float_t'] = np.float64
+    out = Eikonal.dictOut({'geodesics': [np.zeros((2, 3))]})
+    assert out['geodesic'].shape == (2, 3)
+    out['geodesic'] = np.ones(2)
+    assert len(out['geodesics']) == 1
 
 
 def test_riemann_dual_and_with_costs():
