@@ -712,7 +712,7 @@ static int64_t jacobi(const Problem& P, double atol, double rtol, int64_t niterm
 
 extern "C" {
 
-int orc_version() { return 4; }
+int orc_version() { return 5; }
 
 void orc_set_order(int order, double threshold) {
   g_order = order;
@@ -1045,6 +1045,20 @@ void orc_wall_distance(int d, const int64_t* shape, const int* periodic, const u
     cur.swap(nxt);
   }
   for (int64_t n = 0; n < G.N; ++n) dist[n] = (uint8_t)(cur[n] > bound ? 255 : cur[n]);
+}
+
+// Visibility of the segments [x_i, x_i + e_i] (n queries, cell-major grid
+// positions and offsets) under the walk of Walls.h:18-50.
+void orc_visible(int d, const int64_t* shape, const int* periodic, const uint8_t* wall_dist,
+                 int64_t n, const int64_t* x, const int32_t* e, uint8_t* out) {
+  Grid G;
+  G.init(d, shape, periodic);
+  for (int64_t q = 0; q < n; ++q) {
+    int64_t xx[3] = {0, 0, 0};
+    int ee[3] = {0, 0, 0};
+    for (int k = 0; k < d; ++k) { xx[k] = x[q * d + k]; ee[k] = e[q * d + k]; }
+    out[q] = visible(G, xx, ee, wall_dist) ? 1 : 0;
+  }
 }
 
 }  // extern "C"

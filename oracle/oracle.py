@@ -63,6 +63,7 @@ def _load():
     lib.orc_set_order.argtypes = [c_int, c_double]
     lib.orc_set_factor.argtypes = [c_double, P, P]
     lib.orc_wall_distance.argtypes = [c_int, P, P, P, c_int, P]
+    lib.orc_visible.argtypes = [c_int, P, P, P, c_int64, P, P, P]
     return lib
 
 
@@ -205,6 +206,21 @@ class Problem:
         lib().orc_forward_ad(self.N, self.nsym, _p(uu), _p(_c(flow['weights'], np.float64)),
                              _p(_c(flow['indices'], np.int64)), nrhs, _p(r), _p(du))
         return np.moveaxis(du.reshape((nrhs,) + self.shape), 0, -1)
+
+
+def visible(wall_dist, x, e, periodic=None):
+    """The oracle's wall visibility walk (Walls.h:18-50) for the segments
+    [x_q, x_q + e_q]: x, e integer arrays [n, d] on the grid of wall_dist."""
+    wall_dist = _c(wall_dist, np.uint8)
+    shape = wall_dist.shape
+    d = len(shape)
+    x = _c(np.asarray(x).reshape(-1, d), np.int64)
+    e = _c(np.asarray(e).reshape(-1, d), np.int32)
+    out = np.zeros(len(x), dtype=np.uint8)
+    per = _c(list(periodic or [0] * d) + [0] * (3 - d), np.int32)
+    lib().orc_visible(d, _p(_c(list(shape) + [1] * (3 - d), np.int64)), _p(per), _p(wall_dist),
+                      len(x), _p(x), _p(e), _p(out))
+    return out.astype(bool)
 
 
 def wall_distance(walls, bound=10, periodic=None):

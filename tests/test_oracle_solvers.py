@@ -250,3 +250,115 @@ def test_forward_and_reverse_are_transposes(O):
     du = P.forward_ad(u, fl, r)
     lam = P.adjoint(u, fl, s)
     assert np.sum(s * du) == pytest.approx(np.sum(lam * r), rel=1e-12)
+
+
+def _round_half_away(n, d):
+    """n / d rounded to the nearest integer, halves away from zero (d > 0)."""
+    q = (2 * abs(n) + d) // (2 * d)
+    return q if n >= 0 else -q
+
+
+def _visible_brute(walls, x, e, periodic):
+    """Definition of visibility: no sample x + round(e t / |e|_inf),
+    t = 1 .. |e|_inf, lies on a wall (samples outside a non-periodic grid are
+    free)."""
+    linf = max(abs(c) for c in e)
+    shape = walls.shape
+    for t in range(1, linf + 1):
+        y = []
+        inside = True
+        for k, (xk, ek) in enumerate(zip(x, e)):
+            yk = xk + _round_half_away(ek * t, linf)
+            if periodic[k]:
+                yk %= shape[k]
+            elif not 0 <= yk < shape[k]:
+                inside = False
+            y.append(yk)
+        if inside and walls[tuple(y)]:
+            return False
+    return True
+
+
+@pytest.mark.parametrize("d,periodic", [(2, (0, 0)), (2, (1, 0)), (3, (0, 0, 0)), (3, (0, 1, 1))])
+def test_wall_visibility_walk_matches_its_definition(O, d, periodic):
+    """The oracle's visibility walk (Walls.h:18-50), which skips ahead using
+    the saturated wall distance, against the plain definition on random wall
+    masks, for every primitive offset with |e|_inf <= 5 (the Selling offsets
+    of the anisotropic test metrics are among them).  With the wall distance
+    computed up to a bound above every |e|_1, the walk's early exits are exact;
+    with the default bound 10 it is checked on the offsets of |e|_1 <= 10."""
+    import itertools
+    import math
+    rng = np.random.default_rng(7 + d)
+    shape = (23, 19) if d == 2 else (11, 9, 10)
+    walls = rng.random(shape) < (0.12 if d == 2 else 0.08)
+    r = 5
+    offsets = [e for e in itertools.product(range(-r, r + 1), repeat=d)
+               if any(e) and math.gcd(*[abs(c) for c in e]) == 1]
+    cells = [x for x in itertools.product(*[range(s) for s in shape]) if not walls[x]]
+    cells = [cells[i] for i in rng.choice(len(cells), size=min(len(cells), 60), replace=False)]
+    for bound, emax in ((16, 15), (10, 10)):
+        dist = O.wall_distance(walls.astype(np.uint8), bound=bound, periodic=list(periodic))
+        xs, es = [], []
+        for x in cells:
+            for e in offsets:
+                if sum(abs(c) for c in e) <= emax:
+                    xs.append(x)
+                    es.append(e)
+        got = O.visible(dist, np.array(xs), np.array(es), periodic=list(periodic))
+        ref = np.array([_visible_brute(walls, x, e, periodic) for x, e in zip(xs, es)])
+        assert (got == ref).all(), [(xs[i], es[i]) for i in np.flatnonzero(got != ref)[:5]]
+        # both outcomes occur: the check is not vacuous
+        assert 0.05 < ref.mean() < 0.95
+
+
+def test_periodic_axis_equals_replicated_seeds(O):
+    """Periodic wrap of the oracle's grid (Grid.h:68-80): on a grid periodic
+    along axis 0 with period n, the fast-marching solution equals the middle
+    third of the non-periodic solution on 3n cells with the seeds replicated
+    at s, s + n, s + 2n (every upwind chain of a middle cell stays within half
+    a period of it, away from the enlarged grid's border).  Riemannian
+    anisotropic metric, so that long Selling offsets cross the wrap."""
+    rng = np.random.default_rng(3)
+    n, m = 24, 17
+    h = np.array([1. / n, 1. / m])
+    A = rng.normal(size=(2, 2))
+    M = A @ A.T + 0.3 * np.eye(2)
+    metric = np.broadcast_to(M[:, :, None, None], (2, 2, n, m)).copy()
+    metric *= (1. + 0.3 * np.sin(2 * np.pi * np.arange(n) / n))[None, None, :, None]
+    seeds = [(3, 5), (17, 12)]
+    P = O.make_problem('Riemann', (n, m), h, metric=metric, seeds_idx=np.array(seeds),
+                       periodic=[1, 0])
+    u = P.solve_fm()
+    big = np.concatenate([metric] * 3, axis=2)
+    seeds3 = np.array([(s0 + k * n, s1) for k in range(3) for s0, s1 in seeds])
+    P3 = O.make_problem('Riemann', (3 * n, m), h, metric=big, seeds_idx=seeds3)
+    u3 = P3.solve_fm()
+    np.testing.assert_allclose(u, u3[n:2 * n], rtol=0, atol=1e-12)
+    # and the wrap is used: the non-periodic solution differs
+    Pn = O.make_problem('Riemann', (n, m), h, metric=metric, seeds_idx=np.array(seeds))
+    assert np.max(np.abs(Pn.solve_fm() - u)) > 1e-2
+
+
+def test_periodic_isotropic_torus_distance(O):
+    """Constant isotropic metric on a grid periodic along both axes: the
+    solution converges to the torus distance, the minimum over the seed's
+    periodic images of |x - s - k L|, at the O(h log 1/h) rate of a point
+    source (errors 0.030, 0.018, 0.011 for n = 32, 64, 128), while the
+    distance without the images stays O(1) away."""
+    errs = []
+    for n in (32, 64, 128):
+        h = 1. / n
+        s = (n // 4, n // 3)
+        P = O.make_problem('Isotropic', (n, n), (h, h), seeds_idx=np.array([s]), periodic=[1, 1])
+        u = P.solve_fm()
+        x = (np.arange(n) + 0.5) * h
+        X, Y = np.meshgrid(x, x, indexing='ij')
+        sx, sy = (s[0] + 0.5) * h, (s[1] + 0.5) * h
+        dx = np.abs(X - sx)
+        dy = np.abs(Y - sy)
+        exact = np.hypot(np.minimum(dx, 1 - dx), np.minimum(dy, 1 - dy))
+        errs.append(np.max(np.abs(u - exact)))
+        assert np.max(np.abs(np.hypot(dx, dy) - exact)) > 0.4
+    assert errs[1] < 0.7 * errs[0] and errs[2] < 0.7 * errs[1]
+    assert errs[2] < 0.012
