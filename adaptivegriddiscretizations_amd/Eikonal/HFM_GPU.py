@@ -616,9 +616,20 @@ class Interface:
         io.seeds = ptr(self.seeds_t)
         io.trigger = ptr(self.trigger_t)
         io.wall_dist = ptr(self.wallDist_t) if self.wallDist_t is not None else None
-        io.scheme_weights = ptr(getattr(self, 'scheme_w_t', None))
-        io.scheme_codes = ptr(getattr(self, 'scheme_c_t', None))
+        sw, sc = getattr(self, 'scheme_w_t', None), getattr(self, 'scheme_c_t', None)
+        io.scheme_weights = ptr(sw)
+        io.scheme_codes = ptr(sc)
+        io.scheme_weights_len = sw.numel() if sw is not None else 0
+        io.scheme_codes_len = sc.numel() if sc is not None else 0
         return io
+
+    def solver_handle(self):
+        """The solver handle (worklist workspace, cached pass graph) of this
+        run, bound to its stream; created on first use."""
+        h = getattr(self, '_solver_handle', None)
+        if h is None:
+            h = self._solver_handle = _lib.Solver(self.stream, self._desc())
+        return h
 
     # ------------------------------------------------------------ solve
     def Solve(self):
@@ -628,7 +639,7 @@ class Interface:
         desc, io, st = self._desc(), self._io(), _lib.EikonalStats()
         torch.cuda.synchronize()
         t0 = time.time()
-        status = _lib.lib.agd_eikonal_solve(desc, io, st, self.stream)
+        status = self.solver_handle().solve(desc, io, st)
         wall = time.time() - t0
         self.stats['eikonal'] = {**st.as_dict(), 'time': wall, 'gpu_time': st.gpu_ms * 1e-3}
         if self.verbosity >= 1:

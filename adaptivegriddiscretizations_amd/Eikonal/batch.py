@@ -87,6 +87,7 @@ class Batch:
             self.inst[i].atol = it.atol
             self.inst[i].rtol = it.rtol
         self.stream = stream_handle()
+        self.solver = _lib.Solver(self.stream)  # worklist workspace, cached pass graph
         self.ncells = int(sum(int(np.prod(it.shape)) for it in self.ifaces))
 
     def desc(self):
@@ -107,7 +108,7 @@ class Batch:
         io.trigger = ptr(self.trigger_t)
         st = L.EikonalStats()
         d = self.desc() if desc is None else desc
-        status = L.lib.agd_eikonal_solve_batched(d, len(self.ifaces), self.inst, io, st, self.stream)
+        status = self.solver.solve_batched(d, len(self.ifaces), self.inst, io, st)
         if status == L.AGD_NOCONV and not all(it.raiseOnNonConvergence for it in self.ifaces):
             self.ifaces[0].Warn("batched solve did not converge")
         else:
@@ -120,6 +121,12 @@ class Batch:
         for i, it in enumerate(self.ifaces):
             it.values_t.copy_(self.values_t[self.offsets[i]:self.offsets[i + 1]])
             it._values = None
+
+    def refresh_values(self):
+        """Re-gather the batch's initial values from the problems' own
+        buffers (after they were changed in place, e.g. repaired starts)."""
+        self.values_t.copy_(torch.cat([it.values_t for it in self.ifaces]))
+        self.values0_t.copy_(self.values_t)
 
     def values(self, i):
         """Cell-major values of problem i."""

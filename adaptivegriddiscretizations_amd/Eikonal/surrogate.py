@@ -327,14 +327,15 @@ def solve_batched(metric, seeds, atol, rtol, values=None, batch=16, check=False)
     """Solves of every field in batches of `batch` problems (warm when
     `values` is given: every tile starts active).  Returns the values
     (count, n, n), the summed batch GPU time (ms), the passes and, with
-    `check`, the number of cells per problem left below their scheme update
-    (agd_eikonal_check)."""
-    import ctypes
+    `check`, the number of cells per problem whose solved values lie below
+    their scheme update (agd_solver_check; 0 after a solve from a
+    supersolution)."""
     from . import batch as B
     count, _, n, _ = metric.shape
     out = torch.empty((count, n, n), dtype=torch.float32, device=metric.device)
     ms, passes = 0., 0
     bad = []
+    per_batch = []
     for b0 in range(0, count, batch):
         hfms = []
         for i in range(b0, min(count, b0 + batch)):
@@ -345,39 +346,62 @@ def solve_batched(metric, seeds, atol, rtol, values=None, batch=16, check=False)
         b = B.Batch(hfms)
         st = b.solve(reset=False)
         for j, it in enumerate(b.ifaces):
-            out[b0 + j] = b.values(j)
+            out[b0 + j] = b.values(j)   # also copies the solved values into it.values_t
             if check:
-                nv = ctypes.c_int64(0)
-                _check(_lib.lib.agd_eikonal_check(it._desc(), it._io(), atol, rtol,
-                                                  ctypes.byref(nv), it.stream))
-                bad.append(int(nv.value))
+                bad.append(b.solver.check_residual(it._desc(), it._io(), atol, rtol))
         ms += float(st.gpu_ms)
         passes += int(st.launches)
+        per_batch.append(int(st.launches))
+    solve_batched.last_batch_passes = per_batch
     return (out, ms, passes, bad) if check else (out, ms, passes)
 
 
-def solve_warm(net, metric, seeds, atol, rtol, batch=16, inflate=1.05):
-    """Warm-started solves judged by the solver: problems with cells left
-    below their scheme update are solved again from a cold start.  Returns
-    values, total GPU ms (network prediction included), passes, fallback
-    count."""
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+def solve_warm(net, metric, seeds, atol, rtol, batch=16, inflate=1.05, max_rounds=8):
+    """Warm-started solves with the solver as judge before solving: every
+    field's start (the inflated prediction) is repaired into a supersolution
+    of the scheme by agd_solver_repair (cells below their scheme update are
+    raised to +inf, round after round, each round one pass over the grid);
+    a field still violating after max_rounds rounds starts cold (+inf), so a
+    bad prediction costs its repair rounds and nothing else.  Every tile
+    starts active.  Returns the values and a dict: GPU ms of the prediction,
+    the repair and the batched solves, solve passes (summed over batches),
+    repair rounds per field, cold fallbacks."""
+    from . import batch as B
+    count, _, n, _ = metric.shape
+    ev = lambda: torch.cuda.Event(enable_timing=True)
+    e0, e1 = ev(), ev()
     e0.record()
     warm = warm_values(net, metric, seeds, inflate)
     e1.record()
     e1.synchronize()
-    ms_pred = e0.elapsed_time(e1)
-    u, ms, passes, bad = solve_batched(metric, seeds, atol, rtol, values=warm, batch=batch,
-                                       check=True)
-    ms += ms_pred
-    redo = [i for i, v in enumerate(bad) if v > 0]
-    if redo:
-        idx = torch.as_tensor(redo, device=metric.device)
-        u2, ms2, p2 = solve_batched(metric[idx], seeds[idx], atol, rtol, batch=batch)
-        u[idx] = u2
-        ms += ms2
-        passes += p2
-    return u, ms, passes, len(redo)
+    rep = {'pred_ms': e0.elapsed_time(e1), 'repair_ms': 0., 'solve_ms': 0., 'passes': 0,
+           'batch_passes': [], 'rounds': [], 'fallbacks': 0}
+    out = torch.empty((count, n, n), dtype=torch.float32, device=metric.device)
+    inf = float('inf')
+    for b0 in range(0, count, batch):
+        hfms = [dict_in(metric[i], seeds[i], n, atol=atol, rtol=rtol, values=warm[i],
+                        warmStartAllTiles=True) for i in range(b0, min(count, b0 + batch))]
+        b = B.Batch(hfms)
+        r0, r1 = ev(), ev()
+        r0.record()
+        for it in b.ifaces:
+            left, rounds = b.solver.repair(it._desc(), it._io(), atol, rtol, max_rounds)
+            rep['rounds'].append(rounds)
+            if left > 0:
+                it.values_t.fill_(inf)
+                rep['fallbacks'] += 1
+        b.refresh_values()
+        r1.record()
+        r1.synchronize()
+        rep['repair_ms'] += r0.elapsed_time(r1)
+        st = b.solve(reset=False)
+        for j in range(len(b.ifaces)):
+            out[b0 + j] = b.values(j)
+        rep['solve_ms'] += float(st.gpu_ms)
+        rep['passes'] += int(st.launches)
+        rep['batch_passes'].append(int(st.launches))
+    rep['total_ms'] = rep['pred_ms'] + rep['repair_ms'] + rep['solve_ms']
+    return out, rep
 
 
 def build_dataset(count, n=1024, factor=4, seed=0, chunk=256, atol=1e-6, rtol=1e-6, log=None):

@@ -44,6 +44,7 @@ class EikonalIO(ctypes.Structure):
         ("values", c_void_p), ("geom", c_void_p), ("rhs", c_void_p),
         ("seeds", c_void_p), ("trigger", c_void_p), ("wall_dist", c_void_p),
         ("scheme_weights", c_void_p), ("scheme_codes", c_void_p),
+        ("scheme_weights_len", c_int64), ("scheme_codes_len", c_int64),
     ]
 
 
@@ -84,6 +85,17 @@ SIGNATURES = {
     "agd_version": (c_int32, []),
     "agd_tile_supported": (c_int32, [c_int32, c_int32, c_int32_p]),
     "agd_device_count": (c_int32, [c_int32_p]),
+    "agd_solver_create": (c_int32, [POINTER(EikonalDesc), c_void_p, POINTER(c_void_p)]),
+    "agd_solver_destroy": (c_int32, [c_void_p]),
+    "agd_solver_solve": (c_int32, [c_void_p, POINTER(EikonalDesc), POINTER(EikonalIO),
+                                   POINTER(EikonalStats)]),
+    "agd_solver_solve_batched": (c_int32, [c_void_p, POINTER(EikonalDesc), c_int32,
+                                           POINTER(BatchInstance), POINTER(EikonalIO),
+                                           POINTER(EikonalStats)]),
+    "agd_solver_check": (c_int32, [c_void_p, POINTER(EikonalDesc), POINTER(EikonalIO), c_double,
+                                   c_double, POINTER(c_int64)]),
+    "agd_solver_repair": (c_int32, [c_void_p, POINTER(EikonalDesc), POINTER(EikonalIO), c_double,
+                                    c_double, c_int32, POINTER(c_int64), POINTER(c_int32)]),
     "agd_eikonal_solve": (c_int32, [POINTER(EikonalDesc), POINTER(EikonalIO),
                                     POINTER(EikonalStats), c_void_p]),
     "agd_eikonal_flow": (c_int32, [POINTER(EikonalDesc), POINTER(EikonalIO), POINTER(FlowIO),
@@ -93,6 +105,7 @@ SIGNATURES = {
     "agd_eikonal_check": (c_int32, [POINTER(EikonalDesc), POINTER(EikonalIO), c_double, c_double,
                                     POINTER(c_int64), c_void_p]),
     "agd_scheme_supported": (c_int32, [c_int32, c_int32, c_int32_p]),
+    "agd_scheme_lengths": (c_int32, [c_int32, c_int32, c_int32, c_int64_p, c_int64_p]),
     "agd_eikonal_scheme": (c_int32, [POINTER(EikonalDesc), POINTER(EikonalIO), c_void_p]),
     "agd_eikonal_adjoint": (c_int32, [POINTER(EikonalDesc), c_void_p, c_void_p, c_void_p,
                                       c_void_p, c_int32, c_void_p, c_void_p, c_double,
@@ -207,3 +220,52 @@ def i64(seq):
 def i32(seq):
     arr = (c_int32 * 3)(*([int(s) for s in seq] + [1] * (3 - len(seq))))
     return arr
+
+
+class Solver:
+    """A solver handle (agd_solver): the worklist workspace of the solves run
+    through it, bound to the current device and to `stream` (a hipStream_t
+    as an int, 0 = the null stream).  Handles share no state, so solves
+    through different handles may run concurrently on different streams."""
+
+    def __init__(self, stream=0, desc=None):
+        h = c_void_p()
+        check(lib.agd_solver_create(desc, stream, ctypes.byref(h)))
+        self._h = h
+        self.stream = stream
+
+    @property
+    def handle(self):
+        if self._h is None:
+            raise ValueError("the solver handle was closed")
+        return self._h
+
+    def solve(self, desc, io, stats):
+        return lib.agd_solver_solve(self.handle, desc, io, stats)
+
+    def solve_batched(self, desc, ninst, instances, io, stats):
+        return lib.agd_solver_solve_batched(self.handle, desc, ninst, instances, io, stats)
+
+    def check_residual(self, desc, io, atol, rtol):
+        n = c_int64(0)
+        check(lib.agd_solver_check(self.handle, desc, io, atol, rtol, ctypes.byref(n)))
+        return int(n.value)
+
+    def repair(self, desc, io, atol, rtol, max_rounds):
+        """Raise the cells of io.values below their scheme update to +inf,
+        round after round; returns (violations left, rounds run)."""
+        left, rounds = c_int64(0), c_int32(0)
+        check(lib.agd_solver_repair(self.handle, desc, io, atol, rtol, max_rounds,
+                                    ctypes.byref(left), ctypes.byref(rounds)))
+        return int(left.value), int(rounds.value)
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            lib.agd_solver_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -25,6 +25,9 @@
 
 #include <hipcub/hipcub.hpp>
 
+#include <cstring>
+#include <vector>
+
 #include "common.hpp"
 #include "device.hpp"
 #include "worklist.hpp"
@@ -333,13 +336,15 @@ static int linear_impl(bool forward, const agd_eikonal_desc* d, const S* values,
 
   // ---- 2. tile Gauss-Seidel, one right-hand side at a time
   const int blocks = blocks_for(g.ntiles, ls.ni);
+  Workspace ws;  // this call's own worklist, released on return
   int64_t niter = 0, visits = 0, passes = 0;
   float ms = 0.f;
   double kernel_ms = 0.;
   bool converged = true;
   int status = AGD_OK;
   for (int r = 0; r < nrhs && status == AGD_OK; ++r) {
-    LinArgs<S> a{};
+    LinArgs<S> a;
+    std::memset(&a, 0, sizeof a);  // zeroed padding: the bytes key the graph cache
     a.lam = lam + int64_t(r) * N;
     a.rhs = sens + int64_t(r) * N;
     a.diag = wsum;
@@ -368,12 +373,16 @@ static int linear_impl(bool forward, const agd_eikonal_desc* d, const S* values,
     };
     auto launch = [&](int32_t koff, hipStream_t s, bool phase) {
       a.w.koff = koff;
-    a.w.phase_from_koff = phase ? 1 : 0;
+      a.w.phase_from_koff = phase ? 1 : 0;
       hipLaunchKernelGGL(ls.update, dim3(blocks), dim3(ls.ni), 0, s, a);
     };
+    auto key = [&](std::vector<unsigned char>& k) {
+      key_append(k, ls.update);
+      key_append(k, a);
+    };
     WorklistResult res;
-    status = run_worklist(g.ntiles, blocks, d->nitermax_o, d->check_interval, stream, init, launch,
-                          res, d->time_launches != 0, d->use_graph != 0);
+    status = run_worklist(ws, g.ntiles, blocks, d->nitermax_o, d->check_interval, stream, init,
+                          launch, key, res, d->time_launches != 0, d->use_graph != 0);
     kernel_ms += res.kernel_ms;
     niter += res.niter;
     visits += res.visits;

@@ -15,9 +15,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstring>
 #include <mutex>
+#include <new>
+#include <string>
 #include <type_traits>
-#include <unordered_map>
 #include <vector>
 
 #include "common.hpp"
@@ -26,7 +28,7 @@
 #include "worklist.hpp"
 
 static_assert(sizeof(agd_eikonal_desc) == 248, "agd_eikonal_desc layout changed");
-static_assert(sizeof(agd_eikonal_io) == 64, "agd_eikonal_io layout changed");
+static_assert(sizeof(agd_eikonal_io) == 80, "agd_eikonal_io layout changed");
 
 namespace agd {
 
@@ -310,6 +312,28 @@ static int64_t tti_codes2_offset(const agd_eikonal_desc* d) {
   return int64_t(kTTIMix) * 2 * nsym + 1;
 }
 
+// Components of the precomputed scheme's weight and code buffers (the io
+// struct's layout comment in agd_hip.h).
+static void scheme_components(int model, int ndim, int order, int64_t& wc, int64_t& cc) {
+  const int64_t nsym = nsym_of(model, ndim);
+  const int64_t nmix = nmix_of(model);
+  wc = nmix * nsym;
+  cc = nmix * 2 * nsym;
+  if (model == AGD_TTI) cc += 1 + (order == 2 ? nmix * 2 * nsym : 0);
+}
+
+// The precomputed scheme buffers hold what the descriptor's layout needs.
+static int check_scheme_lengths(const agd_eikonal_desc* d, const agd_eikonal_io* io, int64_t size_tot) {
+  int64_t wc = 0, cc = 0;
+  scheme_components(d->model, d->ndim, d->order, wc, cc);
+  AGD_REQUIRE(io->scheme_weights_len >= wc * size_tot,
+              "scheme_weights_len is shorter than the scheme layout of this model and order");
+  AGD_REQUIRE(io->scheme_codes_len >= cc * size_tot,
+              "scheme_codes_len is shorter than the scheme layout of this model and order "
+              "(TTI with order 2 also stores the further-neighbour codes)");
+  return AGD_OK;
+}
+
 static GridDesc make_grid(const agd_eikonal_desc* d) {
   GridDesc g{};
   g.ntiles = 1;
@@ -329,65 +353,144 @@ static GridDesc make_grid(const agd_eikonal_desc* d) {
   return g;
 }
 
-// Solver workspace, cached per device and grown on demand.
-static std::mutex g_ws_mutex;
-static std::unordered_map<int, Workspace> g_ws;
+// ------------------------------------------------------------------ workspace
+template <class T> static void free_dev(T*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
 
-int get_workspace(int64_t ntiles, int64_t shard_cap, int64_t nhist, Workspace*& out) {
-  int dev = 0;
-  AGD_HIP_TRY(hipGetDevice(&dev));
-  std::lock_guard<std::mutex> lock(g_ws_mutex);
-  Workspace& w = g_ws[dev];
-  if (!w.ev0) {
-    AGD_HIP_TRY(hipEventCreate(&w.ev0));
-    AGD_HIP_TRY(hipEventCreate(&w.ev1));
-    AGD_HIP_TRY(hipHostMalloc((void**)&w.host_counts, 2 * kRow * sizeof(int32_t), hipHostMallocDefault));
-    AGD_HIP_TRY(hipEventCreateWithFlags(&w.chunk_ev[0], hipEventDisableTiming));
-    AGD_HIP_TRY(hipEventCreateWithFlags(&w.chunk_ev[1], hipEventDisableTiming));
-    AGD_HIP_TRY(hipStreamCreateWithFlags(&w.capture_stream, hipStreamNonBlocking));
-    AGD_HIP_TRY(hipMalloc((void**)&w.counts, kCountsLen * sizeof(int32_t)));
-    AGD_HIP_TRY(hipMalloc((void**)&w.check_count, 64));
+void Workspace::release() {
+  if (!fixed) {
+    free_dev(stamp);
+    free_dev(needs);
+    free_dev(pending);
+    free_dev(bab);
+    free_dev(lists);
+    free_dev(counts);
+    free_dev(hist);
+    free_dev(score);
+    free_dev(check_count);
+    free_dev(inst);
+    free_dev(tile_inst);
+    if (host_counts) (void)hipHostFree(host_counts);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    for (hipEvent_t e : {ev0, ev1, chunk_ev[0], chunk_ev[1]})
+      if (e) (void)hipEventDestroy(e);
+    if (capture_stream) (void)hipStreamDestroy(capture_stream);
   }
-  if (w.tile_cap < ntiles) {
-    if (w.stamp) {
-      AGD_HIP_TRY(hipFree(w.stamp));
-      AGD_HIP_TRY(hipFree(w.score));
-      AGD_HIP_TRY(hipFree(w.needs));
-      AGD_HIP_TRY(hipFree(w.pending));
-    }
+  stamp = needs = bab = lists = counts = hist = score = check_count = tile_inst = nullptr;
+  pending = nullptr;
+  inst = nullptr;
+  host_counts = nullptr;
+  exec = nullptr;
+  exec_key.clear();
+  ev0 = ev1 = chunk_ev[0] = chunk_ev[1] = nullptr;
+  capture_stream = nullptr;
+  tile_cap = shard_cap = hist_cap = inst_cap = tinst_cap = 0;
+  fixed = false;
+}
+
+int Workspace::ensure(int64_t ntiles, int64_t scap, int64_t nhist) {
+  if (fixed) {
+    AGD_REQUIRE(ntiles <= tile_cap && scap <= shard_cap && nhist <= hist_cap,
+                "the solve does not fit the library's fixed device region");
+    return AGD_OK;
+  }
+  if (!ev0) {
+    AGD_HIP_TRY(hipEventCreate(&ev0));
+    AGD_HIP_TRY(hipEventCreate(&ev1));
+    AGD_HIP_TRY(hipHostMalloc((void**)&host_counts, 2 * kRow * sizeof(int32_t), hipHostMallocDefault));
+    AGD_HIP_TRY(hipEventCreateWithFlags(&chunk_ev[0], hipEventDisableTiming));
+    AGD_HIP_TRY(hipEventCreateWithFlags(&chunk_ev[1], hipEventDisableTiming));
+    AGD_HIP_TRY(hipStreamCreateWithFlags(&capture_stream, hipStreamNonBlocking));
+    AGD_HIP_TRY(hipMalloc((void**)&counts, kCountsAlloc * sizeof(int32_t)));
+    AGD_HIP_TRY(hipMalloc((void**)&check_count, 64));
+    AGD_HIP_TRY(hipMalloc((void**)&bab, kBabLen * sizeof(int32_t)));
+  }
+  if (tile_cap < ntiles) {
+    free_dev(stamp);
+    free_dev(score);
+    free_dev(needs);
+    free_dev(pending);
     const int64_t cap = std::max<int64_t>(ntiles, 1024);
-    AGD_HIP_TRY(hipMalloc((void**)&w.stamp, cap * sizeof(int32_t)));
-    AGD_HIP_TRY(hipMalloc((void**)&w.score, 2 * cap * sizeof(int32_t)));
-    AGD_HIP_TRY(hipMalloc((void**)&w.needs, cap * sizeof(int32_t)));
-    AGD_HIP_TRY(hipMalloc((void**)&w.pending, cap * sizeof(double)));
-    w.tile_cap = cap;
+    AGD_HIP_TRY(hipMalloc((void**)&stamp, cap * sizeof(int32_t)));
+    AGD_HIP_TRY(hipMalloc((void**)&score, 2 * cap * sizeof(int32_t)));
+    AGD_HIP_TRY(hipMalloc((void**)&needs, cap * sizeof(int32_t)));
+    AGD_HIP_TRY(hipMalloc((void**)&pending, cap * sizeof(double)));
+    tile_cap = cap;
   }
-  if (!w.bab) AGD_HIP_TRY(hipMalloc((void**)&w.bab, kBabLen * sizeof(int32_t)));
-  if (w.shard_cap < shard_cap) {
-    if (w.lists) AGD_HIP_TRY(hipFree(w.lists));
-    const int64_t cap = std::max<int64_t>(shard_cap, 256);
-    AGD_HIP_TRY(hipMalloc((void**)&w.lists, 2 * kShards * cap * sizeof(int32_t)));
-    w.shard_cap = cap;
+  if (shard_cap < scap) {
+    free_dev(lists);
+    const int64_t cap = std::max<int64_t>(scap, 256);
+    AGD_HIP_TRY(hipMalloc((void**)&lists, 2 * kShards * cap * sizeof(int32_t)));
+    shard_cap = cap;
   }
-  if (w.hist_cap < nhist) {
-    if (w.hist) AGD_HIP_TRY(hipFree(w.hist));
+  if (hist_cap < nhist) {
+    free_dev(hist);
     const int64_t cap = std::max<int64_t>(nhist, 4096);
-    AGD_HIP_TRY(hipMalloc((void**)&w.hist, cap * sizeof(int32_t)));
-    w.hist_cap = cap;
+    AGD_HIP_TRY(hipMalloc((void**)&hist, cap * sizeof(int32_t)));
+    hist_cap = cap;
   }
-  out = &w;
+  return AGD_OK;
+}
+
+int Workspace::ensure_batch(int64_t ninst, int64_t ntiles, size_t inst_bytes) {
+  AGD_REQUIRE(!fixed, "batched solves need an allocating workspace");
+  if (inst_cap < ninst) {
+    free_dev(inst);
+    AGD_HIP_TRY(hipMalloc(&inst, size_t(ninst) * inst_bytes));
+    inst_cap = ninst;
+  }
+  if (tinst_cap < ntiles) {
+    free_dev(tile_inst);
+    AGD_HIP_TRY(hipMalloc((void**)&tile_inst, size_t(ntiles) * sizeof(int32_t)));
+    tinst_cap = ntiles;
+  }
+  return AGD_OK;
+}
+
+// Fixed-region layout: every buffer starts on a 256-byte boundary.
+static size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+size_t Workspace::fixed_bytes(int64_t ntiles, int64_t scap, int64_t nhist) {
+  const size_t t = size_t(ntiles);
+  return align256(t * 4) + align256(t * 4) + align256(t * 8) + align256(kBabLen * 4) +
+         align256(size_t(2) * kShards * size_t(scap) * 4) + align256(kCountsAlloc * 4) +
+         align256(size_t(nhist) * 4) + align256(2 * t * 4) + align256(64);
+}
+
+int Workspace::bind_fixed(void* base, size_t bytes, int64_t ntiles, int64_t scap, int64_t nhist) {
+  AGD_REQUIRE(fixed_bytes(ntiles, scap, nhist) <= bytes,
+              "the solve does not fit the library's fixed device region");
+  release();
+  fixed = true;
+  char* p = static_cast<char*>(base);
+  auto take = [&](size_t b) {
+    char* q = p;
+    p += align256(b);
+    return q;
+  };
+  stamp = reinterpret_cast<int32_t*>(take(size_t(ntiles) * 4));
+  needs = reinterpret_cast<int32_t*>(take(size_t(ntiles) * 4));
+  pending = reinterpret_cast<double*>(take(size_t(ntiles) * 8));
+  bab = reinterpret_cast<int32_t*>(take(kBabLen * 4));
+  lists = reinterpret_cast<int32_t*>(take(size_t(2) * kShards * size_t(scap) * 4));
+  counts = reinterpret_cast<int32_t*>(take(kCountsAlloc * 4));
+  hist = reinterpret_cast<int32_t*>(take(size_t(nhist) * 4));
+  score = reinterpret_cast<int32_t*>(take(2 * size_t(ntiles) * 4));
+  check_count = reinterpret_cast<int32_t*>(take(64));
+  tile_cap = ntiles;
+  shard_cap = scap;
+  hist_cap = nhist;
+  host_counts = host_local;
   return AGD_OK;
 }
 
 int blocks_for(int ntiles, int lanes_per_block) {
-  static int cus = 0;
-  if (cus == 0) {
-    int dev = 0;
-    hipDeviceProp_t p;
-    cus = (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&p, dev) == hipSuccess)
-              ? p.multiProcessorCount
-              : 256;
-  }
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+    cus = 256;
   // Enough resident waves per CU to hide the neighbour-gather latency; the
   // workgroups stride over the tile list.
   const int waves_per_block = (lanes_per_block + 63) / 64;
@@ -396,7 +499,7 @@ int blocks_for(int ntiles, int lanes_per_block) {
 }
 
 template <class S>
-static int solve_impl(const agd_eikonal_desc* d, const agd_eikonal_io* io,
+static int solve_impl(Workspace& ws, const agd_eikonal_desc* d, const agd_eikonal_io* io,
                       agd_eikonal_stats* stats, hipStream_t stream) {
   KernelSet<S> ks;
   AGD_REQUIRE(lookup<S>(d->model, d->ndim, d->shape_i, ks), "unsupported tile shape for this model");
@@ -411,6 +514,10 @@ static int solve_impl(const agd_eikonal_desc* d, const agd_eikonal_io* io,
   const bool pre = io->scheme_weights != nullptr || io->scheme_codes != nullptr;
   AGD_REQUIRE(!pre || (io->scheme_weights && io->scheme_codes),
               "scheme_weights and scheme_codes go together");
+  if (pre) {
+    const int stl = check_scheme_lengths(d, io, g.size_tot);
+    if (stl) return stl;
+  }
   AGD_REQUIRE(!pre || ks.update_pre, "no precomputed-scheme variant for this model");
   AGD_REQUIRE(pre || ks.update, "this model runs from a precomputed scheme (agd_eikonal_scheme)");
   const bool order2 = d->order == 2;
@@ -427,7 +534,9 @@ static int solve_impl(const agd_eikonal_desc* d, const agd_eikonal_io* io,
   // Fields of 4 GiB or more go to the general variant (64-bit addressing).
   const bool big = uint64_t(g.size_tot) * sizeof(S) > uint64_t(0xffffffffu);
   AGD_REQUIRE(!pre || !big, "a precomputed scheme needs fields below 4 GiB (2^30 FP32 cells)");
-  EikArgs<S> a{};
+  // Zeroed bytes, padding included: the argument bytes key the graph cache.
+  EikArgs<S> a;
+  std::memset(&a, 0, sizeof a);
   a.u = static_cast<S*>(io->values);
   a.geom = static_cast<const S*>(io->geom);
   a.sw = static_cast<const S*>(io->scheme_weights);
@@ -456,7 +565,7 @@ static int solve_impl(const agd_eikonal_desc* d, const agd_eikonal_io* io,
     a.w.tile_cap = int32_t(ws->tile_cap);
     a.w.shard_cap = int32_t(ws->shard_cap);
     a.w.fixed = nullptr;
-    AGD_HIP_TRY(hipMemsetAsync(ws->score, 0x7f, size_t(2 * ws->tile_cap) * sizeof(int32_t), stream));
+    fill_i32(ws->score, 2 * ws->tile_cap, 0x7f7f7f7f, stream);
     if (global) {
       // Every pass visits the full list (kept in the score buffer, unused
       // by this solver); the AGSI tags only decide when to stop.
@@ -472,30 +581,36 @@ static int solve_impl(const agd_eikonal_desc* d, const agd_eikonal_io* io,
       a.bab.ctl = ws->bab;
       a.bab.needs = ws->needs;
       a.bab.pending = ws->pending;
-      AGD_HIP_TRY(hipMemsetAsync(ws->needs, 0, size_t(g.ntiles) * sizeof(int32_t), stream));
+      fill_i32(ws->needs, g.ntiles, 0, stream);
       hipLaunchKernelGGL(bab_init_kernel, dim3(unsigned(ceil_div(g.ntiles, 256))), dim3(256), 0,
                          stream, g.ntiles, ws->pending, ws->bab);
     }
     return AGD_OK;
   };
+  // walls, periodic axes, source factorization and bound_active_blocks run
+  // in the general variant
+  auto kern = pre ? ks.update_pre
+                  : order2 ? ks.update_order2
+                           : ((io->wall_dist || g.periodic || factor || big || bab)
+                                  ? ks.update_walls
+                                  : ks.update);
   auto launch = [&](int32_t koff, hipStream_t s, bool phase) {
     a.w.koff = koff;
     a.w.phase_from_koff = phase ? 1 : 0;
-    // walls, periodic axes, source factorization and bound_active_blocks run
-    // in the general variant
-    auto kern = pre ? ks.update_pre
-                    : order2 ? ks.update_order2
-                             : ((io->wall_dist || g.periodic || factor || big || bab)
-                                    ? ks.update_walls
-                                    : ks.update);
     hipLaunchKernelGGL(kern, dim3(blocks), dim3(ni), 0, s, a);
     if (bab)
       hipLaunchKernelGGL(bab_threshold_kernel, dim3(1), dim3(64), 0, s, a.bab.ctl,
                          int64_t(d->bound_active_blocks), d->minchg_delta_min);
   };
+  auto key = [&](std::vector<unsigned char>& k) {
+    key_append(k, kern);
+    key_append(k, a);
+    key_append(k, d->bound_active_blocks);
+    key_append(k, d->minchg_delta_min);
+  };
   WorklistResult r;
-  int st = run_worklist(g.ntiles, blocks, d->nitermax_o, d->check_interval, stream, init, launch,
-                        r, d->time_launches != 0, d->use_graph != 0);
+  int st = run_worklist(ws, g.ntiles, blocks, d->nitermax_o, d->check_interval, stream, init, launch,
+                        key, r, d->time_launches != 0, d->use_graph != 0);
   if (st) return st;
   if (stats) {
     stats->kernel_ms = r.kernel_ms;
@@ -517,7 +632,7 @@ static int solve_impl(const agd_eikonal_desc* d, const agd_eikonal_io* io,
 template <class S>
 static int flow_impl(const agd_eikonal_desc* d, const agd_eikonal_io* io, const agd_flow_io* f,
                      hipStream_t stream, int32_t* violations = nullptr, double chk_atol = 0.,
-                     double chk_rtol = 0.) {
+                     double chk_rtol = 0., bool repair = false) {
   KernelSet<S> ks;
   AGD_REQUIRE(lookup<S>(d->model, d->ndim, d->shape_i, ks), "unsupported tile shape for this model");
   AGD_REQUIRE(io && io->values && io->rhs && io->seeds && (f || violations), "missing buffers");
@@ -537,6 +652,7 @@ static int flow_impl(const agd_eikonal_desc* d, const agd_eikonal_io* io, const 
   a.violations = violations;
   a.chk_atol = S(chk_atol);
   a.chk_rtol = S(chk_rtol);
+  a.repair = repair ? 1 : 0;
   AGD_REQUIRE(d->order >= 0 && d->order <= 2, "order must be 1 or 2 (0 means 1)");
   a.order2 = d->order == 2 ? 1 : 0;
   a.order2_thr = S(d->order2_threshold);
@@ -546,17 +662,8 @@ static int flow_impl(const agd_eikonal_desc* d, const agd_eikonal_io* io, const 
   return AGD_OK;
 }
 
-// Per-device cache of the instance table and tile -> instance map.
-struct BatchBuffers {
-  InstDesc* inst = nullptr;
-  int32_t* tile_inst = nullptr;
-  int64_t inst_cap = 0, tile_cap = 0;
-};
-static std::mutex g_batch_mutex;
-static std::unordered_map<int, BatchBuffers> g_batch;
-
 template <class S>
-static int solve_batched_impl(const agd_eikonal_desc* d, int32_t ninst,
+static int solve_batched_impl(Workspace& ws, const agd_eikonal_desc* d, int32_t ninst,
                               const agd_batch_instance* bi, const agd_eikonal_io* io,
                               agd_eikonal_stats* stats, hipStream_t stream) {
   KernelSet<S> ks;
@@ -603,33 +710,20 @@ static int solve_batched_impl(const agd_eikonal_desc* d, int32_t ninst,
   gb.size_tot = tiles * ni;
   for (int k = 0; k < 3; ++k) gb.shape_tot[k] = gb.shape_o[k] = 1;
 
-  BatchBuffers* bb = nullptr;
-  {
-    int dev = 0;
-    AGD_HIP_TRY(hipGetDevice(&dev));
-    std::lock_guard<std::mutex> lock(g_batch_mutex);
-    bb = &g_batch[dev];
-    if (bb->inst_cap < ninst) {
-      if (bb->inst) AGD_HIP_TRY(hipFree(bb->inst));
-      AGD_HIP_TRY(hipMalloc((void**)&bb->inst, size_t(ninst) * sizeof(InstDesc)));
-      bb->inst_cap = ninst;
-    }
-    if (bb->tile_cap < tiles) {
-      if (bb->tile_inst) AGD_HIP_TRY(hipFree(bb->tile_inst));
-      AGD_HIP_TRY(hipMalloc((void**)&bb->tile_inst, size_t(tiles) * sizeof(int32_t)));
-      bb->tile_cap = tiles;
-    }
-  }
-  AGD_HIP_TRY(hipMemcpyAsync(bb->inst, h.data(), size_t(ninst) * sizeof(InstDesc),
+  int st = ws.ensure_batch(ninst, tiles, sizeof(InstDesc));
+  if (st) return st;
+  InstDesc* inst_d = static_cast<InstDesc*>(ws.inst);
+  AGD_HIP_TRY(hipMemcpyAsync(inst_d, h.data(), size_t(ninst) * sizeof(InstDesc),
                              hipMemcpyHostToDevice, stream));
   hipLaunchKernelGGL(tile_inst_kernel, dim3(unsigned(ceil_div(tiles, 256))), dim3(256), 0, stream,
-                     ninst, bb->inst, int32_t(tiles), bb->tile_inst);
+                     ninst, inst_d, int32_t(tiles), ws.tile_inst);
   AGD_HIP_TRY(hipGetLastError());
   AGD_HIP_TRY(hipStreamSynchronize(stream));  // h is released on return
 
   const int blocks = blocks_for(int(tiles), ni);
   const bool global = d->solver == AGD_GLOBAL;
-  EikArgs<S> a{};
+  EikArgs<S> a;
+  std::memset(&a, 0, sizeof a);
   a.u = static_cast<S*>(io->values);
   a.geom = static_cast<const S*>(io->geom);
   a.rhs = static_cast<const S*>(io->rhs);
@@ -638,8 +732,8 @@ static int solve_batched_impl(const agd_eikonal_desc* d, int32_t ninst,
   a.niter_i = d->niter_i;
   a.solver = global ? AGD_AGSI : d->solver;
   a.fim_front = d->fim_front_width;
-  a.inst = bb->inst;
-  a.tile_inst = bb->tile_inst;
+  a.inst = inst_d;
+  a.tile_inst = ws.tile_inst;
   auto init = [&](Workspace* ws) -> int {
     a.w.lists = ws->lists;
     a.w.counts = ws->counts;
@@ -649,7 +743,7 @@ static int solve_batched_impl(const agd_eikonal_desc* d, int32_t ninst,
     a.w.tile_cap = int32_t(ws->tile_cap);
     a.w.shard_cap = int32_t(ws->shard_cap);
     a.w.fixed = nullptr;
-    AGD_HIP_TRY(hipMemsetAsync(ws->score, 0x7f, size_t(2 * ws->tile_cap) * sizeof(int32_t), stream));
+    fill_i32(ws->score, 2 * ws->tile_cap, 0x7f7f7f7f, stream);
     if (global) {
       hipLaunchKernelGGL(all_ids_kernel, dim3(unsigned(ceil_div(tiles, 256))), dim3(256), 0, stream,
                          int32_t(tiles), ws->score, ws->counts + kFixedIdx);
@@ -667,9 +761,13 @@ static int solve_batched_impl(const agd_eikonal_desc* d, int32_t ninst,
     a.w.phase_from_koff = phase ? 1 : 0;
     hipLaunchKernelGGL(ks.update_batch, dim3(blocks), dim3(ni), 0, s, a);
   };
+  auto key = [&](std::vector<unsigned char>& k) {
+    key_append(k, ks.update_batch);
+    key_append(k, a);
+  };
   WorklistResult r;
-  int st = run_worklist(int32_t(tiles), blocks, d->nitermax_o, d->check_interval, stream, init,
-                        launch, r, d->time_launches != 0, d->use_graph != 0);
+  st = run_worklist(ws, int32_t(tiles), blocks, d->nitermax_o, d->check_interval, stream, init,
+                    launch, key, r, d->time_launches != 0, d->use_graph != 0);
   if (st) return st;
   if (stats) {
     stats->kernel_ms = r.kernel_ms;
@@ -696,6 +794,8 @@ static int scheme_impl(const agd_eikonal_desc* d, const agd_eikonal_io* io, hipS
   AGD_REQUIRE(io && io->geom && io->scheme_weights && io->scheme_codes,
               "geom, scheme_weights and scheme_codes are required");
   const GridDesc g = make_grid(d);
+  const int stl = check_scheme_lengths(d, io, g.size_tot);
+  if (stl) return stl;
   SchemeArgs<S> a{};
   a.geom = static_cast<const S*>(io->geom);
   a.wall = io->wall_dist;
@@ -715,21 +815,186 @@ static int scheme_impl(const agd_eikonal_desc* d, const agd_eikonal_io* io, hipS
 
 using namespace agd;
 
+// A solver handle: the workspace of the solves run through it, on the device
+// current at creation and on one stream.
+struct agd_solver {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  Workspace ws;
+};
+
+namespace agd {
+
+static int check_handle(const agd_solver* s) {
+  AGD_REQUIRE(s != nullptr, "null solver handle");
+  int dev = 0;
+  AGD_HIP_TRY(hipGetDevice(&dev));
+  AGD_REQUIRE(dev == s->device, "the solver handle belongs to another device than the current one");
+  return AGD_OK;
+}
+
+static int solve_entry(Workspace& ws, const agd_eikonal_desc* desc, const agd_eikonal_io* io,
+                       agd_eikonal_stats* stats, hipStream_t stream) {
+  int st = check_desc(desc);
+  if (st) return st;
+  if (desc->scalar == AGD_F32) return solve_impl<float>(ws, desc, io, stats, stream);
+  return solve_impl<double>(ws, desc, io, stats, stream);
+}
+
+static int batched_entry(Workspace& ws, const agd_eikonal_desc* desc, int32_t ninst,
+                         const agd_batch_instance* instances, const agd_eikonal_io* io,
+                         agd_eikonal_stats* stats, hipStream_t stream) {
+  int st = check_desc(desc);
+  if (st) return st;
+  if (desc->scalar == AGD_F32)
+    return solve_batched_impl<float>(ws, desc, ninst, instances, io, stats, stream);
+  return solve_batched_impl<double>(ws, desc, ninst, instances, io, stats, stream);
+}
+
+// Residual check with the counter `cnt` (one int32 of device memory).
+static int check_entry(int32_t* cnt, const agd_eikonal_desc* desc, const agd_eikonal_io* io,
+                       double atol, double rtol, int64_t* violations, hipStream_t s) {
+  int st = check_desc(desc);
+  if (st) return st;
+  AGD_REQUIRE(violations, "null output");
+  fill_i32(cnt, 1, 0, s);
+  st = desc->scalar == AGD_F32 ? flow_impl<float>(desc, io, nullptr, s, cnt, atol, rtol)
+                               : flow_impl<double>(desc, io, nullptr, s, cnt, atol, rtol);
+  if (st) return st;
+  int32_t n = 0;
+  AGD_HIP_TRY(hipMemcpyAsync(&n, cnt, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  AGD_HIP_TRY(hipStreamSynchronize(s));
+  *violations = n;
+  return AGD_OK;
+}
+
+// Repair rounds with the counter `cnt`: each counts the cells below their
+// scheme update and raises them to +inf; stops after a round that counts
+// nothing or after max_rounds rounds.
+static int repair_entry(int32_t* cnt, const agd_eikonal_desc* desc, const agd_eikonal_io* io,
+                        double atol, double rtol, int32_t max_rounds, int64_t* left,
+                        int32_t* rounds, hipStream_t s) {
+  int st = check_desc(desc);
+  if (st) return st;
+  AGD_REQUIRE(left && rounds, "null output");
+  AGD_REQUIRE(max_rounds >= 1, "max_rounds must be >= 1");
+  int32_t n = 0, r = 0;
+  do {
+    fill_i32(cnt, 1, 0, s);
+    st = desc->scalar == AGD_F32 ? flow_impl<float>(desc, io, nullptr, s, cnt, atol, rtol, true)
+                                 : flow_impl<double>(desc, io, nullptr, s, cnt, atol, rtol, true);
+    if (st) return st;
+    AGD_HIP_TRY(hipMemcpyAsync(&n, cnt, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    AGD_HIP_TRY(hipStreamSynchronize(s));
+    ++r;
+  } while (n > 0 && r < max_rounds);
+  *left = n;
+  *rounds = r;
+  return AGD_OK;
+}
+
+}  // namespace agd
+
 extern "C" {
+
+int agd_solver_create(const agd_eikonal_desc* desc, void* stream, agd_solver** out) {
+  AGD_REQUIRE(out != nullptr, "null output");
+  *out = nullptr;
+  if (desc) {
+    const int st = check_desc(desc);
+    if (st) return st;
+  }
+  agd_solver* s = new (std::nothrow) agd_solver;
+  if (!s) {
+    set_error("out of host memory");
+    return AGD_ENOMEM;
+  }
+  s->stream = as_stream(stream);
+  if (hipGetDevice(&s->device) != hipSuccess) {
+    delete s;
+    set_error("hipGetDevice failed");
+    return AGD_EHIP;
+  }
+  if (desc) {
+    // Size the workspace for the descriptor's grid (it still grows on demand).
+    const GridDesc g = make_grid(desc);
+    int ni = 1;
+    for (int k = 0; k < desc->ndim; ++k) ni *= desc->shape_i[k];
+    const int st = s->ws.ensure(g.ntiles, shard_capacity(g.ntiles, blocks_for(g.ntiles, ni)),
+                                hist_len(desc->nitermax_o, desc->check_interval));
+    if (st) {
+      delete s;
+      return st;
+    }
+  }
+  *out = s;
+  return AGD_OK;
+}
+
+int agd_solver_destroy(agd_solver* s) {
+  if (!s) return AGD_OK;
+  int cur = 0;
+  const bool swap = hipGetDevice(&cur) == hipSuccess && cur != s->device &&
+                    hipSetDevice(s->device) == hipSuccess;
+  delete s;
+  if (swap) (void)hipSetDevice(cur);
+  return AGD_OK;
+}
+
+int agd_solver_solve(agd_solver* s, const agd_eikonal_desc* desc, const agd_eikonal_io* io,
+                     agd_eikonal_stats* stats) {
+  const int st = check_handle(s);
+  if (st) return st;
+  return solve_entry(s->ws, desc, io, stats, s->stream);
+}
+
+int agd_solver_solve_batched(agd_solver* s, const agd_eikonal_desc* desc, int32_t ninst,
+                             const agd_batch_instance* instances, const agd_eikonal_io* io,
+                             agd_eikonal_stats* stats) {
+  const int st = check_handle(s);
+  if (st) return st;
+  return batched_entry(s->ws, desc, ninst, instances, io, stats, s->stream);
+}
+
+int agd_solver_check(agd_solver* s, const agd_eikonal_desc* desc, const agd_eikonal_io* io,
+                     double atol, double rtol, int64_t* violations) {
+  int st = check_handle(s);
+  if (st) return st;
+  st = s->ws.ensure(1, 1, 1);
+  if (st) return st;
+  return check_entry(s->ws.check_count, desc, io, atol, rtol, violations, s->stream);
+}
+
+int agd_solver_repair(agd_solver* s, const agd_eikonal_desc* desc, const agd_eikonal_io* io,
+                      double atol, double rtol, int32_t max_rounds, int64_t* violations_left,
+                      int32_t* rounds) {
+  int st = check_handle(s);
+  if (st) return st;
+  st = s->ws.ensure(1, 1, 1);
+  if (st) return st;
+  return repair_entry(s->ws.check_count, desc, io, atol, rtol, max_rounds, violations_left, rounds,
+                      s->stream);
+}
 
 int agd_eikonal_solve_batched(const agd_eikonal_desc* desc, int32_t ninst,
                               const agd_batch_instance* instances, const agd_eikonal_io* io,
                               agd_eikonal_stats* stats, void* stream) {
-  int st = check_desc(desc);
-  if (st) return st;
-  if (desc->scalar == AGD_F32)
-    return solve_batched_impl<float>(desc, ninst, instances, io, stats, as_stream(stream));
-  return solve_batched_impl<double>(desc, ninst, instances, io, stats, as_stream(stream));
+  Workspace ws;  // this call's own, released on return
+  return batched_entry(ws, desc, ninst, instances, io, stats, as_stream(stream));
 }
 
 int agd_scheme_supported(int32_t model, int32_t ndim, const int32_t* shape_i) {
   KernelSet<float> ks;
   return (shape_i && lookup<float>(model, ndim, shape_i, ks) && ks.scheme) ? 1 : 0;
+}
+
+int agd_scheme_lengths(int32_t model, int32_t ndim, int32_t order, int64_t* weight_components,
+                       int64_t* code_components) {
+  AGD_REQUIRE(ndim == 2 || ndim == 3, "ndim must be 2 or 3");
+  AGD_REQUIRE(model >= AGD_ISOTROPIC && model <= AGD_TTI, "unknown model");
+  AGD_REQUIRE(weight_components && code_components, "null output");
+  scheme_components(model, ndim, order, *weight_components, *code_components);
+  return AGD_OK;
 }
 
 int agd_eikonal_scheme(const agd_eikonal_desc* desc, const agd_eikonal_io* io, void* stream) {
@@ -746,10 +1011,8 @@ int agd_tile_supported(int32_t model, int32_t ndim, const int32_t* shape_i) {
 
 int agd_eikonal_solve(const agd_eikonal_desc* desc, const agd_eikonal_io* io,
                       agd_eikonal_stats* stats, void* stream) {
-  int st = check_desc(desc);
-  if (st) return st;
-  if (desc->scalar == AGD_F32) return solve_impl<float>(desc, io, stats, as_stream(stream));
-  return solve_impl<double>(desc, io, stats, as_stream(stream));
+  Workspace ws;  // this call's own, released on return
+  return solve_entry(ws, desc, io, stats, as_stream(stream));
 }
 
 int agd_eikonal_flow(const agd_eikonal_desc* desc, const agd_eikonal_io* io,
@@ -762,23 +1025,13 @@ int agd_eikonal_flow(const agd_eikonal_desc* desc, const agd_eikonal_io* io,
 
 int agd_eikonal_check(const agd_eikonal_desc* desc, const agd_eikonal_io* io, double atol,
                       double rtol, int64_t* violations, void* stream) {
-  int st = check_desc(desc);
-  if (st) return st;
-  AGD_REQUIRE(violations, "null output");
   hipStream_t s = as_stream(stream);
-  Workspace* ws = nullptr;
-  st = get_workspace(1, 1, 1, ws);
-  if (st) return st;
-  AGD_HIP_TRY(hipMemsetAsync(ws->check_count, 0, sizeof(int32_t), s));
-  st = desc->scalar == AGD_F32
-           ? flow_impl<float>(desc, io, nullptr, s, ws->check_count, atol, rtol)
-           : flow_impl<double>(desc, io, nullptr, s, ws->check_count, atol, rtol);
-  if (st) return st;
-  int32_t n = 0;
-  AGD_HIP_TRY(hipMemcpyAsync(&n, ws->check_count, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  AGD_HIP_TRY(hipStreamSynchronize(s));
-  *violations = n;
-  return AGD_OK;
+  int32_t* cnt = nullptr;
+  AGD_HIP_TRY(hipMalloc((void**)&cnt, sizeof(int32_t)));
+  const int st = check_entry(cnt, desc, io, atol, rtol, violations, s);
+  (void)hipStreamSynchronize(s);
+  (void)hipFree(cnt);
+  return st;
 }
 
 int agd_stencil_export(int32_t model, int32_t ndim, int32_t scalar, int64_t n, const void* geom,

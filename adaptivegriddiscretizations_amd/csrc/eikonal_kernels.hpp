@@ -730,6 +730,10 @@ template <class S> struct FlowArgs {
   // the fixed point, which the decreasing iteration cannot raise).
   int32_t* violations;
   S chk_atol, chk_rtol;
+  // With violations: 1 also raises every counted cell to +inf (a value at
+  // +inf is above the fixed point; repeated until nothing is counted, this
+  // turns a start into a supersolution of the scheme).
+  int32_t repair;
   FactorDesc<S> fac;
 };
 
@@ -893,6 +897,10 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_flow(FlowArgs<S> a) {
     const bool bad = !seed && val > u_old + a.chk_atol + a.chk_rtol * abs_(u_old);
     const uint64_t m = __ballot(bad);
     if ((threadIdx.x & 63) == 0 && m) atomicAdd(a.violations, int(__popcll(m)));
+    if (a.repair) {
+      __syncthreads();  // the tile's own reads are done before any store
+      if (bad) a.u[n_t] = inf_v<S>();
+    }
     return;
   }
   __syncthreads();  // every lane has read its neighbours before any store

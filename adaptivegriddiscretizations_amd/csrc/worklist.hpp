@@ -29,7 +29,9 @@ constexpr int kCountStride = 16;                         // one 64-byte line per
 constexpr int kRow = kShards * kCountStride;             // one pass's counters
 constexpr int kFixedIdx = 3 * kRow;                      // fixed-list length (global iteration)
 constexpr int kBaseIdx = 3 * kRow + kCountStride;        // pass base
-constexpr int kCountsLen = 3 * kRow + 2 * kCountStride;
+constexpr int kCountsLen = 3 * kRow + 2 * kCountStride;  // cleared at the start of a solve
+constexpr int kSumIdx = kCountsLen;                      // int64[2]: non-empty passes, visits
+constexpr int kCountsAlloc = kCountsLen + 4;
 
 // bound_active_blocks control block (one per device workspace): per-shard
 // counters of the pass just run (tiles that propagated, tiles frozen, and
@@ -42,6 +44,10 @@ constexpr int kBabMax = 3 * kRow;
 constexpr int kBabThr = 4 * kRow;                         // double[2]: prev, next
 constexpr int kBabLen = 4 * kRow + 2 * kCountStride;
 
+// Worklist workspace of the solves run through one owner: a solver handle
+// (agd_solver), a single call of an entry point without a handle, or the
+// fixed device region of the allocation-free entry point.  Nothing is shared
+// between workspaces, so solves with different workspaces may run at once.
 struct Workspace {
   int32_t* stamp = nullptr;
   int32_t* needs = nullptr;   // bound_active_blocks: last pass a real update was requested for
@@ -52,19 +58,67 @@ struct Workspace {
   int32_t* hist = nullptr;    // hist_cap: list length of every pass
   int32_t* score = nullptr;   // 2 * tile_cap (FIM scores; fixed list of global iteration)
   int32_t* check_count = nullptr;  // residual-check counter
-  int32_t* host_counts = nullptr;  // pinned, 2 * kRow (two chunks in flight)
-  int64_t tile_cap = 0, shard_cap = 0, hist_cap = 0;
+  void* inst = nullptr;            // batched solves: instance table (inst_cap InstDesc)
+  int32_t* tile_inst = nullptr;    // batched solves: tile -> instance (tinst_cap)
+  int64_t tile_cap = 0, shard_cap = 0, hist_cap = 0, inst_cap = 0, tinst_cap = 0;
+  // Host side.  host_counts: pinned (owned) or host_local (fixed region),
+  // 2 * kRow entries, two chunks in flight.
+  int32_t* host_counts = nullptr;
+  int32_t host_local[2 * kRow];
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipEvent_t chunk_ev[2] = {nullptr, nullptr};  // list length of a chunk copied
   // Private stream used only to record pass chunks into a graph (the legacy
   // null stream, torch's default, cannot be captured).  Nothing executes on
   // it: the instantiated graph is launched on the caller's stream.
   hipStream_t capture_stream = nullptr;
+  // Last instantiated pass chunk and the bytes (kernel, arguments, grid,
+  // chunk length) it was captured with: a solve with identical bytes
+  // replays it instead of capturing again.
+  hipGraphExec_t exec = nullptr;
+  std::vector<unsigned char> exec_key;
+  // Fixed region (allocation-free entry point): nothing is ever allocated,
+  // grown or freed, no events are recorded and passes launch eagerly.
+  bool fixed = false;
+
+  Workspace() = default;
+  Workspace(const Workspace&) = delete;
+  Workspace& operator=(const Workspace&) = delete;
+  ~Workspace() { release(); }
+
+  // Grows the device buffers to ntiles tiles, sub-lists of shard_cap entries
+  // and nhist passes (allocating the host side on first use).
+  int ensure(int64_t ntiles, int64_t shard_cap, int64_t nhist);
+  // Batched solves: instance table and tile map of ninst instances, ntiles tiles.
+  int ensure_batch(int64_t ninst, int64_t ntiles, size_t inst_bytes);
+  // Frees everything owned (no-op for a fixed region).
+  void release();
+  // Lays the buffers out in [base, base + bytes) (fixed region); AGD_EINVAL
+  // if they do not fit.
+  int bind_fixed(void* base, size_t bytes, int64_t ntiles, int64_t shard_cap, int64_t nhist);
+  // Bytes bind_fixed needs.
+  static size_t fixed_bytes(int64_t ntiles, int64_t shard_cap, int64_t nhist);
 };
 
-// Per-device cached workspace (grown on demand).  Defined in eikonal.hip.
-int get_workspace(int64_t ntiles, int64_t shard_cap, int64_t nhist, Workspace*& out);
+// Workgroups of a pass launch: enough resident waves per CU of the current
+// device to hide the gather latency (the workgroups stride over the list).
 int blocks_for(int ntiles, int lanes_per_block);
+
+// Fills n int32 words with v (a kernel rather than hipMemsetAsync, so that
+// the solve stays capturable by a caller).
+__global__ inline void fill_i32_kernel(int32_t* p, int64_t n, int32_t v) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+inline void fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(fill_i32_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, p, n, v);
+}
+
+// Appends the bytes of `x` to a graph cache key.
+template <class T> inline void key_append(std::vector<unsigned char>& k, const T& x) {
+  const unsigned char* p = reinterpret_cast<const unsigned char*>(&x);
+  k.insert(k.end(), p, p + sizeof(T));
+}
 
 // Capacity of one sub-list.  A pass appends at most 2d+1 <= 7 tiles per
 // tile visit and the workgroups b = s mod kShards of a grid of G visit at
@@ -195,6 +249,32 @@ __device__ inline int32_t pass_entry(const PassView& v, int li) {
 
 __global__ inline void advance_kernel(int32_t* counts, int32_t g) { counts[kBaseIdx] += g; }
 
+// out[0] = passes with a non-empty list, out[1] = tile visits, over hist[0..n).
+// One workgroup of 256 lanes.
+__global__ inline void __launch_bounds__(256) hist_sum_kernel(const int32_t* hist, int64_t n,
+                                                              int64_t* out) {
+  __shared__ int64_t part[2][4];
+  int64_t ne = 0, vis = 0;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+    const int32_t h = hist[i];
+    ne += h > 0 ? 1 : 0;
+    vis += h;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    ne += __shfl_xor(ne, off);
+    vis += __shfl_xor(vis, off);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    part[0][threadIdx.x >> 6] = ne;
+    part[1][threadIdx.x >> 6] = vis;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    out[0] = part[0][0] + part[0][1] + part[0][2] + part[0][3];
+    out[1] = part[1][0] + part[1][1] + part[1][2] + part[1][3];
+  }
+}
+
 // ---- bound_active_blocks (Propagation.h:53-99,120-126, _solvers.py:195-224)
 // Order-preserving map of a float onto int32 (atomicMin/atomicMax on floats).
 __device__ inline int32_t ord_int(float f) {
@@ -272,38 +352,49 @@ struct WorklistResult {
   bool converged = false;
 };
 
+// Passes a solve needs room for in the pass histogram.
+inline int64_t hist_len(int64_t nitermax, int check_interval) {
+  return nitermax + std::max(1, check_interval) + 8;
+}
+
 // Runs passes until the list empties or nitermax passes were issued.
 // `init(ws)` enqueues the initial list into the sub-lists of pass 0 (list
 // row 0, counter row 0); `launch(koff, stream)` enqueues one pass of
 // iteration base + koff on `stream`, on a grid of `grid` workgroups.
-// With `graph`, chunks of `check_interval` passes are captured once into a
-// hipGraph and replayed; the host reads the next list length after each
-// chunk.  Optional per-pass timing records an event pair around every pass
-// (and then launches eagerly).
-template <class Init, class Launch>
-int run_worklist(int32_t ntiles, int32_t grid, int64_t nitermax, int check_interval,
-                 hipStream_t stream, Init init, Launch launch, WorklistResult& res,
+// With `graph`, chunks of `check_interval` passes are captured into a
+// hipGraph and replayed; `key(k)` appends to k the bytes the captured
+// launches depend on (kernel, arguments, grid), and the workspace's last
+// graph is replayed when they are unchanged.  The host reads the next list
+// length after each chunk.  Optional per-pass timing records an event pair
+// around every pass (and then launches eagerly).  A fixed-region workspace
+// launches eagerly and records no event.
+template <class Init, class Launch, class Key>
+int run_worklist(Workspace& wsr, int32_t ntiles, int32_t grid, int64_t nitermax, int check_interval,
+                 hipStream_t stream, Init init, Launch launch, Key key, WorklistResult& res,
                  bool time_launches = false, bool graph = false) {
-  Workspace* ws = nullptr;
+  Workspace* ws = &wsr;
   const int ci = std::max(1, check_interval);
-  int st = get_workspace(ntiles, shard_capacity(ntiles, grid), nitermax + ci + 8, ws);
+  const int64_t nhist = hist_len(nitermax, ci);
+  int st = ws->ensure(ntiles, shard_capacity(ntiles, grid), nhist);
   if (st) return st;
+  if (ws->fixed) {
+    time_launches = false;
+    graph = false;
+  }
   int32_t* counts = ws->counts;
-  AGD_HIP_TRY(hipMemsetAsync(ws->stamp, 0xff, size_t(ntiles) * sizeof(int32_t), stream));
-  AGD_HIP_TRY(hipMemsetAsync(counts, 0, kCountsLen * sizeof(int32_t), stream));
-  AGD_HIP_TRY(hipMemsetAsync(ws->hist, 0, size_t(nitermax + ci + 8) * sizeof(int32_t), stream));
+  fill_i32(ws->stamp, ntiles, -1, stream);
+  fill_i32(counts, kCountsLen, 0, stream);
+  fill_i32(ws->hist, nhist, 0, stream);
   st = init(ws);
   if (st) return st;
   AGD_HIP_TRY(hipGetLastError());
 
-  // Events and the graph are released on every return path.
+  // Events are released on every return path.
   struct Owned {
     std::vector<hipEvent_t> evs;
-    hipGraphExec_t exec = nullptr;
     ~Owned() {
       for (auto e : evs)
         if (e) (void)hipEventDestroy(e);
-      if (exec) (void)hipGraphExecDestroy(exec);
     }
   } own;
   std::vector<hipEvent_t>& evs = own.evs;
@@ -326,27 +417,43 @@ int run_worklist(int32_t ntiles, int32_t grid, int64_t nitermax, int check_inter
   // Per-launch timing needs event pairs around every launch, which a graph
   // does not record: timed runs launch eagerly.
   graph = graph && !time_launches;
-  hipGraphExec_t& exec = own.exec;
+  hipGraphExec_t exec = nullptr;
   if (graph) {
-    hipGraph_t g = nullptr;
-    hipStream_t cs = ws->capture_stream;
-    AGD_HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-    for (int i = 0; i < ci; ++i) launch(i, cs, ci % 6 == 0);
-    hipLaunchKernelGGL(advance_kernel, dim3(1), dim3(1), 0, cs, counts, ci);
-    // End the capture whatever happened inside it, so the stream is usable again.
-    const hipError_t cap_err = hipGetLastError();
-    const hipError_t end_err = hipStreamEndCapture(cs, &g);
-    if (cap_err != hipSuccess || end_err != hipSuccess) {
-      if (g) (void)hipGraphDestroy(g);
-      AGD_HIP_TRY(cap_err);
-      AGD_HIP_TRY(end_err);
+    std::vector<unsigned char> k;
+    key(k);
+    key_append(k, grid);
+    key_append(k, ci);
+    if (ws->exec && k == ws->exec_key) {
+      exec = ws->exec;
+    } else {
+      if (ws->exec) {
+        (void)hipGraphExecDestroy(ws->exec);
+        ws->exec = nullptr;
+        ws->exec_key.clear();
+      }
+      hipGraph_t g = nullptr;
+      hipStream_t cs = ws->capture_stream;
+      AGD_HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+      for (int i = 0; i < ci; ++i) launch(i, cs, ci % 6 == 0);
+      hipLaunchKernelGGL(advance_kernel, dim3(1), dim3(1), 0, cs, counts, ci);
+      // End the capture whatever happened inside it, so the stream is usable again.
+      const hipError_t cap_err = hipGetLastError();
+      const hipError_t end_err = hipStreamEndCapture(cs, &g);
+      if (cap_err != hipSuccess || end_err != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        AGD_HIP_TRY(cap_err);
+        AGD_HIP_TRY(end_err);
+      }
+      const hipError_t inst_err = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      AGD_HIP_TRY(inst_err);
+      ws->exec = exec;
+      ws->exec_key = std::move(k);
     }
-    const hipError_t inst_err = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    AGD_HIP_TRY(inst_err);
   }
 
-  AGD_HIP_TRY(hipEventRecord(ws->ev0, stream));
+  const bool timed = ws->ev0 != nullptr;
+  if (timed) AGD_HIP_TRY(hipEventRecord(ws->ev0, stream));
   int64_t k = 0;
   bool converged = false;
   auto list_length = [&](const int32_t* row) {
@@ -406,23 +513,27 @@ int run_worklist(int32_t ntiles, int32_t grid, int64_t nitermax, int check_inter
     }
   }
   AGD_HIP_TRY(hipGetLastError());
-  AGD_HIP_TRY(hipEventRecord(ws->ev1, stream));
-  AGD_HIP_TRY(hipEventSynchronize(ws->ev1));
-  AGD_HIP_TRY(hipEventElapsedTime(&res.ms, ws->ev0, ws->ev1));
+  if (timed) {
+    AGD_HIP_TRY(hipEventRecord(ws->ev1, stream));
+    AGD_HIP_TRY(hipEventSynchronize(ws->ev1));
+    AGD_HIP_TRY(hipEventElapsedTime(&res.ms, ws->ev0, ws->ev1));
+  } else {
+    res.ms = 0.f;
+  }
   if (time_launches) {
     const int st2 = drain();
     if (st2) return st2;
   }
-  std::vector<int32_t> hist(k);
-  if (k > 0)
-    AGD_HIP_TRY(hipMemcpy(hist.data(), ws->hist, size_t(k) * sizeof(int32_t), hipMemcpyDeviceToHost));
+  // Non-empty passes and visits, summed on the device from the histogram.
+  int64_t* sums = reinterpret_cast<int64_t*>(counts + kSumIdx);
+  hipLaunchKernelGGL(hist_sum_kernel, dim3(1), dim3(256), 0, stream, ws->hist, k, sums);
+  AGD_HIP_TRY(hipGetLastError());
+  int64_t hs[2] = {0, 0};
+  AGD_HIP_TRY(hipMemcpyAsync(hs, sums, sizeof(hs), hipMemcpyDeviceToHost, stream));
+  AGD_HIP_TRY(hipStreamSynchronize(stream));
   res.passes = k;
-  res.niter = 0;
-  res.visits = 0;
-  for (int64_t i = 0; i < k; ++i) {
-    if (hist[i] > 0) ++res.niter;
-    res.visits += hist[i];
-  }
+  res.niter = hs[0];
+  res.visits = hs[1];
   res.converged = converged;
   return AGD_OK;
 }

@@ -45,7 +45,7 @@ def log(msg):
 def make_workload(name, small=False):
     from adaptivegriddiscretizations_amd import workloads
     if name == 'riemann3':
-        return workloads.riemann3_aniso(n=150 if small else 200)
+        return workloads.riemann3_aniso(n=120 if small else 200)
     if name == 'riemann2':
         return workloads.riemann2_aniso(n=1000 if small else 2000)
     if name == 'isotropic2':
@@ -73,29 +73,71 @@ def solve_once(iface, desc):
     if iface.precompute_scheme:
         iface.compute_scheme()
     st = _lib.EikonalStats()
-    _lib.check(_lib.lib.agd_eikonal_solve(desc, iface._io(), st, torch.cuda.current_stream().cuda_stream))
+    _lib.check(iface.solver_handle().solve(desc, iface._io(), st))
     return st
 
 
-def cpu_baseline(name, budget_s=30.):
-    """Single-thread FP64 fast marching of the oracle on a reduced sample."""
+def _host_info():
+    """CPU model, logical cores and OMP_NUM_THREADS of the host running the
+    baseline."""
+    model = None
+    try:
+        with open('/proc/cpuinfo') as f:
+            for line in f:
+                if line.startswith('model name'):
+                    model = line.split(':', 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(),
+            "omp_num_threads": os.environ.get('OMP_NUM_THREADS')}
+
+
+def cpu_baseline(name):
+    """The reference scheme on the host (the oracle, compiled C++): an
+    OpenMP tile adaptive Gauss-Seidel (FP64, tiles and sweeps of the device
+    solver, same tolerances) on the full workload, on every core OpenMP is
+    given, plus single-thread FP64 fast marching on a reduced sample (the
+    algorithmic class of the reference's CPU library).  Stencil construction
+    is included in both."""
     sys.path.insert(0, os.path.join(ROOT, 'oracle'))
     import numpy as np
     import oracle as O
-    wl = make_workload(name, small=True)
-    sides = np.asarray(wl.sides, dtype=np.float64)
-    h = wl.h.astype(np.float32).astype(np.float64)
-    idx = O.seed_indices(wl.seeds, sides[:, 0], h)
-    kw = {'metric': wl.metric.astype(np.float64)} if wl.metric is not None else {}
-    cost = 1. if wl.cost is None else wl.cost.astype(np.float64)
-    P = O.make_problem(wl.model[:-1], wl.dims, h, cost=cost, seeds_idx=idx, **kw).cast(np.float32)
+
+    def problem(wl):
+        sides = np.asarray(wl.sides, dtype=np.float64)
+        h = wl.h.astype(np.float32).astype(np.float64)
+        idx = O.seed_indices(wl.seeds, sides[:, 0], h)
+        kw = {'metric': wl.metric.astype(np.float64)} if wl.metric is not None else {}
+        cost = 1. if wl.cost is None else wl.cost.astype(np.float64)
+        return O.make_problem(wl.model[:-1], wl.dims, h, cost=cost, seeds_idx=idx, **kw).cast(np.float32)
+
+    from adaptivegriddiscretizations_amd.Eikonal.HFM_GPU import default_traits
+    wl = make_workload(name)
+    tr, _ = default_traits(wl.model)
+    P = problem(wl)
+    tol = 5e-6  # the device solver's default rtol (and atol for unit mean cost)
     t0 = time.perf_counter()
-    u = P.solve_fm()
+    u, passes, visits = P.solve_tiles_omp(tol, tol, tile=tr['shape_i'][0], niter=tr['niter_i'])
     dt = time.perf_counter() - t0
     assert np.isfinite(u).all()
-    return {"value": wl.ncells / dt, "unit": "solved cells/s", "cores": 1, "kind": "port",
-            "sample": f"{wl.name} {'x'.join(map(str, wl.dims))} ({wl.ncells} cells, same generator), "
-                      f"FP64 fast marching incl. stencil construction, {dt:.2f} s"}
+    del P, u
+    threads = O.omp_threads()
+    wls = make_workload(name, small=True)
+    Ps = problem(wls)
+    t0 = time.perf_counter()
+    us = Ps.solve_fm()
+    dts = time.perf_counter() - t0
+    assert np.isfinite(us).all()
+    return {"value": wl.ncells / dt, "unit": "solved cells/s", "cores": threads, "kind": "port",
+            "sample": f"{wl.name} {'x'.join(map(str, wl.dims))} (the full workload): OpenMP tile "
+                      f"adaptive Gauss-Seidel, FP64, tiles {tr['shape_i']}, {tr['niter_i']} sweeps, "
+                      f"tol {tol}, {threads} threads: {dt:.2f} s, {passes} passes, {visits} tile visits",
+            "host": _host_info(),
+            "fast_marching_1_thread": {
+                "value": wls.ncells / dts, "unit": "solved cells/s", "cores": 1,
+                "sample": f"{wls.name} {'x'.join(map(str, wls.dims))} ({wls.ncells} cells, same "
+                          f"generator), FP64 fast marching, {dts:.2f} s"}}
 
 
 def main():
@@ -205,7 +247,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"{wl.name} {'x'.join(map(str, wl.dims))}, one seed, cold start to convergence",
+            "workload": f"{wl.name} {'x'.join(map(str, wl.dims))}, {len(wl.seeds)} seed"
+                        f"{'s' if len(wl.seeds) > 1 else ''}, cold start to convergence",
             "model": wl.model, "grid": list(wl.dims), "tile": list(iface.shape_i),
             "niter_i": iface.niter_i, "solver": iface.solver, "atol": iface.atol, "rtol": iface.rtol,
             "parallelism": "single GPU", "pass_launch": "eager" if args.no_graph else "hipGraph",
@@ -338,23 +381,28 @@ def bench_warmstart(args):
     cold_b, ms_cb, p_cb = S.solve_batched(metric, seeds, atol, rtol, batch=16)
     torch.cuda.synchronize()
     wall_cold_b = time.perf_counter() - t0
-    # warm (prediction included), batched, judged by the solver
+    # warm (prediction and repair included), batched, judged by the solver before solving
     t0 = time.perf_counter()
-    warm, ms_w, p_w, fb = S.solve_warm(net, metric, seeds, atol, rtol, batch=16)
+    warm, rw = S.solve_warm(net, metric, seeds, atol, rtol, batch=16)
     torch.cuda.synchronize()
     wall_warm = time.perf_counter() - t0
-    # warm, one at a time
+    ms_w = rw['total_ms']
+    # warm, one at a time (the surrogate's share without batching)
     t0 = time.perf_counter()
-    warm1, ms_w1, p_w1, fb1 = S.solve_warm(net, metric, seeds, atol, rtol, batch=1)
+    warm1, rw1 = S.solve_warm(net, metric, seeds, atol, rtol, batch=1)
     torch.cuda.synchronize()
     wall_warm1 = time.perf_counter() - t0
+    ms_w1 = rw1['total_ms']
     fin = torch.isfinite(cold)          # cells no stencil reaches stay +inf in both
     same_inf = bool((torch.isfinite(warm) == fin).all())
     dev = torch.where(fin, (warm - cold).abs(), torch.zeros_like(cold)).amax(dim=(1, 2))
     err = float((dev / torch.where(fin, cold, torch.zeros_like(cold)).amax(dim=(1, 2))).max())
     if not same_inf:
         err = float('inf')
+    fb = rw['fallbacks']
     log(f"cold: {ms_c:.1f} ms GPU ({wall_cold:.2f} s wall), {p_c / args.test:.1f} passes each; "
+        f"cold batched {ms_cb:.1f} ms; warm one at a time {ms_w1:.1f} ms "
+        f"({rw1['passes'] / args.test:.1f} passes + {np.mean(rw1['rounds']):.2f} repair rounds each); "
         f"warm+batched: {ms_w:.1f} ms GPU ({wall_warm:.2f} s wall), fallbacks {fb}; error vs cold {err:.2e}")
     cells = args.test * n * n
     result = {
@@ -369,11 +417,15 @@ def bench_warmstart(args):
                    "parallelism": "single GPU"},
         "time_to_solution_ms": {"cold_sequential": ms_c, "cold_batched": ms_cb,
                                 "warm_batched": ms_w, "warm_sequential": ms_w1},
+        "warm_breakdown_ms": {"batched": {k: rw[k] for k in ('pred_ms', 'repair_ms', 'solve_ms')},
+                              "sequential": {k: rw1[k] for k in ('pred_ms', 'repair_ms', 'solve_ms')}},
         "wall_s": {"cold_sequential": wall_cold, "cold_batched": wall_cold_b,
                    "warm_batched": wall_warm, "warm_sequential": wall_warm1},
-        "passes_per_field": {"cold": p_c / args.test, "warm": p_w1 / args.test},
+        "passes_per_field": {"cold": p_c / args.test, "warm": rw1['passes'] / args.test,
+                             "repair_rounds": float(np.mean(rw1['rounds']))},
         "speedup_warm_batched_vs_cold": ms_c / ms_w,
-        "speedup_warm_only": ms_c / ms_w1,
+        "speedup_batching_only": ms_c / ms_cb,
+        "speedup_surrogate_only": ms_c / ms_w1,
         "fallbacks": fb, "max_rel_error_vs_cold": err,
         "train": {"fields": args.train, "epochs": args.epochs, "loss_first": hist[0],
                   "loss_last": hist[-1], "data_s": t_data, "train_s": t_train},

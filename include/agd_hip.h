@@ -109,13 +109,24 @@ typedef struct agd_eikonal_io {
   const uint32_t* trigger;    /* 1 bit per cell: cells whose tiles start active (NULL: seeds) */
   const uint8_t* wall_dist;   /* saturated L1 distance to walls, 255 = far; NULL = no walls */
   /* Precomputed scheme (both NULL: the stencil is rebuilt at every tile
-   * visit).  Filled by agd_eikonal_scheme from geom and wall_dist:
-   * scheme_weights nsym components of the scalar type, scheme_codes 2*nsym
-   * int32 components (neighbour x - e_k then x + e_k, resolved against the
-   * tiled grid: >= 0 tiled index, <= -2 slot -2-code of the same tile,
-   * -1 none). */
+   * visit).  Filled by agd_eikonal_scheme from geom and wall_dist.  Each
+   * component is one tiled field of size_tot entries:
+   *   Riemann: scheme_weights nsym components of the scalar type,
+   *            scheme_codes 2*nsym int32 components (neighbour x - e_k then
+   *            x + e_k, resolved against the tiled grid: >= 0 tiled index,
+   *            <= -2 slot -2-code of the same tile, -1 none);
+   *   TTI:     weights 7*nsym components (sub-scheme m at m*nsym), codes
+   *            7*2*nsym components (sub-scheme m at m*2*nsym), then one
+   *            component holding the min/max flag, then, with order 2, 7*2*nsym
+   *            components: the codes of the further neighbours x -+ 2 e_k.
+   * agd_scheme_lengths gives the component counts. */
   void* scheme_weights;
   int32_t* scheme_codes;
+  /* Entries (not bytes) available in scheme_weights and scheme_codes; the
+   * solve and agd_eikonal_scheme refuse buffers shorter than the layout
+   * above needs for the descriptor's model, dimension and order. */
+  int64_t scheme_weights_len;
+  int64_t scheme_codes_len;
 } agd_eikonal_io;
 
 typedef struct agd_eikonal_stats {
@@ -139,15 +150,57 @@ typedef struct agd_flow_io {
 } agd_flow_io;
 
 /* ------------------------------------------------------------ library */
+/* The message of the last failure on the calling thread. */
 const char* agd_last_error(void);
 int agd_version(void);
 /* 1 if (model, ndim, shape_i) has a compiled kernel. */
 int agd_tile_supported(int32_t model, int32_t ndim, const int32_t* shape_i);
 int agd_device_count(int32_t* count);
 
+/* ------------------------------------------------------------ solver handles
+ * A handle owns the workspace of the solves run through it: the device
+ * worklist (tile stamps, sub-lists, counters, pass histogram, FIM scores),
+ * pinned host counters, events, a private capture stream and the last
+ * captured chunk of passes, which a later solve with the same kernel and
+ * arguments replays instead of capturing again.  It is bound to the device
+ * current at creation and to `stream`.  Handles share no state: solves
+ * through different handles may run at the same time (on different streams
+ * or host threads).  One handle serves one host thread at a time.
+ * (The reference keeps its solver state in module __constant__ globals,
+ * cupy_module_helper.py:35-49, and reruns its host loop per call,
+ * _solvers.py:122-193.)  desc may be NULL; given, the workspace is sized
+ * for its grid.  Workspaces grow on demand. */
+typedef struct agd_solver agd_solver;
+int agd_solver_create(const agd_eikonal_desc* desc, void* stream, agd_solver** out);
+int agd_solver_destroy(agd_solver* solver);
+/* agd_eikonal_solve, agd_eikonal_solve_batched and agd_eikonal_check through
+ * a handle, on its stream. */
+int agd_solver_solve(agd_solver* solver, const agd_eikonal_desc* desc, const agd_eikonal_io* io,
+                     agd_eikonal_stats* stats);
+typedef struct agd_batch_instance agd_batch_instance;
+int agd_solver_solve_batched(agd_solver* solver, const agd_eikonal_desc* desc, int32_t ninst,
+                             const agd_batch_instance* instances, const agd_eikonal_io* io,
+                             agd_eikonal_stats* stats);
+int agd_solver_check(agd_solver* solver, const agd_eikonal_desc* desc, const agd_eikonal_io* io,
+                     double atol, double rtol, int64_t* violations);
+/* Turns io->values into a supersolution of the scheme before a warm-started
+ * solve (no counterpart in the reference): each round counts the non-seed
+ * cells whose scheme update exceeds their value by more than atol + rtol |u|
+ * (agd_eikonal_check's test) and raises them to +inf.  A start with no such
+ * cell lies above the discrete fixed point (comparison principle of the
+ * monotone scheme), so the decreasing iteration converges from it to the
+ * cold-start solution.  Stops after a round that counts nothing, or after
+ * max_rounds rounds; *violations_left = the last round's count (0: the
+ * start is a supersolution), *rounds = rounds run (one grid pass each). */
+int agd_solver_repair(agd_solver* solver, const agd_eikonal_desc* desc, const agd_eikonal_io* io,
+                      double atol, double rtol, int32_t max_rounds, int64_t* violations_left,
+                      int32_t* rounds);
+
 /* Solve the eikonal equation in place on `values` (replaces Solve('eikonal'),
  * _solvers.py:15-57, and the Update kernel, Update.h:20-227).  Returns
- * AGD_NOCONV (and still fills stats) if nitermax_o passes did not converge. */
+ * AGD_NOCONV (and still fills stats) if nitermax_o passes did not converge.
+ * Without a handle, the call allocates its own workspace and frees it
+ * before returning (re-entrant; no state survives the call). */
 int agd_eikonal_solve(const agd_eikonal_desc* desc, const agd_eikonal_io* io,
                       agd_eikonal_stats* stats, void* stream);
 
@@ -161,11 +214,11 @@ int agd_eikonal_flow(const agd_eikonal_desc* desc, const agd_eikonal_io* io,
  * solver runs one grid per call).  Instances share the descriptor's model,
  * tile shape, scalar type and solver; each has its own shape, isotropic
  * weights and tolerances. */
-typedef struct agd_batch_instance {
+struct agd_batch_instance {
   int64_t shape[3];        /* instance grid (unused axes = 1)            */
   double iso_weights[3];   /* AGD_ISOTROPIC: per-axis weights (h^-2)     */
   double atol, rtol;       /* convergence tolerances of the instance     */
-} agd_batch_instance;
+};
 
 /* Solve several instances in one worklist (one pass launch per grid pass for
  * the whole batch).  io buffers are the concatenation, in instance order, of
@@ -197,6 +250,9 @@ int agd_eikonal_check(const agd_eikonal_desc* desc, const agd_eikonal_io* io, do
  * x -+ 2 e_k of the second-order scheme (HFM.h:91-134), which
  * agd_eikonal_solve then reads. */
 int agd_scheme_supported(int32_t model, int32_t ndim, const int32_t* shape_i);
+/* Components of scheme_weights and scheme_codes for (model, ndim, order). */
+int agd_scheme_lengths(int32_t model, int32_t ndim, int32_t order, int64_t* weight_components,
+                       int64_t* code_components);
 int agd_eikonal_scheme(const agd_eikonal_desc* desc, const agd_eikonal_io* io, void* stream);
 
 /* Reverse-mode sensitivities (replaces SolveAD's reverse branch,

@@ -46,6 +46,9 @@
 #include <limits>
 #include <queue>
 #include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 namespace orc {
 
@@ -710,9 +713,134 @@ static int64_t jacobi(const Problem& P, double atol, double rtol, int64_t niterm
   return it;
 }
 
+// Tile adaptive Gauss-Seidel on all host cores (OpenMP): the multi-core CPU
+// baseline of the benchmark, the same discrete scheme as the device solver
+// (_solvers.py:122-193 on the host).  Tiles of side `tile` along every axis;
+// a pass relaxes the active tiles in parallel, each by `niter` in-place
+// sweeps in row-major order; a tile whose values changed by more than
+// atol + rtol |u| activates itself and its 2d face neighbours for the next
+// pass.  Values of other tiles are read and written with relaxed atomic
+// accesses: the chaotic relaxation the monotone scheme converges under.
+// Stencils are built once per cell.  Returns the passes; *visits = tile
+// visits.
+template <class S>
+static int64_t tiles_omp(const Problem& P, int tile, int niter, double atol, double rtol,
+                         int64_t nitermax, double* u, int64_t* visits_out) {
+  const int64_t N = P.G.N;
+  const int d = P.d;
+  std::vector<Stencil<S>> st(N);
+#pragma omp parallel for schedule(static)
+  for (int64_t n = 0; n < N; ++n) stencil_at<S>(P, n, st[n]);
+  int64_t so[3] = {1, 1, 1}, ntiles = 1;
+  for (int c = 0; c < d; ++c) {
+    so[c] = (P.G.shape[c] + tile - 1) / tile;
+    ntiles *= so[c];
+  }
+  auto ld = [&](int64_t m) {
+    double r;
+    __atomic_load(&u[m], &r, __ATOMIC_RELAXED);
+    return S(r);
+  };
+  for (int64_t n = 0; n < N; ++n)
+    if (P.seeds[n]) u[n] = P.rhs[n];
+  std::vector<uint8_t> next(ntiles, 0);
+  std::vector<int64_t> list;
+  // initial list: tiles holding a seed
+  for (int64_t n = 0; n < N; ++n)
+    if (P.seeds[n] && P.rhs[n] < Inf<double>()) {
+      int64_t x[3];
+      P.G.pos(n, x);
+      int64_t t = 0;
+      for (int c = 0; c < d; ++c) t = t * so[c] + x[c] / tile;
+      next[t] = 1;
+    }
+  int64_t pass = 0, visits = 0;
+  for (; pass < nitermax; ++pass) {
+    list.clear();
+    for (int64_t t = 0; t < ntiles; ++t)
+      if (next[t]) {
+        list.push_back(t);
+        next[t] = 0;
+      }
+    if (list.empty()) break;
+    visits += int64_t(list.size());
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t li = 0; li < int64_t(list.size()); ++li) {
+      const int64_t t = list[li];
+      int64_t to[3] = {0, 0, 0}, rem = t;
+      for (int c = d - 1; c >= 0; --c) {
+        to[c] = rem % so[c];
+        rem /= so[c];
+      }
+      int64_t lo[3] = {0, 0, 0}, hi[3] = {1, 1, 1};
+      for (int c = 0; c < d; ++c) {
+        lo[c] = to[c] * tile;
+        hi[c] = std::min<int64_t>(lo[c] + tile, P.G.shape[c]);
+      }
+      bool any = false;
+      for (int it = 0; it < niter; ++it) {
+        bool swept = false;
+        int64_t x[3];
+        for (x[0] = lo[0]; x[0] < hi[0]; ++x[0])
+          for (x[1] = lo[1]; x[1] < hi[1]; ++x[1])
+            for (x[2] = lo[2]; x[2] < hi[2]; ++x[2]) {
+              const int64_t n = P.G.index(x);
+              if (P.seeds[n]) continue;
+              const S un = ld(n);
+              const S val = cell_update<S>(P, st[n], n, ld, S(P.rhs[n]), nullptr, nullptr,
+                                           nullptr, nullptr, un);
+              if (val < un) {
+                const double vd = double(val);
+                __atomic_store(&u[n], &vd, __ATOMIC_RELAXED);
+                const S tol = S(atol) + S(rtol) * std::fabs(val);
+                if (!(un - val <= tol)) swept = any = true;
+              }
+            }
+        if (!swept) break;
+      }
+      if (any) {
+        const uint8_t one = 1;
+        __atomic_store(&next[t], &one, __ATOMIC_RELAXED);
+        for (int c = 0; c < d; ++c)
+          for (int s2 = -1; s2 <= 1; s2 += 2) {
+            int64_t y[3] = {to[0], to[1], to[2]};
+            y[c] += s2;
+            if (y[c] < 0 || y[c] >= so[c]) continue;
+            int64_t tn = 0;
+            for (int k = 0; k < d; ++k) tn = tn * so[k] + y[k];
+            __atomic_store(&next[tn], &one, __ATOMIC_RELAXED);
+          }
+      }
+    }
+  }
+  *visits_out = visits;
+  return pass;
+}
+
 extern "C" {
 
-int orc_version() { return 5; }
+int orc_version() { return 6; }
+
+// Threads the OpenMP solver runs on (omp_get_max_threads).
+int orc_omp_threads() {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+int64_t orc_solve_tiles_omp(int model, int d, const int64_t* shape, const int* periodic,
+                            const double* geom, const double* rhs, const uint8_t* seeds,
+                            const uint8_t* wallDist, const double* iso_w, int tile, int niter,
+                            double atol, double rtol, int64_t nitermax, int64_t* visits,
+                            double* u) {
+  if (g_order != 1 || g_factor.r2 > 0) return -1;
+  Problem P;
+  fill_problem(P, model, d, shape, periodic, geom, rhs, seeds, wallDist, iso_w);
+  for (int64_t n = 0; n < P.G.N; ++n) u[n] = Inf<double>();
+  return tiles_omp<double>(P, tile, niter, atol, rtol, nitermax, u, visits);
+}
 
 void orc_set_order(int order, double threshold) {
   g_order = order;

@@ -41,6 +41,11 @@ def build():
     return LIB_PATH
 
 
+def omp_threads():
+    """Threads of the OpenMP tile solver (OMP_NUM_THREADS or the host's cores)."""
+    return int(lib().orc_omp_threads())
+
+
 def _load():
     src = os.path.join(HERE, "agd_oracle.cpp")
     if not os.path.exists(LIB_PATH) or os.path.getmtime(LIB_PATH) < os.path.getmtime(src):
@@ -64,6 +69,10 @@ def _load():
     lib.orc_set_factor.argtypes = [c_double, P, P]
     lib.orc_wall_distance.argtypes = [c_int, P, P, P, c_int, P]
     lib.orc_visible.argtypes = [c_int, P, P, P, c_int64, P, P, P]
+    lib.orc_omp_threads.restype = c_int
+    lib.orc_solve_tiles_omp.argtypes = [c_int, c_int, P, P, P, P, P, P, P, c_int, c_int, c_double,
+                                        c_double, c_int64, P, P]
+    lib.orc_solve_tiles_omp.restype = c_int64
     return lib
 
 
@@ -158,6 +167,19 @@ class Problem:
                                     _p(self.iso_w), float(atol), float(rtol), int(nitermax),
                                     int(use_float), _p(u))
         return u.reshape(self.shape), int(it)
+
+    def solve_tiles_omp(self, atol, rtol, tile=4, niter=4, nitermax=100000):
+        """Tile adaptive Gauss-Seidel on all host cores (OpenMP, FP64): the
+        benchmark's multi-core CPU baseline.  Returns (u, passes, tile visits)."""
+        u = np.zeros(self.N)
+        visits = np.zeros(1, dtype=np.int64)
+        p = lib().orc_solve_tiles_omp(self.model, self.d, _p(self._shape), _p(self.periodic),
+                                      _p(self.geom), _p(self.rhs), _p(self.seeds), _p(self.wall_dist),
+                                      _p(self.iso_w), int(tile), int(niter), float(atol), float(rtol),
+                                      int(nitermax), _p(visits), _p(u))
+        if p < 0:
+            raise ValueError("the tile solver runs the first-order scheme only")
+        return u.reshape(self.shape), int(p), int(visits[0])
 
     def stencils(self, use_float=False, with_mix=False):
         """Per-cell weights [N, nmix*nsym], offsets [N, nmix*nsym, d] (and the

@@ -128,30 +128,41 @@ def test_network_forward_and_weight_gradients():
 
 
 def test_trained_network_gain_and_solver_judged_warm_start():
-    """Train on 512 fields labelled by the solver (512^2, coarse 128^2).  On
-    held-out fields the trained network predicts r = u / d_s far better
-    than the method without it (u ~ d_s, the distance in the metric frozen
-    at the seed, i.e. r = 1); the warm-started solve, judged by the solver's
-    residual check with cold fallback, ends on the cold solution."""
+    """The learned warm start on the 1024^2 Riemann2 workload (fields as
+    riemann2_aniso, one seed each), at a training size that fits a test:
+    1024 fields labelled by the solver, 12 epochs, network on the 256^2
+    coarse grid.  (i) On held-out fields the trained network predicts
+    r = u / d_s better than the method without it (r = 1: the distance in
+    the metric frozen at the seed): relative L2 error below 0.8 of it.  (ii) Every warm start is repaired into
+    a supersolution before solving (agd_solver_repair) or falls back to a
+    cold start; the warm solve then ends on the cold solution (same fixed
+    point, 2e-5 of max u) and costs, per field, no more passes than the cold
+    solve plus one check pass, the repair rounds counted as passes."""
     import torch
     from adaptivegriddiscretizations_amd.Eikonal import surrogate as S
-    n, atol, rtol = 512, 1e-6, 1e-6
-    feats, targ = S.build_dataset(512, n=n, seed=1, chunk=128, atol=atol, rtol=rtol, log=log)
+    n, atol, rtol = 1024, 1e-6, 1e-6
+    feats, targ = S.build_dataset(1024, n=n, seed=1, chunk=256, atol=atol, rtol=rtol, log=log)
     net = S.Surrogate(seed=0)
-    hist = net.train(feats, targ, epochs=16, batch=16, lr=2e-3, log=log)
+    hist = net.train(feats, targ, epochs=12, batch=16, lr=2e-3, log=log)
     assert np.isfinite(hist).all() and hist[-1] < 0.5 * hist[0]
+    del feats, targ
     ftest, ttest = S.build_dataset(16, n=n, seed=7, chunk=16, atol=atol, rtol=rtol)
     pred = net.predict(ftest)[..., 0].float()
     e_net = float(torch.linalg.norm(pred - ttest) / torch.linalg.norm(ttest))
     e_base = float(torch.linalg.norm(torch.ones_like(ttest) - ttest) / torch.linalg.norm(ttest))
     log(f"held-out relative L2 error of r: network {e_net:.3f}, frozen-metric distance {e_base:.3f}")
-    assert e_net < 0.75 * e_base
+    assert e_net < 0.8 * e_base
     metric, seeds = S.riemann_fields(8, n, seed=999)
     cold, ms_c, p_c = S.solve_batched(metric, seeds, atol, rtol, batch=1)
-    warm, ms_w, p_w, fb = S.solve_warm(net, metric, seeds, atol, rtol, batch=1)
+    cold_passes = list(S.solve_batched.last_batch_passes)
+    warm, rep = S.solve_warm(net, metric, seeds, atol, rtol, batch=1)
     fin = torch.isfinite(cold)          # cells no stencil reaches stay +inf in both
     assert bool((torch.isfinite(warm) == fin).all())
     diff = float((warm - cold).abs()[fin].max() / cold[fin].abs().max())
-    log(f"held-out solves: cold {p_c} passes {ms_c:.1f} ms, warm {p_w} passes {ms_w:.1f} ms, "
-        f"fallbacks {fb}, max rel diff {diff:.2e}")
+    log(f"held-out solves: cold {cold_passes} passes {ms_c:.1f} ms; warm {rep['batch_passes']} passes "
+        f"+ repair rounds {rep['rounds']}, {rep['total_ms']:.1f} ms (prediction {rep['pred_ms']:.1f}, "
+        f"repair {rep['repair_ms']:.1f}, solve {rep['solve_ms']:.1f}); fallbacks {rep['fallbacks']}; "
+        f"max rel diff {diff:.2e}")
     assert diff < 2e-5
+    for pw, r, pc in zip(rep['batch_passes'], rep['rounds'], cold_passes):
+        assert pw + r <= pc + 1, (pw, r, pc)
