@@ -98,6 +98,8 @@ SIGNATURES = {
                                     c_double, c_int32, POINTER(c_int64), POINTER(c_int32)]),
     "agd_eikonal_solve": (c_int32, [POINTER(EikonalDesc), POINTER(EikonalIO),
                                     POINTER(EikonalStats), c_void_p]),
+    "agd_eikonal_solve_noalloc": (c_int32, [POINTER(EikonalDesc), POINTER(EikonalIO),
+                                            POINTER(EikonalStats), c_void_p]),
     "agd_eikonal_flow": (c_int32, [POINTER(EikonalDesc), POINTER(EikonalIO), POINTER(FlowIO),
                                    c_void_p]),
     "agd_eikonal_solve_batched": (c_int32, [POINTER(EikonalDesc), c_int32, POINTER(BatchInstance),

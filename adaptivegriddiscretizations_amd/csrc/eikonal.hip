@@ -868,6 +868,35 @@ static int check_entry(int32_t* cnt, const agd_eikonal_desc* desc, const agd_eik
   return AGD_OK;
 }
 
+// ------------------------------------------------------------------ fixed region
+// Worklist of agd_eikonal_solve_noalloc: one array per device, held until
+// the process ends.  Host code reaches it only through the address that
+// hipGetSymbolAddress returns on each device (scratch_ptr), never by name.
+constexpr size_t kScratchBytes = size_t(12) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::once_flag g_scratch_once[kMaxDevices];
+static void* g_scratch_addr[kMaxDevices];
+static hipError_t g_scratch_err[kMaxDevices];
+
+// The device address of `scratch` on the current device (looked up once per
+// device, with that device current).
+static int scratch_ptr(void*& out) {
+  int dev = 0;
+  AGD_HIP_TRY(hipGetDevice(&dev));
+  AGD_REQUIRE(dev >= 0 && dev < kMaxDevices, "device index beyond the fixed-region table");
+  std::call_once(g_scratch_once[dev], [dev] {
+    void* p = nullptr;
+    g_scratch_err[dev] = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    g_scratch_addr[dev] = p;
+  });
+  AGD_HIP_TRY(g_scratch_err[dev]);
+  AGD_REQUIRE(g_scratch_addr[dev] != nullptr, "no device address for the fixed region");
+  out = g_scratch_addr[dev];
+  return AGD_OK;
+}
+
 // Repair rounds with the counter `cnt`: each counts the cells below their
 // scheme update and raises them to +inf; stops after a round that counts
 // nothing or after max_rounds rounds.
@@ -896,6 +925,32 @@ static int repair_entry(int32_t* cnt, const agd_eikonal_desc* desc, const agd_ei
 }  // namespace agd
 
 extern "C" {
+
+int agd_eikonal_solve_noalloc(const agd_eikonal_desc* desc, const agd_eikonal_io* io,
+                              agd_eikonal_stats* stats, void* stream) {
+  int st = check_desc(desc);
+  if (st) return st;
+  // Sizes first (no device call): the worklist this solve needs.
+  const GridDesc g = make_grid(desc);
+  int ni = 1;
+  for (int k = 0; k < desc->ndim; ++k) ni *= desc->shape_i[k];
+  const int64_t scap = shard_capacity(g.ntiles, blocks_for(g.ntiles, ni));
+  const int64_t nhist = hist_len(desc->nitermax_o, desc->check_interval);
+  const size_t need = Workspace::fixed_bytes(g.ntiles, scap, nhist);
+  if (need > kScratchBytes) {
+    set_error("invalid argument: the solve needs " + std::to_string(need) +
+              " bytes of worklist, beyond the fixed region of " + std::to_string(kScratchBytes) +
+              " bytes (about 150,000 tiles and 2,000 passes); use agd_eikonal_solve");
+    return AGD_EINVAL;
+  }
+  void* base = nullptr;
+  st = scratch_ptr(base);
+  if (st) return st;
+  Workspace ws;  // views the fixed region: nothing is allocated or freed
+  st = ws.bind_fixed(base, kScratchBytes, g.ntiles, scap, nhist);
+  if (st) return st;
+  return solve_entry(ws, desc, io, stats, as_stream(stream));
+}
 
 int agd_solver_create(const agd_eikonal_desc* desc, void* stream, agd_solver** out) {
   AGD_REQUIRE(out != nullptr, "null output");

@@ -204,6 +204,19 @@ int agd_solver_repair(agd_solver* solver, const agd_eikonal_desc* desc, const ag
 int agd_eikonal_solve(const agd_eikonal_desc* desc, const agd_eikonal_io* io,
                       agd_eikonal_stats* stats, void* stream);
 
+/* agd_eikonal_solve without any allocation (no counterpart in the
+ * reference): for callers that cannot allocate device memory, such as code
+ * under stream capture, tight serving loops or harnesses that forbid
+ * hipMalloc.  The worklist lives in a fixed 12 MiB device array of the
+ * library (one per device, held until the process ends), which bounds the
+ * grid to about 150,000 tiles (200^3 cells in 4x4x4 tiles, 3000^2 in 8x8)
+ * and nitermax_o to about 2,000 passes; larger solves are refused with
+ * AGD_EINVAL.  Passes launch eagerly and stats->gpu_ms is 0 (no events).
+ * Every call on a device shares that device's array, so calls that overlap
+ * on one device (several streams or threads) race: serialise them. */
+int agd_eikonal_solve_noalloc(const agd_eikonal_desc* desc, const agd_eikonal_io* io,
+                              agd_eikonal_stats* stats, void* stream);
+
 /* One global pass exporting the upwind flow (replaces Solve('flow'),
  * _PostProcess.py:79, Update.h:171-209).  Updates `values` like the
  * reference's flow pass (DECREASING rule).  */
