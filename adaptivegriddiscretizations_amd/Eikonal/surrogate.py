@@ -356,16 +356,19 @@ def solve_batched(metric, seeds, atol, rtol, values=None, batch=16, check=False)
     return (out, ms, passes, bad) if check else (out, ms, passes)
 
 
-def solve_warm(net, metric, seeds, atol, rtol, batch=16, inflate=1.05, max_rounds=8):
+def solve_warm(net, metric, seeds, atol, rtol, batch=16, inflate=1.05, max_rounds=1):
     """Warm-started solves with the solver as judge before solving: every
-    field's start (the inflated prediction) is repaired into a supersolution
-    of the scheme by agd_solver_repair (cells below their scheme update are
-    raised to +inf, round after round, each round one pass over the grid);
-    a field still violating after max_rounds rounds starts cold (+inf), so a
-    bad prediction costs its repair rounds and nothing else.  Every tile
-    starts active.  Returns the values and a dict: GPU ms of the prediction,
-    the repair and the batched solves, solve passes (summed over batches),
-    repair rounds per field, cold fallbacks."""
+    field's start (the inflated prediction) is checked against the scheme by
+    agd_solver_repair: a start whose cells all lie at or above their scheme
+    update is a supersolution, from which the decreasing iteration reaches
+    the cold-start fixed point.  With max_rounds = 1 the check is one pass
+    and any violation sends the field to a cold start (+inf), so a bad
+    prediction costs one pass; with more rounds the violating cells are
+    raised to +inf and the start checked again (a repair that cascades
+    downstream of every violation).  Every tile starts active.  Returns the
+    values and a dict: GPU ms of the prediction, the check and the batched
+    solves, solve passes (summed, and per batch), check rounds and
+    violations found per field, cold fallbacks."""
     from . import batch as B
     count, _, n, _ = metric.shape
     ev = lambda: torch.cuda.Event(enable_timing=True)
@@ -375,7 +378,7 @@ def solve_warm(net, metric, seeds, atol, rtol, batch=16, inflate=1.05, max_round
     e1.record()
     e1.synchronize()
     rep = {'pred_ms': e0.elapsed_time(e1), 'repair_ms': 0., 'solve_ms': 0., 'passes': 0,
-           'batch_passes': [], 'rounds': [], 'fallbacks': 0}
+           'batch_passes': [], 'rounds': [], 'violations': [], 'fallbacks': 0}
     out = torch.empty((count, n, n), dtype=torch.float32, device=metric.device)
     inf = float('inf')
     for b0 in range(0, count, batch):
@@ -387,6 +390,7 @@ def solve_warm(net, metric, seeds, atol, rtol, batch=16, inflate=1.05, max_round
         for it in b.ifaces:
             left, rounds = b.solver.repair(it._desc(), it._io(), atol, rtol, max_rounds)
             rep['rounds'].append(rounds)
+            rep['violations'].append(left)
             if left > 0:
                 it.values_t.fill_(inf)
                 rep['fallbacks'] += 1

@@ -251,7 +251,8 @@ def main():
                         f"{'s' if len(wl.seeds) > 1 else ''}, cold start to convergence",
             "model": wl.model, "grid": list(wl.dims), "tile": list(iface.shape_i),
             "niter_i": iface.niter_i, "solver": iface.solver, "atol": iface.atol, "rtol": iface.rtol,
-            "parallelism": "single GPU", "pass_launch": "eager" if args.no_graph else "hipGraph",
+            "parallelism": "single GPU",
+            "pass_launch": "eager" if args.no_graph else "hipGraph",
             "precompute_scheme": bool(iface.precompute_scheme),
         },
         "solved_cells_per_s": wl.ncells * args.steps / wall,

@@ -10,7 +10,7 @@ def log(msg):
     sys.stdout.flush()
 
 
-def oracle_problem(O, wl, walls=None, seed_values=None, cast=True):
+def oracle_problem(O, wl, walls=None, seed_values=None, cast=True, periodic=None):
     """Oracle problem matching workload `wl` (FP64 arithmetic).  With cast,
     geometry and rhs are rounded to float32 as the FP32 device path stores
     them; the FP64 device path keeps them in double."""
@@ -27,7 +27,7 @@ def oracle_problem(O, wl, walls=None, seed_values=None, cast=True):
         kw['metric'] = wl.extra['costs'].astype(np.float64)
     cost = 1. if wl.cost is None else wl.cost.astype(np.float64)
     P = O.make_problem(model, wl.dims, h, cost=cost, seeds_idx=idx, seed_values=seed_values,
-                       walls=walls, **kw)
+                       walls=walls, periodic=periodic, **kw)
     return (P.cast(np.float32) if cast else P), idx
 
 

@@ -66,3 +66,54 @@ def test_interpolation_reproduces_multilinear_fields():
     val = np.sum(weights * f[corners[..., 0], corners[..., 1]], axis=0)
     ex = 1. + 0.2 * pts[:, 0] - 0.1 * pts[:, 1] + 0.03 * pts[:, 0] * pts[:, 1]
     np.testing.assert_allclose(val, ex, rtol=1e-13)
+
+
+def _reflect_spline1(values, pts, periodic):
+    """The reference's order-1 spline evaluation with its 'reflect' boundary
+    (Interpolation.py:113-147, spline_coefs returning the values themselves
+    at order 1): base index floor(x), weights (1 - t, t) on (base, base + 1),
+    indices wrapped modulo 2n then mirrored (i >= n -> 2n - 1 - i) on
+    non-periodic axes, modulo n on periodic ones."""
+    d = pts.shape[1]
+    out = np.zeros(len(pts))
+    b = np.floor(pts).astype(np.int64)
+    t = pts - b
+    for m in range(1 << d):
+        hi = np.array([(m >> c) & 1 for c in range(d)])
+        idx = b + hi
+        w = np.prod(np.where(hi == 1, t, 1 - t), axis=1)
+        for c in range(d):
+            n = values.shape[c]
+            if periodic[c]:
+                idx[:, c] = np.mod(idx[:, c], n)
+            else:
+                i = np.mod(idx[:, c], 2 * n)
+                idx[:, c] = np.where(i >= n, 2 * n - 1 - i, i)
+        out += w * values[tuple(idx.T)]
+    return out
+
+
+@pytest.mark.parametrize("periodic", [(False, False), (True, False)])
+def test_interpolation_matches_reference_reflect_mode(periodic):
+    """The cost metric at an off-grid seed and its spread neighbours is
+    interpolated at order 1: the product clamps positions onto the grid, the
+    reference mirrors indices ('reflect').  For positions in [-1, n] on a
+    non-periodic axis -- every seed and every spread neighbour lies in the
+    box [-1/2, n - 1/2] -- both give the same value (index -1 mirrors to 0,
+    index n to n - 1, each with its linear weight); beyond, they differ.
+    Periodic axes wrap in both."""
+    rng = np.random.default_rng(11)
+    vals = rng.uniform(size=SHAPE)
+    lo = np.array([-1. if not p else -7. for p in periodic])
+    hi = np.array([n if not p else n + 7. for n, p in zip(SHAPE, periodic)])
+    pts = lo + (hi - lo) * rng.uniform(size=(4000, 2))
+    pts[:20] = np.array([[-1., 0.], [SHAPE[0], 3.], [-0.5, SHAPE[1] - 0.5], [0., SHAPE[1]]] * 5)
+    corners, weights = F.interpolation_stencil(pts, SHAPE, periodic)
+    ours = np.sum(weights * vals[tuple(np.moveaxis(corners, -1, 0))], axis=0)
+    ref = _reflect_spline1(vals, pts, periodic)
+    assert np.max(np.abs(ours - ref)) < 1e-12
+    if not periodic[0]:
+        far = np.array([[-1.6, 3.2], [SHAPE[0] + 0.7, 4.1]])
+        c2, w2 = F.interpolation_stencil(far, SHAPE, periodic)
+        ours2 = np.sum(w2 * vals[tuple(np.moveaxis(c2, -1, 0))], axis=0)
+        assert np.max(np.abs(ours2 - _reflect_spline1(vals, far, periodic))) > 1e-3
