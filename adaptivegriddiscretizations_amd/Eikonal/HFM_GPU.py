@@ -590,6 +590,7 @@ class Interface:
         d.check_interval = getattr(self, 'check_interval', 18)
         d.activate_all = int(getattr(self, 'warmStartAllTiles', False))
         d.use_graph = int(getattr(self, 'use_graph', True))
+        d.time_launches = int(getattr(self, 'time_launches', False))
         d.order = int(getattr(self, 'order', 1))
         d.order2_threshold = float(getattr(self, 'order2_threshold', 0.3))
         if getattr(self, 'bound_active_blocks', 0):

@@ -146,7 +146,7 @@ def main():
     ap.add_argument('--steps', type=int, default=10)
     ap.add_argument('--warmup', type=int, default=2)
     ap.add_argument('--config', default='riemann3',
-                    choices=['riemann3', 'riemann2', 'isotropic2', 'batch256', 'warmstart'])
+                    choices=['riemann3', 'riemann2', 'isotropic2', 'batch256', 'warmstart', 'adjoint'])
     ap.add_argument('--train', type=int, default=4096, help='warmstart: training fields')
     ap.add_argument('--test', type=int, default=512, help='warmstart: held-out fields')
     ap.add_argument('--epochs', type=int, default=12, help='warmstart: training epochs')
@@ -184,6 +184,8 @@ def main():
         return bench_batch(args)
     if args.config == 'warmstart':
         return bench_warmstart(args)
+    if args.config == 'adjoint':
+        return bench_adjoint(args)
     t0 = time.time()
     wl = make_workload(args.config)
     log(f"workload {wl.name} {wl.dims} generated in {time.time() - t0:.1f} s")
@@ -347,6 +349,77 @@ def bench_batch(args):
         "sequential_ms": seq * 1e3,
         "batching_speedup": seq / (wall / args.steps),
         "max_rel_diff_vs_single": worst,
+    }
+    print(json.dumps(result), flush=True)
+
+
+def bench_adjoint(args):
+    """Reverse pass of the 200^3 Riemann3 workload: the gradient of the sum of
+    its 64 receivers' arrival times with respect to the cost (what the
+    torch.autograd.Function's backward runs: transposed flow graph, then the
+    adjoint tile Gauss-Seidel), after the forward solve and flow export.  A
+    step = one backward pass; the adjoint update kernel is timed per pass in
+    one more backward with an event pair around every pass."""
+    import numpy as np
+    import torch
+    wl = make_workload('riemann3')
+    iface = prepare(wl)
+    iface.keep_flow = True
+    e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    e[0].record()
+    iface.Solve()
+    e[1].record()
+    iface.PostProcess()
+    e[2].record()
+    e[2].synchronize()
+    fwd = {"solve_ms": e[0].elapsed_time(e[1]), "flow_ms": e[1].elapsed_time(e[2])}
+    sens = torch.zeros(wl.dims + (1,), device='cuda', dtype=torch.float32)
+    for r in wl.receivers:
+        sens[tuple(int(x) for x in r) + (0,)] += 1.
+    for _ in range(args.warmup):
+        iface.reverse(sens)
+    torch.cuda.synchronize()
+    times = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        e[0].record()
+        lam, _ = iface.reverse(sens)
+        e[1].record()
+        e[1].synchronize()
+        times.append(e[0].elapsed_time(e[1]))
+    wall = time.perf_counter() - t0
+    st = iface.stats['reverseAD']
+    # roofline of the adjoint update: per-pass event timing
+    iface.time_launches = True
+    iface.reverse(sens)
+    st_t = dict(iface.stats['reverseAD'])
+    iface.time_launches = False
+    # bytes per adjoint cell-update: lambda read + write, right-hand side,
+    # diagonal, row pointers (2), in-tile split, and the row's (source, weight)
+    # pairs; the mean row length is the flow graph's nonzeros per cell
+    fw, fi = iface.flow_weights_t, iface.flow_indices_t
+    nnz = int(((fw != 0) & (fi != np.iinfo(np.int32).max)).sum())
+    row = nnz / wl.ncells
+    bpc = 8 + 4 + 4 + 8 + 4 + 8 * row
+    achieved = st_t['cell_updates'] * bpc / (st_t['kernel_ms'] * 1e-3) / 1e9
+    log(f"forward: solve {fwd['solve_ms']:.2f} ms, flow {fwd['flow_ms']:.2f} ms; backward "
+        f"{np.mean(times):.2f} ms, passes {st['niter_o']}, cell-updates {st['cell_updates']}, "
+        f"adjoint kernel {st_t['kernel_ms']:.2f} ms, mean row length {row:.2f}")
+    result = {
+        "metric": "converged cell-updates/s and %HBM roofline; time-to-solution warm vs cold @1e-6",
+        "value": st['cell_updates'] * args.steps / (sum(times) * 1e-3), "unit": "adjoint cell-updates/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{wl.name} {'x'.join(map(str, wl.dims))}, gradient of the sum of "
+                               f"{len(wl.receivers)} receivers' arrival times w.r.t. the cost (one backward pass)",
+                   "model": wl.model, "parallelism": "single GPU"},
+        "forward_ms": fwd, "backward_ms": float(np.mean(times)),
+        "passes_per_step": st['niter_o'], "cell_updates_per_step": st['cell_updates'],
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "adj_update",
+                     "bytes_per_cell_update": bpc, "mean_row_length": row,
+                     "avg_launch_us": st_t['kernel_ms'] / max(st_t['launches'], 1) * 1e3},
+        "cpu_baseline": None,
     }
     print(json.dumps(result), flush=True)
 
