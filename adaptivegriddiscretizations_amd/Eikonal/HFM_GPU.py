@@ -160,8 +160,6 @@ class Interface:
             help="Use source factorization, to improve accuracy"))
         if self.factoringRadius < 0:
             raise ValueError("factoringRadius must be >= 0")
-        if self.factoringRadius and self.model_ == 'TTI':
-            raise NotImplementedError("source factorization of the TTI model is not provided")
         if self.factoringRadius:
             # The point choice concerns the CPU solver's factorization; the
             # GPU kernel always freezes the metric at the seed.
@@ -400,7 +398,9 @@ class Interface:
         """The metric with the grid steps and the cost folded in, evaluated
         at a few points (_Interface.py:202-211)."""
         metric = drift = None
-        if self.model_ == 'Diagonal':
+        if self.model_ == 'TTI':
+            metric = (self.metric.linear, self.metric.quadratic, self.metric.inverse_transformation)
+        elif self.model_ == 'Diagonal':
             metric = self.metric.costs
         elif self.model_ in ('Riemann', 'Rander'):
             metric = self.metric.m
@@ -543,7 +543,8 @@ class Interface:
         # it (seven sub-schemes per cell exceed the register budget).
         self.scheme_w_t = self.scheme_c_t = None
         self.precompute_scheme = False
-        if (self.order == 1 or self.model_ == 'TTI') and not self.factoringRadius and _lib.lib.agd_scheme_supported(
+        if (self.order == 1 or self.model_ == 'TTI') and (not self.factoringRadius or self.model_ == 'TTI') \
+                and _lib.lib.agd_scheme_supported(
                 self.modelId, self.ndim, _lib.i32(self.shape_i)):
             self.precompute_scheme = bool(self.GetValue(
                 'precompute_scheme', default=self.model_ == 'TTI',

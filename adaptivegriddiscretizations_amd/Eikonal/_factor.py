@@ -64,13 +64,51 @@ def interpolation_stencil(pts, shape, periodic):
     return np.stack(corners), np.stack(weights)
 
 
+def tti_norm(l, q3, A, v, niter=6):
+    """The TTI norm N(v) = N0(A v) at rows v[k, d]: N0(w) is the largest <p, w>
+    over the dual ball f(p) <= 0, f(p) = l0 P0 + l1 P1 + (q00 P0^2 +
+    2 q10 P0 P1 + q11 P1^2) / 2 - 1 with P0 = X^2 [+ Y^2] and P1 = Z^2 the
+    squares of p's lateral and last coordinates (Seismic/tti.py:64-69).  The
+    maximiser is found by sequential quadratic programming from p = 0
+    (Seismic/implicit_base.py:83-95, 140-179: niter steps
+    p += M (k w - grad f), M = hess f^-1, k = sqrt((<grad f, M grad f> - 2 f)
+    / <w, M w>)); N0(w) = <p, w>.  l[k, 2], q3[k, 3] = (q00, q10, q11),
+    A[k, d, d] (each also broadcast from one row)."""
+    v = np.atleast_2d(np.asarray(v, dtype=np.float64))
+    k, d = v.shape
+    l = np.broadcast_to(np.asarray(l, dtype=np.float64).reshape(-1, 2), (k, 2))
+    q3 = np.broadcast_to(np.asarray(q3, dtype=np.float64).reshape(-1, 3), (k, 3))
+    A = np.broadcast_to(np.asarray(A, dtype=np.float64).reshape(-1, d, d), (k, d, d))
+    w = np.einsum('kij,kj->ki', A, v)
+    zero = np.all(w == 0., axis=1)
+    w = np.where(zero[:, None], 1., w)
+    cls = np.array([0] * (d - 1) + [1])                  # lateral (0) or last (1) coordinate
+    qm = np.stack([np.stack([q3[:, 0], q3[:, 1]], -1), np.stack([q3[:, 1], q3[:, 2]], -1)], -2)
+    qc = qm[:, cls][:, :, cls]                            # q between the coordinates' classes
+    p = np.zeros_like(w)
+    for _ in range(niter):
+        P = np.stack([np.sum(p[:, :d - 1] ** 2, axis=1), p[:, d - 1] ** 2], -1)
+        g = l + np.einsum('kij,kj->ki', qm, P)            # df / dP
+        f = np.sum(l * P, axis=1) + 0.5 * np.einsum('ki,kij,kj->k', P, qm, P) - 1.
+        gc = g[:, cls]
+        V = 2. * p * gc                                   # grad f
+        H = 2. * gc[:, :, None] * np.eye(d) + 4. * qc * p[:, :, None] * p[:, None, :]
+        M = np.linalg.inv(H)
+        kk = np.sqrt((np.einsum('ki,kij,kj->k', V, M, V) - 2. * f) / np.einsum('ki,kij,kj->k', w, M, w))
+        p = p + np.einsum('kij,kj->ki', M, kk[:, None] * w - V)
+    return np.where(zero, 0., np.sum(p * w, axis=1))
+
+
 class CostMetric:
     """The metric of a solve in grid units at a few points: Isotropic costs
     h_i c, Diagonal costs_i h_i c, Riemann m_ij h_i h_j c^2, Rander
-    (m_ij h_i h_j c^2, w_i h_i c), with c the cost (Metrics.*.with_costs(h)
-    followed by with_cost(cost)), interpolated at order 1.
+    (m_ij h_i h_j c^2, w_i h_i c), TTI (l, q, A diag(h) c), with c the cost
+    (Metrics.*.with_costs(h) followed by with_cost(cost)), interpolated at
+    order 1.
 
-    diag[k, d] (diagonal models) or m[k, d, d] and w[k, d] (or None)."""
+    diag[k, d] (diagonal models), m[k, d, d] and w[k, d] (or None), or for
+    TTI the tuple (l[k, 2], q3[k, 3], A[k, d, d]); metric is then the tuple
+    (linear, quadratic, inverse_transformation or None)."""
 
     def __init__(self, model, h, cost, metric=None, drift=None):
         self.model = model
@@ -82,6 +120,14 @@ class CostMetric:
     def _at_cells(self, cells):
         d = len(self.h)
         c = gather_cells(self.cost, (), cells)
+        if self.model == 'TTI':
+            lin, quad, A = self.metric
+            l = gather_cells(lin, (2,), cells)
+            q = gather_cells(quad, (2, 2), cells)
+            q3 = np.stack([q[:, 0, 0], q[:, 1, 0], q[:, 1, 1]], axis=1)
+            Am = (np.broadcast_to(np.eye(d), (len(cells), d, d)) if A is None
+                  else gather_cells(A, (d, d), cells))
+            return (l, q3, Am * self.h[None, None, :] * c[:, None, None]), None
         if self.model in DIAGONAL_MODELS:
             costs = np.ones((len(cells), d)) if self.model == 'Isotropic' \
                 else gather_cells(self.metric, (d,), cells)
@@ -98,16 +144,21 @@ class CostMetric:
         corners, weights = interpolation_stencil(pts, shape, periodic)
         nc, k, d = corners.shape
         a, w = self._at_cells(corners.reshape(-1, d))
-        a = a.reshape((nc, k) + a.shape[1:])
-        wt = weights.reshape((nc, k) + (1,) * (a.ndim - 2))
-        a = np.sum(a * wt, axis=0)
+
+        def interp(f):
+            f = f.reshape((nc, k) + f.shape[1:])
+            return np.sum(f * weights.reshape((nc, k) + (1,) * (f.ndim - 2)), axis=0)
+
+        a = tuple(interp(f) for f in a) if isinstance(a, tuple) else interp(a)
         if w is not None:
-            w = np.sum(w.reshape(nc, k, d) * weights[..., None], axis=0)
+            w = interp(w)
         return a, w
 
     def norm(self, a, w, v):
         """|v|: sqrt(sum (c_i v_i)^2), or sqrt(v m v) (+ <w, v> for Rander)."""
         v = np.asarray(v, dtype=np.float64)
+        if self.model == 'TTI':
+            return tti_norm(a[0], a[1], a[2], v.reshape(-1, v.shape[-1]))
         if self.model in DIAGONAL_MODELS:
             return np.sqrt(np.sum((a * v) ** 2, axis=-1))
         out = np.sqrt(np.einsum('...i,...ij,...j->...', v, a, v))
@@ -117,8 +168,12 @@ class CostMetric:
 
     def factor_metric(self, a):
         """The factorization metric of one point (_Kernel.py:304-308): the
-        squared costs (diagonal models), else the lower triangle
-        (m00, m10, m11, m20, m21, m22) of m; the drift is left out."""
+        squared costs (diagonal models), [l0, l1, q00, q10, q11, A row-major]
+        (TTI), else the lower triangle (m00, m10, m11, m20, m21, m22) of m;
+        the drift is left out."""
+        if self.model == 'TTI':
+            return [float(x) for x in np.concatenate([np.asarray(f, dtype=np.float64).reshape(-1)
+                                                      for f in a])]
         a = np.asarray(a, dtype=np.float64)
         if self.model in DIAGONAL_MODELS:
             return list((a.reshape(-1)) ** 2)

@@ -34,7 +34,7 @@ class EikonalDesc(ctypes.Structure):
         ("iso_weights", c_double * 3), ("check_interval", c_int32), ("activate_all", c_int32),
         ("time_launches", c_int32), ("use_graph", c_int32),
         ("order", c_int32), ("reserved", c_int32), ("order2_threshold", c_double),
-        ("factor_radius2", c_double), ("factor_origin", c_double * 3), ("factor_metric", c_double * 6),
+        ("factor_radius2", c_double), ("factor_origin", c_double * 3), ("factor_metric", c_double * 14),
         ("bound_active_blocks", c_int64), ("minchg_delta_min", c_double),
     ]
 

@@ -612,6 +612,140 @@ __device__ inline void tti_sub(const S* D0, const S* D1, S d0, S d1, S* w, int (
   else selling3<S>(m, w, e);
 }
 
+// ------------------------------------------------------------------ TTI factor
+// Source factorization of the TTI model (TTI_.h:139-212, GoldenSearch.h,
+// EuclideanFactor.h:34-50).  The factor metric is
+// fm = [l0, l1, q00, q10, q11, A (D x D, row-major)], the norm N(x) = N0(A x)
+// with N0 the primal norm of the dual ball l(s) + 1/2 q(s, s) = 1,
+// s = (X^2 [+ Y^2], Z^2).  N0 is approached by a golden-section search over
+// the tangent ellipses of that ball: ellipse v (v in [a, -b], the contact
+// point x = ((u+v)/2, (u-v)/2) on the curve) has the primal norm
+// sqrt(s0 / g0 + s1 / g1), g the gradient ratio at x; the search maximises
+// or minimises as the determinant of the first two ellipses says.  Written
+// without contraction, as the oracle.
+constexpr int kGoldenSteps = 8;  // niter_golden_search (TTI_.h:141-143)
+
+template <class S> struct TTIFactorData {
+  S l[2], q[3];
+  S L0, L1, Q0, Q1, Q2;  // l and q in the basis (1/2, 1/2), (1/2, -1/2)
+  S a, b;                // (a, 0) and (0, b) lie on the curve
+};
+
+template <class S> __device__ inline void tti_factor_data(const S* fm, TTIFactorData<S>& t) {
+#pragma clang fp contract(off)
+  t.l[0] = fm[0];
+  t.l[1] = fm[1];
+  t.q[0] = fm[2];
+  t.q[1] = fm[3];
+  t.q[2] = fm[4];
+  t.a = tti_solve2_above<S>(S(-2), t.l[0], t.q[0], S(0));
+  t.b = tti_solve2_above<S>(S(-2), t.l[1], t.q[2], S(0));
+  t.L0 = t.l[0] * S(0.5) + t.l[1] * S(0.5);
+  t.L1 = t.l[0] * S(0.5) + t.l[1] * S(-0.5);
+  t.Q0 = tti_vqv<S>(S(0.5), S(0.5), t.q, S(0.5), S(0.5));
+  t.Q1 = tti_vqv<S>(S(0.5), S(0.5), t.q, S(0.5), S(-0.5));
+  t.Q2 = tti_vqv<S>(S(0.5), S(-0.5), t.q, S(0.5), S(-0.5));
+}
+
+// Primal norm of the ellipse of parameter v at s; (d0, d1) = 1 / g.
+template <class S>
+__device__ inline S tti_ellipse_norm(const TTIFactorData<S>& t, S s0, S s1, S v, S& d0, S& d1) {
+#pragma clang fp contract(off)
+  const S u = tti_solve2_above<S>(S(-2) + S(2) * t.L1 * v + t.Q2 * v * v, t.L0 + t.Q1 * v, t.Q0, abs_(v));
+  S g0, g1;
+  tti_grad_ratio<S>(t.l, t.q, (u + v) / S(2), (u - v) / S(2), g0, g1);
+  d0 = S(1) / g0;
+  d1 = S(1) / g1;
+  return sqrt_(s0 * d0 + s1 * d1);
+}
+
+// N(x); grad (nullable) receives A^T (diag(d) A x) / N(x), the gradient of
+// the last ellipse of the search.
+template <class S, int D>
+__device__ inline S tti_factor_norm(const S* fm, const TTIFactorData<S>& t, const S* x, S* grad) {
+#pragma clang fp contract(off)
+  const S* A = fm + 5;
+  S Ax[D];
+#pragma unroll
+  for (int i = 0; i < D; ++i) {
+    S r = S(0);
+#pragma unroll
+    for (int j = 0; j < D; ++j) r += A[i * D + j] * x[j];
+    Ax[i] = r;
+  }
+  S s0 = Ax[0] * Ax[0];
+  const S s1 = Ax[D - 1] * Ax[D - 1];
+  if constexpr (D == 3) s0 += Ax[1] * Ax[1];
+  const S phi = S(0.6180339887498949), psi = S(0.3819660112501051);
+  S mid[2] = {phi * t.a + psi * (-t.b), psi * t.a + phi * (-t.b)};
+  S val[2], e0[2], e1[2];
+  val[0] = tti_ellipse_norm<S>(t, s0, s1, mid[0], e0[0], e0[1]);
+  val[1] = tti_ellipse_norm<S>(t, s0, s1, mid[1], e1[0], e1[1]);
+  const bool mix_min = e0[0] * e1[1] - e0[1] * e1[0] <= S(0);
+  S d0 = S(0), d1 = S(0);
+  S norm = val[1];
+  for (int i = 0; i < kGoldenSteps; ++i) {
+    const S dx = mid[1] - mid[0];
+    if (mix_min == (val[0] < val[1])) {
+      mid[1] = mid[0];
+      val[1] = val[0];
+      mid[0] -= dx * phi;
+      val[0] = tti_ellipse_norm<S>(t, s0, s1, mid[0], d0, d1);
+      norm = val[0];
+    } else {
+      mid[0] = mid[1];
+      val[0] = val[1];
+      mid[1] += dx * phi;
+      val[1] = tti_ellipse_norm<S>(t, s0, s1, mid[1], d0, d1);
+      norm = val[1];
+    }
+  }
+  if (grad) {
+#pragma unroll
+    for (int i = 0; i < D; ++i) Ax[i] = Ax[i] * (((D == 2 ? i : i - 1) <= 0 ? d0 : d1) / norm);
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      S r = S(0);
+#pragma unroll
+      for (int i = 0; i < D; ++i) r += A[i * D + j] * Ax[i];
+      grad[j] = r;
+    }
+  }
+  return norm;
+}
+
+// Shifts of the neighbours x - e and x + e (fact), and of x - 2e and x + 2e
+// (fact2, nullable) of the cell at x (relative to the origin), given
+// nx = N(x) and its gradient (generic_factor_sym, EuclideanFactor.h:34-50).
+template <class S, int D>
+__device__ inline void tti_factor_pair(const S* fm, const TTIFactorData<S>& t, const S* x, S nx,
+                                       const S* grad, const int* e, S* fact, S* fact2) {
+#pragma clang fp contract(off)
+  S xp[D], xm[D];
+  S ge = S(0);
+#pragma unroll
+  for (int c = 0; c < D; ++c) {
+    xp[c] = x[c] + S(e[c]);
+    xm[c] = x[c] - S(e[c]);
+    ge += grad[c] * S(e[c]);
+  }
+  const S np = tti_factor_norm<S, D>(fm, t, xp, nullptr);
+  const S nm = tti_factor_norm<S, D>(fm, t, xm, nullptr);
+  fact[0] = -ge + nx - nm;
+  fact[1] = ge + nx - np;
+  if (fact2) {
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      xp[c] = xp[c] + S(e[c]);
+      xm[c] = xm[c] - S(e[c]);
+    }
+    const S np2 = tti_factor_norm<S, D>(fm, t, xp, nullptr);
+    const S nm2 = tti_factor_norm<S, D>(fm, t, xm, nullptr);
+    fact2[0] = S(2) * fact[0] - (nx - S(2) * nm + nm2);
+    fact2[1] = S(2) * fact[1] - (nx - S(2) * np + np2);
+  }
+}
+
 // ------------------------------------------------------------------ update
 // Compare-exchange of the sorting network (NetworkSort.h:17-22): position i
 // keeps its pair when v[i] < v[j], otherwise the pairs swap (ties swap).

@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <mutex>
 #include <new>
@@ -27,7 +28,7 @@
 #include "eikonal_kernels.hpp"
 #include "worklist.hpp"
 
-static_assert(sizeof(agd_eikonal_desc) == 248, "agd_eikonal_desc layout changed");
+static_assert(sizeof(agd_eikonal_desc) == 312, "agd_eikonal_desc layout changed");
 static_assert(sizeof(agd_eikonal_io) == 80, "agd_eikonal_io layout changed");
 
 namespace agd {
@@ -290,8 +291,6 @@ static int check_desc(const agd_eikonal_desc* d) {
     AGD_REQUIRE(ceil_div(d->shape[k], d->shape_i[k]) <= (d->ndim == 2 ? 32768 : 1024),
                 "too many tiles along an axis (2D: 32768, 3D: 1024)");
   AGD_REQUIRE(d->factor_radius2 >= 0., "factor_radius2 must be >= 0 (0 disables factorization)");
-  AGD_REQUIRE(d->factor_radius2 == 0. || d->model != AGD_TTI,
-              "source factorization is not provided for the TTI model");
   return AGD_OK;
 }
 
@@ -301,8 +300,32 @@ template <class S> static FactorDesc<S> make_factor(const agd_eikonal_desc* d) {
   FactorDesc<S> f{};
   f.r2 = S(d->factor_radius2);
   for (int k = 0; k < 3; ++k) f.origin[k] = S(d->factor_origin[k]);
-  for (int k = 0; k < 6; ++k) f.m[k] = S(d->factor_metric[k]);
+  for (int k = 0; k < 14; ++k) f.m[k] = S(d->factor_metric[k]);
+  if (d->factor_radius2 > 0.) {
+    // The box of the ball, one cell wider on each side than the exact one
+    // (the device tests membership in the scalar type), clipped to the grid.
+    const double r = std::sqrt(d->factor_radius2);
+    f.nbox = 1;
+    for (int k = 0; k < 3; ++k) {
+      f.lo[k] = 0;
+      f.n[k] = 1;
+    }
+    for (int k = 0; k < d->ndim; ++k) {
+      const double lo = std::max(0., std::floor(d->factor_origin[k] - r) - 1.);
+      const double hi = std::min(double(d->shape[k] - 1), std::ceil(d->factor_origin[k] + r) + 1.);
+      f.lo[k] = int32_t(lo);
+      f.n[k] = hi >= lo ? int32_t(hi - lo + 1.) : 0;
+      f.nbox *= f.n[k];
+    }
+  }
   return f;
+}
+
+// Scalars of the TTI shift table: NMIX * 2 * nsym shifts (twice that with
+// order 2) per cell of the factor box.
+template <class S> static int64_t tti_factor_table_len(const agd_eikonal_desc* d, const FactorDesc<S>& f) {
+  const int nsym = d->ndim * (d->ndim + 1) / 2;
+  return int64_t(kTTIMix) * 2 * nsym * (d->order == 2 ? 2 : 1) * f.nbox;
 }
 
 // TTI scheme codes: NMIX * 2 * nsym neighbour codes, the mix flag, then
@@ -361,6 +384,7 @@ template <class T> static void free_dev(T*& p) {
 
 void Workspace::release() {
   if (!fixed) {
+    free_dev(ftab);
     free_dev(stamp);
     free_dev(needs);
     free_dev(pending);
@@ -387,7 +411,19 @@ void Workspace::release() {
   ev0 = ev1 = chunk_ev[0] = chunk_ev[1] = nullptr;
   capture_stream = nullptr;
   tile_cap = shard_cap = hist_cap = inst_cap = tinst_cap = 0;
+  ftab = nullptr;
+  ftab_bytes = 0;
   fixed = false;
+}
+
+int Workspace::ensure_ftab(size_t bytes) {
+  AGD_REQUIRE(!fixed, "source factorization of the TTI model needs an allocating workspace");
+  if (ftab_bytes < bytes) {
+    free_dev(ftab);
+    AGD_HIP_TRY(hipMalloc(&ftab, bytes));
+    ftab_bytes = bytes;
+  }
+  return AGD_OK;
 }
 
 int Workspace::ensure(int64_t ntiles, int64_t scap, int64_t nhist) {
@@ -526,7 +562,8 @@ static int solve_impl(Workspace& ws, const agd_eikonal_desc* d, const agd_eikona
   AGD_REQUIRE(!order2 || (!pre && ks.update_order2) || (pre && tti),
               "the second-order scheme has no precomputed-scheme variant (except TTI's)");
   const bool factor = d->factor_radius2 > 0.;
-  AGD_REQUIRE(!factor || !pre, "source factorization has no precomputed-scheme variant");
+  AGD_REQUIRE(!factor || !pre || tti,
+              "source factorization has no precomputed-scheme variant (except TTI's)");
   const bool bab = d->bound_active_blocks > 0;
   AGD_REQUIRE(!bab || d->solver == AGD_AGSI, "bound_active_blocks runs with the AGSI solver");
   AGD_REQUIRE(!bab || d->minchg_delta_min > 0., "bound_active_blocks needs minchg_delta_min > 0");
@@ -555,6 +592,22 @@ static int solve_impl(Workspace& ws, const agd_eikonal_desc* d, const agd_eikona
   a.fim_front = d->fim_front_width;
   a.order2_thr = S(d->order2_threshold);
   a.fac = make_factor<S>(d);
+  if (tti && factor && a.fac.nbox > 0) {
+    // TTI: the factorization shifts of the ball's cells, once per solve.
+    const int64_t len = tti_factor_table_len<S>(d, a.fac);
+    const int st = ws.ensure_ftab(size_t(len) * sizeof(S));
+    if (st) return st;
+    a.fac.tab = static_cast<const S*>(ws.ftab);
+    TTIFactorArgs<S> fa;
+    std::memset(&fa, 0, sizeof fa);
+    fa.geom = a.geom;
+    fa.g = g;
+    fa.fac = a.fac;
+    fa.tab = static_cast<S*>(ws.ftab);
+    fa.order2 = order2 ? 1 : 0;
+    hipLaunchKernelGGL(ks.tti_factor, dim3(unsigned(g.ntiles)), dim3(ni), 0, stream, fa);
+    AGD_HIP_TRY(hipGetLastError());
+  }
 
   auto init = [&](Workspace* ws) -> int {
     a.w.lists = ws->lists;

@@ -34,9 +34,39 @@ namespace agd {
 // Source factorization parameters (see factor_shifts below).
 template <class S> struct FactorDesc {
   S origin[3];
-  S r2;    // squared radius; 0 disables
-  S m[6];  // M: diagonal (Isotropic, Diagonal) or lower triangle
+  S r2;     // squared radius; 0 disables
+  S m[14];  // M: diagonal (Isotropic, Diagonal) or lower triangle; TTI:
+            // [l0, l1, q00, q10, q11, A (row-major)]
+  // TTI: the shifts of every cell of the ball, computed once per solve
+  // (eik_tti_factor) for the box [lo, lo + n) of cells: component
+  // (m * 2 NS + 2k + s) (then, with order 2, the further neighbours after
+  // NMIX * 2 NS components) of box cell b at comp * nbox + b.
+  const S* tab;
+  int32_t lo[3], n[3];
+  int64_t nbox;
 };
+
+// Index of the grid cell xt in the factor box of f, -1 outside the ball
+// (the predicate of factor_shifts, FiniteDifferences.h:30-33).
+template <class S, int D>
+__device__ inline int64_t factor_box_index(const FactorDesc<S>& f, const int* xt) {
+#pragma clang fp contract(off)
+  S n2 = S(0);
+#pragma unroll
+  for (int c = 0; c < D; ++c) {
+    const S x = S(xt[c]) - f.origin[c];
+    n2 = fma_(x, x, n2);
+  }
+  if (!(n2 < f.r2)) return -1;
+  int64_t b = 0;
+#pragma unroll
+  for (int c = 0; c < D; ++c) {
+    const int r = xt[c] - f.lo[c];
+    if (r < 0 || r >= f.n[c]) return -1;
+    b = b * f.n[c] + r;
+  }
+  return b;
+}
 
 // One instance of a batched solve: its own grid and constants; its tiles
 // occupy [tile_base, tile_base + g.ntiles) of the batch's tile space.
@@ -469,6 +499,10 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
       // held in registers; out-of-tile values are read at every sweep.
       const int64_t N = a.g.size_tot;
       const bool mix_min = ld32(a.scode + int64_t(MT::NMIX * NJ) * N, uint32_t(n_t)) != 0;
+      // source factorization: the cell's row of the shift table, or -1
+      const int64_t fb = a.fac.r2 > S(0) ? factor_box_index<S, D>(a.fac, xt) : -1;
+      const int64_t nbox = a.fac.nbox;
+      const int64_t fb2 = fb + int64_t(MT::NMIX * NJ) * nbox;  // further neighbours
       __syncthreads();  // u_s complete
       const bool active = !seed;
       for (int it = 0; it < a.niter_i; ++it) {
@@ -484,6 +518,7 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
             for (int sd = 0; sd < 2; ++sd) {
               const int c = ld32(a.scode + int64_t(m * NJ + 2 * k + sd) * N, uint32_t(n_t));
               side[sd] = c <= -2 ? u_s[-2 - c] : (c >= 0 ? ld32(a.u, uint32_t(c)) : inf_v<S>());
+              if (fb >= 0) side[sd] += a.fac.tab[int64_t(m * NJ + 2 * k + sd) * nbox + fb];
             }
             const int sd = (side[0] > side[1]) ? 1 : 0;  // Greater(v,-) keeps the -e side on ties
             v[k] = side[sd];
@@ -491,6 +526,7 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
               // the further neighbour on the chosen side (second order)
               const int c2 = ld32(a.scode2 + int64_t(m * NJ + 2 * k + sd) * N, uint32_t(n_t));
               v2[k] = c2 <= -2 ? u_s[-2 - c2] : (c2 >= 0 ? ld32(a.u, uint32_t(c2)) : inf_v<S>());
+              if (fb >= 0) v2[k] += a.fac.tab[int64_t(m * NJ + 2 * k + sd) * nbox + fb2];
             }
           }
           if (a.scode2) order2_correct<S, NS>(u_s[tid], a.order2_thr, v, v2, w);
@@ -709,6 +745,76 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_scheme(SchemeArgs<S> a) {
   }
 }
 
+// ------------------------------------------------------------------ TTI factor
+// The source-factorization shifts of every TTI sub-scheme at the cells of
+// the factoring ball (FiniteDifferences.h:30-45 with TTI_.h:191-211), written
+// into the table of FactorDesc once per solve.  The sub-stencils are rebuilt
+// from the geometry as eik_scheme builds them, so that entry (m, k) belongs to
+// the offset whose codes the scheme stored.
+template <class S> struct TTIFactorArgs {
+  const S* geom;
+  GridDesc g;
+  FactorDesc<S> fac;
+  S* tab;
+  int32_t order2;  // also the shifts of the further neighbours
+};
+
+template <class S, int D, int T0, int T1, int T2>
+__global__ void __launch_bounds__(T0* T1* T2) eik_tti_factor(TTIFactorArgs<S> a) {
+  using TL = Tile<D, T0, T1, T2>;
+  using MT = ModelTraits<AGD_TTI, D>;
+  constexpr int NI = TL::NI;
+  constexpr int NS = MT::NSYM;
+  constexpr int NG = MT::NGEOM;
+  const int tid = threadIdx.x;
+  const int n_o = blockIdx.x;
+  int xi[3], xo[3], xt[3] = {0, 0, 0};
+  TL::pos(tid, xi);
+  tile_pos<D>(n_o, a.g, xo);
+#pragma unroll
+  for (int k = 0; k < D; ++k) xt[k] = xo[k] * TL::T[k] + xi[k];
+  const int64_t b = factor_box_index<S, D>(a.fac, xt);
+  if (b < 0) return;
+  const int64_t N = a.g.size_tot;
+  const int64_t n_t = int64_t(n_o) * NI + tid;
+  S geo[NG];
+#pragma unroll
+  for (int k = 0; k < NG; ++k) geo[k] = a.geom[k * N + n_t];
+  S diag[kTTIMix][2];
+  (void)tti_diags<S>(geo, geo + 2, diag);
+  S D0[NS], D1[NS];
+  tti_base<S, D>(geo, D0, D1);
+  TTIFactorData<S> t;
+  tti_factor_data<S>(a.fac.m, t);
+  S x[D], grad[D];
+  {
+#pragma clang fp contract(off)
+#pragma unroll
+    for (int c = 0; c < D; ++c) x[c] = S(xt[c]) - a.fac.origin[c];
+  }
+  const S nx = tti_factor_norm<S, D>(a.fac.m, t, x, grad);
+  const int64_t nb = a.fac.nbox;
+  const int64_t off2 = int64_t(kTTIMix) * 2 * NS * nb;
+#pragma unroll 1
+  for (int m = 0; m < kTTIMix; ++m) {
+    S w[NS];
+    int e[NS][3];
+    tti_sub<S, D>(D0, D1, diag[m][0], diag[m][1], w, e);
+#pragma unroll 1
+    for (int k = 0; k < NS; ++k) {
+      S f[2], f2[2];
+      tti_factor_pair<S, D>(a.fac.m, t, x, nx, grad, e[k], f, a.order2 ? f2 : nullptr);
+      const int64_t c = int64_t(m * 2 * NS + 2 * k);
+      a.tab[c * nb + b] = f[0];
+      a.tab[(c + 1) * nb + b] = f[1];
+      if (a.order2) {
+        a.tab[off2 + c * nb + b] = f2[0];
+        a.tab[off2 + (c + 1) * nb + b] = f2[1];
+      }
+    }
+  }
+}
+
 // ------------------------------------------------------------------ flow
 // One global pass with niter_i = 1 and flow export (Update.h:149-209).
 template <class S> struct FlowArgs {
@@ -762,6 +868,21 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_flow(FlowArgs<S> a) {
   for (int k = 0; k < NG; ++k) geo[k] = a.geom[k * N + n_t];
   const int wx = a.wall ? int(a.wall[n_t]) : 255;
 
+  // TTI source factorization: N(x) and its gradient, once per cell.
+  bool tfac = false;
+  TTIFactorData<S> tdat;
+  S xrel[D], tgrad[D], tnx = S(0);
+  if constexpr (MT::NMIX > 1) {
+    tfac = a.fac.r2 > S(0) && factor_box_index<S, D>(a.fac, xt) >= 0;
+    if (tfac) {
+#pragma clang fp contract(off)
+      tti_factor_data<S>(a.fac.m, tdat);
+#pragma unroll
+      for (int c = 0; c < D; ++c) xrel[c] = S(xt[c]) - a.fac.origin[c];
+      tnx = tti_factor_norm<S, D>(a.fac.m, tdat, xrel, tgrad);
+    }
+  }
+
   // Neighbour values (min over -e/+e, ties keep -e), sides and indices of a
   // stencil.
   auto gather = [&](const Stencil<S, MODEL, D>& st, S* v, S* w, int* side, int64_t* nb,
@@ -775,6 +896,12 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_flow(FlowArgs<S> a) {
     }
     if constexpr (MT::NMIX == 1) {
       if (a.fac.r2 > S(0)) factor_shifts<S, MODEL, D, NS>(a.fac, xt, st.e, sh, v2 ? sh2 : nullptr);
+    } else {
+      if (tfac) {
+#pragma unroll 1
+        for (int k = 0; k < NS; ++k)
+          tti_factor_pair<S, D>(a.fac.m, tdat, xrel, tnx, tgrad, st.e[k], sh + 2 * k, v2 ? sh2 + 2 * k : nullptr);
+      }
     }
 #pragma unroll
     for (int k = 0; k < NS; ++k) {
@@ -933,6 +1060,7 @@ template <class S> struct ModelKernels {
   void (*scheme)(SchemeArgs<S>);
   void (*scheme_general)(SchemeArgs<S>);
   void (*flow)(FlowArgs<S>);
+  void (*tti_factor)(TTIFactorArgs<S>);  // TTI only: shift table of the factoring ball
   int ni;
 };
 
@@ -967,6 +1095,8 @@ ModelKernels<S> make_model_kernels() {
     k.scheme_general = nullptr;
   }
   k.flow = eik_flow<S, MODEL, D, T0, T1, T2>;
+  if constexpr (MODEL == AGD_TTI) k.tti_factor = eik_tti_factor<S, D, T0, T1, T2>;
+  else k.tti_factor = nullptr;
   k.ni = T0 * T1 * T2;
   return k;
 }

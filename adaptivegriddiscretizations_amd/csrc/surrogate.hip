@@ -82,55 +82,73 @@ __global__ void __launch_bounds__(256) conv3x3_fwd(const __bf16* __restrict__ in
   }
 }
 
-// dW[tap, co, ci] += sum_p gout[p, co] * in[p + off(tap), ci] over a chunk of
-// `chunk` pixels (a multiple of 32).  One wave per (chunk, tap, 16 co, 16 ci)
-// block: A = gout^T (16 channels x 32 pixels), B = shifted input (32 pixels x
-// 16 channels); FP32 partial sums added atomically.
+// dW[tap, co, ci] += sum_p gout[p, co] * in[p + off(tap), ci], an implicit
+// GEMM with M = output channels, N = input channels, K = pixels.  One
+// workgroup (4 waves) per (pixel chunk, tap, 32-co block, 32-ci block); per K
+// step of 32 pixels the workgroup stages gout[32 px][32 co] and the
+// tap-shifted input [32 px][32 ci] in LDS with 16-byte loads (8 channels of
+// one pixel per lane, the pixel's row and column from one 32-bit division),
+// stored transposed so that each lane reads its MFMA fragment (8
+// consecutive pixels of one channel) with one 16-byte LDS read; wave w
+// accumulates the 16x16 block (co half w/2, ci half w%2) in registers and adds
+// it to dw once per chunk (FP32 atomics).
+constexpr int kWgChunk = 1024;  // pixels per workgroup
 __global__ void __launch_bounds__(256) conv3x3_wgrad(const __bf16* __restrict__ in,
                                                      const __bf16* __restrict__ gout,
                                                      float* __restrict__ dw, int N, int H, int W,
-                                                     int Cin, int Cout, int64_t chunk,
-                                                     int64_t nwaves) {
-  const int64_t gw = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
-  if (gw >= nwaves) return;
-  const int l = threadIdx.x & 63;
-  const int nco = Cout / 16, nci = Cin / 16;
-  int64_t t = gw;
-  const int bi = int(t % nci); t /= nci;
-  const int bo = int(t % nco); t /= nco;
-  const int tap = int(t % 9); t /= 9;
-  const int64_t p0 = t * chunk;
-  const int64_t npix = int64_t(N) * H * W;
-  const int64_t p1 = p0 + chunk < npix ? p0 + chunk : npix;
+                                                     int Cin, int Cout, int nchunks) {
+  __shared__ __attribute__((aligned(16))) __bf16 sg[32][40];  // [co][pixel], padded rows
+  __shared__ __attribute__((aligned(16))) __bf16 si[32][40];  // [ci][pixel]
+  const int nci = Cin / 32, nco = Cout / 32;
+  int t = blockIdx.x;
+  const int bi = t % nci; t /= nci;
+  const int bo = t % nco; t /= nco;
+  const int tap = t % 9;
+  const int chunk = t / 9;
   const int dy = tap / 3 - 1, dx = tap % 3 - 1;
-  const int co = bo * 16 + (l & 15), ci = bi * 16 + (l & 15);
+  const int npix = N * H * W;
+  const int p0 = chunk * kWgChunk;
+  const int p1 = min(p0 + kWgChunk, npix);
+  const int tid = threadIdx.x;
+  const int wv = tid >> 6, l = tid & 63;
+  // staging role: lanes 0..127 load gout, 128..255 the shifted input
+  const bool ld_g = tid < 128;
+  const int r = tid & 127;
+  const int px_local = r >> 2, c8 = (r & 3) * 8;
+  // MFMA role
+  const int coh = wv >> 1, cih = wv & 1;
   const int kq = (l >> 4) * 8;
   f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-  for (int64_t p = p0; p < p1; p += 32) {
-    bf16x8 a, b;
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      const int64_t q = p + kq + i;  // pixel of this reduction index
-      __bf16 av = __bf16(0.f), bv = __bf16(0.f);
-      if (q < p1) {
-        const int xq = int(q % W);
-        const int64_t r = q / W;
-        const int yq = int(r % H);
-        const int64_t nq = r / H;
-        av = gout[q * Cout + co];
-        const int ys = yq + dy, xs = xq + dx;
-        if (ys >= 0 && ys < H && xs >= 0 && xs < W) bv = in[((nq * H + ys) * W + xs) * Cin + ci];
+  for (int p = p0; p < p1; p += 32) {
+    const int q = p + px_local;
+    bf16x8 v = zero8();
+    if (q < p1) {
+      if (ld_g) {
+        v = *reinterpret_cast<const bf16x8*>(gout + int64_t(q) * Cout + bo * 32 + c8);
+      } else {
+        const int x = q % W;
+        const int rowi = q / W;  // n * H + y
+        const int y = rowi % H;
+        const int ys = y + dy, xs = x + dx;
+        if (ys >= 0 && ys < H && xs >= 0 && xs < W)
+          v = *reinterpret_cast<const bf16x8*>(in + (int64_t(rowi + dy) * W + xs) * Cin + bi * 32 + c8);
       }
-      a[i] = av;
-      b[i] = bv;
     }
+    __bf16(*dst)[40] = ld_g ? sg : si;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) dst[c8 + i][px_local] = v[i];
+    __syncthreads();
+    const bf16x8 a = *reinterpret_cast<const bf16x8*>(&sg[coh * 16 + (l & 15)][kq]);
+    const bf16x8 b = *reinterpret_cast<const bf16x8*>(&si[cih * 16 + (l & 15)][kq]);
     acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+    __syncthreads();
   }
-  // D rows (co) bo*16 + (l/16)*4 + i, column (ci) l%16
+  // D rows (co) (l/16)*4 + i of the wave's block, column (ci) l%16
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
-    const int cor = bo * 16 + (l >> 4) * 4 + i;
-    atomicAdd(dw + (int64_t(tap) * Cout + cor) * Cin + ci, acc[i]);
+    const int co = bo * 32 + coh * 16 + (l >> 4) * 4 + i;
+    const int ci = bi * 32 + cih * 16 + (l & 15);
+    atomicAdd(dw + (int64_t(tap) * Cout + co) * Cin + ci, acc[i]);
   }
 }
 
@@ -302,12 +320,13 @@ int agd_nn_conv3x3_wgrad(const void* in, const void* gout, float* dw, int32_t N,
   AGD_REQUIRE(in && gout && dw && N >= 1 && H >= 1 && W >= 1, "bad convolution arguments");
   AGD_REQUIRE(Cin % 32 == 0 && Cout % 32 == 0, "channels must be padded to multiples of 32");
   const int64_t npix = int64_t(N) * H * W;
-  const int64_t chunk = 4096;
-  const int64_t nchunks = (npix + chunk - 1) / chunk;
-  const int64_t nwaves = nchunks * 9 * (Cout / 16) * (Cin / 16);
-  hipLaunchKernelGGL(conv3x3_wgrad, dim3(unsigned((nwaves + 3) / 4)), dim3(256), 0,
-                     as_stream(stream), static_cast<const __bf16*>(in),
-                     static_cast<const __bf16*>(gout), dw, N, H, W, Cin, Cout, chunk, nwaves);
+  AGD_REQUIRE(npix < (int64_t(1) << 31), "too many pixels for the weight-gradient kernel");
+  const int64_t nchunks = (npix + kWgChunk - 1) / kWgChunk;
+  const int64_t nblocks = nchunks * 9 * (Cout / 32) * (Cin / 32);
+  AGD_REQUIRE(nblocks < (int64_t(1) << 31), "too many weight-gradient blocks");
+  hipLaunchKernelGGL(conv3x3_wgrad, dim3(unsigned(nblocks)), dim3(256), 0, as_stream(stream),
+                     static_cast<const __bf16*>(in), static_cast<const __bf16*>(gout), dw, N, H, W,
+                     Cin, Cout, int(nchunks));
   AGD_HIP_TRY(hipGetLastError());
   return AGD_OK;
 }

@@ -60,6 +60,8 @@ struct Workspace {
   int32_t* check_count = nullptr;  // residual-check counter
   void* inst = nullptr;            // batched solves: instance table (inst_cap InstDesc)
   int32_t* tile_inst = nullptr;    // batched solves: tile -> instance (tinst_cap)
+  void* ftab = nullptr;            // TTI source factorization: shift table (ftab_bytes)
+  size_t ftab_bytes = 0;
   int64_t tile_cap = 0, shard_cap = 0, hist_cap = 0, inst_cap = 0, tinst_cap = 0;
   // Host side.  host_counts: pinned (owned) or host_local (fixed region),
   // 2 * kRow entries, two chunks in flight.
@@ -90,6 +92,8 @@ struct Workspace {
   int ensure(int64_t ntiles, int64_t shard_cap, int64_t nhist);
   // Batched solves: instance table and tile map of ninst instances, ntiles tiles.
   int ensure_batch(int64_t ninst, int64_t ntiles, size_t inst_bytes);
+  // TTI source factorization: a shift table of at least `bytes`.
+  int ensure_ftab(size_t bytes);
   // Frees everything owned (no-op for a fixed region).
   void release();
   // Lays the buffers out in [base, base + bytes) (fixed region); AGD_EINVAL

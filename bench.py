@@ -146,7 +146,8 @@ def main():
     ap.add_argument('--steps', type=int, default=10)
     ap.add_argument('--warmup', type=int, default=2)
     ap.add_argument('--config', default='riemann3',
-                    choices=['riemann3', 'riemann2', 'isotropic2', 'batch256', 'warmstart', 'adjoint'])
+                    choices=['riemann3', 'riemann2', 'isotropic2', 'batch256', 'warmstart', 'adjoint',
+                             'conv'])
     ap.add_argument('--train', type=int, default=4096, help='warmstart: training fields')
     ap.add_argument('--test', type=int, default=512, help='warmstart: held-out fields')
     ap.add_argument('--epochs', type=int, default=12, help='warmstart: training epochs')
@@ -186,6 +187,8 @@ def main():
         return bench_warmstart(args)
     if args.config == 'adjoint':
         return bench_adjoint(args)
+    if args.config == 'conv':
+        return bench_conv(args)
     t0 = time.time()
     wl = make_workload(args.config)
     log(f"workload {wl.name} {wl.dims} generated in {time.time() - t0:.1f} s")
@@ -419,6 +422,102 @@ def bench_adjoint(args):
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "adj_update",
                      "bytes_per_cell_update": bpc, "mean_row_length": row,
                      "avg_launch_us": st_t['kernel_ms'] / max(st_t['launches'], 1) * 1e3},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(result), flush=True)
+
+
+def bench_conv(args):
+    """The warm-start surrogate's 3x3 convolutions (BF16 MFMA, FP32
+    accumulation) at the shapes of one training batch (16 coarse 256x256
+    grids, levels 256/128/64/32 with 32/64/128/128 channels): forward and
+    weight-gradient kernels timed with events on the launch stream, and one
+    whole training step (forward, backward, Adam).  A step = one training
+    step; FLOPs 2 * 9 * Cin * Cout per pixel and kernel."""
+    import numpy as np
+    import torch
+    from adaptivegriddiscretizations_amd import _lib
+    from adaptivegriddiscretizations_amd.Eikonal import surrogate as S
+    from adaptivegriddiscretizations_amd.Eikonal._device import ptr
+    stream = torch.cuda.current_stream().cuda_stream
+    rng = np.random.default_rng(0)
+    nb, side = 16, 256
+    layers = []
+    for lvl, c in enumerate((32, 64, 128, 128)):
+        n = side >> lvl
+        conv = S.Conv3x3(c, c, rng)
+        x = (torch.rand((nb, n, n, conv.cin_p), device='cuda') - 0.5).to(torch.bfloat16)
+        gy = (torch.rand((nb, n, n, conv.cout_p), device='cuda') - 0.5).to(torch.bfloat16)
+        y = torch.empty((nb, n, n, conv.cout_p), dtype=torch.bfloat16, device='cuda')
+        flops = 2. * 9 * conv.cin_p * conv.cout_p * nb * n * n
+        res = {}
+        for kind in ('fwd', 'wgrad'):
+            def run():
+                if kind == 'fwd':
+                    S._check(_lib.lib.agd_nn_conv3x3(ptr(x), ptr(conv.wb), ptr(y), nb, n, n, conv.cin_p,
+                                                     conv.cout_p, 1, stream))
+                else:
+                    S._check(_lib.lib.agd_nn_conv3x3_wgrad(ptr(x), ptr(gy), ptr(conv.g), nb, n, n,
+                                                           conv.cin_p, conv.cout_p, stream))
+            for _ in range(3):
+                run()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            reps = 20
+            e0.record()
+            for _ in range(reps):
+                run()
+            e1.record()
+            e1.synchronize()
+            us = e0.elapsed_time(e1) / reps * 1e3
+            res[kind] = {"us": us, "tflops": flops / (us * 1e-6) / 1e12}
+        layers.append({"level": lvl, "shape": [nb, n, n, c, c], "gflop": flops / 1e9, **res})
+        log(f"level {lvl} {n}x{n} {c}->{c}: fwd {res['fwd']['us']:.1f} us "
+            f"({res['fwd']['tflops']:.0f} TFLOP/s), wgrad {res['wgrad']['us']:.1f} us "
+            f"({res['wgrad']['tflops']:.0f} TFLOP/s)")
+    # one training step of the whole network on a batch
+    net = S.Surrogate()
+    feats = (torch.rand((nb, side, side, S.CP), device='cuda') - 0.5).to(torch.bfloat16)
+    targ = torch.rand((nb, side, side), device='cuda')
+    def train_step():
+        c = {}
+        pred = net.forward(feats, cache=c)
+        loss, g = net.loss_and_grad(pred, targ)
+        net.backward(g, c)
+        net.step(1e-3)
+    for _ in range(args.warmup):
+        train_step()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record()
+    for _ in range(args.steps):
+        train_step()
+    e1.record()
+    e1.synchronize()
+    wall = time.perf_counter() - t0
+    ms = e0.elapsed_time(e1) / args.steps
+    # network FLOPs per step: forward, input gradient and weight gradient of every conv
+    net_flops = 0.
+    for lvl, (c1, c2) in enumerate(net.enc):
+        npx = nb * (side >> lvl) ** 2
+        net_flops += 3 * 2. * 9 * npx * (c1.cin_p * c1.cout_p + c2.cin_p * c2.cout_p)
+    for j, (c1, c2) in enumerate(net.dec):
+        npx = nb * (side >> (2 - j)) ** 2
+        net_flops += 3 * 2. * 9 * npx * (c1.cin_p * c1.cout_p + c2.cin_p * c2.cout_p)
+    net_flops += 3 * 2. * 9 * nb * side * side * net.head.cin_p * net.head.cout_p
+    best = max(layers, key=lambda r: r['fwd']['tflops'])
+    peak = 2500.
+    result = {
+        "metric": "surrogate training step: conv3x3 BF16 MFMA throughput", "value": net_flops / (ms * 1e-3) / 1e12,
+        "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "bf16", "data": "synthetic",
+        "config": {"workload": f"surrogate encoder-decoder, batch {nb} of {side}x{side}, widths 32-64-128-128",
+                   "parallelism": "single GPU"},
+        "step_ms": ms, "layers": layers,
+        "roofline": {"bound": "mfma", "achieved": best['fwd']['tflops'], "peak": peak, "unit": "TFLOP/s",
+                     "frac": best['fwd']['tflops'] / peak, "traffic": None, "kernel": "conv3x3_fwd",
+                     "shape": best['shape']},
         "cpu_baseline": None,
     }
     print(json.dumps(result), flush=True)

@@ -81,16 +81,21 @@ typedef struct agd_eikonal_desc {
                              where the solution is smooth (HFM.h:91-134) */
   int32_t reserved;       /* 0                                           */
   double order2_threshold;/* order 2: |u-2v+v2| < threshold * max(u-v) (0.3) */
-  /* Source factorization (EuclideanFactor.h:8-30, Constants.h:50-61): the
-   * cells with |x - factor_origin|^2 < factor_radius2 (grid units, Euclidean)
-   * shift their neighbour values by the finite-difference error of
-   * T(x) = |x - factor_origin|_M, M = factor_metric.  0 disables it.  Not with
-   * TTI, a precomputed scheme or batched solves. */
+  /* Source factorization (EuclideanFactor.h:8-50, Constants.h:50-61,
+   * TTI_.h:139-212): the cells with |x - factor_origin|^2 < factor_radius2
+   * (grid units, Euclidean) shift their neighbour values by the
+   * finite-difference error of T(x) = |x - factor_origin|_M, M = factor_metric.
+   * 0 disables it.  Not with batched solves, nor with the precomputed scheme
+   * of Riemann (TTI's precomputed scheme takes it: a table of the ball's
+   * shifts is computed at the start of every solve, in a workspace the
+   * allocation-free entry point does not have). */
   double factor_radius2;
   double factor_origin[3];/* seed position in grid units                 */
-  double factor_metric[6];/* M in grid units: squared costs (Isotropic, Diagonal:
-                             ndim entries) or lower triangle (d00, d10, d11,
-                             d20, d21, d22) for Riemann and Rander */
+  double factor_metric[14];/* M in grid units: squared costs (Isotropic, Diagonal:
+                             ndim entries), lower triangle (d00, d10, d11,
+                             d20, d21, d22) for Riemann and Rander, or for TTI
+                             [l0, l1, q00, q10, q11, A (ndim x ndim, row-major)]
+                             with T(x) = N0(A x) (_Kernel.py:301-308) */
   /* bound_active_blocks (_solvers.py:145-181,195-224, Propagation.h:53-99):
    * with the AGSI solver, a tile whose smallest changed value is at or above
    * a moving threshold is frozen (kept in the list without propagating)

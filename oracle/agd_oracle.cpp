@@ -270,6 +270,125 @@ static void tti_scheme(int d, const S* g, Stencil<S>& st) {
   }
 }
 
+// ---------------------------------------------------------------- TTI factor
+// TTI_.h:139-212 with GoldenSearch.h and EuclideanFactor.h:34-50: the source
+// factorization of the TTI model uses the norm N(x) = N0(A x) of the factor
+// metric fm = [l0, l1, q00, q10, q11, A (d x d, row-major)], N0 approached by
+// niter_golden_search = 8 golden-section steps over the tangent ellipses of
+// the dual ball.  Ellipse v, v in [a, -b], touches the curve at
+// x = ((u+v)/2, (u-v)/2) (u as in tti_diags) and has the primal norm
+// sqrt(s0 / g0 + s1 / g1) with s = (X^2 [+ Y^2], Z^2) of A x and g the
+// gradient ratio at x.  The search maximises, or minimises when the first
+// two ellipses' reciprocal ratios have a non-positive determinant.
+static const int kGoldenSteps = 8;
+
+template <class S>
+struct TTIFactorData {
+  S l[2], q[3], L[2], Q[3], a, b;
+};
+
+// TTI_.h:86-97 (tti_data_init).
+template <class S>
+static void tti_factor_data(const S* fm, TTIFactorData<S>& t) {
+  for (int i = 0; i < 2; ++i) t.l[i] = fm[i];
+  for (int i = 0; i < 3; ++i) t.q[i] = fm[2 + i];
+  t.a = solve2_above<S>(S(-2), t.l[0], t.q[0], S(0));
+  t.b = solve2_above<S>(S(-2), t.l[1], t.q[2], S(0));
+  const S e0[2] = {S(0.5), S(0.5)}, e1[2] = {S(0.5), S(-0.5)};
+  t.L[0] = t.l[0] * e0[0] + t.l[1] * e0[1];
+  t.L[1] = t.l[0] * e1[0] + t.l[1] * e1[1];
+  t.Q[0] = scal_vmv2f<S>(e0, t.q, e0);
+  t.Q[1] = scal_vmv2f<S>(e0, t.q, e1);
+  t.Q[2] = scal_vmv2f<S>(e1, t.q, e1);
+}
+
+// TTI_.h:148-156: the ellipse of parameter v; diag = 1 / g.
+template <class S>
+static S tti_ellipse(const TTIFactorData<S>& t, const S s[2], S v, S diag[2]) {
+  const S u = solve2_above<S>(S(-2) + S(2) * t.L[1] * v + t.Q[2] * v * v, t.L[0] + t.Q[1] * v, t.Q[0],
+                              std::fabs(v));
+  const S x[2] = {(u + v) / 2, (u - v) / 2};
+  S g[2];
+  grad_ratio<S>(t.l, t.q, x, g);
+  diag[0] = S(1) / g[0];
+  diag[1] = S(1) / g[1];
+  return std::sqrt(s[0] * diag[0] + s[1] * diag[1]);
+}
+
+// TTI_.h:159-189: N(x) by the golden-section search, and (grad non-null) the
+// gradient A^T diag(d) A x / N(x) of the last ellipse evaluated.
+template <class S>
+static S tti_factor_norm(int d, const S* fm, const TTIFactorData<S>& t, const S* x, S* grad) {
+  const S* A = fm + 5;
+  S Ax[3] = {0, 0, 0};
+  for (int i = 0; i < d; ++i) {
+    S r = 0;
+    for (int j = 0; j < d; ++j) r += A[i * d + j] * x[j];
+    Ax[i] = r;
+  }
+  S s[2] = {Ax[0] * Ax[0], Ax[d - 1] * Ax[d - 1]};
+  if (d == 3) s[0] += Ax[1] * Ax[1];
+  const S phi = S(0.6180339887498949), psi = S(0.3819660112501051);
+  const S bounds[2] = {t.a, -t.b};
+  S mid[2] = {phi * bounds[0] + psi * bounds[1], psi * bounds[0] + phi * bounds[1]};
+  S dg[2][2], diag[2] = {0, 0};
+  S val[2] = {tti_ellipse<S>(t, s, mid[0], dg[0]), tti_ellipse<S>(t, s, mid[1], dg[1])};
+  const bool mix_is_min = dg[0][0] * dg[1][1] - dg[0][1] * dg[1][0] <= 0;
+  int next = 1;
+  for (int i = 0; i < kGoldenSteps; ++i) {
+    const S dx = mid[1] - mid[0];
+    if (mix_is_min == (val[0] < val[1])) {
+      mid[1] = mid[0];
+      val[1] = val[0];
+      mid[0] -= dx * phi;
+      next = 0;
+    } else {
+      mid[0] = mid[1];
+      val[0] = val[1];
+      mid[1] += dx * phi;
+      next = 1;
+    }
+    val[next] = tti_ellipse<S>(t, s, mid[next], diag);
+  }
+  const S norm = val[next];
+  if (grad) {
+    for (int i = 0; i < d; ++i) Ax[i] = Ax[i] * (diag[d == 2 ? i : std::max(i - 1, 0)] / norm);
+    for (int j = 0; j < d; ++j) {
+      S r = 0;
+      for (int i = 0; i < d; ++i) r += A[i * d + j] * Ax[i];
+      grad[j] = r;
+    }
+  }
+  return norm;
+}
+
+// TTI_.h:191-211 and generic_factor_sym (EuclideanFactor.h:34-50): shifts of
+// x - e, x + e and (f2) of x - 2e, x + 2e, given nx = N(x) and its gradient.
+template <class S>
+static void tti_factor_pair(int d, const S* fm, const TTIFactorData<S>& t, const S* x, S nx,
+                            const S* grad, const int* e, S f[2], S* f2) {
+  S xp[3] = {0, 0, 0}, xm[3] = {0, 0, 0}, ge = 0;
+  for (int c = 0; c < d; ++c) {
+    xp[c] = x[c] + S(e[c]);
+    xm[c] = x[c] - S(e[c]);
+    ge += grad[c] * S(e[c]);
+  }
+  const S np = tti_factor_norm<S>(d, fm, t, xp, nullptr);
+  const S nm = tti_factor_norm<S>(d, fm, t, xm, nullptr);
+  f[0] = -ge + nx - nm;
+  f[1] = ge + nx - np;
+  if (f2) {
+    for (int c = 0; c < d; ++c) {
+      xp[c] = xp[c] + S(e[c]);
+      xm[c] = xm[c] - S(e[c]);
+    }
+    const S np2 = tti_factor_norm<S>(d, fm, t, xp, nullptr);
+    const S nm2 = tti_factor_norm<S>(d, fm, t, xm, nullptr);
+    f2[0] = 2 * f[0] - (nx - 2 * nm + nm2);
+    f2[1] = 2 * f[1] - (nx - 2 * np + np2);
+  }
+}
+
 // Isotropic_.h:20-41, Riemann_.h:16-29 and TTI_.h:112-134.
 template <class S>
 static void make_stencil(int model, int d, const S* g, const S* iso_w, Stencil<S>& st) {
@@ -443,7 +562,7 @@ static double g_order2_threshold = 0.3;
 struct Factor {
   double r2 = 0.;       // 0: off
   double origin[3] = {0., 0., 0.};
-  double m[6] = {0., 0., 0., 0., 0., 0.};  // diagonal, or lower triangle
+  double m[14] = {};  // diagonal, lower triangle, or TTI [l, q, A]
 };
 static Factor g_factor;
 
@@ -506,7 +625,10 @@ static void neighbours(const Problem& P, const Stencil<S>& st, int base, int64_t
   S xr[3] = {0, 0, 0};
   bool fac = false;
   const bool diag = P.model == ISOTROPIC || P.model == DIAGONAL;
-  S fm[6];
+  S fm[14];
+  const bool tti = P.model == TTI;
+  TTIFactorData<S> td;
+  S tnx = 0, tgrad[3] = {0, 0, 0};
   if (P.factor.r2 > 0) {
     S n2 = 0;
     for (int c = 0; c < P.d; ++c) {
@@ -514,7 +636,11 @@ static void neighbours(const Problem& P, const Stencil<S>& st, int base, int64_t
       n2 = fmaT(xr[c], xr[c], n2);
     }
     fac = n2 < S(P.factor.r2);
-    for (int c = 0; c < 6; ++c) fm[c] = S(P.factor.m[c]);
+    for (int c = 0; c < 14; ++c) fm[c] = S(P.factor.m[c]);
+    if (fac && tti) {
+      tti_factor_data<S>(fm, td);
+      tnx = tti_factor_norm<S>(P.d, fm, td, xr, tgrad);
+    }
   }
   for (int k = 0; k < st.nsym; ++k) {
     const int* ek = st.e[base + k];
@@ -525,9 +651,12 @@ static void neighbours(const Problem& P, const Stencil<S>& st, int base, int64_t
     S sh[2] = {shift, -shift}, sh2[2] = {0, 0};
     if (fac) {
       const S ef[3] = {S(ek[0]), S(ek[1]), S(ek[2])};
-      S f[2], f2[2];
-      euclid_factor<S>(factor_dot<S>(P.d, diag, fm, xr, xr), factor_dot<S>(P.d, diag, fm, xr, ef),
-                       factor_dot<S>(P.d, diag, fm, ef, ef), f, f2);
+      S f[2], f2[2] = {0, 0};
+      if (tti)
+        tti_factor_pair<S>(P.d, fm, td, xr, tnx, tgrad, ek, f, P.order == 2 ? f2 : nullptr);
+      else
+        euclid_factor<S>(factor_dot<S>(P.d, diag, fm, xr, xr), factor_dot<S>(P.d, diag, fm, xr, ef),
+                         factor_dot<S>(P.d, diag, fm, ef, ef), f, f2);
       sh[0] = f[0] + sh[0];
       sh[1] = f[1] + sh[1];
       sh2[0] = f2[0];
@@ -817,9 +946,39 @@ static int64_t tiles_omp(const Problem& P, int tile, int niter, double atol, dou
   return pass;
 }
 
+// The TTI factorization at n points x[n][d] (relative to the origin) for the
+// offsets e[n][3]: norm[n], grad[n][d], fact[n][2], fact2[n][2] (each output
+// nullable), in FP32 (f32 = 1) or FP64.
+template <class S>
+static void tti_factor_points(int d, const double* fm64, int64_t n, const double* x, const int32_t* e,
+                              double* norm, double* grad, double* fact, double* fact2) {
+  S fm[14];
+  for (int k = 0; k < 14; ++k) fm[k] = S(fm64[k]);
+  TTIFactorData<S> t;
+  tti_factor_data<S>(fm, t);
+  for (int64_t i = 0; i < n; ++i) {
+    S xs[3] = {0, 0, 0}, g[3] = {0, 0, 0};
+    for (int c = 0; c < d; ++c) xs[c] = S(x[i * d + c]);
+    const S nx = tti_factor_norm<S>(d, fm, t, xs, g);
+    if (norm) norm[i] = double(nx);
+    if (grad)
+      for (int c = 0; c < d; ++c) grad[i * d + c] = double(g[c]);
+    if (fact && e) {
+      S f[2], f2[2];
+      tti_factor_pair<S>(d, fm, t, xs, nx, g, e + 3 * i, f, fact2 ? f2 : nullptr);
+      fact[2 * i] = double(f[0]);
+      fact[2 * i + 1] = double(f[1]);
+      if (fact2) {
+        fact2[2 * i] = double(f2[0]);
+        fact2[2 * i + 1] = double(f2[1]);
+      }
+    }
+  }
+}
+
 extern "C" {
 
-int orc_version() { return 6; }
+int orc_version() { return 7; }
 
 // Threads the OpenMP solver runs on (omp_get_max_threads).
 int orc_omp_threads() {
@@ -848,14 +1007,20 @@ void orc_set_order(int order, double threshold) {
 }
 
 // Source factorization for the following solves (r2 = 0 switches it off):
-// origin[3] in grid units, m[6] the metric at the origin in grid units.
+// origin[3] in grid units, m[14] the metric at the origin in grid units.
 void orc_set_factor(double r2, const double* origin, const double* m) {
   g_factor = Factor{};
   g_factor.r2 = r2;
   if (r2 > 0) {
     for (int k = 0; k < 3; ++k) g_factor.origin[k] = origin[k];
-    for (int k = 0; k < 6; ++k) g_factor.m[k] = m[k];
+    for (int k = 0; k < 14; ++k) g_factor.m[k] = m[k];
   }
+}
+
+void orc_tti_factor(int d, int f32, const double* fm, int64_t n, const double* x, const int32_t* e,
+                    double* norm, double* grad, double* fact, double* fact2) {
+  if (f32) tti_factor_points<float>(d, fm, n, x, e, norm, grad, fact, fact2);
+  else tti_factor_points<double>(d, fm, n, x, e, norm, grad, fact, fact2);
 }
 
 // Selling decomposition of n symmetric matrices (row n: symdim coefficients,

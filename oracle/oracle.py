@@ -67,6 +67,7 @@ def _load():
     lib.orc_forward_ad.argtypes = [c_int64, c_int, P, P, P, c_int, P, P]
     lib.orc_set_order.argtypes = [c_int, c_double]
     lib.orc_set_factor.argtypes = [c_double, P, P]
+    lib.orc_tti_factor.argtypes = [c_int, c_int, P, c_int64, P, P, P, P, P, P]
     lib.orc_wall_distance.argtypes = [c_int, P, P, P, c_int, P]
     lib.orc_visible.argtypes = [c_int, P, P, P, c_int64, P, P, P]
     lib.orc_omp_threads.restype = c_int
@@ -461,19 +462,20 @@ def make_problem(model, shape, h, metric=None, drift=None, cost=1., seeds_idx=()
 
 # ---------------------------------------------------------------- source factorization
 class factor:
-    """Context manager: source factorization (EuclideanFactor.h:8-30,
-    Constants.h:50-61) for the following solves and flows: radius in grid
-    units around origin (grid units), metric m6 at the origin in grid units
-    (squared costs for Isotropic/Diagonal, lower triangle otherwise; see
+    """Context manager: source factorization (EuclideanFactor.h:8-50,
+    Constants.h:50-61, TTI_.h:139-212) for the following solves and flows:
+    radius in grid units around origin (grid units), metric m at the origin in
+    grid units (squared costs for Isotropic/Diagonal, lower triangle for
+    Riemann/Rander, [l0, l1, q00, q10, q11, A row-major] for TTI; see
     factor_metric)."""
 
-    def __init__(self, radius, origin, m6):
+    def __init__(self, radius, origin, m):
         self.r2 = float(radius) ** 2
         self.origin = _c(list(np.asarray(origin, dtype=np.float64).reshape(-1)) + [0.] * 3, np.float64)[:3].copy()
-        self.m6 = _c(list(np.asarray(m6, dtype=np.float64).reshape(-1)) + [0.] * 6, np.float64)[:6].copy()
+        self.m = _c(list(np.asarray(m, dtype=np.float64).reshape(-1)) + [0.] * 14, np.float64)[:14].copy()
 
     def __enter__(self):
-        lib().orc_set_factor(self.r2, _p(self.origin), _p(self.m6))
+        lib().orc_set_factor(self.r2, _p(self.origin), _p(self.m))
         return self
 
     def __exit__(self, *exc):
@@ -537,6 +539,18 @@ def cost_metric_at(model, pts, h, shape, metric=None, drift=None, cost=1., perio
         return np.moveaxis(sel, tuple(range(len(core))), tuple(range(2, 2 + len(core))))
 
     c = at(field(cost, ()), ())
+    if mid == TTI:
+        # metric = (linear (2, ...), quadratic (2, 2, ...), A (d, d, ...) or None):
+        # with_costs(h) then with_cost(cost) scale A's columns by h and A by the
+        # cost (Seismic/implicit_base.py:48-56); each field is interpolated.
+        lin, quad, A = metric
+        li = np.einsum('ck,ckd->kd', wts, at(field(lin, (2,)), (2,)))
+        qf = np.asarray(quad, dtype=np.float64)
+        q3 = np.stack([qf[0, 0], qf[1, 0], qf[1, 1]])
+        qi = np.einsum('ck,ckd->kd', wts, at(field(q3, (3,)), (3,)))
+        A = np.eye(d) if A is None else A
+        Ah = at(field(A, (d, d)), (d, d)) * h[None, None, None, :] * c[..., None, None]
+        return 'tti', li, qi, np.einsum('ck,ckij->kij', wts, Ah)
     if mid in (ISOTROPIC, DIAGONAL):
         costs = np.ones((d,)) if mid == ISOTROPIC else metric
         cc = at(field(costs, (d,)), (d,)) * h * c[..., None]
@@ -552,10 +566,80 @@ def cost_metric_at(model, pts, h, shape, metric=None, drift=None, cost=1., perio
     return 'sym', mi, wi
 
 
+def tti_primal_norm(l, q3, A, v, nsample=4001):
+    """The TTI norm N(v) = N0(A v), N0 the support function of the dual ball
+    l0 P0 + l1 P1 + (q00 P0^2 + 2 q10 P0 P1 + q11 P1^2) / 2 <= 1 with
+    P0 = X^2 [+ Y^2], P1 = Z^2 (Seismic/tti.py:64-69), by brute force: the
+    ball's boundary in the quarter plane (rho, z) = t (cos th, sin th), t^2
+    the smallest positive root of the quadratic along the ray, maximising
+    rho |w_lateral| + z |w_last| over th on a grid and then by golden-section
+    refinement of the best bracket.  l[k, 2], q3[k, 3], A[k, d, d], v[k, d]."""
+    l = np.atleast_2d(np.asarray(l, dtype=np.float64))
+    q3 = np.atleast_2d(np.asarray(q3, dtype=np.float64))
+    A = np.asarray(A, dtype=np.float64)
+    v = np.atleast_2d(np.asarray(v, dtype=np.float64))
+    d = v.shape[1]
+    A = np.broadcast_to(A.reshape((-1, d, d)), (len(v), d, d))
+    w = np.einsum('kij,kj->ki', A, v)
+    r = np.sqrt(np.sum(w[:, :d - 1] ** 2, axis=1))
+    z = np.abs(w[:, d - 1])
+    l = np.broadcast_to(l, (len(v), 2))
+    q3 = np.broadcast_to(q3, (len(v), 3))
+
+    def value(th):  # th[k, m]
+        c2, s2 = np.cos(th) ** 2, np.sin(th) ** 2
+        L = l[:, :1] * c2 + l[:, 1:] * s2
+        Q = q3[:, :1] * c2 * c2 + 2. * q3[:, 1:2] * c2 * s2 + q3[:, 2:] * s2 * s2
+        tau = 2. / (L + np.sqrt(L * L + 2. * Q))  # root of Q tau^2 / 2 + L tau - 1
+        t = np.sqrt(tau)
+        return t * (np.cos(th) * r[:, None] + np.sin(th) * z[:, None])
+
+    th = np.linspace(0., np.pi / 2, nsample)[None, :].repeat(len(v), 0)
+    vals = value(th)
+    j = np.argmax(vals, axis=1)
+    step = th[0, 1] - th[0, 0]
+    lo = np.clip(th[np.arange(len(v)), j] - step, 0., np.pi / 2)
+    hi = np.clip(th[np.arange(len(v)), j] + step, 0., np.pi / 2)
+    g = (np.sqrt(5.) - 1.) / 2.
+    for _ in range(80):
+        a = hi - g * (hi - lo)
+        b = lo + g * (hi - lo)
+        fa, fb = value(a[:, None])[:, 0], value(b[:, None])[:, 0]
+        keep_lo = fa > fb
+        hi = np.where(keep_lo, b, hi)
+        lo = np.where(keep_lo, lo, a)
+    best = value((0.5 * (lo + hi))[:, None])[:, 0]
+    return np.maximum(best, vals.max(axis=1))
+
+
+def tti_factor(d, fm, x, e=None, f32=False, order2=False):
+    """The TTI factorization of the oracle at points x[n, d] (relative to the
+    origin) for the metric fm = [l0, l1, q00, q10, q11, A row-major]: returns
+    (norm[n], grad[n, d]) and, with offsets e[n, d], also fact[n, 2] (shifts
+    of x - e, x + e) and, with order2, fact2[n, 2] (x - 2e, x + 2e)."""
+    x = _c(np.atleast_2d(x), np.float64)
+    n = len(x)
+    fmv = _c(list(np.asarray(fm, dtype=np.float64).reshape(-1)) + [0.] * 14, np.float64)[:14].copy()
+    norm = np.zeros(n)
+    grad = np.zeros((n, d))
+    ee = fact = fact2 = None
+    if e is not None:
+        ee = np.zeros((n, 3), dtype=np.int32)
+        ee[:, :d] = np.asarray(e).reshape(n, d)
+        fact = np.zeros((n, 2))
+        fact2 = np.zeros((n, 2)) if order2 else None
+    lib().orc_tti_factor(d, int(f32), _p(fmv), n, _p(x), _p(ee), _p(norm), _p(grad), _p(fact), _p(fact2))
+    if e is None:
+        return norm, grad
+    return (norm, grad, fact, fact2) if order2 else (norm, grad, fact)
+
+
 def metric_norm(cm, diff):
     """Norms of diff[k, d] for the cost metric cm (one point, broadcast, or
     one per row)."""
     diff = np.asarray(diff, dtype=np.float64)
+    if cm[0] == 'tti':
+        return tti_primal_norm(cm[1], cm[2], cm[3], diff)
     if cm[0] == 'diag':
         return np.sqrt(np.sum((cm[1] * diff) ** 2, axis=-1))
     m, w = cm[1], cm[2]
@@ -567,10 +651,13 @@ def metric_norm(cm, diff):
 
 def factor_metric(cm):
     """The factorization metric of _Kernel.py:301-308 from the cost metric at
-    the seed: squared costs (diagonal models), else the lower triangle of m
-    (the drift of Rander is left out)."""
+    the seed: squared costs (diagonal models), [l, q, A] (TTI), else the
+    lower triangle of m (the drift of Rander is left out)."""
     if cm[0] == 'diag':
         return (cm[1].reshape(-1) ** 2)
+    if cm[0] == 'tti':
+        A = cm[3].reshape(cm[3].shape[-2:])
+        return np.concatenate([cm[1].reshape(-1), cm[2].reshape(-1), A.reshape(-1)])
     m = cm[1].reshape(cm[1].shape[-2:])
     return flatten_lower(m)
 

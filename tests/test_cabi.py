@@ -35,9 +35,9 @@ def test_library_exports_every_declared_symbol():
 def test_struct_layouts_match_header():
     from adaptivegriddiscretizations_amd import _lib
     # offsets follow the natural alignment of the C declarations
-    assert ctypes.sizeof(_lib.EikonalDesc) == 248
+    assert ctypes.sizeof(_lib.EikonalDesc) == 312
     assert _lib.EikonalDesc.factor_radius2.offset == 152
-    assert _lib.EikonalDesc.bound_active_blocks.offset == 232
+    assert _lib.EikonalDesc.bound_active_blocks.offset == 296
     assert _lib.EikonalDesc.atol.offset == 72
     assert _lib.EikonalDesc.nitermax_o.offset == 88
     assert ctypes.sizeof(_lib.EikonalIO) == 80
