@@ -67,8 +67,9 @@ def RunGPU(hfmIn, *args, cache=None, host_outputs=False, **kwargs):
 class Interface:
     """Carries out RunGPU (_Interface.py:23-174).  Not meant for direct use."""
 
-    def __init__(self, hfmIn, host_outputs=False):
+    def __init__(self, hfmIn, host_outputs=False, batched=False):
         self.hfmIn = hfmIn
+        self.batched = batched  # member of a batched solve (tiles below 64 cells allowed)
         if hfmIn['arrayOrdering'] != 'RowMajor':
             raise ValueError("Only RowMajor indexing supported")
         self.host_outputs = host_outputs
@@ -192,7 +193,8 @@ class Interface:
         self.niter_i = int(traits['niter_i'])
         if len(self.shape_i) != self.ndim:
             raise ValueError(f"shape_i {self.shape_i} does not match dimension {self.ndim}")
-        if not _lib.lib.agd_tile_supported(_MODELS[self.model_], self.ndim, _lib.i32(self.shape_i)):
+        supported = _lib.lib.agd_batch_tile_supported if self.batched else _lib.lib.agd_tile_supported
+        if not supported(_MODELS[self.model_], self.ndim, _lib.i32(self.shape_i)):
             raise ValueError(f"No kernel compiled for tile shape {self.shape_i}")
         self.size_i = int(np.prod(self.shape_i))
 

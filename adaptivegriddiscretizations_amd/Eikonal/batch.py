@@ -38,7 +38,7 @@ def _interfaces(hfmIns, traits):
         h['precompute_scheme'] = False
         if 'verbosity' not in h:
             h['verbosity'] = 0
-        it = HFM_GPU.Interface(h)
+        it = HFM_GPU.Interface(h, batched=True)
         it.SetKernelTraits()
         it.SetGeometry()
         it.SetArgs()
@@ -73,8 +73,8 @@ class Batch:
         self.values_t = cat('values_t')
         self.values0_t = self.values_t.clone()
         self.rhs_t = cat('rhs_t')
-        self.seeds_t = cat('seeds_t')      # size_tot is a multiple of 64: word aligned
-        self.trigger_t = cat('trigger_t')
+        self.seeds_t = self._cat_bits('seeds_t')
+        self.trigger_t = self._cat_bits('trigger_t')
         self.geom = None
         if it0.geom is not None:
             self.geom = torch.cat([it.geom for it in self.ifaces], dim=1).contiguous()
@@ -89,6 +89,21 @@ class Batch:
         self.stream = stream_handle()
         self.solver = _lib.Solver(self.stream)  # worklist workspace, cached pass graph
         self.ncells = int(sum(int(np.prod(it.shape)) for it in self.ifaces))
+
+    def _cat_bits(self, name):
+        """Concatenation of the problems' bit arrays (1 bit per cell, 32 to a
+        word): word by word when every size_tot is a multiple of 32, else
+        bit by bit (tiles of 16 cells)."""
+        words = [getattr(it, name) for it in self.ifaces]
+        if all(n % 32 == 0 for n in self.sizes):
+            return torch.cat(words)
+        sh = torch.arange(32, device=words[0].device, dtype=torch.int64)
+        bits = torch.cat([((w.to(torch.int64).unsqueeze(-1) >> sh) & 1).reshape(-1)[:n]
+                          for w, n in zip(words, self.sizes)])
+        pad = (-bits.numel()) % 32
+        bits = torch.cat([bits, bits.new_zeros(pad)]).reshape(-1, 32)
+        packed = torch.sum(bits << sh, dim=1)                       # < 2^32
+        return torch.where(packed >= 2 ** 31, packed - 2 ** 32, packed).to(torch.int32)
 
     def desc(self):
         d = self.ifaces[0]._desc()
@@ -168,7 +183,8 @@ def _key(hfmIns):
 def candidate_traits(model):
     """Tile shapes and in-tile sweep counts tried by the autotuner."""
     if model.endswith('2'):
-        return [{'shape_i': s, 'niter_i': n} for s in ((8, 8), (16, 16), (8, 16)) for n in (4, 8, 16)]
+        return [{'shape_i': s, 'niter_i': n}
+                for s in ((4, 4), (4, 8), (8, 4), (8, 8), (16, 16), (8, 16)) for n in (4, 8, 16)]
     return [{'shape_i': s, 'niter_i': n} for s in ((4, 4, 4), (8, 8, 4), (4, 4, 8)) for n in (2, 4, 8)]
 
 
@@ -180,7 +196,7 @@ def autotune(hfmIns, candidates=None, repeats=2):
     model = hfmIns[0]['model']
     table = []
     for tr in (candidates or candidate_traits(model)):
-        if not _lib.lib.agd_tile_supported(
+        if not _lib.lib.agd_batch_tile_supported(
                 {'Isotropic': _lib.ISOTROPIC, 'Diagonal': _lib.DIAGONAL, 'Riemann': _lib.RIEMANN,
                  'Rander': _lib.RANDER}[model[:-1]], int(model[-1]), _lib.i32(tr['shape_i'])):
             continue

@@ -84,6 +84,7 @@ SIGNATURES = {
     "agd_last_error": (c_char_p, []),
     "agd_version": (c_int32, []),
     "agd_tile_supported": (c_int32, [c_int32, c_int32, c_int32_p]),
+    "agd_batch_tile_supported": (c_int32, [c_int32, c_int32, c_int32_p]),
     "agd_device_count": (c_int32, [c_int32_p]),
     "agd_solver_create": (c_int32, [POINTER(EikonalDesc), c_void_p, POINTER(c_void_p)]),
     "agd_solver_destroy": (c_int32, [c_void_p]),

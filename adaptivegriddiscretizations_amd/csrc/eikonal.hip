@@ -73,9 +73,14 @@ __global__ void __launch_bounds__(64) init_list_ids(const uint32_t* trigger, int
   if (t < ntiles) {
     on = all != 0;
     const int64_t b0 = int64_t(t) * NI, b1 = b0 + NI;
-    for (int64_t b = b0; b < b1 && !on; b += 32) {
-      // NI is a multiple of 64: whole words
-      on = trigger[b >> 5] != 0u;
+    if constexpr (NI >= 32) {
+      for (int64_t b = b0; b < b1 && !on; b += 32) {
+        // NI is a multiple of 32: whole words
+        on = trigger[b >> 5] != 0u;
+      }
+    } else {
+      // NI (16) divides 32: the tile's bits within one word
+      on = on || ((trigger[b0 >> 5] >> (b0 & 31)) & ((1u << NI) - 1u)) != 0u;
     }
   }
   const int shard = blockIdx.x % kShards;
@@ -232,10 +237,17 @@ struct TileKernels {
 
 template <int D, int T0, int T1, int T2> TileKernels make_tile_kernels() {
   TileKernels k;
-  k.init = eik_init_list<D, T0 * T1 * T2>;
   k.init_ids = init_list_ids<T0 * T1 * T2>;
-  k.all_tiles = all_tiles_kernel<D>;
-  k.wallpass = wall_dist_pass<D, T0, T1, T2>;
+  if constexpr (T0 * T1 * T2 < 64) {
+    // tiles below a wave: batched solves only
+    k.init = nullptr;
+    k.all_tiles = nullptr;
+    k.wallpass = nullptr;
+  } else {
+    k.init = eik_init_list<D, T0 * T1 * T2>;
+    k.all_tiles = all_tiles_kernel<D>;
+    k.wallpass = wall_dist_pass<D, T0, T1, T2>;
+  }
   return k;
 }
 
@@ -246,6 +258,9 @@ static bool lookup_tile_kernels(int d, const int32_t* si, TileKernels& out) {
       case 808: out = make_tile_kernels<2, 8, 8, 1>(); return true;
       case 1616: out = make_tile_kernels<2, 16, 16, 1>(); return true;
       case 816: out = make_tile_kernels<2, 8, 16, 1>(); return true;
+      case 404: out = make_tile_kernels<2, 4, 4, 1>(); return true;
+      case 408: out = make_tile_kernels<2, 4, 8, 1>(); return true;
+      case 804: out = make_tile_kernels<2, 8, 4, 1>(); return true;
     }
   } else if (d == 3) {
     switch (key) {
@@ -539,6 +554,7 @@ static int solve_impl(Workspace& ws, const agd_eikonal_desc* d, const agd_eikona
                       agd_eikonal_stats* stats, hipStream_t stream) {
   KernelSet<S> ks;
   AGD_REQUIRE(lookup<S>(d->model, d->ndim, d->shape_i, ks), "unsupported tile shape for this model");
+  AGD_REQUIRE(ks.tiles_per_block == 1, "tiles of fewer than 64 cells are for batched solves only");
   AGD_REQUIRE(io && io->values && io->rhs && io->seeds, "values, rhs and seeds are required");
   AGD_REQUIRE(d->model == AGD_ISOTROPIC || io->geom, "geometry required for this model");
   AGD_REQUIRE(d->solver != AGD_FIM || d->fim_front_width >= 2, "fim_front_width must be >= 2");
@@ -688,6 +704,7 @@ static int flow_impl(const agd_eikonal_desc* d, const agd_eikonal_io* io, const 
                      double chk_rtol = 0., bool repair = false) {
   KernelSet<S> ks;
   AGD_REQUIRE(lookup<S>(d->model, d->ndim, d->shape_i, ks), "unsupported tile shape for this model");
+  AGD_REQUIRE(ks.flow, "tiles of fewer than 64 cells are for batched solves only");
   AGD_REQUIRE(io && io->values && io->rhs && io->seeds && (f || violations), "missing buffers");
   const GridDesc g = make_grid(d);
   FlowArgs<S> a{};
@@ -773,7 +790,9 @@ static int solve_batched_impl(Workspace& ws, const agd_eikonal_desc* d, int32_t 
   AGD_HIP_TRY(hipGetLastError());
   AGD_HIP_TRY(hipStreamSynchronize(stream));  // h is released on return
 
-  const int blocks = blocks_for(int(tiles), ni);
+  // workgroups of ks.block lanes, each visiting ks.tiles_per_block tiles at once
+  const int kb = ks.tiles_per_block;
+  const int blocks = blocks_for(int(ceil_div(tiles, kb)), ks.block);
   const bool global = d->solver == AGD_GLOBAL;
   EikArgs<S> a;
   std::memset(&a, 0, sizeof a);
@@ -812,7 +831,7 @@ static int solve_batched_impl(Workspace& ws, const agd_eikonal_desc* d, int32_t 
   auto launch = [&](int32_t koff, hipStream_t s, bool phase) {
     a.w.koff = koff;
     a.w.phase_from_koff = phase ? 1 : 0;
-    hipLaunchKernelGGL(ks.update_batch, dim3(blocks), dim3(ni), 0, s, a);
+    hipLaunchKernelGGL(ks.update_batch, dim3(blocks), dim3(ks.block), 0, s, a);
   };
   auto key = [&](std::vector<unsigned char>& k) {
     key_append(k, ks.update_batch);
@@ -820,7 +839,7 @@ static int solve_batched_impl(Workspace& ws, const agd_eikonal_desc* d, int32_t 
   };
   WorklistResult r;
   st = run_worklist(ws, int32_t(tiles), blocks, d->nitermax_o, d->check_interval, stream, init,
-                    launch, key, r, d->time_launches != 0, d->use_graph != 0);
+                    launch, key, r, d->time_launches != 0, d->use_graph != 0, kb);
   if (st) return st;
   if (stats) {
     stats->kernel_ms = r.kernel_ms;
@@ -1114,7 +1133,12 @@ int agd_eikonal_scheme(const agd_eikonal_desc* desc, const agd_eikonal_io* io, v
 
 int agd_tile_supported(int32_t model, int32_t ndim, const int32_t* shape_i) {
   KernelSet<float> ks;
-  return (shape_i && lookup<float>(model, ndim, shape_i, ks)) ? 1 : 0;
+  return (shape_i && lookup<float>(model, ndim, shape_i, ks) && ks.tiles_per_block == 1) ? 1 : 0;
+}
+
+int agd_batch_tile_supported(int32_t model, int32_t ndim, const int32_t* shape_i) {
+  KernelSet<float> ks;
+  return (shape_i && lookup<float>(model, ndim, shape_i, ks) && ks.update_batch) ? 1 : 0;
 }
 
 int agd_eikonal_solve(const agd_eikonal_desc* desc, const agd_eikonal_io* io,

@@ -338,8 +338,16 @@ __device__ inline void resolve_codes(const Stencil<S, MODEL, D>& st, const int* 
 // VAR_GENERAL with the second-order correction (HFM.h:91-134).
 enum { VAR_FAST = 0, VAR_GENERAL = 1, VAR_PRE = 2, VAR_BATCH = 3, VAR_ORDER2 = 4 };
 
+// Tiles of fewer than 64 cells (2D 4x4, 4x8, 8x4; batched solves only) are
+// packed K = 64 / NI to a single-wave workgroup: lane group g = tid / NI
+// visits list entry (workgroup's first entry + g) with its own LDS slice, its
+// convergence flag a ballot restricted to its lanes, and its activation
+// lanes g * NI + (0 .. 2D).
+template <int NI> constexpr int tiles_per_block() { return NI < 64 ? 64 / NI : 1; }
+template <int NI> constexpr int block_lanes() { return NI < 64 ? 64 : NI; }
+
 template <class S, int MODEL, int D, int T0, int T1, int T2, int VAR>
-__global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
+__global__ void __launch_bounds__(block_lanes<T0 * T1 * T2>()) eik_update(EikArgs<S> a) {
   constexpr bool ORDER2 = VAR == VAR_ORDER2;
   constexpr bool GENERAL = VAR == VAR_GENERAL || ORDER2;
   constexpr bool WALLS = GENERAL;
@@ -352,21 +360,28 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
     if constexpr (OFF32) return ld32(base, uint32_t(idx));
     else return base[idx];
   };
-  static_assert((T0 * T1 * T2) % 64 == 0, "tiles are whole waves (pass_begin scans per wave)");
   using TL = Tile<D, T0, T1, T2>;
   using MT = ModelTraits<MODEL, D>;
   constexpr int NI = TL::NI;
   constexpr int NS = MT::NSYM;
   constexpr int NG = MT::NGEOM;
   constexpr int NJ = 2 * NS;
-  __shared__ S u_s[NI + 1];  // slot NI holds 0: the read of neighbours outside the tile
+  constexpr int K = tiles_per_block<NI>();
+  static_assert(NI % 64 == 0 || (BATCH && 64 % NI == 0),
+                "tiles are whole waves, or (batched solves) pack whole into one wave");
+  // tile g's values at [g * (NI + 1), g * (NI + 1) + NI); slot NI holds 0:
+  // the read of neighbours outside the tile
+  __shared__ S u_s[K * (NI + 1)];
   __shared__ uint8_t wall_s[WALLS ? NI : 1];
 
   const int tid = threadIdx.x;
+  const int lt = K > 1 ? tid % NI : tid;  // cell of the tile
+  const int grp = K > 1 ? tid / NI : 0;   // tile of the workgroup
+  S* const us = u_s + grp * (NI + 1);
   const PassView pv = pass_begin(a.w);
   const int count = pv.count;
   int xi[3];
-  TL::pos(tid, xi);
+  TL::pos(lt, xi);
   const int32_t tag = pv.iter + 1;
   const bool fim = a.solver == AGD_FIM;
   // bound_active_blocks (general variants only): the threshold of this pass
@@ -374,8 +389,22 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
   const bool bab = GENERAL && a.bab.ctl != nullptr;
   const S bab_thr = bab ? S(reinterpret_cast<const double*>(a.bab.ctl + kBabThr)[1]) : inf_v<S>();
 
-  for (int li = blockIdx.x; li < count; li += gridDim.x) {
-    const int pk = pass_entry(pv, li);
+  for (int li = blockIdx.x * K; li < count; li += gridDim.x * K) {
+    // A lane group past the end of the list repeats the last entry with
+    // every store and activation switched off.
+    int pk;
+    bool valid = true;
+    if constexpr (K == 1) {
+      pk = pass_entry(pv, li);
+    } else {
+      pk = 0;
+#pragma unroll
+      for (int g = 0; g < K; ++g) {
+        const int e = pass_entry(pv, min(li + g, count - 1));
+        if (g == grp) pk = e;
+      }
+      valid = li + grp < count;
+    }
     int xo[3];
     // Grid and constants of the tile's instance (the solve's own when not
     // batched).  Batched list entries are tile ids, else packed coordinates.
@@ -384,8 +413,8 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
     S atol, rtol, iso_w[3];
     int n_o;
     if constexpr (BATCH) {
-      n_o = __builtin_amdgcn_readfirstlane(pk);
-      const int ii = __builtin_amdgcn_readfirstlane(a.tile_inst[n_o]);
+      n_o = K == 1 ? __builtin_amdgcn_readfirstlane(pk) : pk;
+      const int ii = K == 1 ? __builtin_amdgcn_readfirstlane(a.tile_inst[n_o]) : a.tile_inst[n_o];
       const InstDesc* I = a.inst + ii;
       gi = I->g;
       tbase = I->tile_base;
@@ -406,16 +435,17 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
     int xt[3] = {0, 0, 0};
 #pragma unroll
     for (int k = 0; k < D; ++k) xt[k] = xo[k] * TL::T[k] + xi[k];
-    const int n_t = n_o * NI + tid;
+    const int n_t = n_o * NI + lt;
 
     // ---- activation inputs first: the tile and its face neighbours
-    // (lanes 0..2D of wave 0), and their FIM scores of the previous pass.
+    // (lanes 0..2D of the tile's lanes), and their FIM scores of the
+    // previous pass.
     int nb = -1, nbp = pk, sc = 0;
-    if (tid < 2 * D + 1) {
-      if (tid == 2 * D) {
+    if (lt < 2 * D + 1) {
+      if (lt == 2 * D) {
         nb = n_o;
       } else {
-        nb = tile_neighbor<D>(xo, tid >> 1, 2 * (tid & 1) - 1, gi, &nbp);
+        nb = tile_neighbor<D>(xo, lt >> 1, 2 * (lt & 1) - 1, gi, &nbp);
         if (BATCH && nb >= 0) {
           nb += tbase;
           nbp = nb;
@@ -445,8 +475,8 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
       if (a.wall) wx = a.wall[n_t];
       wall_s[tid] = uint8_t(wx);
     }
-    u_s[tid] = seed ? rhs : u_old;
-    if (tid == 0) u_s[NI] = S(0);
+    us[lt] = seed ? rhs : u_old;
+    if (lt == 0) us[NI] = S(0);
 
     int code[NJ];
     int code2[ORDER2 ? NJ : 1];
@@ -517,7 +547,7 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
 #pragma unroll
             for (int sd = 0; sd < 2; ++sd) {
               const int c = ld32(a.scode + int64_t(m * NJ + 2 * k + sd) * N, uint32_t(n_t));
-              side[sd] = c <= -2 ? u_s[-2 - c] : (c >= 0 ? ld32(a.u, uint32_t(c)) : inf_v<S>());
+              side[sd] = c <= -2 ? us[-2 - c] : (c >= 0 ? ld32(a.u, uint32_t(c)) : inf_v<S>());
               if (fb >= 0) side[sd] += a.fac.tab[int64_t(m * NJ + 2 * k + sd) * nbox + fb];
             }
             const int sd = (side[0] > side[1]) ? 1 : 0;  // Greater(v,-) keeps the -e side on ties
@@ -525,17 +555,17 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
             if (a.scode2) {
               // the further neighbour on the chosen side (second order)
               const int c2 = ld32(a.scode2 + int64_t(m * NJ + 2 * k + sd) * N, uint32_t(n_t));
-              v2[k] = c2 <= -2 ? u_s[-2 - c2] : (c2 >= 0 ? ld32(a.u, uint32_t(c2)) : inf_v<S>());
+              v2[k] = c2 <= -2 ? us[-2 - c2] : (c2 >= 0 ? ld32(a.u, uint32_t(c2)) : inf_v<S>());
               if (fb >= 0) v2[k] += a.fac.tab[int64_t(m * NJ + 2 * k + sd) * nbox + fb2];
             }
           }
-          if (a.scode2) order2_correct<S, NS>(u_s[tid], a.order2_thr, v, v2, w);
+          if (a.scode2) order2_correct<S, NS>(us[lt], a.order2_thr, v, v2, w);
           const S um = hfm_solve<S, NS>(v, w, rhs);
           if (mix_min == (best > um)) best = um;
         }
         __syncthreads();
-        const bool ch = active && (u_s[tid] > best);
-        if (ch) u_s[tid] = best;
+        const bool ch = active && (us[lt] > best);
+        if (ch) us[lt] = best;
         if (!block_any<NI>(ch)) break;
       }
     } else {
@@ -569,7 +599,7 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
         S w2v[ORDER2 ? NS : 1];  // further-neighbour values (order 2)
         S x[NJ];
 #pragma unroll
-        for (int j = 0; j < NJ; ++j) x[j] = u_s[slot[j]];
+        for (int j = 0; j < NJ; ++j) x[j] = us[slot[j]];
         // Keeps every read unconditional (the compiler would otherwise sink
         // each into a branch with its own wait on the LDS).
 #pragma unroll
@@ -584,26 +614,36 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
             // the further neighbour on the chosen side
             const int j2 = 2 * k + ((a0 > a1) ? 1 : 0);
             const int c2 = code2[j2];
-            w2v[k] = c2 <= -2 ? u_s[-2 - c2] + sh2[j2] : (c2 >= 0 ? g2[j2] + sh2[j2] : inf_v<S>());
+            w2v[k] = c2 <= -2 ? us[-2 - c2] + sh2[j2] : (c2 >= 0 ? g2[j2] + sh2[j2] : inf_v<S>());
           }
         }
-        if constexpr (ORDER2) order2_correct<S, NS>(u_s[tid], a.order2_thr, v, w2v, w);
+        if constexpr (ORDER2) order2_correct<S, NS>(us[lt], a.order2_thr, v, w2v, w);
         const S un = hfm_solve<S, NS>(v, w, rhs);
         __syncthreads();
-        const bool ch = active && (u_s[tid] > un);
-        if (ch) u_s[tid] = un;
+        const bool ch = active && (us[lt] > un);
+        if (ch) us[lt] = un;
+        // (packed tiles: the sweeps go on while any tile of the wave
+        // changes; a tile that did not change gives the same values again)
         if (!block_any<NI>(ch)) break;
       }
     }
 
     // ---- store and convergence (Update.h:163-219)
-    const S un = u_s[tid];
-    if constexpr (OFF32) st32(a.u, uint32_t(n_t), un);
-    else a.u[n_t] = un;
+    const S un = us[lt];
+    if (valid) {
+      if constexpr (OFF32) st32(a.u, uint32_t(n_t), un);
+      else a.u[n_t] = un;
+    }
     const S diff = abs_(u_old - un);
     const S tol = atol + rtol * abs_(un);
     const bool changed = !(isnan_(diff) || diff <= tol);
-    any = block_any<NI>(changed);
+    if constexpr (K == 1) {
+      any = block_any<NI>(changed);
+    } else {
+      __syncthreads();
+      const uint64_t m = (uint64_t(1) << NI) - 1;
+      any = ((uint64_t(__ballot(changed)) >> (grp * NI)) & m) != 0;
+    }
     if (bab) minchg = block_min<NI>(changed ? un : inf_v<S>());
     }  // real
 
@@ -632,13 +672,14 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
           }
         }
       } else if (!fim) {
-        enqueue_wave(any && nb >= 0, nb, nbp, tag, a.w.stamp, pv.list_out, pv.count_out);
+        enqueue_wave(valid && any && nb >= 0, nb, nbp, tag, a.w.stamp, pv.list_out, pv.count_out);
       } else {
         const int front = a.fim_front;
-        const int prev = __shfl(sc, 2 * D);
+        const int l0 = grp * NI;  // the tile's first lane
+        const int prev = __shfl(sc, l0 + 2 * D);
         int mx = 0;
 #pragma unroll
-        for (int i = 0; i < 2 * D; ++i) mx = max(mx, __shfl(sc, i));
+        for (int i = 0; i < 2 * D; ++i) mx = max(mx, __shfl(sc, l0 + i));
         int next;
         bool prop;
         if (!any) {
@@ -649,8 +690,8 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_update(EikArgs<S> a) {
           if (front > 2) next = max(next, mx - 1);
           prop = next > 1;
         }
-        if (tid == 2 * D) pv.score_next[n_o] = (pv.iter << 8) | next;
-        const bool want = (tid == 2 * D) ? any : (prop && tid < 2 * D && nb >= 0);
+        if (lt == 2 * D && valid) pv.score_next[n_o] = (pv.iter << 8) | next;
+        const bool want = valid && ((lt == 2 * D) ? any : (prop && lt < 2 * D && nb >= 0));
         enqueue_wave(want, nb, nbp, tag, a.w.stamp, pv.list_out, pv.count_out);
       }
     }
@@ -1061,43 +1102,59 @@ template <class S> struct ModelKernels {
   void (*scheme_general)(SchemeArgs<S>);
   void (*flow)(FlowArgs<S>);
   void (*tti_factor)(TTIFactorArgs<S>);  // TTI only: shift table of the factoring ball
-  int ni;
+  int ni;               // cells per tile
+  int block;            // lanes per workgroup of the update kernels
+  int tiles_per_block;  // tiles a workgroup visits at once (tiles below a wave)
 };
 
 template <class S, int MODEL, int D, int T0, int T1, int T2>
 ModelKernels<S> make_model_kernels() {
   ModelKernels<S> k;
-  if constexpr (MODEL == AGD_TTI) {
-    // TTI runs from its precomputed scheme only.
-    k.update = nullptr;
-    k.update_walls = nullptr;
-    k.update_batch = nullptr;
-    k.update_order2 = nullptr;
-  } else {
-    k.update = eik_update<S, MODEL, D, T0, T1, T2, VAR_FAST>;
-    k.update_walls = eik_update<S, MODEL, D, T0, T1, T2, VAR_GENERAL>;
-    k.update_order2 = eik_update<S, MODEL, D, T0, T1, T2, VAR_ORDER2>;
-    if constexpr (std::is_same<S, float>::value)
+  k.ni = T0 * T1 * T2;
+  k.block = block_lanes<T0 * T1 * T2>();
+  k.tiles_per_block = tiles_per_block<T0 * T1 * T2>();
+  if constexpr (T0 * T1 * T2 < 64) {
+    // tiles below a wave: the batched FP32 variant only
+    k.update = k.update_walls = k.update_pre = k.update_order2 = nullptr;
+    k.scheme = k.scheme_general = nullptr;
+    k.flow = nullptr;
+    k.tti_factor = nullptr;
+    if constexpr (std::is_same<S, float>::value && MODEL != AGD_TTI)
       k.update_batch = eik_update<S, MODEL, D, T0, T1, T2, VAR_BATCH>;
     else
       k.update_batch = nullptr;
-  }
-  // Precomputed schemes pay off where the stencil is costly to build
-  // (Selling decompositions: one for Riemann, seven for TTI, which has no
-  // other variant); the drift of Rander is not stored.
-  if constexpr (MODEL == AGD_RIEMANN || MODEL == AGD_TTI) {
-    k.update_pre = eik_update<S, MODEL, D, T0, T1, T2, VAR_PRE>;
-    k.scheme = eik_scheme<S, MODEL, D, T0, T1, T2, false>;
-    k.scheme_general = eik_scheme<S, MODEL, D, T0, T1, T2, true>;
   } else {
-    k.update_pre = nullptr;
-    k.scheme = nullptr;
-    k.scheme_general = nullptr;
+    if constexpr (MODEL == AGD_TTI) {
+      // TTI runs from its precomputed scheme only.
+      k.update = nullptr;
+      k.update_walls = nullptr;
+      k.update_batch = nullptr;
+      k.update_order2 = nullptr;
+    } else {
+      k.update = eik_update<S, MODEL, D, T0, T1, T2, VAR_FAST>;
+      k.update_walls = eik_update<S, MODEL, D, T0, T1, T2, VAR_GENERAL>;
+      k.update_order2 = eik_update<S, MODEL, D, T0, T1, T2, VAR_ORDER2>;
+      if constexpr (std::is_same<S, float>::value)
+        k.update_batch = eik_update<S, MODEL, D, T0, T1, T2, VAR_BATCH>;
+      else
+        k.update_batch = nullptr;
+    }
+    // Precomputed schemes pay off where the stencil is costly to build
+    // (Selling decompositions: one for Riemann, seven for TTI, which has no
+    // other variant); the drift of Rander is not stored.
+    if constexpr (MODEL == AGD_RIEMANN || MODEL == AGD_TTI) {
+      k.update_pre = eik_update<S, MODEL, D, T0, T1, T2, VAR_PRE>;
+      k.scheme = eik_scheme<S, MODEL, D, T0, T1, T2, false>;
+      k.scheme_general = eik_scheme<S, MODEL, D, T0, T1, T2, true>;
+    } else {
+      k.update_pre = nullptr;
+      k.scheme = nullptr;
+      k.scheme_general = nullptr;
+    }
+    k.flow = eik_flow<S, MODEL, D, T0, T1, T2>;
+    if constexpr (MODEL == AGD_TTI) k.tti_factor = eik_tti_factor<S, D, T0, T1, T2>;
+    else k.tti_factor = nullptr;
   }
-  k.flow = eik_flow<S, MODEL, D, T0, T1, T2>;
-  if constexpr (MODEL == AGD_TTI) k.tti_factor = eik_tti_factor<S, D, T0, T1, T2>;
-  else k.tti_factor = nullptr;
-  k.ni = T0 * T1 * T2;
   return k;
 }
 
@@ -1105,7 +1162,8 @@ inline int tile_key(int d, const int32_t* si) {
   return d == 2 ? (si[0] * 100 + si[1]) : (si[0] * 10000 + si[1] * 100 + si[2]);
 }
 
-// The compiled tile shapes: 2D 8x8, 16x16, 8x16; 3D 4x4x4, 8x8x4, 4x4x8.
+// The compiled tile shapes: 2D 8x8, 16x16, 8x16; 3D 4x4x4, 8x8x4, 4x4x8;
+// and, for batched solves only, 2D 4x4, 4x8, 8x4.
 template <class S, int MODEL>
 bool lookup_model_kernels(int d, const int32_t* si, ModelKernels<S>& out) {
   const int key = tile_key(d, si);
@@ -1114,6 +1172,9 @@ bool lookup_model_kernels(int d, const int32_t* si, ModelKernels<S>& out) {
       case 808: out = make_model_kernels<S, MODEL, 2, 8, 8, 1>(); return true;
       case 1616: out = make_model_kernels<S, MODEL, 2, 16, 16, 1>(); return true;
       case 816: out = make_model_kernels<S, MODEL, 2, 8, 16, 1>(); return true;
+      case 404: out = make_model_kernels<S, MODEL, 2, 4, 4, 1>(); return true;
+      case 408: out = make_model_kernels<S, MODEL, 2, 4, 8, 1>(); return true;
+      case 804: out = make_model_kernels<S, MODEL, 2, 8, 4, 1>(); return true;
     }
   } else if (d == 3) {
     switch (key) {

@@ -125,14 +125,17 @@ template <class T> inline void key_append(std::vector<unsigned char>& k, const T
 }
 
 // Capacity of one sub-list.  A pass appends at most 2d+1 <= 7 tiles per
-// tile visit and the workgroups b = s mod kShards of a grid of G visit at
-// most ceil(G/kShards) * ceil(n/G) tiles; the initial list (64-lane
-// workgroups over the tiles) puts at most ntiles/kShards + 64 in a
-// sub-list; no sub-list exceeds ntiles (one append per tile and pass).
-inline int64_t shard_capacity(int64_t ntiles, int64_t grid) {
+// tile visit, and the workgroups b = s mod kShards of a grid of G, each
+// visiting K tiles at once, visit at most
+// ceil(G/kShards) * K * ceil(n/(G K)) <= n/kShards + G K/kShards + n/G + K
+// tiles; the initial list (64-lane workgroups over the tiles) puts at most
+// ntiles/kShards + 64 in a sub-list; no sub-list exceeds ntiles (one append
+// per tile and pass).
+inline int64_t shard_capacity(int64_t ntiles, int64_t grid, int64_t tiles_per_block = 1) {
   auto cd = [](int64_t a, int64_t b) { return (a + b - 1) / b; };
   grid = std::max<int64_t>(grid, 1);
-  const int64_t visits = cd(ntiles, kShards) + cd(grid, kShards) + cd(ntiles, grid) + 1;
+  const int64_t k = std::max<int64_t>(tiles_per_block, 1);
+  const int64_t visits = cd(ntiles, kShards) + cd(grid * k, kShards) + cd(ntiles, grid) + k;
   return std::min<int64_t>(ntiles, 7 * visits + 64);
 }
 
@@ -375,11 +378,11 @@ inline int64_t hist_len(int64_t nitermax, int check_interval) {
 template <class Init, class Launch, class Key>
 int run_worklist(Workspace& wsr, int32_t ntiles, int32_t grid, int64_t nitermax, int check_interval,
                  hipStream_t stream, Init init, Launch launch, Key key, WorklistResult& res,
-                 bool time_launches = false, bool graph = false) {
+                 bool time_launches = false, bool graph = false, int tiles_per_block = 1) {
   Workspace* ws = &wsr;
   const int ci = std::max(1, check_interval);
   const int64_t nhist = hist_len(nitermax, ci);
-  int st = ws->ensure(ntiles, shard_capacity(ntiles, grid), nhist);
+  int st = ws->ensure(ntiles, shard_capacity(ntiles, grid, tiles_per_block), nhist);
   if (st) return st;
   if (ws->fixed) {
     time_launches = false;

@@ -160,6 +160,10 @@ const char* agd_last_error(void);
 int agd_version(void);
 /* 1 if (model, ndim, shape_i) has a compiled kernel. */
 int agd_tile_supported(int32_t model, int32_t ndim, const int32_t* shape_i);
+/* 1 if (model, ndim, shape_i) has a compiled batched kernel: the shapes above
+ * and, for agd_eikonal_solve_batched only, the 2D tiles of fewer than 64 cells
+ * (4x4, 4x8, 8x4; the reference takes any shape_i, kernel_traits.py:55-90). */
+int agd_batch_tile_supported(int32_t model, int32_t ndim, const int32_t* shape_i);
 int agd_device_count(int32_t* count);
 
 /* ------------------------------------------------------------ solver handles

@@ -65,5 +65,35 @@ def test_autotune_picks_a_candidate_and_is_reused():
         log(f"  {tr}: {ms:.2f} ms, {passes} passes")
     assert best in [t for t, _, _ in table]
     assert best == min(table, key=lambda r: r[1])[0]
+    # the candidates span 4x4 to 16x16 tiles (tiles below a wave included)
+    shapes = {tuple(t['shape_i']) for t, _, _ in table}
+    assert {(4, 4), (4, 8), (8, 4), (8, 8), (16, 16)} <= shapes
     outs = batch.RunBatch(hfms, tune=True)
     check_against_single(hfms[:2], outs[:2])
+
+
+@pytest.mark.parametrize("shape_i", [(4, 4), (4, 8), (8, 4)])
+@pytest.mark.parametrize("solver", ["FIM", "AGSI"])
+def test_subwave_tiles_batch_matches_single_solves(solver, shape_i):
+    """Tiles of 16 and 32 cells, packed 4 or 2 to a wave: every problem of
+    the batch reaches the fixed point of its single solve (8x8 tiles, the
+    same scheme), including odd tile counts whose seed bits do not start on
+    a word boundary, and an empty lane group at the end of every pass."""
+    from adaptivegriddiscretizations_amd.Eikonal import batch
+    rng = np.random.default_rng(11)
+    shapes = [(70, 90), (33, 200), (17, 45), (64, 64), (150, 77), (5, 9)]
+    hfms = [iso_problem(rng, a, b, 1 + i % 3) for i, (a, b) in enumerate(shapes)]
+    for h in hfms:
+        h['solver'] = solver
+    outs = batch.RunBatch(hfms, traits={'shape_i': shape_i, 'niter_i': 8})
+    log(f"batch of {len(hfms)} ({solver}, tiles {shape_i}): passes "
+        f"{outs[0]['stats']['eikonal']['launches']}")
+    check_against_single(hfms, outs)
+
+
+def test_subwave_tiles_are_refused_outside_batches():
+    rng = np.random.default_rng(12)
+    hfm = iso_problem(rng, 40, 40, 1)
+    hfm['traits'] = {'shape_i': (4, 4)}
+    with pytest.raises(ValueError, match="tile shape"):
+        hfm.Run()

@@ -184,3 +184,43 @@ def test_rander_zermelo_round_trip():
     # unit speed + drift: moving along the drift costs less than against it
     v = np.array([1., 0.])
     assert r.norm(v) < r.norm(-v)
+
+
+def test_batch_bit_arrays_concatenate_at_bit_offsets():
+    """Batched solves with 16-cell tiles: the problems' seed/trigger bit
+    arrays (32 cells per word) concatenate at bit offsets that are not word
+    aligned; the packed result holds every problem's bits at its offset."""
+    import types
+    import torch
+    from adaptivegriddiscretizations_amd.Eikonal import batch
+    rng = np.random.default_rng(3)
+    sizes = [48, 80, 16, 64]
+    bits = [rng.integers(0, 2, size=n) for n in sizes]
+
+    def pack(b):
+        b = np.concatenate([b, np.zeros((-len(b)) % 32, dtype=b.dtype)]).reshape(-1, 32)
+        w = (b.astype(np.uint64) << np.arange(32, dtype=np.uint64)).sum(axis=1)
+        return torch.from_numpy(w.astype(np.uint32).view(np.int32).copy())
+
+    fake = types.SimpleNamespace(sizes=sizes,
+                                 ifaces=[types.SimpleNamespace(seeds_t=pack(b)) for b in bits])
+    got = batch.Batch._cat_bits(fake, 'seeds_t').numpy().view(np.uint32)
+    ref = pack(np.concatenate(bits)).numpy().view(np.uint32)
+    assert np.array_equal(got, ref)
+    # word-aligned sizes take the plain concatenation
+    fake.sizes = [64, 32]
+    fake.ifaces = fake.ifaces[:2]
+    fake.ifaces[0].seeds_t = pack(bits[1][:64])
+    fake.ifaces[1].seeds_t = pack(bits[0][:32])
+    got = batch.Batch._cat_bits(fake, 'seeds_t').numpy().view(np.uint32)
+    assert np.array_equal(got, pack(np.concatenate([bits[1][:64], bits[0][:32]])).numpy().view(np.uint32))
+
+
+def test_subwave_tile_shapes_are_batched_only():
+    from adaptivegriddiscretizations_amd import _lib
+    for shape in ((4, 4), (4, 8), (8, 4)):
+        assert _lib.lib.agd_batch_tile_supported(_lib.ISOTROPIC, 2, _lib.i32(shape)) == 1
+        assert _lib.lib.agd_tile_supported(_lib.ISOTROPIC, 2, _lib.i32(shape)) == 0
+    assert _lib.lib.agd_batch_tile_supported(_lib.ISOTROPIC, 2, _lib.i32((8, 8))) == 1
+    assert _lib.lib.agd_tile_supported(_lib.ISOTROPIC, 2, _lib.i32((8, 8))) == 1
+    assert _lib.lib.agd_batch_tile_supported(_lib.ISOTROPIC, 2, _lib.i32((2, 2))) == 0
