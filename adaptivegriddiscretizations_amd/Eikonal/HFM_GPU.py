@@ -685,9 +685,15 @@ class Interface:
 
     # ------------------------------------------------------------ flow
     def PostProcess(self):
-        """_PostProcess.py:46-95: one global pass exporting the upwind flow."""
+        """_PostProcess.py:46-95: one global pass exporting the upwind flow.
+        Geodesics alone need no exported flow when it is recomputed online
+        (_Kernel.py:148-150, 223-224)."""
+        self.geodesic_online_flow = bool(self.GetValue(
+            'geodesic_online_flow', default=False,
+            help="Compute the flow online when extracting geodesics (saves memory)")) if self.hasTips else False
         self.flow_needed = (self.exportGeodesicFlow or self.reverseAD or self.forwardAD
-                            or self.hasTips or getattr(self, 'keep_flow', False))
+                            or (self.hasTips and not self.geodesic_online_flow)
+                            or getattr(self, 'keep_flow', False))
         if not self.flow_needed:
             return
         if self.verbosity >= 1:
@@ -756,9 +762,6 @@ class Interface:
         if self.verbosity >= 1:
             print("Computing the geodesics")
         tips = self.hfmIn.PointFromIndex(self.tips, to=True).astype(self.float_t)
-        self.GetValue('geodesic_online_flow', default=False,
-                      help="Compute the flow online when extracting geodesics (here the flow of the "
-                           "post-processing pass is always used)")
         step = float(self.GetValue('geodesic_step', default=0.25,
                                    help='Step size, in pixels, for the geodesic ODE solver'))
         eucl_delay = int(self.GetValue(
@@ -818,9 +821,14 @@ class Interface:
             gd.ngeo, gd.max_len = n, typical_len
             gd.eucl_delay, gd.nymin_delay, gd.pastseed_delay = eucl_delay, nymin_delay, pastseed_delay
             gd.step, gd.causality_tolerance = step, causality
-            _lib.check(_lib.lib.agd_geodesic_ode(desc, gd, ptr(self.values_t), ptr(self.flow_vector_t),
-                                                 ptr(self.flow_weightsum_t), ptr(eucl), ptr(x_s),
-                                                 ptr(len_s), ptr(stop_s), self.stream))
+            if self.geodesic_online_flow:
+                # the flow of each cell the ODE enters, recomputed from the scheme
+                _lib.check(_lib.lib.agd_geodesic_ode_online(desc, self._io(), gd, ptr(eucl), ptr(x_s),
+                                                            ptr(len_s), ptr(stop_s), self.stream))
+            else:
+                _lib.check(_lib.lib.agd_geodesic_ode(desc, gd, ptr(self.values_t), ptr(self.flow_vector_t),
+                                                     ptr(self.flow_weightsum_t), ptr(eucl), ptr(x_s),
+                                                     ptr(len_s), ptr(stop_s), self.stream))
             x_h, len_h, stop_h = x_s.cpu().numpy(), len_s.cpu().numpy(), stop_s.cpu().numpy()
             nxt = []
             for i, x, ln, sp in zip(corresp, x_h, len_h, stop_h):

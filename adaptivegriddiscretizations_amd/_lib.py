@@ -120,6 +120,8 @@ SIGNATURES = {
                                     c_void_p, c_void_p, c_void_p]),
     "agd_geodesic_ode": (c_int32, [POINTER(EikonalDesc), POINTER(GeodesicDesc), c_void_p, c_void_p,
                                    c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "agd_geodesic_ode_online": (c_int32, [POINTER(EikonalDesc), POINTER(EikonalIO), POINTER(GeodesicDesc),
+                                          c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "agd_block_expand": (c_int32, [c_int32, c_int64_p, c_int32_p, c_int32, c_int32, c_void_p,
                                    c_int64, c_void_p, c_double, c_void_p]),
     "agd_block_squeeze": (c_int32, [c_int32, c_int64_p, c_int32_p, c_int32, c_int32, c_void_p,

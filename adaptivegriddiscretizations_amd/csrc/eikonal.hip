@@ -698,6 +698,23 @@ static int solve_impl(Workspace& ws, const agd_eikonal_desc* d, const agd_eikona
   return AGD_OK;
 }
 
+// Flow-pass arguments of a solve: values, geometry and constants, with no
+// output buffers.
+template <class S> static FlowArgs<S> flow_args(const agd_eikonal_desc* d, const agd_eikonal_io* io) {
+  FlowArgs<S> a{};
+  a.u = static_cast<S*>(io->values);
+  a.geom = static_cast<const S*>(io->geom);
+  a.rhs = static_cast<const S*>(io->rhs);
+  a.seeds = io->seeds;
+  a.wall = io->wall_dist;
+  a.g = make_grid(d);
+  for (int k = 0; k < 3; ++k) a.iso_w[k] = S(d->iso_weights[k]);
+  a.order2 = d->order == 2 ? 1 : 0;
+  a.order2_thr = S(d->order2_threshold);
+  a.fac = make_factor<S>(d);
+  return a;
+}
+
 template <class S>
 static int flow_impl(const agd_eikonal_desc* d, const agd_eikonal_io* io, const agd_flow_io* f,
                      hipStream_t stream, int32_t* violations = nullptr, double chk_atol = 0.,
@@ -706,15 +723,9 @@ static int flow_impl(const agd_eikonal_desc* d, const agd_eikonal_io* io, const 
   AGD_REQUIRE(lookup<S>(d->model, d->ndim, d->shape_i, ks), "unsupported tile shape for this model");
   AGD_REQUIRE(ks.flow, "tiles of fewer than 64 cells are for batched solves only");
   AGD_REQUIRE(io && io->values && io->rhs && io->seeds && (f || violations), "missing buffers");
+  AGD_REQUIRE(d->order >= 0 && d->order <= 2, "order must be 1 or 2 (0 means 1)");
   const GridDesc g = make_grid(d);
-  FlowArgs<S> a{};
-  a.u = static_cast<S*>(io->values);
-  a.geom = static_cast<const S*>(io->geom);
-  a.rhs = static_cast<const S*>(io->rhs);
-  a.seeds = io->seeds;
-  a.wall = io->wall_dist;
-  a.g = g;
-  for (int k = 0; k < 3; ++k) a.iso_w[k] = S(d->iso_weights[k]);
+  FlowArgs<S> a = flow_args<S>(d, io);
   a.fw = f ? static_cast<S*>(f->flow_weights) : nullptr;
   a.wsum = f ? static_cast<S*>(f->flow_weightsum) : nullptr;
   a.fidx = f ? f->flow_indices : nullptr;
@@ -723,11 +734,29 @@ static int flow_impl(const agd_eikonal_desc* d, const agd_eikonal_io* io, const 
   a.chk_atol = S(chk_atol);
   a.chk_rtol = S(chk_rtol);
   a.repair = repair ? 1 : 0;
-  AGD_REQUIRE(d->order >= 0 && d->order <= 2, "order must be 1 or 2 (0 means 1)");
-  a.order2 = d->order == 2 ? 1 : 0;
-  a.order2_thr = S(d->order2_threshold);
-  a.fac = make_factor<S>(d);
   hipLaunchKernelGGL(ks.flow, dim3(g.ntiles), dim3(ks.ni), 0, stream, a);
+  AGD_HIP_TRY(hipGetLastError());
+  return AGD_OK;
+}
+
+template <class S>
+static int geodesic_online_impl(const agd_eikonal_desc* d, const agd_eikonal_io* io,
+                                const agd_geodesic_desc* gd, const uint8_t* eucl, void* x,
+                                int32_t* len, int8_t* stop, hipStream_t stream) {
+  KernelSet<S> ks;
+  AGD_REQUIRE(lookup<S>(d->model, d->ndim, d->shape_i, ks), "unsupported tile shape for this model");
+  AGD_REQUIRE(ks.geodesic_online, "tiles of fewer than 64 cells are for batched solves only");
+  AGD_REQUIRE(io && io->values && io->rhs && io->seeds, "values, rhs and seeds are required");
+  AGD_REQUIRE(d->model == AGD_ISOTROPIC || io->geom, "geometry required for this model");
+  AGD_REQUIRE(d->order >= 0 && d->order <= 2, "order must be 1 or 2 (0 means 1)");
+  AGD_REQUIRE(eucl && x && len && stop, "missing buffers");
+  int hlen = 0;
+  if (int st = check_geodesic_desc(gd, hlen)) return st;
+  if (gd->ngeo == 0) return AGD_OK;
+  GeoArgs<S> a = geodesic_args<S>(geo_grid(d), gd, hlen, io->values, eucl, x, len, stop);
+  const FlowArgs<S> fa = flow_args<S>(d, io);
+  hipLaunchKernelGGL(ks.geodesic_online, dim3(unsigned(ceil_div(gd->ngeo, 64))), dim3(64), 0, stream,
+                     a, fa);
   AGD_HIP_TRY(hipGetLastError());
   return AGD_OK;
 }
@@ -1145,6 +1174,16 @@ int agd_eikonal_solve(const agd_eikonal_desc* desc, const agd_eikonal_io* io,
                       agd_eikonal_stats* stats, void* stream) {
   Workspace ws;  // this call's own, released on return
   return solve_entry(ws, desc, io, stats, as_stream(stream));
+}
+
+int agd_geodesic_ode_online(const agd_eikonal_desc* desc, const agd_eikonal_io* io,
+                            const agd_geodesic_desc* gd, const uint8_t* eucl, void* x, int32_t* len,
+                            int8_t* stop, void* stream) {
+  int st = check_desc(desc);
+  if (st) return st;
+  if (desc->scalar == AGD_F32)
+    return geodesic_online_impl<float>(desc, io, gd, eucl, x, len, stop, as_stream(stream));
+  return geodesic_online_impl<double>(desc, io, gd, eucl, x, len, stop, as_stream(stream));
 }
 
 int agd_eikonal_flow(const agd_eikonal_desc* desc, const agd_eikonal_io* io,

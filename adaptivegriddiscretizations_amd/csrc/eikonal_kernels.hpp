@@ -27,6 +27,7 @@
 
 #include "common.hpp"
 #include "device.hpp"
+#include "geodesic_kernels.hpp"
 #include "worklist.hpp"
 
 namespace agd {
@@ -884,21 +885,27 @@ template <class S> struct FlowArgs {
   FactorDesc<S> fac;
 };
 
-template <class S, int MODEL, int D, int T0, int T1, int T2>
-__global__ void __launch_bounds__(T0* T1* T2) eik_flow(FlowArgs<S> a) {
-  using TL = Tile<D, T0, T1, T2>;
+// The upwind update of one cell and its flow (Update.h:149-209, the body
+// of the flow pass): value val of the scheme at the cell, flow weights
+// w_k (v - v_k)_+ (3/2 of it where the second-order correction applied),
+// their sum, the flow vector sum_k fw_k (side_k e_k) in grid units and the
+// chosen neighbours' tiled indices.  Reads the cell and its neighbours only
+// (the flow pass and the geodesic ODE's recomputed flow share it).
+template <class S, int NS> struct CellFlow {
+  S u_old, val;
+  bool seed;
+  S fw[NS];
+  int64_t nb[NS];
+  S sum;
+  S vec[3];
+};
+
+template <class S, int MODEL, int D, class TL>
+__device__ inline void flow_cell(const FlowArgs<S>& a, const int* xt, int64_t n_t,
+                                 CellFlow<S, ModelTraits<MODEL, D>::NSYM>& out) {
   using MT = ModelTraits<MODEL, D>;
-  constexpr int NI = TL::NI;
   constexpr int NS = MT::NSYM;
   constexpr int NG = MT::NGEOM;
-  const int tid = threadIdx.x;
-  const int n_o = blockIdx.x;
-  int xi[3], xo[3], xt[3] = {0, 0, 0};
-  TL::pos(tid, xi);
-  tile_pos<D>(n_o, a.g, xo);
-#pragma unroll
-  for (int k = 0; k < D; ++k) xt[k] = xo[k] * TL::T[k] + xi[k];
-  const int64_t n_t = int64_t(n_o) * NI + tid;
   const int64_t N = a.g.size_tot;
 
   const S u_old = a.u[n_t];
@@ -1047,20 +1054,51 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_flow(FlowArgs<S> a) {
     val = hfm_solve<S, NS>(vs, ws, rhs);
   }
 
-  S fw[NS];
   S sum = S(0);
-  S vec[3] = {S(0), S(0), S(0)};
   const bool live = !seed && val != inf_v<S>();
+#pragma unroll
+  for (int c = 0; c < 3; ++c) out.vec[c] = S(0);
 #pragma unroll
   for (int k = 0; k < NS; ++k) {
     const S diff = val - v[k];
-    fw[k] = live ? w[k] * (diff > S(0) ? diff : S(0)) : S(0);
-    if ((o2mask >> k) & 1u) fw[k] *= S(1.5);
-    sum += fw[k];
+    S f = live ? w[k] * (diff > S(0) ? diff : S(0)) : S(0);
+    if ((o2mask >> k) & 1u) f *= S(1.5);
+    out.fw[k] = f;
+    out.nb[k] = nb[k];
+    sum += f;
     const int sg = 2 * side[k] - 1;
 #pragma unroll
-    for (int c = 0; c < D; ++c) vec[c] += fw[k] * S(sg * st.e[k][c]);
+    for (int c = 0; c < D; ++c) out.vec[c] += f * S(sg * st.e[k][c]);
   }
+  out.sum = sum;
+  out.u_old = u_old;
+  out.val = val;
+  out.seed = seed;
+}
+
+template <class S, int MODEL, int D, int T0, int T1, int T2>
+__global__ void __launch_bounds__(T0* T1* T2) eik_flow(FlowArgs<S> a) {
+  using TL = Tile<D, T0, T1, T2>;
+  using MT = ModelTraits<MODEL, D>;
+  constexpr int NI = TL::NI;
+  constexpr int NS = MT::NSYM;
+  const int tid = threadIdx.x;
+  const int n_o = blockIdx.x;
+  int xi[3], xo[3], xt[3] = {0, 0, 0};
+  TL::pos(tid, xi);
+  tile_pos<D>(n_o, a.g, xo);
+#pragma unroll
+  for (int k = 0; k < D; ++k) xt[k] = xo[k] * TL::T[k] + xi[k];
+  const int64_t n_t = int64_t(n_o) * NI + tid;
+  const int64_t N = a.g.size_tot;
+  CellFlow<S, NS> r;
+  flow_cell<S, MODEL, D, TL>(a, xt, n_t, r);
+  const S u_old = r.u_old, val = r.val;
+  const bool seed = r.seed;
+  const S* fw = r.fw;
+  const int64_t* nb = r.nb;
+  const S sum = r.sum;
+  const S* vec = r.vec;
   if (a.violations) {
     const bool bad = !seed && val > u_old + a.chk_atol + a.chk_rtol * abs_(u_old);
     const uint64_t m = __ballot(bad);
@@ -1089,6 +1127,31 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_flow(FlowArgs<S> a) {
   }
 }
 
+// ------------------------------------------------------------------ geodesics
+// Flow recomputed at each cell the geodesic ODE enters
+// (geodesic_online_flow, GeodesicODE_Opt.h:101-238): the flow pass's cell
+// computation on the solver's values, geometry and constants, so that no
+// flow arrays are kept.
+template <class S, int MODEL, int D, class TL> struct GeoFlowOnline {
+  const FlowArgs<S>* fa;
+  const GeoGrid* g;
+  __device__ void at(int64_t n, S (&vec)[D], S& ws) const {
+    int y[3] = {0, 0, 0};
+    geo_pos<D>(n, *g, y);
+    CellFlow<S, ModelTraits<MODEL, D>::NSYM> r;
+    flow_cell<S, MODEL, D, TL>(*fa, y, n, r);
+    ws = r.sum;
+#pragma unroll
+    for (int k = 0; k < D; ++k) vec[k] = r.vec[k];
+  }
+};
+
+template <class S, int MODEL, int D, int T0, int T1, int T2>
+__global__ void __launch_bounds__(64) eik_geodesic_online(GeoArgs<S> a, FlowArgs<S> fa) {
+  const GeoFlowOnline<S, MODEL, D, Tile<D, T0, T1, T2>> src{&fa, &a.g};
+  geodesic_run<S, D>(a, src);
+}
+
 // ------------------------------------------------------------------ tables
 // Model kernels of one (model, dimension, tile shape); null entries are
 // variants that do not exist for the model.
@@ -1102,6 +1165,7 @@ template <class S> struct ModelKernels {
   void (*scheme_general)(SchemeArgs<S>);
   void (*flow)(FlowArgs<S>);
   void (*tti_factor)(TTIFactorArgs<S>);  // TTI only: shift table of the factoring ball
+  void (*geodesic_online)(GeoArgs<S>, FlowArgs<S>);  // geodesics with the flow recomputed
   int ni;               // cells per tile
   int block;            // lanes per workgroup of the update kernels
   int tiles_per_block;  // tiles a workgroup visits at once (tiles below a wave)
@@ -1119,6 +1183,7 @@ ModelKernels<S> make_model_kernels() {
     k.scheme = k.scheme_general = nullptr;
     k.flow = nullptr;
     k.tti_factor = nullptr;
+    k.geodesic_online = nullptr;
     if constexpr (std::is_same<S, float>::value && MODEL != AGD_TTI)
       k.update_batch = eik_update<S, MODEL, D, T0, T1, T2, VAR_BATCH>;
     else
@@ -1152,6 +1217,7 @@ ModelKernels<S> make_model_kernels() {
       k.scheme_general = nullptr;
     }
     k.flow = eik_flow<S, MODEL, D, T0, T1, T2>;
+    k.geodesic_online = eik_geodesic_online<S, MODEL, D, T0, T1, T2>;
     if constexpr (MODEL == AGD_TTI) k.tti_factor = eik_tti_factor<S, D, T0, T1, T2>;
     else k.tti_factor = nullptr;
   }

@@ -227,14 +227,19 @@ def main():
     assert np.isfinite(u32).all(), "non-finite values in the benchmark solve"
 
     fp64_err = None
+    fp64_run = None
     if not args.no_fp64:
+        # the same solve in FP64 arithmetic (what multiprecision runs here):
+        # its error against FP32, and its own cost (second of two solves)
         import numpy as _np
         iface64 = prepare(wl, traits={'Scalar': _np.float64, 'shape_i': iface.shape_i,
                                       'niter_i': iface.niter_i},
                           atol=iface.atol, rtol=iface.rtol)
+        solve_once(iface64, iface64._desc())
         st64 = solve_once(iface64, iface64._desc())
         u64 = iface64.values.cpu().numpy()
         fp64_err = float(np.max(np.abs(u32 - u64)) / np.max(np.abs(u64)))
+        fp64_run = {"ms": st64.gpu_ms, "passes": int(st64.launches), "cell_updates": int(st64.cell_updates)}
         log(f"fp64 run: {st64.gpu_ms:.1f} ms, max|u32-u64|/max u64 = {fp64_err:.3e}")
         del iface64
 
@@ -264,6 +269,7 @@ def main():
         "cell_updates_per_step": cell_updates / args.steps,
         "passes_per_step": launches / args.steps,
         "fp64_max_rel_err": fp64_err,
+        "fp64_solve": fp64_run,
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": None,

@@ -336,6 +336,14 @@ int agd_geodesic_eucl(const agd_eikonal_desc* desc, const uint32_t* seeds, int32
 int agd_geodesic_ode(const agd_eikonal_desc* desc, const agd_geodesic_desc* gd, const void* values,
                      const void* flow_vector, const void* flow_weightsum, const uint8_t* eucl,
                      void* x, int32_t* len, int8_t* stop, void* stream);
+/* The same, with the flow at each cell the ODE enters recomputed from the
+ * scheme instead of read from exported arrays (geodesic_online_flow,
+ * GeodesicODE_Opt.h:101-238, _GetGeodesics.py:92-93): desc and io as for
+ * agd_eikonal_flow (values, geom, rhs, seeds, walls; the model, order and
+ * source factorization of desc), no flow buffers.  Tiles of 64 cells or more. */
+int agd_geodesic_ode_online(const agd_eikonal_desc* desc, const agd_eikonal_io* io,
+                            const agd_geodesic_desc* gd, const uint8_t* eucl, void* x, int32_t* len,
+                            int8_t* stop, void* stream);
 
 /* ------------------------------------------------------------ layout */
 /* block_expand (FiniteDifferences.py:164-195): src cell-major

@@ -140,3 +140,47 @@ def test_isotropic3_straight_geodesics(multiprecision):
     # within a pixel of the segment: the first-order scheme's upwind flow is
     # not exactly radial (0.76 pixel measured on this grid)
     assert worst < 1.0
+
+
+@pytest.mark.parametrize("model", ["Isotropic2", "Riemann2", "TTI2"])
+def test_online_flow_geodesics_match_exported_flow(model):
+    """geodesic_online_flow: the ODE recomputes the flow of every cell it
+    enters from the scheme instead of reading the flow pass's arrays.  Same
+    stopping criteria and lengths, points within 1e-3 pixel (the flow pass
+    also lowers values by at most the solver tolerance, which the online
+    path does not see), and no flow arrays kept."""
+    from adaptivegriddiscretizations_amd import workloads
+    from adaptivegriddiscretizations_amd.Eikonal import HFM_GPU
+    n = 96
+    if model == "Isotropic2":
+        x = np.linspace(-1, 1, n)
+        X = np.stack(np.meshgrid(x, x, indexing='ij'))
+        base = {'model': model, 'cost': (1 + 0.4 * np.sin(3 * X[0]) * np.cos(2 * X[1])).astype(np.float32),
+                'seed': [0.1, -0.2]}
+    elif model == "Riemann2":
+        wl = workloads.riemann2_aniso(n=n, seed=31, sigma=4., amax=4., nseeds=1)
+        base = {'model': model, 'metric': Metrics.Riemann(wl.metric), 'seeds': wl.seeds}
+    else:
+        wl = workloads.tti_tilted(n=n, d=2, seed=32)
+        base = {'model': model, 'metric': wl.extra['tti'], 'seeds': wl.seeds}
+    res, ifaces = {}, {}
+    for online in (False, True):
+        hfmIn = Eikonal.dictIn({**base, 'exportValues': True, 'verbosity': 0,
+                                'geodesic_online_flow': online})
+        hfmIn.SetRect([[-1, 1], [-1, 1]], dims=[n, n])
+        hfmIn.SetUniformTips((4, 4))
+        ifaces[online] = HFM_GPU.Interface(hfmIn)
+        res[online] = ifaces[online].Run()
+    a, b = res[False], res[True]
+    assert 'flow' not in b
+    assert getattr(ifaces[True], 'flow_vector_t', None) is None   # no flow pass, no flow arrays
+    assert getattr(ifaces[False], 'flow_vector_t', None) is not None
+    assert a['geodesic_stopping_criteria'] == b['geodesic_stopping_criteria']
+    worst = 0.
+    for ga, gb in zip(a['geodesics'], b['geodesics']):
+        assert abs(ga.shape[1] - gb.shape[1]) <= 1
+        m = min(ga.shape[1], gb.shape[1])
+        worst = max(worst, float(np.max(np.abs(ga[:, :m] - gb[:, :m]))) * n / 2.)
+    log(f"{model} online flow: stops {set(b['geodesic_stopping_criteria'])}, worst point gap "
+        f"{worst:.2e} pixel")
+    assert worst <= 1e-3
