@@ -258,20 +258,47 @@ class dictIn(_AliasedStore):
         return 2 * d - 1 if self.SE else d
 
     @property
+    def nTheta(self):
+        """Cells of the angle axis of a curvature-penalised model, covering
+        [0, 2 pi) (DictIn.py:323-339); set after SetRect of the physical box."""
+        if not self.SE:
+            raise ValueError("nTheta belongs to the curvature penalized models")
+        shape = self.shape
+        if len(shape) != self.vdim:
+            raise ValueError("Angular resolution not set")
+        return shape[-1]
+
+    @nTheta.setter
+    def nTheta(self, value):
+        if not self.SE:
+            raise ValueError("nTheta belongs to the curvature penalized models")
+        if self.vdim != 3:
+            raise NotImplementedError("only the 2D curvature penalized models are provided")
+        phys = self.shape[:2]
+        self['dims'] = self.array_float_caster(list(phys) + [value])
+
+    @property
     def gridScales(self):
-        """Cell side along each axis."""
+        """Cell side along each axis; for a curvature-penalised model
+        (gridScale, gridScale, 2 pi / nTheta)."""
         if self.SE:
-            raise NotImplementedError("Curvature penalized models are not provided")
+            h = self['gridScale']
+            return self.array_float_caster((h, h, 2. * np.pi / self.nTheta))
         if 'gridScales' in self:
             return self['gridScales']
         return self.array_float_caster((self['gridScale'],) * self.vdim)
 
     @property
     def corners(self):
-        """Lower and upper corners of the box."""
+        """Lower and upper corners of the box; the angle axis of a
+        curvature-penalised model starts half a cell below 0, so that its cell
+        centres are the angles 2 pi k / nTheta."""
         h = _host(self.gridScales)
         n = _host(self['dims'])
-        lo = _host(self['origin']) if 'origin' in self else np.zeros_like(n)
+        nphys = (len(n) + 1) // 2 if self.SE else len(n)
+        lo = _host(self['origin']) if 'origin' in self else np.zeros_like(n[:nphys])
+        if self.SE and len(lo) < len(n):
+            lo = np.concatenate([lo, -h[len(lo):] / 2])
         return lo, lo + h * n
 
     def Axes(self, dims=None):
@@ -353,6 +380,22 @@ class dictIn(_AliasedStore):
         if x.shape[-1] != self.vdim:
             raise ValueError(f"points of dimension {self.vdim} expected, got shape {x.shape}")
         return (x - first) / h if to else first + h * x
+
+    def OrientedPoints(self, pointU):
+        """Each physical point paired with every angle of the grid: shape
+        (nTheta, ..., 3) (DictIn.py:464-482)."""
+        if not self.SE:
+            raise ValueError("OrientedPoints only makes sense for models SE space.")
+        if self.vdim != 3:
+            raise ValueError("Sorry, oriented point not implemented for SE(3) models.")
+        p = np.asarray(_host(pointU), dtype=self.float_t)
+        if p.shape[-1] != 2:
+            raise ValueError(f"Unoriented points expected to have 2 dimensions, found {p.shape[-1]}.")
+        theta = _host(self.Axes()[2])
+        out = np.empty((len(theta),) + p.shape[:-1] + (3,), dtype=self.float_t)
+        out[..., :2] = p
+        out[..., 2] = np.asarray(theta, dtype=self.float_t).reshape((-1,) + (1,) * (p.ndim - 1))
+        return out
 
     def IndexFromPoint(self, point):
         """Nearest cell index of each point (ties to even) and the offset of

@@ -10,6 +10,7 @@ _Interface.py:128-146:
 Device buffers are torch tensors on the current HIP device (allocation and
 stream plumbing only); all computation happens in the HIP kernels.
 """
+import ctypes
 import numbers
 import time
 from collections import OrderedDict
@@ -25,8 +26,20 @@ from ._device import (torch, device_tensor, host_array, ptr, stream_handle, empt
                       full, zeros)
 
 _MODELS = {'Isotropic': _lib.ISOTROPIC, 'Diagonal': _lib.DIAGONAL,
-           'Riemann': _lib.RIEMANN, 'Rander': _lib.RANDER, 'TTI': _lib.TTI}
-_NMIX = {'TTI': 7}
+           'Riemann': _lib.RIEMANN, 'Rander': _lib.RANDER, 'TTI': _lib.TTI,
+           'ReedsShepp': _lib.REEDSSHEPP2, 'ReedsSheppForward': _lib.REEDSSHEPPFWD2,
+           'Elastica': _lib.ELASTICA2, 'Dubins': _lib.DUBINS2}
+# Curvature-penalised models: 2D position and an angle, on (x, y, theta) grids.
+_CURVATURE = ('ReedsShepp', 'ReedsSheppForward', 'Elastica', 'Dubins')
+
+
+def scheme_slots(model_, ndim):
+    """Offsets per (sub-)scheme of a model, each stored as a pair of
+    neighbours (the forward offsets of the curvature models included)."""
+    if model_ in ('Isotropic', 'Diagonal'):
+        return ndim
+    return {'ReedsShepp': 6, 'ReedsSheppForward': 7, 'Elastica': 30, 'Dubins': 6}.get(
+        model_, ndim * (ndim + 1) // 2)
 
 _SOLVER_NAMES = {'AGSI': 'adaptive_gauss_siedel_iteration', 'FIM': 'fast_iterative_method'}
 _SOLVER_IDS = {'adaptive_gauss_siedel_iteration': _lib.AGSI,
@@ -42,6 +55,10 @@ def default_traits(model):
         'Riemann2': ((8, 8), 8, 4), 'Riemann3': ((4, 4, 4), 4, 6),
         'Rander2': ((8, 8), 12, 8), 'Rander3': ((4, 4, 4), 8, 8),
         'TTI2': ((8, 8), 10, 6), 'TTI3': ((4, 4, 4), 3, 6),
+        # curvature models on (x, y, theta) grids (kernel_traits.py:23-37; the
+        # 4x4x2 tiles there are 4x4x4 here, the smallest 3D tile)
+        'ReedsShepp2': ((4, 4, 4), 6, None), 'ReedsSheppForward2': ((4, 4, 4), 6, 5),
+        'Elastica2': ((4, 4, 4), 1, 4), 'Dubins2': ((4, 4, 4), 1, None),
     }
     if model not in table:
         raise NotImplementedError(
@@ -136,7 +153,12 @@ class Interface:
         """_Kernel.py:23-92."""
         if self.verbosity >= 1:
             print("Setting the kernel traits.")
-        if self.model_ not in _MODELS or self.ndim not in (2, 3) or int(self.model[-1]) != self.ndim:
+        self.curvature = self.model_ in _CURVATURE
+        if self.model.startswith('Convex'):
+            raise NotImplementedError("the convex variants of the curvature models are not provided")
+        phys = int(self.model[-1]) if self.model[-1].isdigit() else -1
+        if self.model_ not in _MODELS or self.ndim not in (2, 3) or \
+                (phys != self.ndim and not (self.curvature and phys == 2 and self.ndim == 3)):
             default_traits(self.model)  # raises with the list of models
         traits, fim_default = default_traits(self.model)
         user = self.GetValue('traits', default=traits,
@@ -223,7 +245,7 @@ class Interface:
             print("Preparing the domain data (shape,metric,...)")
         self.shape = self.hfmIn.shape
         self.periodic = tuple(bool(p) for p in self.GetValue(
-            'periodic', default=(False,) * self.ndim,
+            'periodic', default=(False, False, True) if self.curvature else (False,) * self.ndim,
             help="Apply periodic boundary conditions on some axes"))
         self.shape_o = tuple(int(s) for s in fd.round_up_ratio(self.shape, self.shape_i))
         self.ntiles = int(np.prod(self.shape_o))
@@ -245,6 +267,8 @@ class Interface:
         if self.model_ == 'Isotropic':
             w = (self.h.astype(np.float32) ** -2).astype(np.float64)
             self.iso_weights = tuple(w.tolist()) + (0.,) * (3 - self.ndim)
+        elif self.curvature:
+            self._set_curvature_geometry()
         else:
             metric = self.GetValue('metric', default=None, verbosity=3,
                                    help="Metric of the minimal path model")
@@ -257,11 +281,13 @@ class Interface:
             help="Export the upwind geodesic flow (direction of the geodesics)")
         tips = self.GetValue('tips', default=None, help="Tips from which to compute the minimal geodesics")
         self.tips = None if tips is None else np.asarray(host_array(tips), dtype=self.float_t).reshape(-1, self.ndim)
-        if self.GetValue('tips_Unoriented', default=None, verbosity=3,
-                         help="Unoriented tips (curvature penalized models)") is not None:
-            raise NotImplementedError("'tips_Unoriented' belongs to the curvature penalized models, "
-                                      "which this solver does not provide")
-        self.hasTips = self.tips is not None
+        self.tips_Unoriented = None
+        if self.curvature:
+            tipsU = self.GetValue('tips_Unoriented', default=None,
+                                  help="Compute a geodesic from the most favorable orientation")
+            if tipsU is not None:
+                self.tips_Unoriented = np.asarray(host_array(tipsU), dtype=self.float_t).reshape(-1, 2)
+        self.hasTips = self.tips is not None or self.tips_Unoriented is not None
 
         # Cost function (_SetGeometry.py:221-233): a denseAD cost, or a
         # separate costVariation, requests forward AD.
@@ -285,6 +311,15 @@ class Interface:
         else:
             self.cost_coef = self.GetValue('costVariation', default=None,
                                            help="First order variation of the cost function")
+        if self.curvature:
+            # a physical (nx, ny) or angular (ntheta) cost field is broadcast
+            # over the grid (_Interface.py:221-234); the curvature schemes are
+            # in units of the physical cell side h (_SetGeometry.py:234)
+            if np.ndim(host_array(self.cost)) > 0:
+                self.cost = np.ascontiguousarray(self._angular_field(self.cost, 'cost'))
+            self.cost = self.cost * self.curvature_cost_scale
+            if self.cost_coef is not None:
+                self.cost_coef = self.cost_coef * self.curvature_cost_scale
 
         # Tolerances (_SetGeometry.py:237-253)
         res = np.finfo(self.tol_float_t).resolution
@@ -313,6 +348,53 @@ class Interface:
             self.minChg_delta_min = float(self.GetValue(
                 'minChg_delta_min', default=float(np.min(self.h)) / 10.,
                 help="Minimal threshold increase with bound_active_blocks method"))
+
+    def _angular_field(self, value, name):
+        """A scalar, a physical field (nx, ny), an angular field (ntheta) or a
+        full field, broadcast to the (x, y, theta) grid (_Interface.py:221-236)."""
+        a = np.asarray(host_array(value), dtype=np.float64)
+        shape = tuple(self.shape)
+        if a.ndim == 0 or a.shape == shape:
+            return np.broadcast_to(a, shape)
+        if a.shape == shape[2:]:
+            return np.broadcast_to(a.reshape((1, 1) + shape[2:]), shape)
+        if a.shape == shape[:2]:
+            return np.broadcast_to(a.reshape(shape[:2] + (1,)), shape)
+        raise ValueError(f"Field {name} has incorrect dimensions {a.shape} for the grid {shape}")
+
+    def _set_curvature_geometry(self):
+        """Curvature models (_SetGeometry.py:83-109, _Kernel.py:318-332): per
+        cell [1/xi, kappa, cos theta, sin theta] in grid units, with 1/xi and
+        kappa divided by h_theta / h (the angular over the physical cell
+        side); theta defaults to the angle of the grid's third axis; the cost
+        is multiplied by h (_SetGeometry.py:234)."""
+        h_base, h_per = float(self.h[0]), float(self.h[2])
+        ratio = h_per / h_base
+        xi = self.GetValue('xi', help="Cost of rotation for the curvature penalized models")
+        kappa = self.GetValue('kappa', default=0., help="Rotation bias for the curvature penalized models")
+        theta = self.GetValue('theta', default=0., verbosity=3,
+                              help="Deviation from horizontality, for the curvature penalized models")
+        eps = float(self.GetValue('eps', default=0.1,
+                                  help='Relaxation parameter for the curvature penalized models'))
+        if not eps > 0:
+            raise ValueError("eps must be > 0")
+        self.curvature_relax = eps * eps
+        ixi = 1. / (self._angular_field(xi, 'xi') * ratio)
+        kap = self._angular_field(kappa, 'kappa') / ratio
+        if np.ndim(host_array(theta)) == 0:
+            # a scalar theta selects the angles of the grid's third axis
+            th = self._angular_field(np.asarray(host_array(self.hfmIn.Axes()[2]), dtype=np.float64), 'theta')
+        else:
+            th = self._angular_field(theta, 'theta')
+        geom = np.stack([ixi, kap, np.cos(th), np.sin(th)])
+        g_d = device_tensor(np.ascontiguousarray(geom), self.float_t)
+        self.geom = empty((4, self.size_tot), self.float_t)
+        esize = np.dtype(self.float_t).itemsize
+        _lib.check(_lib.lib.agd_block_expand(self.ndim, _lib.i64(self.shape), _lib.i32(self.shape_i), 4,
+                                             esize, ptr(g_d), 1, ptr(self.geom),
+                                             0., self.stream))
+        self._keep = (g_d,)
+        self.curvature_cost_scale = h_base
 
     def _set_metric_geometry(self, metric, dualMetric):
         model = self.model_
@@ -460,6 +542,9 @@ class Interface:
         self.trigger_t = zeros((nwords,), np.int32)
 
         seeds = self.GetValue('seeds', default=None, help="Points from where the front propagation starts")
+        seedsU = self.GetValue('seeds_Unoriented', default=None,
+                               help="Unoriented seeds for the curvature penalized models") \
+            if self.curvature else None
         self.reverseAD = self.HasValue('sensitivity')
         self.seedIndices = None
         self.seed_coef = None
@@ -485,6 +570,8 @@ class Interface:
             seedRadius = float(self.GetValue(
                 'seedRadius', default=2. if self.factoringRadius > 0 else 0.,
                 help="Spread the seeds over a radius given in pixels, so as to improve accuracy."))
+            if self.curvature and seedRadius > 0:
+                raise NotImplementedError("seedRadius > 0 with the curvature penalized models")
             if (seedRadius > 0 or self.factoringRadius) and len(seeds) != 1:
                 raise ValueError("seedRadius > 0 and factoringRadius apply to a single seed")
             cm = self._cost_metric() if (seedRadius > 0 or self.factoringRadius) else None
@@ -510,6 +597,30 @@ class Interface:
                 self.factor = {'radius2': self.factoringRadius ** 2,
                                'origin': np.asarray(cont[0], dtype=np.float64),
                                'metric': cm.factor_metric(a0)}
+        if seedsU is not None:
+            # _SetArgs.py:86-100: every orientation of each unoriented seed,
+            # rounded to the nearest cell
+            pU = np.asarray(host_array(seedsU), dtype=self.hfmIn.float_t).reshape(-1, 2)
+            valsU = np.asarray(host_array(self.GetValue(
+                'seedValues_Unoriented', default=np.zeros(len(pU)),
+                help="Initial value for the front propagation")), dtype=np.float64).reshape(-1)
+            if len(valsU) != len(pU):
+                raise ValueError("seedValues_Unoriented must have one entry per unoriented seed")
+            contU = self.hfmIn.PointFromIndex(self.hfmIn.OrientedPoints(pU).reshape(-1, 3), to=True)
+            idxU = np.round(contU).astype(np.int64)
+            idxU[:, 2] %= self.shape[2]
+            valsU = np.tile(valsU, self.shape[2])
+            if seeds is None:
+                idx, seedValues = idxU, valsU
+                self.seed_rows = np.arange(len(idxU))
+            else:
+                if self.seed_coef is not None or self.factor is not None:
+                    raise NotImplementedError("seeds_Unoriented with seed AD or factorization")
+                self.seed_rows = np.concatenate([self.seed_rows, len(seeds) + np.arange(len(idxU))])
+                idx, seedValues = np.concatenate([idx, idxU]), np.concatenate([seedValues, valsU])
+            if np.any(idx < 0) or np.any(idx >= np.asarray(self.shape)):
+                raise ValueError("Seed outside the domain")
+        if seeds is not None or seedsU is not None:
             self.seedIndices = idx
             self.seedValues = seedValues
             tiled = fd.tiled_index(idx, self.shape_i, self.shape_o)
@@ -518,8 +629,9 @@ class Interface:
             _lib.check(_lib.lib.agd_scatter_seeds(self.scalar, len(idx), ptr(tiled_d), ptr(vals_d),
                                                   ptr(self.rhs_t), ptr(self.seeds_t),
                                                   ptr(self.trigger_t), self.stream))
-        elif self.solver != 'global_iteration' and not self.warmStartAllTiles:
-            raise ValueError("Missing value for key seeds")
+        else:
+            if self.solver != 'global_iteration' and not self.warmStartAllTiles:
+                raise ValueError("Missing value for key seeds")
 
         self.forwardAD = self.cost_coef is not None or self.seed_coef is not None
         self.values_coef = None
@@ -545,22 +657,26 @@ class Interface:
         # it (seven sub-schemes per cell exceed the register budget).
         self.scheme_w_t = self.scheme_c_t = None
         self.precompute_scheme = False
+        stored = self.model_ == 'TTI' or self.curvature   # schemes always precomputed
+        if self.curvature and (self.order != 1 or self.factoringRadius):
+            raise NotImplementedError("the curvature models use the first-order scheme, without factorization")
         if (self.order == 1 or self.model_ == 'TTI') and (not self.factoringRadius or self.model_ == 'TTI') \
                 and _lib.lib.agd_scheme_supported(
                 self.modelId, self.ndim, _lib.i32(self.shape_i)):
             self.precompute_scheme = bool(self.GetValue(
-                'precompute_scheme', default=self.model_ == 'TTI',
+                'precompute_scheme', default=stored,
                 help="Precompute and store the finite difference scheme stencils"))
-        if self.model_ == 'TTI' and not self.precompute_scheme:
-            raise ValueError("the TTI model runs from a precomputed scheme (precompute_scheme=True)")
+        if stored and not self.precompute_scheme:
+            raise ValueError(f"the {self.model_} model runs from a precomputed scheme (precompute_scheme=True)")
         if self.precompute_scheme:
-            nsym = self.ndim * (self.ndim + 1) // 2
-            nmix = _NMIX.get(self.model_, 1)
-            self.scheme_w_t = empty((nmix * nsym, self.size_tot), self.float_t)
-            # TTI: the mix flag after the codes, then (second order) the
-            # codes of the further neighbours x -+ 2 e_k
-            ncodes = nmix * 2 * nsym + (nmix > 1) + (nmix * 2 * nsym if nmix > 1 and self.order == 2 else 0)
-            self.scheme_c_t = empty((ncodes, self.size_tot), np.int32)
+            # weights and codes per the library's layout (TTI and the
+            # curvature models: the mix flag after the codes; TTI with order 2
+            # the codes of the further neighbours x -+ 2 e_k after it)
+            wc, cc = ctypes.c_int64(), ctypes.c_int64()
+            _lib.check(_lib.lib.agd_scheme_lengths(self.modelId, self.ndim, self.order, ctypes.byref(wc),
+                                                   ctypes.byref(cc)))
+            self.scheme_w_t = empty((wc.value, self.size_tot), self.float_t)
+            self.scheme_c_t = empty((cc.value, self.size_tot), np.int32)
             self.compute_scheme()
 
         self.nitermax_o = int(self.GetValue('nitermax_o', default=2000,
@@ -596,6 +712,7 @@ class Interface:
         d.time_launches = int(getattr(self, 'time_launches', False))
         d.order = int(getattr(self, 'order', 1))
         d.order2_threshold = float(getattr(self, 'order2_threshold', 0.3))
+        d.curvature_relax = float(getattr(self, 'curvature_relax', 0.))
         if getattr(self, 'bound_active_blocks', 0):
             d.bound_active_blocks = int(self.bound_active_blocks)
             d.minchg_delta_min = float(self.minChg_delta_min)
@@ -698,7 +815,7 @@ class Interface:
             return
         if self.verbosity >= 1:
             print("Post-Processing")
-        nsym = self.ndim if self.model_ in ('Isotropic', 'Diagonal') else self.ndim * (self.ndim + 1) // 2
+        nsym = scheme_slots(self.model_, self.ndim)
         self.nsym = nsym
         self.flow_weights_t = empty((nsym, self.size_tot), self.float_t)
         self.flow_weightsum_t = empty((self.size_tot,), self.float_t)
@@ -761,7 +878,20 @@ class Interface:
             return
         if self.verbosity >= 1:
             print("Computing the geodesics")
-        tips = self.hfmIn.PointFromIndex(self.tips, to=True).astype(self.float_t)
+        tips = np.zeros((0, self.ndim), self.float_t) if self.tips is None else \
+            self.hfmIn.PointFromIndex(self.tips, to=True).astype(self.float_t)
+        if self.tips_Unoriented is not None:
+            # _GetGeodesics.py:13-25: each unoriented tip starts from the
+            # cell of its most favourable orientation
+            pts = self.hfmIn.OrientedPoints(self.tips_Unoriented)      # (ntheta, n, 3)
+            idxU = np.round(self.hfmIn.PointFromIndex(pts, to=True)).astype(np.int64)
+            idxU[..., 2] %= self.shape[2]
+            vals = self.values
+            vals = vals.cpu().numpy() if isinstance(vals, torch.Tensor) else np.asarray(ad.remove_ad(vals))
+            vU = vals[tuple(np.moveaxis(idxU, -1, 0))]                   # (ntheta, n)
+            best = np.argmin(vU, axis=0)
+            tipsU = idxU[best, np.arange(idxU.shape[1])].astype(self.float_t)
+            tips = np.concatenate([tips, tipsU])
         step = float(self.GetValue('geodesic_step', default=0.25,
                                    help='Step size, in pixels, for the geodesic ODE solver'))
         eucl_delay = int(self.GetValue(
@@ -840,8 +970,12 @@ class Interface:
             corresp = nxt
             pastseed_delay = 0
         self.stats['geodesic'] = {'time': time.time() - t0, 'launches': it}
-        geos = [np.concatenate(g, axis=0) for g in geodesics]
-        self.hfmOut['geodesics'] = [self.hfmIn.PointFromIndex(g).T for g in geos]
+        geos = [self.hfmIn.PointFromIndex(np.concatenate(g, axis=0)).T for g in geodesics]
+        ntips = 0 if self.tips is None else len(self.tips)
+        if self.tips is not None:
+            self.hfmOut['geodesics'] = geos[:ntips]
+        if self.tips_Unoriented is not None:
+            self.hfmOut['geodesics_Unoriented'] = geos[ntips:]
         self.hfmOut['geodesic_stopping_criteria'] = stopping
 
     # ------------------------------------------------------------ AD

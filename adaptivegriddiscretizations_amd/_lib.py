@@ -19,6 +19,7 @@ c_uint32_p = POINTER(ctypes.c_uint32)
 AGD_OK, AGD_EINVAL, AGD_EHIP, AGD_NOCONV, AGD_ENOMEM = 0, -1, -2, -3, -4
 # agd_model
 ISOTROPIC, DIAGONAL, RIEMANN, RANDER, TTI = 0, 1, 2, 3, 4
+REEDSSHEPP2, REEDSSHEPPFWD2, ELASTICA2, DUBINS2 = 5, 6, 7, 8
 # agd_scalar
 F32, F64 = 0, 1
 # agd_solver_kind
@@ -36,6 +37,7 @@ class EikonalDesc(ctypes.Structure):
         ("order", c_int32), ("reserved", c_int32), ("order2_threshold", c_double),
         ("factor_radius2", c_double), ("factor_origin", c_double * 3), ("factor_metric", c_double * 14),
         ("bound_active_blocks", c_int64), ("minchg_delta_min", c_double),
+        ("curvature_relax", c_double),
     ]
 
 

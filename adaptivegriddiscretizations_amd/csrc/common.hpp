@@ -45,12 +45,29 @@ inline int ngeom_of(int model, int d) {
     case AGD_RIEMANN: return sd;
     case AGD_RANDER: return sd + d;
     case AGD_TTI: return 5 + d * d;
+    case AGD_REEDSSHEPP2:
+    case AGD_REEDSSHEPPFWD2:
+    case AGD_ELASTICA2:
+    case AGD_DUBINS2: return 4;
   }
   return -1;
 }
-inline int nmix_of(int model) { return model == AGD_TTI ? 7 : 1; }
+inline bool curvature_model(int model) { return model >= AGD_REEDSSHEPP2 && model <= AGD_DUBINS2; }
+inline int nmix_of(int model) { return model == AGD_TTI ? 7 : (model == AGD_DUBINS2 ? 2 : 1); }
+// Offsets of one sub-scheme, each stored as a pair of neighbour codes (the
+// forward offsets of the curvature models included).
 inline int nsym_of(int model, int d) {
-  return (model == AGD_ISOTROPIC || model == AGD_DIAGONAL) ? d : d * (d + 1) / 2;
+  switch (model) {
+    case AGD_ISOTROPIC:
+    case AGD_DIAGONAL: return d;
+    case AGD_REEDSSHEPP2: return 6;
+    case AGD_REEDSSHEPPFWD2: return 7;
+    case AGD_ELASTICA2: return 30;
+    case AGD_DUBINS2: return 6;
+  }
+  return d * (d + 1) / 2;
 }
+// Models whose scheme is always precomputed (agd_eikonal_scheme).
+inline bool stored_scheme_model(int model) { return model == AGD_TTI || curvature_model(model); }
 
 }  // namespace agd

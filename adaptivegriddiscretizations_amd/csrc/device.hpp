@@ -66,15 +66,32 @@ template <class T> __device__ inline void st32(T* base, uint32_t idx, T v) {
 // ------------------------------------------------------------------ models
 template <int MODEL, int D> struct ModelTraits {
   static constexpr int SYMDIM = D * (D + 1) / 2;
-  static constexpr int NSYM = (MODEL == AGD_ISOTROPIC || MODEL == AGD_DIAGONAL) ? D : SYMDIM;
+  // Curvature-penalised models (3D grid (x, y, theta)).
+  static constexpr bool CURV = MODEL >= AGD_REEDSSHEPP2 && MODEL <= AGD_DUBINS2;
+  // Offsets of one sub-scheme, each a pair of neighbours x -+ e (the last
+  // NFWD of them forward: x + e only, the x - e side left out).
+  static constexpr int NSYM = (MODEL == AGD_ISOTROPIC || MODEL == AGD_DIAGONAL) ? D
+                              : MODEL == AGD_REEDSSHEPP2                     ? 6
+                              : MODEL == AGD_REEDSSHEPPFWD2                  ? 7
+                              : MODEL == AGD_ELASTICA2                       ? 30
+                              : MODEL == AGD_DUBINS2                         ? 6
+                                                                             : SYMDIM;
+  static constexpr int NFWD = MODEL == AGD_REEDSSHEPPFWD2 ? 6
+                              : MODEL == AGD_ELASTICA2    ? 30
+                              : MODEL == AGD_DUBINS2      ? 6
+                                                          : 0;
   static constexpr int NGEOM = MODEL == AGD_ISOTROPIC ? 0
                                : MODEL == AGD_DIAGONAL ? D
                                : MODEL == AGD_RIEMANN  ? SYMDIM
                                : MODEL == AGD_RANDER   ? SYMDIM + D
+                               : CURV                  ? 4
                                                        : 5 + D * D;
   static constexpr bool DRIFT = (MODEL == AGD_RANDER);
-  // TTI: the update is the min or max over NMIX Selling schemes.
-  static constexpr int NMIX = MODEL == AGD_TTI ? 7 : 1;
+  // TTI: the update is the min or max over NMIX Selling schemes; Dubins2 the
+  // min over its left and right turning schemes.
+  static constexpr int NMIX = MODEL == AGD_TTI ? 7 : (MODEL == AGD_DUBINS2 ? 2 : 1);
+  // The scheme is always precomputed (agd_eikonal_scheme) and read per sweep.
+  static constexpr bool STORED = MODEL == AGD_TTI || CURV;
 };
 
 // ------------------------------------------------------------------ tiles
@@ -746,6 +763,126 @@ __device__ inline void tti_factor_pair(const S* fm, const TTIFactorData<S>& t, c
   }
 }
 
+// ------------------------------------------------------------------ curvature models
+// The curvature-penalised models on the (x, y, theta) grid (ReedsShepp2.h,
+// ReedsSheppForward2.h, Elastica2.h, Dubins2.h with Decomp_v_.h and
+// Geometry_.h:199-210).  g = [1/xi, kappa, cos theta, sin theta] per cell in
+// grid units; v = (cos theta, sin theta, kappa) is the direction of motion.
+// relax is the relaxation of the self outer products (eps^2, eps = 0.1 by
+// default).  Written without contraction, as the oracle.
+
+// m = (1 - relax) v v^T + relax |v|^2 Id, lower triangle.
+template <class S> __device__ inline void outer_relax3(const S* v, S relax, S* m) {
+#pragma clang fp contract(off)
+  const S rv2 = relax * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  const S rm1 = S(1) - relax;
+  int k = 0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j <= i; ++j, ++k) {
+      m[k] = v[i] * v[j] * rm1;
+      if (i == j) m[k] += rv2;
+    }
+}
+
+// Decomp_v_.h:7-25: Selling decomposition of the relaxed v v^T; offsets
+// deviating from v by more than acos(sqrt(2/3)) get weight 0, the others
+// point against v (<v, e> <= 0: the forward neighbour x + e lies upwind).
+template <class S> __device__ inline void decomp_v(const S* v, S relax, S* w, int (*e)[3]) {
+#pragma clang fp contract(off)
+  S m[6];
+  outer_relax3<S>(v, relax, m);
+  int eb[6][3];
+  selling3<S>(m, w, eb);
+  const S vv = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+  const S cosmin2 = S(2.0 / 3.0);
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const S ve = v[0] * S(eb[k][0]) + v[1] * S(eb[k][1]) + v[2] * S(eb[k][2]);
+    const S ee = S(eb[k][0] * eb[k][0] + eb[k][1] * eb[k][1] + eb[k][2] * eb[k][2]);
+    const bool off = ve * ve < vv * ee * cosmin2;
+    const int sg = (!off && ve > S(0)) ? -1 : 1;
+    if (off) w[k] = S(0);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) e[k][c] = sg * eb[k][c];
+  }
+}
+
+// Fejer's first quadrature rule on [0, pi] with the weight sin(phi), n nodes
+// phi_l = pi (l + 1/2) / n: w_l = (2/n) (1 - 2 sum_{j=1}^{n/2} cos(2 j phi_l) /
+// (4 j^2 - 1)) (Elastica2.h:13-45 uses the rule with n = 5).  Rounded to the
+// precision of the reference's tables, so that the stencils agree with it:
+// cosines and sines to 9 decimals (cos(pi/2) is then exactly 0), weights to 6.
+template <class S> __device__ inline void fejer_node(int l, int n, S& c, S& s, S& w) {
+  const double phi = 3.141592653589793 * (double(l) + 0.5) / double(n);
+  double acc = 0.;
+  for (int j = 1; j <= n / 2; ++j) acc += cos(2. * j * phi) / double(4 * j * j - 1);
+  c = S(rint(cos(phi) * 1e9) / 1e9);
+  s = S(rint(sin(phi) * 1e9) / 1e9);
+  w = S(rint(2. / n * (1. - 2. * acc) * 1e6) / 1e6);
+}
+constexpr int kNFejer = 5;  // nFejer_macro of Elastica2 (kernel_traits.py:30-33)
+
+// Stencil of a curvature model: NMIX sub-schemes of NSYM offset slots each
+// (slot k of mix m at m * NSYM + k; the last NFWD slots are forward
+// offsets).  Returns mix_is_min (Dubins2: the smaller of its two updates).
+template <class S, int MODEL>
+__device__ inline bool curv_stencil(const S* g, S relax, S* w, int (*e)[3]) {
+#pragma clang fp contract(off)
+  const S ixi = g[0], kappa = g[1], cT = g[2], sT = g[3];
+  if constexpr (MODEL == AGD_REEDSSHEPP2) {
+    // ReedsShepp2.h:13-36
+    const S v[3] = {cT, sT, kappa};
+    S m[6];
+    outer_relax3<S>(v, relax, m);
+    const S m5 = v[2] * v[2] + ixi * ixi;
+    m[5] = m[5] > m5 ? m[5] : m5;
+    int eb[6][3];
+    selling3<S>(m, w, eb);
+    const S wv[3] = {v[1], -v[0], S(0)};
+    const S ww = wv[0] * wv[0] + wv[1] * wv[1] + wv[2] * wv[2];
+    const S cut = S(1) - S(2.0 / 3.0);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const S we = wv[0] * S(eb[k][0]) + wv[1] * S(eb[k][1]) + wv[2] * S(eb[k][2]);
+      const S ee = S(eb[k][0] * eb[k][0] + eb[k][1] * eb[k][1] + eb[k][2] * eb[k][2]);
+      if (we * we >= ee * ww * cut) w[k] = S(0);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) e[k][c] = eb[k][c];
+    }
+    return true;
+  } else if constexpr (MODEL == AGD_REEDSSHEPPFWD2) {
+    // ReedsSheppForward2.h:19-33: pure rotation (symmetric), then forward motion
+    w[0] = ixi * ixi;
+    e[0][0] = 0;
+    e[0][1] = 0;
+    e[0][2] = -1;
+    const S v[3] = {cT, sT, kappa};
+    decomp_v<S>(v, relax, w + 1, e + 1);
+    return true;
+  } else if constexpr (MODEL == AGD_DUBINS2) {
+    // Dubins2.h:17-36: left and right turns at the largest curvature
+    const S vl[3] = {cT, sT, kappa + ixi};
+    const S vr[3] = {cT, sT, kappa - ixi};
+    decomp_v<S>(vl, relax, w, e);
+    decomp_v<S>(vr, relax, w + 6, e + 6);
+    return true;
+  } else {
+    // Elastica2.h:51-79: the directions (sin phi v_xy, sin phi kappa + cos phi / xi)
+#pragma unroll 1
+    for (int l = 0; l < kNFejer; ++l) {
+      S cP, sP, wF;
+      fejer_node<S>(l, kNFejer, cP, sP, wF);
+      const S v[3] = {sP * cT, sP * sT, sP * kappa + cP * ixi};
+      decomp_v<S>(v, relax, w + 6 * l, e + 6 * l);
+#pragma unroll
+      for (int i = 0; i < 6; ++i) w[6 * l + i] *= wF;
+    }
+    return true;
+  }
+}
+
 // ------------------------------------------------------------------ update
 // Compare-exchange of the sorting network (NetworkSort.h:17-22): position i
 // keeps its pair when v[i] < v[j], otherwise the pairs swap (ties swap).
@@ -758,7 +895,9 @@ __device__ inline void cex(S& vi, S& wi, S& vj, S& wj) {
 }
 
 template <class S, int N> __device__ inline void sort_pairs(S (&v)[N], S (&w)[N]) {
-  if constexpr (N == 2) {
+  if constexpr (N == 1) {
+    // nothing to sort
+  } else if constexpr (N == 2) {
     cex(v[0], w[0], v[1], w[1]);
   } else if constexpr (N == 3) {
     cex(v[1], w[1], v[2], w[2]); cex(v[0], w[0], v[2], w[2]); cex(v[0], w[0], v[1], w[1]);
@@ -769,6 +908,13 @@ template <class S, int N> __device__ inline void sort_pairs(S (&v)[N], S (&w)[N]
     cex(v[0], w[0], v[3], w[3]); cex(v[1], w[1], v[4], w[4]);
     cex(v[2], w[2], v[4], w[4]); cex(v[1], w[1], v[3], w[3]);
     cex(v[2], w[2], v[3], w[3]);
+  } else {
+    // other lengths (the curvature models' 7 and 30): odd-even
+    // transposition, N rounds of the same compare-exchange
+#pragma unroll
+    for (int r = 0; r < N; ++r)
+#pragma unroll
+      for (int i = r & 1; i + 1 < N; i += 2) cex(v[i], w[i], v[i + 1], w[i + 1]);
   }
 }
 

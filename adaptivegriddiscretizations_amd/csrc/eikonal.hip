@@ -28,7 +28,7 @@
 #include "eikonal_kernels.hpp"
 #include "worklist.hpp"
 
-static_assert(sizeof(agd_eikonal_desc) == 312, "agd_eikonal_desc layout changed");
+static_assert(sizeof(agd_eikonal_desc) == 320, "agd_eikonal_desc layout changed");
 static_assert(sizeof(agd_eikonal_io) == 80, "agd_eikonal_io layout changed");
 
 namespace agd {
@@ -284,6 +284,10 @@ static bool lookup(int model, int d, const int32_t* si, KernelSet<S>& out) {
     case AGD_RIEMANN: ok = lookup_riemann<S>(d, si, mk); break;
     case AGD_RANDER: ok = lookup_rander<S>(d, si, mk); break;
     case AGD_TTI: ok = lookup_tti<S>(d, si, mk); break;
+    case AGD_REEDSSHEPP2:
+    case AGD_REEDSSHEPPFWD2:
+    case AGD_ELASTICA2:
+    case AGD_DUBINS2: ok = lookup_curvature<S>(model, d, si, mk); break;
   }
   return ok && lookup_tile_kernels(d, si, out);
 }
@@ -291,7 +295,13 @@ static bool lookup(int model, int d, const int32_t* si, KernelSet<S>& out) {
 static int check_desc(const agd_eikonal_desc* d) {
   AGD_REQUIRE(d != nullptr, "null descriptor");
   AGD_REQUIRE(d->ndim == 2 || d->ndim == 3, "ndim must be 2 or 3");
-  AGD_REQUIRE(d->model >= AGD_ISOTROPIC && d->model <= AGD_TTI, "unknown model");
+  AGD_REQUIRE(d->model >= AGD_ISOTROPIC && d->model <= AGD_DUBINS2, "unknown model");
+  if (curvature_model(d->model)) {
+    AGD_REQUIRE(d->ndim == 3, "the curvature models solve on (x, y, theta) grids (ndim 3)");
+    AGD_REQUIRE(d->curvature_relax > 0., "curvature_relax must be > 0 (eps^2, 0.01 by default)");
+    AGD_REQUIRE(d->factor_radius2 == 0., "source factorization is not provided for the curvature models");
+    AGD_REQUIRE(d->order <= 1, "the curvature models use the first-order scheme");
+  }
   AGD_REQUIRE(d->scalar == AGD_F32 || d->scalar == AGD_F64, "unknown scalar type");
   for (int k = 0; k < d->ndim; ++k) {
     AGD_REQUIRE(d->shape[k] >= 1 && d->shape[k] < (1 << 30), "shape out of range");
@@ -357,7 +367,7 @@ static void scheme_components(int model, int ndim, int order, int64_t& wc, int64
   const int64_t nmix = nmix_of(model);
   wc = nmix * nsym;
   cc = nmix * 2 * nsym;
-  if (model == AGD_TTI) cc += 1 + (order == 2 ? nmix * 2 * nsym : 0);
+  if (stored_scheme_model(model)) cc += 1 + (model == AGD_TTI && order == 2 ? nmix * 2 * nsym : 0);
 }
 
 // The precomputed scheme buffers hold what the descriptor's layout needs.
@@ -712,6 +722,7 @@ template <class S> static FlowArgs<S> flow_args(const agd_eikonal_desc* d, const
   a.order2 = d->order == 2 ? 1 : 0;
   a.order2_thr = S(d->order2_threshold);
   a.fac = make_factor<S>(d);
+  a.relax = S(d->curvature_relax);
   return a;
 }
 
@@ -906,6 +917,7 @@ static int scheme_impl(const agd_eikonal_desc* d, const agd_eikonal_io* io, hipS
   a.scode = io->scheme_codes;
   a.scode2 = (d->model == AGD_TTI && d->order == 2) ? io->scheme_codes + tti_codes2_offset(d) * g.size_tot
                                                      : nullptr;
+  a.relax = S(d->curvature_relax);
   hipLaunchKernelGGL((io->wall_dist || g.periodic) ? ks.scheme_general : ks.scheme, dim3(g.ntiles),
                      dim3(ks.ni), 0, stream, a);
   AGD_HIP_TRY(hipGetLastError());
@@ -1147,7 +1159,7 @@ int agd_scheme_supported(int32_t model, int32_t ndim, const int32_t* shape_i) {
 int agd_scheme_lengths(int32_t model, int32_t ndim, int32_t order, int64_t* weight_components,
                        int64_t* code_components) {
   AGD_REQUIRE(ndim == 2 || ndim == 3, "ndim must be 2 or 3");
-  AGD_REQUIRE(model >= AGD_ISOTROPIC && model <= AGD_TTI, "unknown model");
+  AGD_REQUIRE(model >= AGD_ISOTROPIC && model <= AGD_DUBINS2, "unknown model");
   AGD_REQUIRE(weight_components && code_components, "null output");
   scheme_components(model, ndim, order, *weight_components, *code_components);
   return AGD_OK;

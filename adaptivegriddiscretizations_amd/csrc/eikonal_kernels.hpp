@@ -489,8 +489,8 @@ __global__ void __launch_bounds__(block_lanes<T0 * T1 * T2>()) eik_update(EikArg
 #pragma unroll
       for (int j = 0; j < NJ; ++j) sh2[j] = S(0);
     }
-    if constexpr (MT::NMIX > 1) {
-      // several sub-schemes: read per sweep below
+    if constexpr (MT::STORED) {
+      // stored sub-schemes: read per sweep below
     } else if constexpr (VAR == VAR_PRE) {
       // Stencil weights and neighbour codes precomputed for the whole solve.
 #pragma unroll
@@ -523,11 +523,12 @@ __global__ void __launch_bounds__(block_lanes<T0 * T1 * T2>()) eik_update(EikArg
         if (a.fac.r2 > S(0)) factor_shifts<S, MODEL, D, NS>(a.fac, xt, st.e, sh, ORDER2 ? sh2 : nullptr);
       }
     }
-    if constexpr (MT::NMIX > 1) {
-      // ---- TTI: the min or max over the sub-schemes of their upwind
-      // updates (HFMIter.h:29-56, strict in-tile iteration).  Weights and
-      // neighbour codes are re-read at every sweep (cache hits) rather than
-      // held in registers; out-of-tile values are read at every sweep.
+    if constexpr (MT::STORED) {
+      // ---- TTI and the curvature models: the min or max over the
+      // sub-schemes of their upwind updates (HFMIter.h:29-56, strict in-tile
+      // iteration).  Weights and neighbour codes are re-read at every sweep
+      // (cache hits) rather than held in registers; out-of-tile values are
+      // read at every sweep.  A forward offset's x - e code is -1.
       const int64_t N = a.g.size_tot;
       const bool mix_min = ld32(a.scode + int64_t(MT::NMIX * NJ) * N, uint32_t(n_t)) != 0;
       // source factorization: the cell's row of the shift table, or -1
@@ -713,6 +714,7 @@ template <class S> struct SchemeArgs {
   S* sw;
   int32_t* scode;
   int32_t* scode2;  // TTI, second order: codes of the further neighbours x -+ 2 e_k, or null
+  S relax;          // curvature models: relaxation of the outer products (eps^2)
 };
 
 template <class S, int MODEL, int D, int T0, int T1, int T2, bool GENERAL>
@@ -741,7 +743,33 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_scheme(SchemeArgs<S> a) {
   S geo[NG > 0 ? NG : 1];
 #pragma unroll
   for (int k = 0; k < NG; ++k) geo[k] = a.geom[k * N + n_t];
-  if constexpr (MT::NMIX > 1) {
+  if constexpr (MT::CURV) {
+    // Curvature models: the stencils of every sub-scheme at once, forward
+    // slots with their x - e code set to -1, then the min/max flag.
+    S cw[MT::NMIX * NS];
+    int ce[MT::NMIX * NS][3];
+    const bool mix_min = curv_stencil<S, MODEL>(geo, a.relax, cw, ce);
+#pragma unroll 1
+    for (int m = 0; m < MT::NMIX; ++m) {
+      Stencil<S, MODEL, D> st;
+#pragma unroll
+      for (int k = 0; k < NS; ++k) {
+        st.w[k] = cw[m * NS + k];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) st.e[k][c] = ce[m * NS + k][c];
+        st.shift[k] = S(0);
+      }
+      int code[2 * NS];
+      resolve_codes<S, MODEL, D, TL, true>(st, xi, xt, xo, wx, wall_s, a.wall, a.g, code, 0, nullptr);
+#pragma unroll
+      for (int k = NS - MT::NFWD; k < NS; ++k) code[2 * k] = -1;
+#pragma unroll
+      for (int k = 0; k < NS; ++k) a.sw[int64_t(m * NS + k) * N + n_t] = st.w[k];
+#pragma unroll
+      for (int j = 0; j < 2 * NS; ++j) a.scode[int64_t(m * 2 * NS + j) * N + n_t] = code[j];
+    }
+    a.scode[int64_t(MT::NMIX * 2 * NS) * N + n_t] = mix_min ? 1 : 0;
+  } else if constexpr (MT::NMIX > 1) {
     // TTI_.h:112-134: one Selling decomposition per tangent ellipse, then
     // the min/max flag after the NMIX*2*NS codes.
     S diag[kTTIMix][2];
@@ -883,6 +911,7 @@ template <class S> struct FlowArgs {
   // turns a start into a supersolution of the scheme).
   int32_t repair;
   FactorDesc<S> fac;
+  S relax;  // curvature models: relaxation of the outer products (eps^2)
 };
 
 // The upwind update of one cell and its flow (Update.h:149-209, the body
@@ -920,7 +949,7 @@ __device__ inline void flow_cell(const FlowArgs<S>& a, const int* xt, int64_t n_
   bool tfac = false;
   TTIFactorData<S> tdat;
   S xrel[D], tgrad[D], tnx = S(0);
-  if constexpr (MT::NMIX > 1) {
+  if constexpr (MODEL == AGD_TTI) {
     tfac = a.fac.r2 > S(0) && factor_box_index<S, D>(a.fac, xt) >= 0;
     if (tfac) {
 #pragma clang fp contract(off)
@@ -942,9 +971,9 @@ __device__ inline void flow_cell(const FlowArgs<S>& a, const int* xt, int64_t n_
       sh[2 * k + 1] = -st.shift[k];
       sh2[2 * k] = sh2[2 * k + 1] = S(0);
     }
-    if constexpr (MT::NMIX == 1) {
+    if constexpr (!MT::STORED) {
       if (a.fac.r2 > S(0)) factor_shifts<S, MODEL, D, NS>(a.fac, xt, st.e, sh, v2 ? sh2 : nullptr);
-    } else {
+    } else if constexpr (MODEL == AGD_TTI) {
       if (tfac) {
 #pragma unroll 1
         for (int k = 0; k < NS; ++k)
@@ -960,8 +989,10 @@ __device__ inline void flow_cell(const FlowArgs<S>& a, const int* xt, int64_t n_
       for (int s = 0; s < 2; ++s) {
         const int sg = 2 * s - 1;
         int e[3] = {sg * st.e[k][0], sg * st.e[k][1], sg * st.e[k][2]};
-        bool vis = true;
-        if (a.wall && wx != 255) {
+        // a forward offset (the last NFWD slots) has no x - e neighbour
+        const bool none = MT::NFWD > 0 && k >= NS - MT::NFWD && s == 0;
+        bool vis = !none;
+        if (vis && a.wall && wx != 255) {
           vis = visible<D>(e, wx, [&](const int* ww) -> int {
             int yt[3] = {xt[0] + ww[0], xt[1] + ww[1], xt[2] + ww[2]};
             const int64_t m = index_tot<D, TL>(yt, a.g);
@@ -994,8 +1025,9 @@ __device__ inline void flow_cell(const FlowArgs<S>& a, const int* xt, int64_t n_
   int64_t nb[NS];
   S val;
   unsigned o2mask = 0;  // order 2: neighbours with the second-order correction
-  if constexpr (MT::NMIX > 1) {
-    // TTI: keep the sub-scheme whose update wins the min/max (HFMIter.h:48-54).
+  if constexpr (MT::STORED) {
+    // TTI and the curvature models: keep the sub-scheme whose update wins
+    // the min/max (HFMIter.h:48-54).
 #pragma unroll
     for (int k = 0; k < NS; ++k) {
       v[k] = inf_v<S>();
@@ -1006,15 +1038,31 @@ __device__ inline void flow_cell(const FlowArgs<S>& a, const int* xt, int64_t n_
       st.shift[k] = S(0);
       st.e[k][0] = st.e[k][1] = st.e[k][2] = 0;
     }
-    S diag[kTTIMix][2];
-    const bool mix_min = tti_diags<S>(geo, geo + 2, diag);
+    bool mix_min;
+    S diag[MT::CURV ? 1 : kTTIMix][2];
     S D0[NS], D1[NS];
-    tti_base<S, D>(geo, D0, D1);
+    S cw[MT::CURV ? MT::NMIX * NS : 1];
+    int ce[MT::CURV ? MT::NMIX * NS : 1][3];
+    if constexpr (MT::CURV) {
+      mix_min = curv_stencil<S, MODEL>(geo, a.relax, cw, ce);
+    } else {
+      mix_min = tti_diags<S>(geo, geo + 2, diag);
+      tti_base<S, D>(geo, D0, D1);
+    }
     val = mix_min ? inf_v<S>() : -inf_v<S>();
 #pragma unroll 1
     for (int m = 0; m < MT::NMIX; ++m) {
       Stencil<S, MODEL, D> cand;
-      tti_sub<S, D>(D0, D1, diag[m][0], diag[m][1], cand.w, cand.e);
+      if constexpr (MT::CURV) {
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+          cand.w[k] = cw[m * NS + k];
+#pragma unroll
+          for (int c = 0; c < 3; ++c) cand.e[k][c] = ce[m * NS + k][c];
+        }
+      } else {
+        tti_sub<S, D>(D0, D1, diag[m][0], diag[m][1], cand.w, cand.e);
+      }
 #pragma unroll
       for (int k = 0; k < NS; ++k) cand.shift[k] = S(0);
       S cv[NS], cw[NS], cv2[NS];
@@ -1184,10 +1232,26 @@ ModelKernels<S> make_model_kernels() {
     k.flow = nullptr;
     k.tti_factor = nullptr;
     k.geodesic_online = nullptr;
-    if constexpr (std::is_same<S, float>::value && MODEL != AGD_TTI)
+    if constexpr (std::is_same<S, float>::value && !ModelTraits<MODEL, D>::STORED)
       k.update_batch = eik_update<S, MODEL, D, T0, T1, T2, VAR_BATCH>;
     else
       k.update_batch = nullptr;
+  } else if constexpr (ModelTraits<MODEL, D>::CURV) {
+    // Curvature models (3D (x, y, theta) grids): precomputed scheme only,
+    // resolved by the general variant (periodic theta axis).
+    if constexpr (D == 3) {
+      k.update = k.update_walls = k.update_order2 = k.update_batch = nullptr;
+      k.update_pre = eik_update<S, MODEL, D, T0, T1, T2, VAR_PRE>;
+      k.scheme = k.scheme_general = eik_scheme<S, MODEL, D, T0, T1, T2, true>;
+      k.flow = eik_flow<S, MODEL, D, T0, T1, T2>;
+      k.geodesic_online = eik_geodesic_online<S, MODEL, D, T0, T1, T2>;
+    } else {
+      k.update = k.update_walls = k.update_order2 = k.update_batch = k.update_pre = nullptr;
+      k.scheme = k.scheme_general = nullptr;
+      k.flow = nullptr;
+      k.geodesic_online = nullptr;
+    }
+    k.tti_factor = nullptr;
   } else {
     if constexpr (MODEL == AGD_TTI) {
       // TTI runs from its precomputed scheme only.
@@ -1258,5 +1322,6 @@ template <class S> bool lookup_diagonal(int d, const int32_t* si, ModelKernels<S
 template <class S> bool lookup_riemann(int d, const int32_t* si, ModelKernels<S>& out);
 template <class S> bool lookup_rander(int d, const int32_t* si, ModelKernels<S>& out);
 template <class S> bool lookup_tti(int d, const int32_t* si, ModelKernels<S>& out);
+template <class S> bool lookup_curvature(int model, int d, const int32_t* si, ModelKernels<S>& out);
 
 }  // namespace agd

@@ -11,6 +11,9 @@ seed given, so a workload is reproducible on any host.
                      theta and log a smooth random fields   (2000^2 headline)
   riemann3_aniso     3D Riemannian, M = Q diag(l1,l2,l3) Q^T, Q from a smooth
                      quaternion field, log l smooth in [0, ln 5]  (200^3)
+  curvature2         a curvature-penalised model (ReedsShepp2, ...) on
+                     [-1,1]^2 x [0, 2 pi): cost 1 + 0.5 tanh(bumps) in the
+                     plane, constant xi and kappa, seed at (0, 0, 0)
 """
 from dataclasses import dataclass, field
 
@@ -37,7 +40,10 @@ class Workload:
     @property
     def h(self):
         s = np.asarray(self.sides, dtype=np.float64)
-        return (s[:, 1] - s[:, 0]) / np.asarray(self.dims, dtype=np.float64)
+        h = (s[:, 1] - s[:, 0]) / np.asarray(self.dims[:len(s)], dtype=np.float64)
+        if len(s) < len(self.dims):      # angle axis of a curvature-penalised model
+            h = np.concatenate([h, [2 * np.pi / self.dims[-1]]])
+        return h
 
     @property
     def ncells(self):
@@ -53,9 +59,16 @@ class Workload:
             d['metric'] = self.extra['tti']
         if self.cost is not None:
             d['cost'] = self.cost
+        for key in ('xi', 'kappa'):
+            if key in self.extra:
+                d[key] = self.extra[key]
         d.update(extra)
         hfmIn = dictIn(d)
-        hfmIn.SetRect(self.sides, dims=list(self.dims))
+        if len(self.sides) < len(self.dims):
+            hfmIn.SetRect(self.sides, dimx=self.dims[0])
+            hfmIn.nTheta = self.dims[-1]
+        else:
+            hfmIn.SetRect(self.sides, dims=list(self.dims))
         return hfmIn
 
 
@@ -157,6 +170,25 @@ def tti_tilted(n=256, d=2, seed=5, sigma=None, max_tilt=0.6, hexagonal=(15., 4.,
     seeds = np.zeros((1, d), dtype=np.float32)
     return Workload(f'tti{d}_tilted', f'TTI{d}', [[-1, 1]] * d, shape, seeds, metric=None,
                     extra={'tti': metric})
+
+
+def curvature2(model='ReedsShepp2', n=64, ntheta=32, xi=0.3, kappa=0., seed=7, nbumps=16):
+    """Curvature-penalised model on [-1,1]^2 x [0, 2 pi) with n x n x ntheta
+    cells: a planar cost 1 + 0.5 tanh(sum of nbumps Gaussian bumps), constant
+    xi and kappa, one seed at the origin heading along +x."""
+    rng = np.random.default_rng(seed)
+    x = -1 + (np.arange(n) + 0.5) * (2. / n)
+    X, Y = np.meshgrid(x, x, indexing='ij')
+    acc = np.zeros((n, n))
+    for _ in range(nbumps):
+        c = rng.uniform(-1, 1, size=2)
+        r = rng.uniform(0.05, 0.3)
+        a = rng.uniform(-2, 2)
+        acc += a * np.exp(-((X - c[0]) ** 2 + (Y - c[1]) ** 2) / (2 * r * r))
+    cost = (1. + 0.5 * np.tanh(acc)).astype(np.float32)
+    seeds = np.zeros((1, 3), dtype=np.float32)
+    return Workload(f'{model.lower()}_{n}x{n}x{ntheta}', model, [[-1, 1], [-1, 1]], (n, n, ntheta), seeds,
+                    cost=cost, extra={'xi': xi, 'kappa': kappa})
 
 
 def rotation_from_quaternion(q):

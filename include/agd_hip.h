@@ -45,6 +45,15 @@ enum agd_model {
   AGD_TTI = 4,       /* TTI_.h: min/max of 7 Selling schemes; geometry
                         [l0, l1, q00, q10, q11, T (d x d)]; precomputed scheme
                         only (agd_eikonal_scheme)                            */
+  /* Curvature-penalised models on the (x, y, theta) grid, ndim 3, theta
+   * periodic; geometry [1/xi, kappa, cos theta, sin theta] per cell in grid
+   * units (_SetGeometry.py:83-109); precomputed scheme only.  Each sub-scheme
+   * stores its offsets as pairs of codes (x - e, x + e); a forward offset
+   * (one-sided, x + e only) has its x - e code set to -1. */
+  AGD_REEDSSHEPP2 = 5,       /* ReedsShepp2.h: Selling of the relaxed v v^T, 6 symmetric offsets */
+  AGD_REEDSSHEPPFWD2 = 6,    /* ReedsSheppForward2.h: 1 symmetric (0,0,1) + 6 forward offsets */
+  AGD_ELASTICA2 = 7,         /* Elastica2.h: 5 Fejer directions x 6 forward offsets */
+  AGD_DUBINS2 = 8,           /* Dubins2.h: min of 2 schemes of 6 forward offsets */
 };
 
 enum agd_scalar { AGD_F32 = 0, AGD_F64 = 1 };
@@ -103,6 +112,10 @@ typedef struct agd_eikonal_desc {
    * pass to keep about bound_active_blocks tiles active.  0 disables it. */
   int64_t bound_active_blocks;
   double minchg_delta_min; /* smallest threshold increase (> 0)           */
+  /* Curvature models: relaxation eps^2 of the self outer products of the
+   * directions of motion (Decomp_v_.h, _Kernel.py:318-321; eps = 0.1 gives
+   * 0.01), > 0; unused by the other models. */
+  double curvature_relax;
 } agd_eikonal_desc;
 
 /* Device pointers, tiled layout.  Scalars are float (AGD_F32) or double. */

@@ -24,6 +24,11 @@
 //       Kernel_LinearUpdate.h:99-216 (solved here exactly, in causal order).
 //   * Walls: visibility walk Walls.h:18-50 and L1 wall distance
 //       _SetGeometry.py:262-282 / Kernel_InfConvolution.h:66-91.
+//   * Curvature-penalised models on (x, y, theta) grids: ReedsShepp2.h:13-36,
+//       ReedsSheppForward2.h:19-33, Elastica2.h:51-79 (Fejer rule, 5 nodes),
+//       Dubins2.h:17-36, with decomp_v (Decomp_v_.h:7-21), the relaxed self
+//       outer product (Geometry_.h:199-210) and forward offsets, whose value
+//       is u(x + e) alone (HFM.h:64-75, FiniteDifferences.h:48).
 //
 // Solvers:
 //   * orc_solve_fm     : Dijkstra-like fast marching in FP64 -- the exact
@@ -148,12 +153,37 @@ static void selling3(const S* m, S* w, int e[][3]) {
 }
 
 // ---------------------------------------------------------------- models
-enum Model { ISOTROPIC = 0, DIAGONAL = 1, RIEMANN = 2, RANDER = 3, TTI = 4 };
+enum Model {
+  ISOTROPIC = 0, DIAGONAL = 1, RIEMANN = 2, RANDER = 3, TTI = 4,
+  REEDSSHEPP2 = 5, REEDSSHEPPFWD2 = 6, ELASTICA2 = 7, DUBINS2 = 8
+};
 static const int kNMix = 7;  // TTI: nmix_macro (kernel_traits.py:47-52)
+static const int kNFejer = 5;  // Elastica2: nFejer_macro (kernel_traits.py:30-33)
+static const int kMaxAct = 30;  // offsets of one sub-stencil (Elastica2: 5 x 6)
+static const int kMaxSlots = 42;  // all sub-stencils (TTI: 7 x 6)
 
 static int symdim(int d) { return d * (d + 1) / 2; }
+static bool curvature(int model) { return model >= REEDSSHEPP2 && model <= DUBINS2; }
+// Offsets of one sub-stencil, each a pair x -+ e; the last nfwd_of of them
+// are forward offsets (x + e only).
 static int nsym_of(int model, int d) {
-  return (model == ISOTROPIC || model == DIAGONAL) ? d : symdim(d);
+  switch (model) {
+    case ISOTROPIC:
+    case DIAGONAL: return d;
+    case REEDSSHEPP2: return 6;
+    case REEDSSHEPPFWD2: return 7;
+    case ELASTICA2: return 6 * kNFejer;
+    case DUBINS2: return 6;
+  }
+  return symdim(d);
+}
+static int nfwd_of(int model) {
+  switch (model) {
+    case REEDSSHEPPFWD2: return 6;
+    case ELASTICA2: return 6 * kNFejer;
+    case DUBINS2: return 6;
+  }
+  return 0;
 }
 static int ngeom_of(int model, int d) {
   switch (model) {
@@ -163,19 +193,19 @@ static int ngeom_of(int model, int d) {
     case RANDER: return symdim(d) + d;
     case TTI: return 5 + d * d;
   }
-  return 0;
+  return curvature(model) ? 4 : 0;
 }
-static int nmix_of(int model) { return model == TTI ? kNMix : 1; }
+static int nmix_of(int model) { return model == TTI ? kNMix : (model == DUBINS2 ? 2 : 1); }
 
 // nmix sub-stencils of nsym weights/offsets each (sub-stencil m at
 // [m*nsym, (m+1)*nsym)); the update is the min (mix_is_min) or the max of the
 // sub-stencils' updates.
 template <class S>
 struct Stencil {
-  int nsym, nmix;
+  int nsym, nmix, nfwd;
   bool mix_is_min;
-  S w[6 * kNMix];
-  int e[6 * kNMix][3];
+  S w[kMaxSlots];
+  int e[kMaxSlots][3];
   S drift[3];
   bool has_drift;
 };
@@ -390,15 +420,124 @@ static void tti_factor_pair(int d, const S* fm, const TTIFactorData<S>& t, const
 }
 
 // Isotropic_.h:20-41, Riemann_.h:16-29 and TTI_.h:112-134.
+// ---------------------------------------------------------------- curvature
+// Geometry_.h:199-210: m = (1 - relax) v v^T + relax |v|^2 Id (lower
+// triangle, row by row), each product rounded separately.
+template <class S>
+static void outer_relax(const S* v, S relax, S* m) {
+  const S rv2 = relax * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  const S rm1 = S(1) - relax;
+  int k = 0;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j <= i; ++j, ++k) {
+      m[k] = v[i] * v[j] * rm1;
+      if (i == j) m[k] += rv2;
+    }
+}
+
+// Decomp_v_.h:7-21: the Selling decomposition of the relaxed v v^T, with
+// weight 0 for offsets e such that <v,e>^2 < (2/3) |v|^2 |e|^2, and the kept
+// offsets turned so that <v,e> <= 0.
+template <class S>
+static void decomp_v(const S* v, S relax, S* w, int (*e)[3]) {
+  S m[6];
+  outer_relax<S>(v, relax, m);
+  selling3<S>(m, w, e);
+  const S vv = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+  for (int k = 0; k < 6; ++k) {
+    const S ve = v[0] * S(e[k][0]) + v[1] * S(e[k][1]) + v[2] * S(e[k][2]);
+    const S ee = S(e[k][0] * e[k][0] + e[k][1] * e[k][1] + e[k][2] * e[k][2]);
+    if (ve * ve < vv * ee * S(2.0 / 3.0)) {
+      w[k] = 0;
+      continue;
+    }
+    if (ve > 0)
+      for (int c = 0; c < 3; ++c) e[k][c] = -e[k][c];
+  }
+}
+
+// Fejer's first rule for the integral over [0, pi] of f(phi) sin(phi): nodes
+// phi_l = pi (l + 1/2) / n, weights (2/n) (1 - 2 sum_{j=1}^{n/2}
+// cos(2 j phi_l) / (4 j^2 - 1)).  Rounded as the reference's table
+// (Elastica2.h:13-45, n = 5): cosines and sines to 9 decimals (so that
+// cos(pi/2) is exactly 0), weights to 6.
+static void fejer(int l, int n, double& c, double& s, double& w) {
+  const double phi = 3.141592653589793 * (l + 0.5) / n;
+  double acc = 0;
+  for (int j = 1; j <= n / 2; ++j) acc += std::cos(2. * j * phi) / double(4 * j * j - 1);
+  c = std::nearbyint(std::cos(phi) * 1e9) / 1e9;
+  s = std::nearbyint(std::sin(phi) * 1e9) / 1e9;
+  w = std::nearbyint(2. / n * (1. - 2. * acc) * 1e6) / 1e6;
+}
+
+// Scheme of a curvature model at a cell with geometry g = [1/xi, kappa,
+// cos theta, sin theta] (grid units); relax = eps^2.
+template <class S>
+static void curvature_scheme(int model, const S* g, S relax, Stencil<S>& st) {
+  const S ixi = g[0], kappa = g[1], cT = g[2], sT = g[3];
+  if (model == REEDSSHEPP2) {
+    // ReedsShepp2.h:13-36: the angular coefficient raised to kappa^2 + 1/xi^2,
+    // offsets nearly orthogonal to the motion (w = v x e_theta) pruned.
+    const S v[3] = {cT, sT, kappa};
+    S m[6];
+    outer_relax<S>(v, relax, m);
+    const S m5 = v[2] * v[2] + ixi * ixi;
+    if (m5 > m[5]) m[5] = m5;
+    selling3<S>(m, st.w, st.e);
+    const S wv[3] = {v[1], -v[0], 0};
+    const S ww = wv[0] * wv[0] + wv[1] * wv[1];
+    for (int k = 0; k < 6; ++k) {
+      const int* e = st.e[k];
+      const S we = wv[0] * S(e[0]) + wv[1] * S(e[1]);
+      const S ee = S(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+      if (we * we >= ee * ww * (S(1) - S(2.0 / 3.0))) st.w[k] = 0;
+    }
+  } else if (model == REEDSSHEPPFWD2) {
+    // ReedsSheppForward2.h:19-33: in-place rotation, then forward motion
+    st.w[0] = ixi * ixi;
+    st.e[0][0] = 0;
+    st.e[0][1] = 0;
+    st.e[0][2] = -1;
+    const S v[3] = {cT, sT, kappa};
+    decomp_v<S>(v, relax, st.w + 1, st.e + 1);
+  } else if (model == DUBINS2) {
+    // Dubins2.h:17-36: turning left or right at curvature 1/xi, the smaller
+    // of the two updates
+    const S vl[3] = {cT, sT, kappa + ixi}, vr[3] = {cT, sT, kappa - ixi};
+    decomp_v<S>(vl, relax, st.w, st.e);
+    decomp_v<S>(vr, relax, st.w + 6, st.e + 6);
+  } else {
+    // Elastica2.h:51-79: the directions (sin phi cos theta, sin phi sin
+    // theta, sin phi kappa + cos phi / xi) at the Fejer nodes, weights times
+    // the node's Fejer weight
+    for (int l = 0; l < kNFejer; ++l) {
+      double c, sn, wf;
+      fejer(l, kNFejer, c, sn, wf);
+      const S cP = S(c), sP = S(sn), wF = S(wf);
+      const S v[3] = {sP * cT, sP * sT, sP * kappa + cP * ixi};
+      decomp_v<S>(v, relax, st.w + 6 * l, st.e + 6 * l);
+      for (int i = 0; i < 6; ++i) st.w[6 * l + i] *= wF;
+    }
+  }
+}
+
+static double g_curvature_relax = 0.01;  // eps^2 (eps = 0.1, _Kernel.py:319-321)
+
 template <class S>
 static void make_stencil(int model, int d, const S* g, const S* iso_w, Stencil<S>& st) {
   st.has_drift = false;
   st.nsym = nsym_of(model, d);
   st.nmix = nmix_of(model);
+  st.nfwd = nfwd_of(model);
   st.mix_is_min = true;
-  for (int k = 0; k < 6 * kNMix; ++k) {
+  for (int k = 0; k < kMaxSlots; ++k) {
     st.w[k] = 0;
     st.e[k][0] = st.e[k][1] = st.e[k][2] = 0;
+  }
+  st.drift[0] = st.drift[1] = st.drift[2] = 0;
+  if (curvature(model)) {
+    curvature_scheme<S>(model, g, S(g_curvature_relax), st);
+    return;
   }
   if (model == TTI) {
     st.drift[0] = st.drift[1] = st.drift[2] = 0;
@@ -437,6 +576,12 @@ static void network_sort(int n, const S* v, int* order) {
   if (n == 2) { net = kNet2; m = 1; }
   else if (n == 3) { net = kNet3; m = 3; }
   else if (n == 6) { net = kNet6; m = 12; }
+  else if (n > 1) {
+    // other sizes (the curvature models' 7 and 30 offsets): insertion sort;
+    // the update depends on the order of equal values only through rounding
+    for (int i = 1; i < n; ++i)
+      for (int j = i; j > 0 && v[order[j]] < v[order[j - 1]]; --j) std::swap(order[j], order[j - 1]);
+  }
   for (int c = 0; c < m; ++c) {
     const int i = net[c][0], j = net[c][1];
     const int oi = order[i], oj = order[j];
@@ -451,7 +596,7 @@ static void network_sort(int n, const S* v, int* order) {
 // weights w_k (val - v_k)_+ when fw != nullptr.
 template <class S>
 static S hfm_solve(int nact, const S* w, const S* v, S rhs, S* fw) {
-  int order[6];
+  int order[kMaxAct];
   network_sort<S>(nact, v, order);
   const S vmin = v[order[0]];
   if (vmin == Inf<S>()) {
@@ -665,7 +810,9 @@ static void neighbours(const Problem& P, const Stencil<S>& st, int base, int64_t
     S best = Inf<S>(), best2 = Inf<S>();
     int bs = 0;
     int64_t bi = -1;
-    for (int s = 0; s < 2; ++s) {
+    // a forward offset (the last nfwd of the sub-stencil) has the x + e side only
+    const bool fwd = k >= st.nsym - st.nfwd;
+    for (int s = fwd ? 1 : 0; s < 2; ++s) {
       const int eps = 2 * s - 1;
       int ee[3] = {eps * ek[0], eps * ek[1], eps * ek[2]};
       int64_t y[3], y2[3];
@@ -680,7 +827,7 @@ static void neighbours(const Problem& P, const Stencil<S>& st, int base, int64_t
         const int64_t ny2 = P.G.index(y2);
         if (ny2 >= 0) val2 = get(ny2) + sh2[s];
       }
-      if (s == 0) { best = val; best2 = val2; bs = 0; bi = ny; }
+      if (s == 0 || fwd) { best = val; best2 = val2; bs = s; bi = ny; }
       else if (best > val) { best = val; best2 = val2; bs = 1; bi = ny; }
     }
     v[k] = best;
@@ -708,8 +855,8 @@ static S cell_update(const Problem& P, const Stencil<S>& st, int64_t n, Get get,
   if (st.nmix == 1 && P.order == 2) {
     // HFM.h:91-134: where |u0 - 2v + v2| < threshold * max_k(u0 - v_k)_+ and
     // u0 > v, use v + (v - v2)/3 with weight 9/4 w (flow weight 3/2 w (u-v)+).
-    S v[6], v2[6], w[6];
-    bool o2[6];
+    S v[kMaxAct], v2[kMaxAct], w[kMaxAct];
+    bool o2[kMaxAct];
     neighbours<S>(P, st, 0, n, get, v, side, nidx, v2);
     S bound = 0;
     for (int k = 0; k < ns; ++k) bound = std::max(bound, u0 - v[k]);
@@ -733,18 +880,19 @@ static S cell_update(const Problem& P, const Stencil<S>& st, int64_t n, Get get,
     return val;
   }
   if (st.nmix == 1) {
-    S v[6];
+    S v[kMaxAct];
     neighbours<S>(P, st, 0, n, get, v, side, nidx);
     return hfm_solve<S>(ns, st.w, v, rhs, fw);
   }
   S best = st.mix_is_min ? Inf<S>() : -Inf<S>();
   for (int m = 0; m < st.nmix; ++m) {
-    S v[6], f[6], v2[6], w[6];
-    int sd[6];
-    int64_t ni[6];
+    S v[kMaxAct], f[kMaxAct], v2[kMaxAct], w[kMaxAct];
+    int sd[kMaxAct];
+    int64_t ni[kMaxAct];
     neighbours<S>(P, st, m * ns, n, get, v, sd, ni, P.order == 2 ? v2 : nullptr);
     for (int k = 0; k < ns; ++k) w[k] = st.w[m * ns + k];
-    bool o2[6] = {false, false, false, false, false, false};
+    bool o2[kMaxAct];
+    for (int k = 0; k < kMaxAct; ++k) o2[k] = false;
     if (P.order == 2) {
       // the second-order correction of each sub-stencil (HFMIter.h:43 passes
       // every sub-stencil's further neighbours to HFMUpdate, HFM.h:91-134)
@@ -978,7 +1126,13 @@ static void tti_factor_points(int d, const double* fm64, int64_t n, const double
 
 extern "C" {
 
-int orc_version() { return 7; }
+int orc_version() { return 8; }
+
+// Relaxation eps^2 of the curvature models' self outer products.
+void orc_set_curvature_relax(double relax) { g_curvature_relax = relax; }
+
+// Node l of the n-point Fejer rule: out = (cos phi_l, sin phi_l, weight).
+void orc_fejer(int l, int n, double* out) { fejer(l, n, out[0], out[1], out[2]); }
 
 // Threads the OpenMP solver runs on (omp_get_max_threads).
 int orc_omp_threads() {
@@ -1048,9 +1202,10 @@ void orc_selling(int d, int64_t n, const double* m, double* w, int32_t* e, int u
 // minimised over +-e), rhs[n] -> u[n], flow weights fw[n][nact] (nullable).
 void orc_hfm_update(int nact, int64_t n, const double* w, const double* v,
                     const double* rhs, double* u, double* fw, int use_float) {
+  if (nact < 1 || nact > kMaxAct) return;
   for (int64_t c = 0; c < n; ++c) {
     if (use_float) {
-      float wf[6], vf[6], f[6];
+      float wf[kMaxAct], vf[kMaxAct], f[kMaxAct];
       for (int k = 0; k < nact; ++k) { wf[k] = float(w[c * nact + k]); vf[k] = float(v[c * nact + k]); }
       u[c] = hfm_solve<float>(nact, wf, vf, float(rhs[c]), fw ? f : nullptr);
       if (fw) for (int k = 0; k < nact; ++k) fw[c * nact + k] = f[k];
@@ -1070,7 +1225,7 @@ void orc_stencils(int model, int d, const int64_t* shape, const double* geom,
   fill_problem(P, model, d, shape, nullptr, geom, nullptr, nullptr, nullptr, iso_w);
   const int ns = nsym_of(model, d) * nmix_of(model);
   for (int64_t n = 0; n < P.G.N; ++n) {
-    static thread_local int ee[6 * kNMix][3];
+    static thread_local int ee[kMaxSlots][3];
     bool mn = true;
     if (use_float) {
       Stencil<float> st;
@@ -1206,16 +1361,21 @@ int orc_flow(int model, int d, const int64_t* shape, const int* periodic, const 
   const int ns = nsym_of(model, d);
   std::vector<double> u0(u, u + N);
   for (int64_t n = 0; n < N; ++n) {
-    double f[6] = {0, 0, 0, 0, 0, 0};
-    int side[6] = {0, 0, 0, 0, 0, 0};
-    int64_t ni[6] = {-1, -1, -1, -1, -1, -1};
-    int e[6][3];
+    double f[kMaxAct];
+    int side[kMaxAct];
+    int64_t ni[kMaxAct];
+    int e[kMaxAct][3];
+    for (int k = 0; k < kMaxAct; ++k) {
+      f[k] = 0;
+      side[k] = 0;
+      ni[k] = -1;
+    }
     if (!seeds[n]) {
       int km = 0;
       if (use_float) {
         Stencil<float> st;
         stencil_at<float>(P, n, st);
-        float ff[6];
+        float ff[kMaxAct];
         const float val = cell_update<float>(P, st, n, [&](int64_t m) { return float(u0[m]); },
                                              float(rhs[n]), ff, side, ni, &km, float(u0[n]));
         if (val < float(u0[n])) u[n] = val;

@@ -31,9 +31,13 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 
 ISOTROPIC, DIAGONAL, RIEMANN, RANDER, TTI = 0, 1, 2, 3, 4
+REEDSSHEPP2, REEDSSHEPPFWD2, ELASTICA2, DUBINS2 = 5, 6, 7, 8
 MODEL_IDS = {'Isotropic': ISOTROPIC, 'Diagonal': DIAGONAL, 'Riemann': RIEMANN, 'Rander': RANDER,
-             'TTI': TTI}
+             'TTI': TTI, 'ReedsShepp2': REEDSSHEPP2, 'ReedsSheppForward2': REEDSSHEPPFWD2,
+             'Elastica2': ELASTICA2, 'Dubins2': DUBINS2}
 NMIX = 7
+# offsets of one sub-stencil of the curvature models (the forward ones included)
+CURVATURE_NSYM = {REEDSSHEPP2: 6, REEDSSHEPPFWD2: 7, ELASTICA2: 30, DUBINS2: 6}
 
 
 def build():
@@ -67,6 +71,8 @@ def _load():
     lib.orc_forward_ad.argtypes = [c_int64, c_int, P, P, P, c_int, P, P]
     lib.orc_set_order.argtypes = [c_int, c_double]
     lib.orc_set_factor.argtypes = [c_double, P, P]
+    lib.orc_set_curvature_relax.argtypes = [c_double]
+    lib.orc_fejer.argtypes = [c_int, c_int, P]
     lib.orc_tti_factor.argtypes = [c_int, c_int, P, c_int64, P, P, P, P, P, P]
     lib.orc_wall_distance.argtypes = [c_int, P, P, P, c_int, P]
     lib.orc_visible.argtypes = [c_int, P, P, P, c_int64, P, P, P]
@@ -138,11 +144,13 @@ class Problem:
 
     @property
     def nsym(self):
+        if self.model in CURVATURE_NSYM:
+            return CURVATURE_NSYM[self.model]
         return self.d if self.model in (ISOTROPIC, DIAGONAL) else self.d * (self.d + 1) // 2
 
     @property
     def nmix(self):
-        return NMIX if self.model == TTI else 1
+        return NMIX if self.model == TTI else (2 if self.model == DUBINS2 else 1)
 
     def cast(self, dtype):
         """Same problem with geometry and rhs rounded to dtype (the FP32
@@ -258,6 +266,52 @@ def wall_distance(walls, bound=10, periodic=None):
 
 
 # ---------------------------------------------------------------- preprocessing
+class curvature_relax:
+    """Context: relaxation eps^2 of the curvature models' self outer products
+    (0.01 by default, eps = 0.1)."""
+
+    def __init__(self, relax):
+        self.relax = float(relax)
+
+    def __enter__(self):
+        lib().orc_set_curvature_relax(self.relax)
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_set_curvature_relax(0.01)
+
+
+def fejer(n=5):
+    """The oracle's n-point Fejer rule on [0, pi] with weight sin: rows
+    (cos phi_l, sin phi_l, w_l)."""
+    out = np.zeros((n, 3))
+    for l in range(n):
+        lib().orc_fejer(l, n, _p(out[l]))
+    return out
+
+
+def curvature_geometry(shape, h, xi, kappa=0., theta=None):
+    """[1/xi, kappa, cos theta, sin theta] per cell in grid units, component
+    major (_SetGeometry.py:83-109): 1/xi and kappa divided by h_theta / h;
+    theta defaults to the angle 2 pi k / n_theta of the third axis."""
+    shape = tuple(shape)
+    ratio = float(h[2]) / float(h[0])
+    def field(a, name):
+        a = np.asarray(a, dtype=np.float64)
+        if a.ndim == 0 or a.shape == shape:
+            return np.broadcast_to(a, shape)
+        if a.shape == shape[2:]:
+            return np.broadcast_to(a.reshape((1, 1) + shape[2:]), shape)
+        if a.shape == shape[:2]:
+            return np.broadcast_to(a.reshape(shape[:2] + (1,)), shape)
+        raise ValueError(f"{name} has shape {a.shape}")
+    if theta is None:
+        theta = 2 * np.pi * np.arange(shape[2]) / shape[2]
+    th = field(theta, 'theta')
+    return np.stack([1. / (field(xi, 'xi') * ratio), field(kappa, 'kappa') / ratio,
+                     np.cos(th), np.sin(th)]).reshape(4, -1)
+
+
 class order:
     """Context manager selecting the scheme order of the oracle (2 = the
     second-order correction of HFM.h:91-134; solve with solve_jacobi)."""

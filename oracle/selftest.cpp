@@ -41,7 +41,7 @@ void orc_tti_factor(int d, int f32, const double* fm, int64_t n, const double* x
 
 namespace {
 
-enum { ISO = 0, DIAG = 1, RIEM = 2, RAND = 3, TTI = 4 };
+enum { ISO = 0, DIAG = 1, RIEM = 2, RAND = 3, TTI = 4, RS2 = 5, RSF2 = 6, ELA2 = 7, DUB2 = 8 };
 int g_fail = 0;
 
 void check(bool ok, const char* what, double v = 0.) {
@@ -83,6 +83,17 @@ std::vector<double> geometry(int model, int d, int64_t N, unsigned seed) {
       if (model == RAND)
         for (int c = 0; c < d; ++c) g[size_t(sd + c) * N + n] = 0.2 * (r() - 0.5);
     }
+  } else if (model >= RS2) {
+    // curvature models on (x, y, theta): [1/xi, kappa, cos theta, sin theta]
+    g.assign(size_t(4) * N, 0.);
+    const int64_t nt = 7;  // angles along the last axis (shape[2])
+    for (int64_t n = 0; n < N; ++n) {
+      const double th = 2. * 3.141592653589793 * double(n % nt) / double(nt);
+      g[size_t(n)] = 1. + r();
+      g[size_t(N + n)] = 0.3 * (r() - 0.5);
+      g[size_t(2 * N + n)] = std::cos(th);
+      g[size_t(3 * N + n)] = std::sin(th);
+    }
   } else if (model == TTI) {
     const int ng = 5 + d * d;
     g.assign(size_t(ng) * N, 0.);
@@ -99,7 +110,7 @@ std::vector<double> geometry(int model, int d, int64_t N, unsigned seed) {
 
 void run_model(int model, int d, bool walls) {
   const int64_t shape[3] = {d == 2 ? 23 : 9, d == 2 ? 17 : 8, d == 2 ? 1 : 7};
-  const int periodic[3] = {0, d == 3 ? 1 : 0, 0};
+  const int periodic[3] = {0, (d == 3 && model < RS2) ? 1 : 0, model >= RS2 ? 1 : 0};
   int64_t N = 1;
   for (int k = 0; k < d; ++k) N *= shape[k];
   std::vector<double> geom = geometry(model, d, N, 17u + unsigned(model) * 3u + unsigned(d));
@@ -139,7 +150,9 @@ void run_model(int model, int d, bool walls) {
                           0., 100000, &visits, u_t.data()) >= 0)
     check(max_rel_diff(u_t, u_j) < 1e-9, (std::string(what) + ": tiles against iteration").c_str(),
           max_rel_diff(u_t, u_j));
-  const int nsym = (model == ISO || model == DIAG) ? d : d * (d + 1) / 2;
+  const int nsym = (model == ISO || model == DIAG) ? d
+                   : model == RS2 ? 6 : model == RSF2 ? 7 : model == ELA2 ? 30 : model == DUB2 ? 6
+                                                                                : d * (d + 1) / 2;
   std::vector<double> fw(NN * nsym), fv(size_t(d) * NN), ws(NN);
   std::vector<int64_t> nidx(NN * nsym);
   std::vector<double> u = u_fm;
@@ -153,6 +166,7 @@ void run_model(int model, int d, bool walls) {
   bool zero = true;
   for (double l : lam) zero = zero && l == 0.;
   check(zero, (std::string(what) + ": zero adjoint").c_str());
+  if (model >= RS2) return;  // first order, no factorization
   // second order and factorization, by iteration
   orc_set_order(2, 0.3);
   std::fill(u_j.begin(), u_j.end(), INFINITY);
@@ -176,12 +190,16 @@ void run_model(int model, int d, bool walls) {
 }  // namespace
 
 int main() {
-  check(orc_version() >= 7, "oracle version");
+  check(orc_version() >= 8, "oracle version");
   for (int d = 2; d <= 3; ++d)
     for (int model = ISO; model <= TTI; ++model) {
       run_model(model, d, false);
       run_model(model, d, true);
     }
+  for (int model = RS2; model <= DUB2; ++model) {
+    run_model(model, 3, false);
+    run_model(model, 3, true);
+  }
   // the TTI factor norm on a few points, both precisions
   const double fm[14] = {18., 13., -90., -110., -60., 0.9, 0.2, 0., -0.2, 0.9, 0., 0., 0., 1.};
   const double x[6] = {3., -1., 2., 0.5, -4., 2.};

@@ -35,7 +35,7 @@ def test_library_exports_every_declared_symbol():
 def test_struct_layouts_match_header():
     from adaptivegriddiscretizations_amd import _lib
     # offsets follow the natural alignment of the C declarations
-    assert ctypes.sizeof(_lib.EikonalDesc) == 312
+    assert ctypes.sizeof(_lib.EikonalDesc) == 320
     assert _lib.EikonalDesc.factor_radius2.offset == 152
     assert _lib.EikonalDesc.bound_active_blocks.offset == 296
     assert _lib.EikonalDesc.atol.offset == 72
@@ -81,7 +81,10 @@ def test_scheme_lengths_and_refused_short_buffers():
     w, c = ctypes.c_int64(), ctypes.c_int64()
     expect = {(_lib.RIEMANN, 2, 1): (3, 6), (_lib.RIEMANN, 3, 1): (6, 12),
               (_lib.TTI, 2, 1): (21, 43), (_lib.TTI, 2, 2): (21, 85),
-              (_lib.TTI, 3, 1): (42, 85), (_lib.TTI, 3, 2): (42, 169)}
+              (_lib.TTI, 3, 1): (42, 85), (_lib.TTI, 3, 2): (42, 169),
+              # curvature models: offsets (forward ones included) and the flag
+              (_lib.REEDSSHEPP2, 3, 1): (6, 13), (_lib.REEDSSHEPPFWD2, 3, 1): (7, 15),
+              (_lib.ELASTICA2, 3, 1): (30, 61), (_lib.DUBINS2, 3, 1): (12, 25)}
     for (model, d, order), wc_cc in expect.items():
         assert _lib.lib.agd_scheme_lengths(model, d, order, ctypes.byref(w), ctypes.byref(c)) == 0
         assert (w.value, c.value) == wc_cc

@@ -1,0 +1,159 @@
+"""Curvature-penalised models on the GPU (ReedsShepp2, ReedsSheppForward2,
+Dubins2, Elastica2 on (x, y, theta) grids, theta periodic): the precomputed
+scheme bit for bit against the oracle on the same FP32 geometry, the solves
+against the oracle's fast marching, the flow of the forward offsets, and
+the geodesics and unoriented seeds and tips of the driver."""
+import numpy as np
+import pytest
+
+from adaptivegriddiscretizations_amd import workloads
+from gpu_helpers import log, rel_err
+from test_gpu_tti import cell_major, prepared
+
+pytestmark = pytest.mark.gpu
+CURV = {'ReedsShepp2': (6, 0, 1), 'ReedsSheppForward2': (7, 6, 1), 'Dubins2': (6, 6, 2),
+        'Elastica2': (30, 30, 1)}     # offsets per sub-scheme, forward ones, sub-schemes
+TOL32 = 1e-4
+
+
+def oracle_from_device(O, it, wl):
+    """Oracle problem on the device's own FP32 geometry and rhs."""
+    geom = cell_major(it, it.geom).astype(np.float64)
+    cost = np.asarray(it.cost, dtype=np.float32).astype(np.float64)
+    rhs = np.broadcast_to(cost, wl.dims).copy()
+    seeds = np.zeros(wl.dims, np.uint8)
+    idx = np.asarray(it.seedIndices)
+    for s, v in zip(idx, it.seedValues):
+        seeds[tuple(s)] = 1
+        rhs[tuple(s)] = v
+    return O.Problem(wl.model, wl.dims, geom.reshape(4, -1), rhs, seeds, [0., 0., 0.],
+                     periodic=[0, 0, 1]), idx
+
+
+def code_mismatches(it, wl, sc, e_ref, ns, nfwd, nmix):
+    """Neighbour codes that do not address cell -+ e (theta wrapped), or -1
+    outside the box; forward offsets must have no x - e code."""
+    N = int(np.prod(wl.dims))
+    dims = np.array(wl.dims)
+    si, so = np.array(it.shape_i), np.array(it.shape_o)
+    NI = int(np.prod(si))
+    cells = np.stack(np.unravel_index(np.arange(N), wl.dims), -1)
+    own = np.ravel_multi_index(tuple((cells // si).T), tuple(so))
+
+    def pos_of(code):
+        t = np.where(code <= -2, own * NI + (-2 - code), code)
+        yo = np.stack(np.unravel_index(t // NI, tuple(so)), -1)
+        yi = np.stack(np.unravel_index(t % NI, tuple(si)), -1)
+        return yo * si + yi
+
+    nbad = 0
+    for m in range(nmix):
+        for k in range(ns):
+            e = e_ref[:, m * ns + k, :]
+            for sd, sg in ((0, -1), (1, 1)):
+                code = sc[:, m * 2 * ns + 2 * k + sd]
+                if sd == 0 and k >= ns - nfwd:
+                    nbad += int(np.sum(code != -1))
+                    continue
+                y = cells + sg * e
+                y[:, 2] %= dims[2]
+                inside = np.all((y[:, :2] >= 0) & (y[:, :2] < dims[:2]), axis=1)
+                ok = np.where(inside, code != -1, code == -1)
+                got = pos_of(np.where(code == -1, 0, code))
+                ok &= np.where(inside, np.all(got == y, axis=1), True)
+                nbad += int(np.sum(~ok))
+    return nbad
+
+
+@pytest.mark.parametrize("model", list(CURV))
+def test_curvature_scheme_matches_oracle(O, model):
+    wl = workloads.curvature2(model, n=24, ntheta=16, xi=0.4, kappa=0.3, seed=31)
+    it = prepared(wl)
+    P, _ = oracle_from_device(O, it, wl)
+    ns, nfwd, nmix = CURV[model]
+    w_ref, e_ref, mix_ref = P.stencils(use_float=True, with_mix=True)
+    N = int(np.prod(wl.dims))
+    sw = cell_major(it, it.scheme_w_t).reshape(nmix * ns, N).T
+    sc = cell_major(it, it.scheme_c_t).reshape(nmix * 2 * ns + 1, N).T
+    assert np.array_equal(sc[:, -1], mix_ref), "min/max flags differ"
+    assert np.array_equal(sw.astype(np.float64), w_ref), \
+        f"weights differ in {int(np.sum(np.any(sw != w_ref, axis=1)))} cells"
+    nbad = code_mismatches(it, wl, sc, e_ref, ns, nfwd, nmix)
+    log(f"{model} scheme: {N} cells, {nmix}x{ns} offsets, nonzero weights {int(np.sum(w_ref > 0))}, "
+        f"code mismatches {nbad}")
+    assert nbad == 0
+
+
+@pytest.mark.parametrize("model,n,nt", [('ReedsShepp2', 48, 32), ('ReedsSheppForward2', 48, 32),
+                                        ('Dubins2', 40, 32), ('Elastica2', 32, 24)])
+def test_curvature_solve_matches_oracle(O, model, n, nt):
+    wl = workloads.curvature2(model, n=n, ntheta=nt, xi=0.4, seed=32)
+    out = wl.dict_in(exportValues=True).Run()
+    u = out['values'].cpu().numpy().astype(np.float64)
+    it = prepared(wl)
+    P, _ = oracle_from_device(O, it, wl)
+    ref = P.solve_fm()
+    err = rel_err(u, ref)
+    log(f"{model} {n}x{n}x{nt}: passes {out['stats']['eikonal']['launches']}, "
+        f"finite {int(np.isfinite(ref).sum())}/{ref.size}, rel err vs oracle FM {err:.2e}")
+    assert err <= TOL32
+
+
+@pytest.mark.parametrize("model", ['ReedsSheppForward2', 'Dubins2'])
+def test_curvature_flow_matches_oracle(O, model):
+    wl = workloads.curvature2(model, n=32, ntheta=24, xi=0.4, seed=33)
+    out = wl.dict_in(exportValues=True, exportGeodesicFlow=True).Run()
+    u = out['values'].cpu().numpy().astype(np.float64)
+    it = prepared(wl)
+    P, _ = oracle_from_device(O, it, wl)
+    fl = P.flow(u, use_float=True)
+    h = wl.h.reshape((3, 1, 1, 1))
+    flow = out['flow'].cpu().numpy()
+    fin = np.isfinite(u)
+    ref = -h * fl['vector']
+    off = np.abs(flow - ref)[:, fin] > 1e-3 * np.max(np.abs(ref[:, fin]))
+    bad = np.mean(off)
+    log(f"{model} flow: fraction of cells off by >1e-3 max: {bad:.2e}")
+    assert bad <= 2e-3
+
+
+def test_reeds_shepp_closed_forms_and_geodesic(O):
+    """Straight ahead and turning in place cost h and xi h_theta per cell
+    (unit cost); a geodesic from a tip ahead of the seed reaches it."""
+    n, nt, xi = 41, 32, 0.3
+    wl = workloads.curvature2('ReedsSheppForward2', n=n, ntheta=nt, xi=xi, seed=34)
+    wl.cost = np.ones((n, n), np.float32)
+    hfm = wl.dict_in(exportValues=True, tips=[[0.5, 0., 0.]])
+    out = hfm.Run()
+    u = out['values'].cpu().numpy().astype(np.float64)
+    c = n // 2
+    h, ht = 2. / n, 2 * np.pi / nt
+    k = np.arange(1, 8)
+    np.testing.assert_allclose(u[c + k, c, 0], k * h, rtol=3e-6)
+    np.testing.assert_allclose(u[c, c, k], k * ht * xi, rtol=3e-6)
+    geo = out['geodesics'][0]
+    log(f"rsf geodesic: {geo.shape[1]} points, stop {out['geodesic_stopping_criteria'][0]}, "
+        f"end {geo[:, -1]}")
+    assert out['geodesic_stopping_criteria'][0] == 'AtSeed'
+    assert np.max(np.abs(geo[:2, -1])) <= 2 * h
+    assert np.all(np.abs(geo[1]) <= 2 * h)          # a straight path along the x axis
+
+
+def test_unoriented_seeds_and_tips(O):
+    n, nt = 32, 16
+    wl = workloads.curvature2('ReedsShepp2', n=n, ntheta=nt, xi=0.5, seed=35)
+    hfm = wl.dict_in(exportValues=True, tips_Unoriented=[[0.6, 0.3]])
+    del hfm['seeds']
+    hfm['seeds_Unoriented'] = [[-0.5, -0.5]]
+    out = hfm.Run()
+    u = out['values'].cpu().numpy().astype(np.float64)
+    i = int(round((-0.5 + 1) / (2. / n) - 0.5))
+    assert np.all(u[i, i, :] == 0.)
+    geo = out['geodesics_Unoriented'][0]
+    log(f"unoriented: {geo.shape[1]} points, stop {out['geodesic_stopping_criteria'][0]}, end {geo[:, -1]}")
+    assert out['geodesic_stopping_criteria'][0] == 'AtSeed'
+    assert np.max(np.abs(geo[:2, -1] + 0.5)) <= 3 * (2. / n)
+    # the tip takes its most favourable orientation
+    ti = int(round((0.6 + 1) / (2. / n) - 0.5)), int(round((0.3 + 1) / (2. / n) - 0.5))
+    k = int(round(geo[2, 0] / (2 * np.pi / nt))) % nt
+    assert u[ti[0], ti[1], k] == np.min(u[ti[0], ti[1], :])

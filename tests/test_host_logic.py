@@ -224,3 +224,36 @@ def test_subwave_tile_shapes_are_batched_only():
     assert _lib.lib.agd_batch_tile_supported(_lib.ISOTROPIC, 2, _lib.i32((8, 8))) == 1
     assert _lib.lib.agd_tile_supported(_lib.ISOTROPIC, 2, _lib.i32((8, 8))) == 1
     assert _lib.lib.agd_batch_tile_supported(_lib.ISOTROPIC, 2, _lib.i32((2, 2))) == 0
+
+
+def test_curvature_grid_geometry():
+    """SE grids (DictIn.py:323-372, 464-482): gridScales (h, h, 2 pi / nTheta),
+    the angle axis starting half a cell below 0, so that cell k sits at
+    angle 2 pi k / nTheta; OrientedPoints pairs each point with every angle."""
+    from adaptivegriddiscretizations_amd.Eikonal import dictIn
+    hfm = dictIn({'model': 'ReedsShepp2'})
+    hfm.SetRect([[-1, 1], [0, 1]], dimx=20)
+    hfm.nTheta = 12
+    assert hfm.shape == (20, 10, 12) and hfm.vdim == 3 and hfm.nTheta == 12
+    np.testing.assert_allclose(hfm.gridScales, [0.1, 0.1, 2 * np.pi / 12], rtol=1e-6)
+    th = np.asarray(hfm.Axes()[2], dtype=np.float64)
+    np.testing.assert_allclose(th, 2 * np.pi * np.arange(12) / 12, atol=1e-6)
+    pts = hfm.OrientedPoints([[0.3, 0.4], [-0.5, 0.2]])
+    assert pts.shape == (12, 2, 3)
+    np.testing.assert_allclose(pts[5, 1], [-0.5, 0.2, th[5]], rtol=1e-6)
+    idx = hfm.PointFromIndex(pts.reshape(-1, 3), to=True)
+    np.testing.assert_allclose(idx[:, 2].reshape(12, 2), np.arange(12)[:, None] + 0 * idx[:2, 2], atol=1e-5)
+    with pytest.raises(ValueError):
+        dictIn({'model': 'Riemann2'}).nTheta
+
+
+def test_curvature_traits_and_refusals():
+    from adaptivegriddiscretizations_amd import _lib
+    from adaptivegriddiscretizations_amd.Eikonal import HFM_GPU
+    for model, fim in (('ReedsShepp2', None), ('ReedsSheppForward2', 5), ('Elastica2', 4), ('Dubins2', None)):
+        traits, f = HFM_GPU.default_traits(model)
+        assert traits['shape_i'] == (4, 4, 4) and f == fim
+        mid = HFM_GPU._MODELS[model[:-1]]
+        assert _lib.lib.agd_tile_supported(mid, 3, _lib.i32((4, 4, 4))) == 1
+        assert _lib.lib.agd_tile_supported(mid, 2, _lib.i32((8, 8))) == 0
+    assert HFM_GPU.scheme_slots('Elastica', 3) == 30 and HFM_GPU.scheme_slots('ReedsSheppForward', 3) == 7
