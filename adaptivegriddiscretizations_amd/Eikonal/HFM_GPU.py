@@ -341,6 +341,9 @@ class Interface:
 
         # Walls
         self.walls = self.GetValue('walls', default=None, help='Obstacles in the domain')
+        if self.walls is not None and self.curvature:
+            # physical (nx, ny) or angular walls are broadcast (_SetGeometry.py:265)
+            self.walls = np.ascontiguousarray(self._angular_field(self.walls, 'walls')).astype(np.uint8)
         self.wallDistBound = int(self.GetValue(
             'wallDistBound', default=10,
             help="Bound on the computed distance to the obstacles.\n(Ideally a sharp upper bound on the stencil width.)"))
@@ -909,7 +912,7 @@ class Interface:
                                         help="Typical expected length of geodesics (number of points)."))
         causality = float(self.GetValue('geodesic_causalityTolerance', default=4.,
                                         help="Used in criterion for rejecting points in flow interpolation"))
-        eucl_bound = self.GetValue('geodesic_targetTolerance', default=6,
+        eucl_bound = self.GetValue('geodesic_targetTolerance', default=12 if self.curvature else 6,
                                    help="Tolerance, in pixels, for declaring a seed as reached.")
         # Distance to the seeds, in fifths of a pixel, up to eucl_bound pixels
         # (the trigger bits: seeds without the walls).
