@@ -638,6 +638,7 @@ class Interface:
 
         self.forwardAD = self.cost_coef is not None or self.seed_coef is not None
         self.values_coef = None
+        self.SetChart()
 
         self.wallDist_t = None
         if self.walls is not None:
@@ -755,17 +756,76 @@ class Interface:
             h = self._solver_handle = _lib.Solver(self.stream, self._desc())
         return h
 
+    # ------------------------------------------------------------ charts
+    def SetChart(self):
+        """_StopCriterion.py:93-165: the mapping between local charts, from
+        physical coordinates to continuous grid indices of the last ndim_s
+        axes (broadcast over the first ones)."""
+        self.hasChart = self.HasValue('chart_mapping')
+        if not self.hasChart:
+            return
+        mapping = self.GetValue('chart_mapping', default=None,
+                                help="Mapping from one local chart to another, for eikonal equations defined "
+                                     "on manifolds. (Please set to NaN inside the walls.)")
+        self.chart_nitermax = int(self.GetValue(
+            'chart_nitermax', default=5,
+            help="Number of times the boundary conditions are updated in the eikonal solver, "
+                 "with values from other local charts."))
+        m = np.asarray(host_array(mapping), dtype=np.float64)
+        ndim_s = m.ndim - 1
+        ndim_b = self.ndim - ndim_s
+        if ndim_s < 1 or ndim_b < 0 or tuple(m.shape[1:]) != tuple(self.shape[ndim_b:]) or len(m) != ndim_s:
+            raise ValueError(f"Inconsistent shape of field chart_mapping : {m.shape}")
+        if self.forwardAD or self.reverseAD:
+            raise ValueError("Sorry, forward and reversed AD not yet supported with charts")
+        lo, _ = self.hfmIn.corners
+        lo = np.asarray(host_array(lo), dtype=np.float64)[ndim_b:]
+        h = np.asarray(self.h, dtype=np.float64)[ndim_b:]
+        shp = (ndim_s,) + (1,) * ndim_s
+        m = (m - lo.reshape(shp)) / h.reshape(shp) - 0.5
+        self.chart_ndim_s = ndim_s
+        self.chart_mapping_t = device_tensor(np.ascontiguousarray(m), self.float_t)
+
+    def _chart_paste(self, scratch, trig, cnt):
+        """One paste of the values between charts; the number of cells lowered."""
+        trig.zero_()
+        cnt.zero_()
+        _lib.check(_lib.lib.agd_chart_paste(self._desc(), ptr(self.values_t), ptr(self.chart_mapping_t),
+                                            self.chart_ndim_s, ptr(scratch), ptr(trig), ptr(cnt), self.stream))
+        return int(cnt.item())
+
     # ------------------------------------------------------------ solve
     def Solve(self):
-        """_solvers.py:15-57."""
+        """_solvers.py:15-57; with charts, after each converged solve the
+        values are pasted between charts and the solve continues from the
+        lowered cells, up to chart_nitermax times (_StopCriterion.py:60-80)."""
         if self.verbosity >= 1:
             print("Running the eikonal GPU kernel")
         desc, io, st = self._desc(), self._io(), _lib.EikonalStats()
         torch.cuda.synchronize()
         t0 = time.time()
         status = self.solver_handle().solve(desc, io, st)
+        totals = st.as_dict()
+        if getattr(self, 'hasChart', False):
+            scratch = empty((self.size_tot,), self.float_t)
+            trig = zeros(((self.size_tot + 31) // 32,), np.int32)
+            cnt = zeros((1,), np.int32)
+            npaste = 0
+            while status == _lib.AGD_OK and npaste < self.chart_nitermax:
+                npaste += 1
+                if self._chart_paste(scratch, trig, cnt) == 0:
+                    break
+                d2, io2, st2 = self._desc(), self._io(), _lib.EikonalStats()
+                d2.activate_all = 0
+                io2.trigger = ptr(trig)
+                status = self.solver_handle().solve(d2, io2, st2)
+                for k in ('niter_o', 'tile_updates', 'cell_updates', 'launches', 'gpu_ms', 'kernel_ms'):
+                    totals[k] += getattr(st2, k)
+                totals['converged'] = st2.converged
+            totals['chart_pastes'] = npaste
+            del scratch, trig, cnt
         wall = time.time() - t0
-        self.stats['eikonal'] = {**st.as_dict(), 'time': wall, 'gpu_time': st.gpu_ms * 1e-3}
+        self.stats['eikonal'] = {**totals, 'time': wall, 'gpu_time': totals['gpu_ms'] * 1e-3}
         if self.verbosity >= 1:
             print(f"GPU kernel eikonal ran for {wall} seconds, and {st.niter_o} iterations.")
         if status == _lib.AGD_NOCONV:
@@ -923,6 +983,26 @@ class Interface:
         _lib.check(_lib.lib.agd_geodesic_eucl(desc, ptr(self.trigger_t), int(np.ceil(eucl_bound)), mult,
                                               int(mult * eucl_bound), ptr(eucl), ptr(tmp), self.stream))
         del tmp
+        chart_var = float('inf')
+        if getattr(self, 'hasChart', False):
+            # _GetGeodesics.py:66-71, _Kernel.py:189-201: the cells where
+            # geodesics jump to another chart carry the eucl code 254
+            jump = self.GetValue('chart_jump', help="Where the geodesics should jump "
+                                 "to another local chart of the manifold")
+            jump = np.ascontiguousarray(np.broadcast_to(np.asarray(host_array(jump), dtype=np.uint8),
+                                                        tuple(self.shape)))
+            jump_c = device_tensor(jump, np.uint8)
+            jump_t = zeros((self.size_tot,), np.uint8)
+            _lib.check(_lib.lib.agd_block_expand(self.ndim, _lib.i64(self.shape), _lib.i32(self.shape_i), 1, 1,
+                                                 ptr(jump_c), 1, ptr(jump_t), 0., self.stream))
+            eucl.masked_fill_(eucl == _lib.EUCL_CHART, 255)
+            eucl.masked_fill_(jump_t != 0, _lib.EUCL_CHART)
+            dev = self.GetValue('chart_jump_deviation', default=np.inf,
+                                help="Do not interpolate the jump coordinates, among pixel corners, "
+                                     " if their (adimensionized) standard deviation exceeds this threshold. "
+                                     "(Use if chart_mapping is discontinuous. Typical value : 5.) ")
+            dev = 5. if dev is True else float(dev)
+            chart_var = dev * dev
         pastseed_delay = int(self.GetValue(
             'pastseed_delay', default=0,
             help="Number of geodesic backtracking steps before checking the PastSeed criterion "
@@ -954,6 +1034,12 @@ class Interface:
             gd.ngeo, gd.max_len = n, typical_len
             gd.eucl_delay, gd.nymin_delay, gd.pastseed_delay = eucl_delay, nymin_delay, pastseed_delay
             gd.step, gd.causality_tolerance = step, causality
+            if getattr(self, 'hasChart', False):
+                gd.chart_ndim_s = self.chart_ndim_s
+                gd.chart_mapping = ptr(self.chart_mapping_t)
+                gd.chart_jump_variance = chart_var
+            else:
+                gd.chart_jump_variance = float('inf')
             if self.geodesic_online_flow:
                 # the flow of each cell the ODE enters, recomputed from the scheme
                 _lib.check(_lib.lib.agd_geodesic_ode_online(desc, self._io(), gd, ptr(eucl), ptr(x_s),

@@ -15,7 +15,7 @@ LIB_NAME = "libagd_hip.so"
 LIB_PATH = os.path.join(PKG_DIR, LIB_NAME)
 SOURCES = ["layout.hip", "eikonal.hip", "adjoint.hip", "eik_isotropic.hip", "eik_diagonal.hip",
            "eik_riemann.hip", "eik_rander.hip", "eik_tti.hip", "eik_curvature.hip", "surrogate.hip",
-           "geodesic.hip"]
+           "geodesic.hip", "chart.hip"]
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 CFLAGS = [

@@ -20,6 +20,7 @@ AGD_OK, AGD_EINVAL, AGD_EHIP, AGD_NOCONV, AGD_ENOMEM = 0, -1, -2, -3, -4
 # agd_model
 ISOTROPIC, DIAGONAL, RIEMANN, RANDER, TTI = 0, 1, 2, 3, 4
 REEDSSHEPP2, REEDSSHEPPFWD2, ELASTICA2, DUBINS2 = 5, 6, 7, 8
+EUCL_CHART = 254
 # agd_scalar
 F32, F64 = 0, 1
 # agd_solver_kind
@@ -69,8 +70,8 @@ class EikonalStats(ctypes.Structure):
 class GeodesicDesc(ctypes.Structure):
     _fields_ = [
         ("ngeo", c_int32), ("max_len", c_int32), ("eucl_delay", c_int32), ("nymin_delay", c_int32),
-        ("pastseed_delay", c_int32), ("reserved", c_int32), ("step", c_double),
-        ("causality_tolerance", c_double),
+        ("pastseed_delay", c_int32), ("chart_ndim_s", c_int32), ("step", c_double),
+        ("causality_tolerance", c_double), ("chart_mapping", c_void_p), ("chart_jump_variance", c_double),
     ]
 
 
@@ -124,6 +125,8 @@ SIGNATURES = {
                                    c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "agd_geodesic_ode_online": (c_int32, [POINTER(EikonalDesc), POINTER(EikonalIO), POINTER(GeodesicDesc),
                                           c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "agd_chart_paste": (c_int32, [POINTER(EikonalDesc), c_void_p, c_void_p, c_int32, c_void_p, c_void_p,
+                                  c_void_p, c_void_p]),
     "agd_block_expand": (c_int32, [c_int32, c_int64_p, c_int32_p, c_int32, c_int32, c_void_p,
                                    c_int64, c_void_p, c_double, c_void_p]),
     "agd_block_squeeze": (c_int32, [c_int32, c_int64_p, c_int32_p, c_int32, c_int32, c_void_p,

@@ -1,0 +1,146 @@
+// chart.hip -- value pasting between the local charts of a manifold
+// (ChartPaste.h:71-167, run by the stopping criterion of the iterative solver,
+// _StopCriterion.py:60-80, after each converged solve).
+//
+// Two passes over the tiled grid, one lane per cell: the first interpolates,
+// for every cell, the values at the point its chart identifies it with (into
+// a scratch array); the second lowers each cell to that value where smaller,
+// sets its trigger bit (so that a following solve starts from its tile) and
+// counts the lowered cells.  Splitting the passes makes the result independent
+// of the order in which lanes run (the reference pastes in place).  Both
+// passes are HBM streams over the values; the interpolation's corner reads
+// are gathers within a few cells of the mapped point.
+#include <cmath>
+
+#include "common.hpp"
+#include "device.hpp"
+#include "geodesic_kernels.hpp"
+
+namespace agd {
+
+template <class S> struct ChartArgs {
+  GeoGrid g;         // periodicity cleared: the paste does not wrap axes
+  const S* u;
+  const S* mapping;  // [ndim_s][size_s], continuous grid indices
+  int32_t ndim_s;
+  int64_t size_s;
+  S* out;            // interpolated value per tiled cell, +inf where none
+};
+
+// ChartPaste.h:99-149 for the cell at tiled index n.
+template <class S, int D> __global__ void chart_eval_kernel(ChartArgs<S> a) {
+  const int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (n >= a.g.size_tot) return;
+  const S inf = inf_v<S>();
+  int x[3];
+  if (!geo_pos<D>(n, a.g, x)) {
+    a.out[n] = inf;
+    return;
+  }
+  const int nb = D - a.ndim_s;
+  // index of x among the cells of the last ndim_s axes
+  int64_t n_s = 0;
+  for (int i = nb; i < D; ++i) n_s = n_s * a.g.shape[i] + x[i];
+  const S tol = S(1e-4);  // boundary_tol
+  int q[3];
+  S r[3];
+  for (int i = 0; i < a.ndim_s; ++i) {
+    const S m = a.mapping[int64_t(i) * a.size_s + n_s];
+    const int qmax = a.g.shape[nb + i] - 1;
+    // NaN, or far outside the grid (kept out of the integer conversion)
+    if (!(m >= S(-2) && m <= S(qmax + 2))) {
+      a.out[n] = inf;
+      return;
+    }
+    const S f = floor(m);
+    q[i] = int(f);
+    r[i] = m - f;
+    // points just outside the last layer are moved onto it
+    if (q[i] == qmax && r[i] <= tol) {
+      q[i] -= 1;
+      r[i] = S(1);
+    }
+    if (q[i] == -1 && r[i] >= S(1) - tol) {
+      q[i] += 1;
+      r[i] = S(0);
+    }
+    if (q[i] >= qmax || q[i] < 0) {
+      a.out[n] = inf;
+      return;
+    }
+  }
+  // multilinear interpolation over the 2^ndim_s corners (a corner of zero
+  // weight and infinite value gives NaN, hence no paste, as in the reference)
+  S val = S(0), wsum = S(0);
+  int y[3] = {x[0], x[1], x[2]};
+  const int ncorner = 1 << a.ndim_s;
+  for (int c = 0; c < ncorner; ++c) {
+    S w = S(1);
+    for (int i = 0; i < a.ndim_s; ++i) {
+      const int e = (c >> i) & 1;
+      y[nb + i] = q[i] + e;
+      w *= e ? r[i] : S(1) - r[i];
+    }
+    const int64_t ny = geo_index<D>(y, a.g);
+    if (ny < 0) continue;
+    val += w * a.u[ny];
+    wsum += w;
+  }
+  a.out[n] = wsum > S(0) ? val / wsum : inf;
+}
+
+// ChartPaste.h:151-165: lower the cells whose pasted value is smaller.
+template <class S>
+__global__ void chart_apply_kernel(int64_t size_tot, S* u, const S* cand, uint32_t* trigger, int32_t* count) {
+  const int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  bool upd = false;
+  if (n < size_tot) {
+    const S c = cand[n];
+    if (c < u[n]) {  // false for NaN and +inf
+      u[n] = c;
+      atomicOr(trigger + (n >> 5), 1u << (n & 31));
+      upd = true;
+    }
+  }
+  const unsigned long long b = __ballot(upd);
+  if ((threadIdx.x & 63) == 0 && b) atomicAdd(count, int32_t(__popcll(b)));
+}
+
+template <class S>
+int chart_paste_impl(const agd_eikonal_desc* d, void* values, const void* mapping, int32_t ndim_s,
+                     void* scratch, uint32_t* trigger, int32_t* count, hipStream_t s) {
+  ChartArgs<S> a{};
+  a.g = geo_grid(d);
+  a.g.periodic = 0;
+  a.u = static_cast<const S*>(values);
+  a.mapping = static_cast<const S*>(mapping);
+  a.ndim_s = ndim_s;
+  a.size_s = 1;
+  for (int i = d->ndim - ndim_s; i < d->ndim; ++i) a.size_s *= d->shape[i];
+  a.out = static_cast<S*>(scratch);
+  const dim3 grid(unsigned(ceil_div(a.g.size_tot, 256))), block(256);
+  if (d->ndim == 2) hipLaunchKernelGGL((chart_eval_kernel<S, 2>), grid, block, 0, s, a);
+  else hipLaunchKernelGGL((chart_eval_kernel<S, 3>), grid, block, 0, s, a);
+  hipLaunchKernelGGL(chart_apply_kernel<S>, grid, block, 0, s, a.g.size_tot, static_cast<S*>(values),
+                     static_cast<const S*>(scratch), trigger, count);
+  AGD_HIP_TRY(hipGetLastError());
+  return AGD_OK;
+}
+
+}  // namespace agd
+
+using namespace agd;
+
+extern "C" {
+
+int agd_chart_paste(const agd_eikonal_desc* desc, void* values, const void* mapping, int32_t ndim_s,
+                    void* scratch, uint32_t* trigger, int32_t* count, void* stream) {
+  if (int st = check_geo_desc(desc)) return st;
+  AGD_REQUIRE(ndim_s >= 1 && ndim_s <= desc->ndim, "ndim_s must be in [1, ndim]");
+  AGD_REQUIRE(values && mapping && scratch && trigger && count, "missing buffers");
+  if (desc->scalar == AGD_F32)
+    return chart_paste_impl<float>(desc, values, mapping, ndim_s, scratch, trigger, count, as_stream(stream));
+  return chart_paste_impl<double>(desc, values, mapping, ndim_s, scratch, trigger, count, as_stream(stream));
+}
+
+}  // extern "C"

@@ -763,6 +763,7 @@ static int geodesic_online_impl(const agd_eikonal_desc* d, const agd_eikonal_io*
   AGD_REQUIRE(eucl && x && len && stop, "missing buffers");
   int hlen = 0;
   if (int st = check_geodesic_desc(gd, hlen)) return st;
+  AGD_REQUIRE(gd->chart_ndim_s <= d->ndim, "chart_ndim_s must be in [0, ndim]");
   if (gd->ngeo == 0) return AGD_OK;
   GeoArgs<S> a = geodesic_args<S>(geo_grid(d), gd, hlen, io->values, eucl, x, len, stop);
   const FlowArgs<S> fa = flow_args<S>(d, io);

@@ -109,6 +109,7 @@ int agd_geodesic_ode(const agd_eikonal_desc* desc, const agd_geodesic_desc* gd, 
   if (int st = check_geo_desc(desc)) return st;
   int hlen = 0;
   if (int st = check_geodesic_desc(gd, hlen)) return st;
+  AGD_REQUIRE(gd->chart_ndim_s <= desc->ndim, "chart_ndim_s must be in [0, ndim]");
   AGD_REQUIRE(values && flow_vector && flow_weightsum && eucl && x && len && stop, "missing buffers");
   if (gd->ngeo == 0) return AGD_OK;
   const GeoGrid g = geo_grid(desc);

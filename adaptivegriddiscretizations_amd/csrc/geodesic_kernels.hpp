@@ -74,6 +74,10 @@ template <class S> struct GeoArgs {
   int8_t* stop;
   int32_t ngeo, max_len, hlen, eucl_delay, nymin_delay, pastseed_delay;
   S step, causality;
+  // charts: mapping [ndim_s][n_s] of the last ndim_s axes (null: no charts)
+  const S* chart;
+  int32_t chart_ndim_s;
+  S chart_var;
 };
 
 template <class S, int D> struct GeoCell {
@@ -184,6 +188,61 @@ __device__ int geo_flow(const GeoArgs<S>& a, const Src& src, const S (&x)[D], S 
   return kGeoContinue;
 }
 
+// ChartJump (GeodesicODE_Opt.h:43-99): the last ndim_s coordinates of x
+// replaced by the multilinear interpolation of the chart mapping at the
+// corners of their cell (clamped to the grid), or, where the corners' mapped
+// points are spread (summed variance >= chart_var), by the mapping of the
+// nearest corner.  A NaN mapping leaves x unchanged.
+template <class S, int D> __device__ inline void chart_jump(const GeoArgs<S>& a, S (&x)[D]) {
+  const int ns = a.chart_ndim_s, nb = D - ns;
+  int64_t size_s = 1;
+  for (int i = nb; i < D; ++i) size_s *= a.g.shape[i];
+  int q[D];
+  S r[D], xs[D], sum[D], sq[D];
+  for (int i = 0; i < ns; ++i) {
+    const S f = floor(x[nb + i]);
+    const S lim = S(a.g.shape[nb + i]);
+    const S fc = f < S(-1) ? S(-1) : (f > lim ? lim : f);  // NaN-free, in int range
+    q[i] = fc == fc ? int(fc) : 0;
+    r[i] = x[nb + i] - f;
+    xs[i] = S(0);
+    sum[i] = S(0);
+    sq[i] = S(0);
+  }
+  const int ncorner = 1 << ns;
+  auto corner_index = [&](int icorner, bool nearest) {
+    int64_t n = 0;
+    for (int i = 0; i < ns; ++i) {
+      int y = q[i] + (nearest ? (r[i] > S(0.5) ? 1 : 0) : ((icorner >> i) & 1));
+      const int hi = a.g.shape[nb + i] - 1;
+      y = y < 0 ? 0 : (y > hi ? hi : y);
+      n = n * a.g.shape[nb + i] + y;
+    }
+    return n;
+  };
+  for (int c = 0; c < ncorner; ++c) {
+    S w = S(1);
+    for (int i = 0; i < ns; ++i) w *= ((c >> i) & 1) ? r[i] : S(1) - r[i];
+    const int64_t n = corner_index(c, false);
+    for (int i = 0; i < ns; ++i) {
+      const S m = a.chart[int64_t(i) * size_s + n];
+      xs[i] += w * m;
+      sum[i] += m;
+      sq[i] += m * m;
+    }
+  }
+  S var = S(0);
+  for (int i = 0; i < ns; ++i) var += sq[i] / S(ncorner) - sum[i] * sum[i] / S(ncorner * ncorner);
+  if (var >= a.chart_var) {
+    const int64_t n = corner_index(0, true);
+    for (int i = 0; i < ns; ++i) xs[i] = a.chart[int64_t(i) * size_s + n];
+  }
+  bool ok = true;
+  for (int i = 0; i < ns; ++i) ok = ok && xs[i] == xs[i];
+  if (ok)
+    for (int i = 0; i < ns; ++i) x[nb + i] = xs[i];
+}
+
 // GeodesicODE (Kernel_GeodesicODE.h:265-413): from the row's first point,
 // midpoint steps x <- x + step * flow(x + step/2 * flow(x)) until a
 // stopping criterion or max_len points.  Histories of the corner of least
@@ -219,6 +278,10 @@ __device__ inline void geodesic_run(const GeoArgs<S>& a, const Src& src) {
     if (stop != kGeoContinue) break;
     nymin_p[l] = c.nymin;
     eucl_p[l] = a.eucl[c.nymin];
+    // a chart cell keeps the previous distance for the tests and jumps after
+    // the step (Kernel_GeodesicODE.h:317-320, 406)
+    const bool chart = a.chart != nullptr && eucl_p[l] == AGD_EUCL_CHART;
+    if (chart) eucl_p[l] = eucl_p[(l - 1 + a.hlen) % a.hlen];
     if (c.nymin == nymin_p[(l - a.nymin_delay + a.hlen) % a.hlen]) {
       stop = kGeoStationnary;
       break;
@@ -241,6 +304,7 @@ __device__ inline void geodesic_run(const GeoArgs<S>& a, const Src& src) {
       x[k] = a.step * flow[k] + xp[k];
       row[int64_t(len) * D + k] = x[k];
     }
+    if (chart) chart_jump<S, D>(a, x);
   }
   a.len[tid] = len;
   a.stop[tid] = int8_t(stop);
@@ -260,6 +324,9 @@ inline int check_geodesic_desc(const agd_geodesic_desc* gd, int& hlen) {
   hlen = 1 + (gd->eucl_delay < gd->nymin_delay ? gd->nymin_delay : gd->eucl_delay);
   AGD_REQUIRE(hlen <= kGeoHist, "PastSeed / Stationnary delays must be below 256");
   AGD_REQUIRE(gd->step > 0. && gd->step < __builtin_huge_val(), "step must be positive");
+  AGD_REQUIRE(gd->chart_ndim_s >= 0 && gd->chart_ndim_s <= 3, "chart_ndim_s must be in [0, ndim]");
+  AGD_REQUIRE(gd->chart_ndim_s == 0 || gd->chart_mapping, "chart_ndim_s > 0 needs chart_mapping");
+  AGD_REQUIRE(!(gd->chart_jump_variance < 0.), "chart_jump_variance must be >= 0");
   return AGD_OK;
 }
 
@@ -282,6 +349,9 @@ inline GeoArgs<S> geodesic_args(const GeoGrid& g, const agd_geodesic_desc* gd, i
   a.pastseed_delay = gd->pastseed_delay;
   a.step = S(gd->step);
   a.causality = S(gd->causality_tolerance);
+  a.chart = gd->chart_ndim_s > 0 ? static_cast<const S*>(gd->chart_mapping) : nullptr;
+  a.chart_ndim_s = gd->chart_ndim_s;
+  a.chart_var = S(gd->chart_jump_variance);
   return a;
 }
 

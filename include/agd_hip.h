@@ -326,10 +326,23 @@ typedef struct agd_geodesic_desc {
   int32_t eucl_delay;      /* PastSeed history delay, in steps             */
   int32_t nymin_delay;     /* Stationnary history delay, in steps          */
   int32_t pastseed_delay;  /* steps before the PastSeed test is enabled    */
-  int32_t reserved;        /* 0                                            */
+  int32_t chart_ndim_s;    /* charts: axes of chart_mapping (0: no charts) */
   double step;             /* ODE step, in grid units (pixels)             */
   double causality_tolerance; /* corner exclusion: value >= min + tol / flow weight sum */
+  /* Charts (GeodesicODE_Opt.h:43-99, Kernel_GeodesicODE.h:317-320, 406):
+   * after a step taken from a cell whose eucl entry is AGD_EUCL_CHART, the
+   * last chart_ndim_s coordinates of the point are replaced by the
+   * multilinear interpolation of chart_mapping (layout of agd_chart_paste;
+   * corners clamped to the grid), or by the mapping of the nearest corner
+   * when the mapped corners' summed variance reaches chart_jump_variance
+   * (+inf: never).  A mapping with a NaN leaves the point unchanged. */
+  const void* chart_mapping;
+  double chart_jump_variance;
 } agd_geodesic_desc;
+
+/* eucl value marking the cells where geodesics jump to another chart; the
+ * capped distances stay below it (255 = beyond the bound). */
+#define AGD_EUCL_CHART 254
 
 /* Capped distance to the seeds read by the PastSeed criterion
  * (_GetGeodesics.py:50-67, inf_convolution.py:24-93): eucl = 0 on the seed
@@ -357,6 +370,27 @@ int agd_geodesic_ode(const agd_eikonal_desc* desc, const agd_geodesic_desc* gd, 
 int agd_geodesic_ode_online(const agd_eikonal_desc* desc, const agd_eikonal_io* io,
                             const agd_geodesic_desc* gd, const uint8_t* eucl, void* x, int32_t* len,
                             int8_t* stop, void* stream);
+
+/* ------------------------------------------------------------ charts */
+/* Value pasting for manifolds described by local charts (ChartPaste.h:71-167,
+ * run by the stopping criterion _StopCriterion.py:60-80 after each converged
+ * solve, up to chart_nitermax times; _StopCriterion.py:93-165 prepares the
+ * mapping).  For every cell x, mapping gives, for the last ndim_s axes, the
+ * continuous grid index (cell centre = integer) of the point identified with
+ * x; the first ndim - ndim_s coordinates are kept.  The values there are the
+ * multilinear interpolation over the 2^ndim_s surrounding cells; points within
+ * 1e-4 of a cell of the last layer are moved onto it, points beyond and NaN
+ * mappings are skipped; the axes are not wrapped (as the reference's paste).
+ * Where the interpolated value is smaller the cell takes it, its trigger bit
+ * is set and *count (a device int32) is incremented.  Evaluation and update are
+ * two passes, so the result does not depend on the order of the cells.
+ * values: tiled, desc->scalar.  mapping: cell-major [ndim_s][n_s] in
+ * desc->scalar, n_s = product of the last ndim_s dimensions (broadcast over
+ * the first ones).  scratch: size_tot elements of desc->scalar.  trigger
+ * (tiled bits) and *count are accumulated into, not cleared.  A solve with
+ * io->trigger = trigger then continues from the pasted values. */
+int agd_chart_paste(const agd_eikonal_desc* desc, void* values, const void* mapping, int32_t ndim_s,
+                    void* scratch, uint32_t* trigger, int32_t* count, void* stream);
 
 /* ------------------------------------------------------------ layout */
 /* block_expand (FiniteDifferences.py:164-195): src cell-major

@@ -1264,10 +1264,13 @@ void orc_tti_diags(int64_t n, const double* l, const double* q, double* diag, in
 }
 
 // Exact fixed point by fast marching (FP64).  u[N] receives the values.
-// Returns the number of accepted cells.
-int64_t orc_solve_fm(int model, int d, const int64_t* shape, const int* periodic,
-                     const double* geom, const double* rhs, const uint8_t* seeds,
-                     const uint8_t* wallDist, const double* iso_w, double* u) {
+// Returns the number of accepted cells.  With u0 (nullable), the fixed point
+// of u = min(u0, scheme(u)) (seeds keep their value): the non-seed cells start
+// as tentative values u0, which is where the decreasing iteration started
+// from u0 ends (the continuation of a solve after a paste between charts).
+static int64_t solve_fm(int model, int d, const int64_t* shape, const int* periodic, const double* geom,
+                        const double* rhs, const uint8_t* seeds, const uint8_t* wallDist,
+                        const double* iso_w, const double* u0, double* u) {
   // the second-order and factored schemes are solved by iteration
   if (g_order != 1 || g_factor.r2 > 0) return -1;
   Problem P;
@@ -1312,6 +1315,9 @@ int64_t orc_solve_fm(int model, int d, const int64_t* shape, const int* periodic
       u[n] = rhs[n];
       if (u[n] < Inf<double>()) heap.push(QE(u[n], n));
       else acc[n] = 1;  // walls: fixed at +inf
+    } else if (u0 && u0[n] < Inf<double>()) {
+      u[n] = u0[n];
+      heap.push(QE(u[n], n));
     }
   int64_t naccepted = 0;
   auto get_acc = [&](int64_t m) { return acc[m] ? u[m] : Inf<double>(); };
@@ -1334,6 +1340,18 @@ int64_t orc_solve_fm(int model, int d, const int64_t* shape, const int* periodic
     }
   }
   return naccepted;
+}
+
+int64_t orc_solve_fm(int model, int d, const int64_t* shape, const int* periodic,
+                     const double* geom, const double* rhs, const uint8_t* seeds,
+                     const uint8_t* wallDist, const double* iso_w, double* u) {
+  return solve_fm(model, d, shape, periodic, geom, rhs, seeds, wallDist, iso_w, nullptr, u);
+}
+
+int64_t orc_solve_fm_from(int model, int d, const int64_t* shape, const int* periodic,
+                          const double* geom, const double* rhs, const uint8_t* seeds,
+                          const uint8_t* wallDist, const double* iso_w, const double* u0, double* u) {
+  return solve_fm(model, d, shape, periodic, geom, rhs, seeds, wallDist, iso_w, u0, u);
 }
 
 int64_t orc_solve_jacobi(int model, int d, const int64_t* shape, const int* periodic,

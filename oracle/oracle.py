@@ -63,6 +63,8 @@ def _load():
     lib.orc_tti_diags.argtypes = [c_int64, P, P, P, P, c_int]
     lib.orc_solve_fm.argtypes = [c_int, c_int, P, P, P, P, P, P, P, P]
     lib.orc_solve_fm.restype = c_int64
+    lib.orc_solve_fm_from.argtypes = [c_int, c_int, P, P, P, P, P, P, P, P, P]
+    lib.orc_solve_fm_from.restype = c_int64
     lib.orc_solve_jacobi.argtypes = [c_int, c_int, P, P, P, P, P, P, P, c_double, c_double,
                                      c_int64, c_int, P]
     lib.orc_solve_jacobi.restype = c_int64
@@ -161,10 +163,17 @@ class Problem:
                     self.wall_dist, list(self.periodic[:self.d]))
         return p
 
-    def solve_fm(self):
+    def solve_fm(self, u0=None):
+        """Exact fixed point; with u0, that of u = min(u0, scheme(u))."""
         u = np.zeros(self.N)
-        n = lib().orc_solve_fm(self.model, self.d, _p(self._shape), _p(self.periodic), _p(self.geom),
-                               _p(self.rhs), _p(self.seeds), _p(self.wall_dist), _p(self.iso_w), _p(u))
+        if u0 is None:
+            n = lib().orc_solve_fm(self.model, self.d, _p(self._shape), _p(self.periodic), _p(self.geom),
+                                   _p(self.rhs), _p(self.seeds), _p(self.wall_dist), _p(self.iso_w), _p(u))
+        else:
+            u0 = _c(np.asarray(u0).reshape(self.N), np.float64)
+            n = lib().orc_solve_fm_from(self.model, self.d, _p(self._shape), _p(self.periodic),
+                                        _p(self.geom), _p(self.rhs), _p(self.seeds), _p(self.wall_dist),
+                                        _p(self.iso_w), _p(u0), _p(u))
         if n < 0:
             raise ValueError("fast marching solves the first-order scheme only (use solve_jacobi)")
         return u.reshape(self.shape)
@@ -237,6 +246,50 @@ class Problem:
         lib().orc_forward_ad(self.N, self.nsym, _p(uu), _p(_c(flow['weights'], np.float64)),
                              _p(_c(flow['indices'], np.int64)), nrhs, _p(r), _p(du))
         return np.moveaxis(du.reshape((nrhs,) + self.shape), 0, -1)
+
+
+def chart_paste(u, mapping):
+    """ChartPaste.h:99-165 on a cell-major grid: mapping[ndim_s, *shape[-ndim_s:]]
+    gives, for the last ndim_s axes, the continuous index of the point each
+    cell is identified with (broadcast over the first axes).  Returns the
+    values lowered to the multilinear interpolation there where smaller
+    (points within 1e-4 of the last layer moved onto it, points beyond and
+    NaN skipped, no wrapping; a zero-weight infinite corner gives NaN, hence no
+    paste), and the mask of lowered cells.  FP64."""
+    u = np.asarray(u, dtype=np.float64)
+    shape = u.shape
+    m = np.asarray(mapping, dtype=np.float64)
+    ns = m.shape[0]
+    nb = u.ndim - ns
+    m = np.broadcast_to(m.reshape((ns,) + (1,) * nb + shape[nb:]), (ns,) + shape)
+    ok = np.all(np.isfinite(m), axis=0)
+    q = np.floor(np.where(np.isfinite(m), m, 0.)).astype(np.int64)
+    r = np.where(np.isfinite(m), m, 0.) - q
+    for i in range(ns):
+        qmax = shape[nb + i] - 1
+        hi = (q[i] == qmax) & (r[i] <= 1e-4)
+        q[i] = np.where(hi, q[i] - 1, q[i])
+        r[i] = np.where(hi, 1., r[i])
+        lo = (q[i] == -1) & (r[i] >= 1 - 1e-4)
+        q[i] = np.where(lo, q[i] + 1, q[i])
+        r[i] = np.where(lo, 0., r[i])
+        ok &= (q[i] >= 0) & (q[i] < qmax)
+    lead = np.indices(shape)[:nb]
+    val = np.zeros(shape)
+    wsum = np.zeros(shape)
+    with np.errstate(invalid='ignore'):
+        for c in range(1 << ns):
+            w = np.ones(shape)
+            idx = list(lead)
+            for i in range(ns):
+                e = (c >> i) & 1
+                w = w * (r[i] if e else 1. - r[i])
+                idx.append(np.clip(q[i] + e, 0, shape[nb + i] - 1))
+            val = val + w * u[tuple(idx)]
+            wsum = wsum + w
+        cand = np.where(ok & (wsum > 0), val / np.where(wsum > 0, wsum, 1.), np.inf)
+        lower = cand < u
+    return np.where(lower, cand, u), lower
 
 
 def visible(wall_dist, x, e, periodic=None):
