@@ -34,7 +34,7 @@ def device_tensor(x, dtype):
     if isinstance(x, torch.Tensor):
         return x.detach().to(device=dev, dtype=tdtype(dtype)).contiguous()
     arr = np.asarray(host_array(x), dtype=dtype)
-    if not arr.flags.c_contiguous:  # (np.ascontiguousarray would turn 0-d into 1-d)
+    if not arr.flags.c_contiguous or not arr.flags.writeable:  # (np.ascontiguousarray turns 0-d into 1-d)
         arr = arr.copy(order='C')
     return torch.from_numpy(arr).to(dev)
 
