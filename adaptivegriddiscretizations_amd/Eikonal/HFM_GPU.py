@@ -98,6 +98,15 @@ class Interface:
         self.verbosity = self.GetValue('verbosity', default=1,
                                        help="Choose the amount of detail displayed on the run")
         self.model = self.GetValue('model', help='Minimal path model to be solved.')
+        # _Interface.py:51-55: the extended curvature models are the standard
+        # ones; the Convex prefix is a variant of the scheme
+        if self.model == 'ElasticaExt2_5':
+            self.model = 'ElasticaExt2'
+        if self.model.endswith('Ext2'):
+            self.model = self.model[:-4] + '2'
+        self.convex = self.model.startswith('Convex')
+        if self.convex:
+            self.model = self.model[6:]
         self.ndim = len(hfmIn['dims'])
         self.stats = {key: {} for key in ('eikonal', 'flow', 'reverseAD')}
 
@@ -154,8 +163,8 @@ class Interface:
         if self.verbosity >= 1:
             print("Setting the kernel traits.")
         self.curvature = self.model_ in _CURVATURE
-        if self.model.startswith('Convex'):
-            raise NotImplementedError("the convex variants of the curvature models are not provided")
+        if self.convex and self.model not in ('ReedsSheppForward2', 'Elastica2', 'Dubins2'):
+            raise ValueError(f"Unsupported model Convex{self.model}")
         phys = int(self.model[-1]) if self.model[-1].isdigit() else -1
         if self.model_ not in _MODELS or self.ndim not in (2, 3) or \
                 (phys != self.ndim and not (self.curvature and phys == 2 and self.ndim == 3)):
@@ -717,6 +726,7 @@ class Interface:
         d.order = int(getattr(self, 'order', 1))
         d.order2_threshold = float(getattr(self, 'order2_threshold', 0.3))
         d.curvature_relax = float(getattr(self, 'curvature_relax', 0.))
+        d.convex = int(getattr(self, 'convex', False))
         if getattr(self, 'bound_active_blocks', 0):
             d.bound_active_blocks = int(self.bound_active_blocks)
             d.minchg_delta_min = float(self.minChg_delta_min)
@@ -806,6 +816,39 @@ class Interface:
         t0 = time.time()
         status = self.solver_handle().solve(desc, io, st)
         totals = st.as_dict()
+
+        def resume(trig):
+            # continue the solve from the tiles of the marked cells
+            d2, io2, st2 = self._desc(), self._io(), _lib.EikonalStats()
+            d2.activate_all = 0
+            io2.trigger = ptr(trig)
+            s = self.solver_handle().solve(d2, io2, st2)
+            for k in ('niter_o', 'tile_updates', 'cell_updates', 'launches', 'gpu_ms', 'kernel_ms'):
+                totals[k] += getattr(st2, k)
+            totals['converged'] = st2.converged
+            return s
+
+        # The curvature models' stencils reach two tiles away, further than
+        # the face neighbours a changed tile re-lists: cells left above
+        # their update are found by a residual pass and the solve resumes
+        # from them (no counterpart in the reference).
+        self.convergence_check = bool(self.GetValue(
+            'convergence_check', default=self.curvature,
+            help="After the solve, find the cells left above their scheme update and resume from them"))
+        if self.convergence_check and status == _lib.AGD_OK:
+            trig = zeros(((self.size_tot + 31) // 32,), np.int32)
+            rounds = 0
+            while rounds < 8:
+                trig.zero_()
+                nstale = self.solver_handle().check_stale(self._desc(), self._io(), self.atol, self.rtol, ptr(trig))
+                if nstale == 0:
+                    break
+                rounds += 1
+                status = resume(trig)
+                if status != _lib.AGD_OK:
+                    break
+            totals['convergence_check_rounds'] = rounds
+            del trig
         if getattr(self, 'hasChart', False):
             scratch = empty((self.size_tot,), self.float_t)
             trig = zeros(((self.size_tot + 31) // 32,), np.int32)
@@ -815,13 +858,7 @@ class Interface:
                 npaste += 1
                 if self._chart_paste(scratch, trig, cnt) == 0:
                     break
-                d2, io2, st2 = self._desc(), self._io(), _lib.EikonalStats()
-                d2.activate_all = 0
-                io2.trigger = ptr(trig)
-                status = self.solver_handle().solve(d2, io2, st2)
-                for k in ('niter_o', 'tile_updates', 'cell_updates', 'launches', 'gpu_ms', 'kernel_ms'):
-                    totals[k] += getattr(st2, k)
-                totals['converged'] = st2.converged
+                status = resume(trig)
             totals['chart_pastes'] = npaste
             del scratch, trig, cnt
         wall = time.time() - t0

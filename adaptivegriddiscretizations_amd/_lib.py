@@ -35,7 +35,7 @@ class EikonalDesc(ctypes.Structure):
         ("atol", c_double), ("rtol", c_double), ("nitermax_o", c_int64),
         ("iso_weights", c_double * 3), ("check_interval", c_int32), ("activate_all", c_int32),
         ("time_launches", c_int32), ("use_graph", c_int32),
-        ("order", c_int32), ("reserved", c_int32), ("order2_threshold", c_double),
+        ("order", c_int32), ("convex", c_int32), ("order2_threshold", c_double),
         ("factor_radius2", c_double), ("factor_origin", c_double * 3), ("factor_metric", c_double * 14),
         ("bound_active_blocks", c_int64), ("minchg_delta_min", c_double),
         ("curvature_relax", c_double),
@@ -98,6 +98,8 @@ SIGNATURES = {
                                            POINTER(EikonalStats)]),
     "agd_solver_check": (c_int32, [c_void_p, POINTER(EikonalDesc), POINTER(EikonalIO), c_double,
                                    c_double, POINTER(c_int64)]),
+    "agd_solver_check_stale": (c_int32, [c_void_p, POINTER(EikonalDesc), POINTER(EikonalIO), c_double,
+                                         c_double, c_void_p, POINTER(c_int64)]),
     "agd_solver_repair": (c_int32, [c_void_p, POINTER(EikonalDesc), POINTER(EikonalIO), c_double,
                                     c_double, c_int32, POINTER(c_int64), POINTER(c_int32)]),
     "agd_eikonal_solve": (c_int32, [POINTER(EikonalDesc), POINTER(EikonalIO),
@@ -259,6 +261,12 @@ class Solver:
     def check_residual(self, desc, io, atol, rtol):
         n = c_int64(0)
         check(lib.agd_solver_check(self.handle, desc, io, atol, rtol, ctypes.byref(n)))
+        return int(n.value)
+
+    def check_stale(self, desc, io, atol, rtol, stale_bits):
+        """Cells left above their scheme update, marked in stale_bits."""
+        n = c_int64(0)
+        check(lib.agd_solver_check_stale(self.handle, desc, io, atol, rtol, stale_bits, ctypes.byref(n)))
         return int(n.value)
 
     def repair(self, desc, io, atol, rtol, max_rounds):

@@ -827,8 +827,12 @@ constexpr int kNFejer = 5;  // nFejer_macro of Elastica2 (kernel_traits.py:30-33
 // Stencil of a curvature model: NMIX sub-schemes of NSYM offset slots each
 // (slot k of mix m at m * NSYM + k; the last NFWD slots are forward
 // offsets).  Returns mix_is_min (Dubins2: the smaller of its two updates).
+// convex: the Convex variants, whose vehicle turns left only (Dubins2: the
+// right turn becomes straight motion; Elastica2: the nodes of right turns
+// weigh 0, the straight one half; ReedsSheppForward2: the rotation slot is
+// forward, which the neighbour codes express).
 template <class S, int MODEL>
-__device__ inline bool curv_stencil(const S* g, S relax, S* w, int (*e)[3]) {
+__device__ inline bool curv_stencil(const S* g, S relax, bool convex, S* w, int (*e)[3]) {
 #pragma clang fp contract(off)
   const S ixi = g[0], kappa = g[1], cT = g[2], sT = g[3];
   if constexpr (MODEL == AGD_REEDSSHEPP2) {
@@ -864,7 +868,7 @@ __device__ inline bool curv_stencil(const S* g, S relax, S* w, int (*e)[3]) {
   } else if constexpr (MODEL == AGD_DUBINS2) {
     // Dubins2.h:17-36: left and right turns at the largest curvature
     const S vl[3] = {cT, sT, kappa + ixi};
-    const S vr[3] = {cT, sT, kappa - ixi};
+    const S vr[3] = {cT, sT, convex ? kappa : kappa - ixi};
     decomp_v<S>(vl, relax, w, e);
     decomp_v<S>(vr, relax, w + 6, e + 6);
     return true;
@@ -874,6 +878,7 @@ __device__ inline bool curv_stencil(const S* g, S relax, S* w, int (*e)[3]) {
     for (int l = 0; l < kNFejer; ++l) {
       S cP, sP, wF;
       fejer_node<S>(l, kNFejer, cP, sP, wF);
+      if (convex) wF *= 2 * l < kNFejer - 1 ? S(1) : (2 * l == kNFejer - 1 ? S(0.5) : S(0));
       const S v[3] = {sP * cT, sP * sT, sP * kappa + cP * ixi};
       decomp_v<S>(v, relax, w + 6 * l, e + 6 * l);
 #pragma unroll

@@ -301,6 +301,10 @@ static int check_desc(const agd_eikonal_desc* d) {
     AGD_REQUIRE(d->curvature_relax > 0., "curvature_relax must be > 0 (eps^2, 0.01 by default)");
     AGD_REQUIRE(d->factor_radius2 == 0., "source factorization is not provided for the curvature models");
     AGD_REQUIRE(d->order <= 1, "the curvature models use the first-order scheme");
+    AGD_REQUIRE(d->convex == 0 || d->model != AGD_REEDSSHEPP2,
+                "ReedsShepp2 has no Convex variant (ReedsSheppForward2, Elastica2, Dubins2 do)");
+  } else {
+    AGD_REQUIRE(d->convex == 0, "convex applies to the curvature models only");
   }
   AGD_REQUIRE(d->scalar == AGD_F32 || d->scalar == AGD_F64, "unknown scalar type");
   for (int k = 0; k < d->ndim; ++k) {
@@ -723,13 +727,15 @@ template <class S> static FlowArgs<S> flow_args(const agd_eikonal_desc* d, const
   a.order2_thr = S(d->order2_threshold);
   a.fac = make_factor<S>(d);
   a.relax = S(d->curvature_relax);
+  a.convex = d->convex;
   return a;
 }
 
 template <class S>
 static int flow_impl(const agd_eikonal_desc* d, const agd_eikonal_io* io, const agd_flow_io* f,
                      hipStream_t stream, int32_t* violations = nullptr, double chk_atol = 0.,
-                     double chk_rtol = 0., bool repair = false) {
+                     double chk_rtol = 0., bool repair = false, uint32_t* stale_bits = nullptr,
+                     bool stale = false) {
   KernelSet<S> ks;
   AGD_REQUIRE(lookup<S>(d->model, d->ndim, d->shape_i, ks), "unsupported tile shape for this model");
   AGD_REQUIRE(ks.flow, "tiles of fewer than 64 cells are for batched solves only");
@@ -745,6 +751,8 @@ static int flow_impl(const agd_eikonal_desc* d, const agd_eikonal_io* io, const 
   a.chk_atol = S(chk_atol);
   a.chk_rtol = S(chk_rtol);
   a.repair = repair ? 1 : 0;
+  a.stale = stale ? 1 : 0;
+  a.stale_bits = stale_bits;
   hipLaunchKernelGGL(ks.flow, dim3(g.ntiles), dim3(ks.ni), 0, stream, a);
   AGD_HIP_TRY(hipGetLastError());
   return AGD_OK;
@@ -919,6 +927,7 @@ static int scheme_impl(const agd_eikonal_desc* d, const agd_eikonal_io* io, hipS
   a.scode2 = (d->model == AGD_TTI && d->order == 2) ? io->scheme_codes + tti_codes2_offset(d) * g.size_tot
                                                      : nullptr;
   a.relax = S(d->curvature_relax);
+  a.convex = d->convex;
   hipLaunchKernelGGL((io->wall_dist || g.periodic) ? ks.scheme_general : ks.scheme, dim3(g.ntiles),
                      dim3(ks.ni), 0, stream, a);
   AGD_HIP_TRY(hipGetLastError());
@@ -965,15 +974,18 @@ static int batched_entry(Workspace& ws, const agd_eikonal_desc* desc, int32_t ni
   return solve_batched_impl<double>(ws, desc, ninst, instances, io, stats, stream);
 }
 
-// Residual check with the counter `cnt` (one int32 of device memory).
+// Residual check with the counter `cnt` (one int32 of device memory); with
+// stale, the converse test (cells above their update), marked in stale_bits.
 static int check_entry(int32_t* cnt, const agd_eikonal_desc* desc, const agd_eikonal_io* io,
-                       double atol, double rtol, int64_t* violations, hipStream_t s) {
+                       double atol, double rtol, int64_t* violations, hipStream_t s,
+                       bool stale = false, uint32_t* stale_bits = nullptr) {
   int st = check_desc(desc);
   if (st) return st;
   AGD_REQUIRE(violations, "null output");
   fill_i32(cnt, 1, 0, s);
-  st = desc->scalar == AGD_F32 ? flow_impl<float>(desc, io, nullptr, s, cnt, atol, rtol)
-                               : flow_impl<double>(desc, io, nullptr, s, cnt, atol, rtol);
+  st = desc->scalar == AGD_F32
+           ? flow_impl<float>(desc, io, nullptr, s, cnt, atol, rtol, false, stale_bits, stale)
+           : flow_impl<double>(desc, io, nullptr, s, cnt, atol, rtol, false, stale_bits, stale);
   if (st) return st;
   int32_t n = 0;
   AGD_HIP_TRY(hipMemcpyAsync(&n, cnt, sizeof(int32_t), hipMemcpyDeviceToHost, s));
@@ -1132,6 +1144,15 @@ int agd_solver_check(agd_solver* s, const agd_eikonal_desc* desc, const agd_eiko
   st = s->ws.ensure(1, 1, 1);
   if (st) return st;
   return check_entry(s->ws.check_count, desc, io, atol, rtol, violations, s->stream);
+}
+
+int agd_solver_check_stale(agd_solver* s, const agd_eikonal_desc* desc, const agd_eikonal_io* io,
+                           double atol, double rtol, uint32_t* stale_bits, int64_t* stale) {
+  int st = check_handle(s);
+  if (st) return st;
+  st = s->ws.ensure(1, 1, 1);
+  if (st) return st;
+  return check_entry(s->ws.check_count, desc, io, atol, rtol, stale, s->stream, true, stale_bits);
 }
 
 int agd_solver_repair(agd_solver* s, const agd_eikonal_desc* desc, const agd_eikonal_io* io,

@@ -554,6 +554,11 @@ __global__ void __launch_bounds__(block_lanes<T0 * T1 * T2>()) eik_update(EikArg
             }
             const int sd = (side[0] > side[1]) ? 1 : 0;  // Greater(v,-) keeps the -e side on ties
             v[k] = side[sd];
+            // curvature models: an offset of weight 0 (pruned, or a Convex
+            // variant's right turn) takes no part in the update; left in, a
+            // much smaller value of it would set the base of the quadratic
+            // and cancel its FP32 digits
+            if constexpr (MT::CURV) v[k] = w[k] > S(0) ? v[k] : inf_v<S>();
             if (a.scode2) {
               // the further neighbour on the chosen side (second order)
               const int c2 = ld32(a.scode2 + int64_t(m * NJ + 2 * k + sd) * N, uint32_t(n_t));
@@ -715,6 +720,7 @@ template <class S> struct SchemeArgs {
   int32_t* scode;
   int32_t* scode2;  // TTI, second order: codes of the further neighbours x -+ 2 e_k, or null
   S relax;          // curvature models: relaxation of the outer products (eps^2)
+  int32_t convex;   // curvature models: the Convex variant
 };
 
 template <class S, int MODEL, int D, int T0, int T1, int T2, bool GENERAL>
@@ -748,7 +754,7 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_scheme(SchemeArgs<S> a) {
     // slots with their x - e code set to -1, then the min/max flag.
     S cw[MT::NMIX * NS];
     int ce[MT::NMIX * NS][3];
-    const bool mix_min = curv_stencil<S, MODEL>(geo, a.relax, cw, ce);
+    const bool mix_min = curv_stencil<S, MODEL>(geo, a.relax, a.convex != 0, cw, ce);
 #pragma unroll 1
     for (int m = 0; m < MT::NMIX; ++m) {
       Stencil<S, MODEL, D> st;
@@ -763,6 +769,7 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_scheme(SchemeArgs<S> a) {
       resolve_codes<S, MODEL, D, TL, true>(st, xi, xt, xo, wx, wall_s, a.wall, a.g, code, 0, nullptr);
 #pragma unroll
       for (int k = NS - MT::NFWD; k < NS; ++k) code[2 * k] = -1;
+      if (MODEL == AGD_REEDSSHEPPFWD2 && a.convex) code[0] = -1;  // rotation to the left only
 #pragma unroll
       for (int k = 0; k < NS; ++k) a.sw[int64_t(m * NS + k) * N + n_t] = st.w[k];
 #pragma unroll
@@ -910,8 +917,14 @@ template <class S> struct FlowArgs {
   // +inf is above the fixed point; repeated until nothing is counted, this
   // turns a start into a supersolution of the scheme).
   int32_t repair;
+  // stale: count instead the cells whose update lies below their value by
+  // more than the tolerance (a value the solve left above the fixed point),
+  // and set their bits in stale_bits when given.
+  int32_t stale;
+  uint32_t* stale_bits;
   FactorDesc<S> fac;
   S relax;  // curvature models: relaxation of the outer products (eps^2)
+  int32_t convex;  // curvature models: the Convex variant
 };
 
 // The upwind update of one cell and its flow (Update.h:149-209, the body
@@ -990,7 +1003,8 @@ __device__ inline void flow_cell(const FlowArgs<S>& a, const int* xt, int64_t n_
         const int sg = 2 * s - 1;
         int e[3] = {sg * st.e[k][0], sg * st.e[k][1], sg * st.e[k][2]};
         // a forward offset (the last NFWD slots) has no x - e neighbour
-        const bool none = MT::NFWD > 0 && k >= NS - MT::NFWD && s == 0;
+        const bool none = (MT::NFWD > 0 && k >= NS - MT::NFWD && s == 0) ||
+                          (MODEL == AGD_REEDSSHEPPFWD2 && a.convex && k == 0 && s == 0);
         bool vis = !none;
         if (vis && a.wall && wx != 255) {
           vis = visible<D>(e, wx, [&](const int* ww) -> int {
@@ -1044,7 +1058,7 @@ __device__ inline void flow_cell(const FlowArgs<S>& a, const int* xt, int64_t n_
     S cw[MT::CURV ? MT::NMIX * NS : 1];
     int ce[MT::CURV ? MT::NMIX * NS : 1][3];
     if constexpr (MT::CURV) {
-      mix_min = curv_stencil<S, MODEL>(geo, a.relax, cw, ce);
+      mix_min = curv_stencil<S, MODEL>(geo, a.relax, a.convex != 0, cw, ce);
     } else {
       mix_min = tti_diags<S>(geo, geo + 2, diag);
       tti_base<S, D>(geo, D0, D1);
@@ -1071,7 +1085,12 @@ __device__ inline void flow_cell(const FlowArgs<S>& a, const int* xt, int64_t n_
       gather(cand, cv, cw, cs, cn, a.order2 ? cv2 : nullptr);
       S vs[NS], ws[NS];
 #pragma unroll
-      for (int k = 0; k < NS; ++k) { vs[k] = cv[k]; ws[k] = cw[k]; }
+      for (int k = 0; k < NS; ++k) {
+        // (curvature models: weight-0 offsets left out, as in the update)
+        if constexpr (MT::CURV) cv[k] = cw[k] > S(0) ? cv[k] : inf_v<S>();
+        vs[k] = cv[k];
+        ws[k] = cw[k];
+      }
       unsigned cmask = 0;
       if (a.order2) {
         cmask = order2_correct<S, NS>(u_old, a.order2_thr, vs, cv2, ws);
@@ -1148,7 +1167,9 @@ __global__ void __launch_bounds__(T0* T1* T2) eik_flow(FlowArgs<S> a) {
   const S sum = r.sum;
   const S* vec = r.vec;
   if (a.violations) {
-    const bool bad = !seed && val > u_old + a.chk_atol + a.chk_rtol * abs_(u_old);
+    const S tol = a.chk_atol + a.chk_rtol * abs_(u_old);
+    const bool bad = !seed && (a.stale ? val < u_old - tol : val > u_old + tol);
+    if (bad && a.stale_bits) atomicOr(a.stale_bits + (n_t >> 5), 1u << (n_t & 31));
     const uint64_t m = __ballot(bad);
     if ((threadIdx.x & 63) == 0 && m) atomicAdd(a.violations, int(__popcll(m)));
     if (a.repair) {

@@ -88,7 +88,9 @@ typedef struct agd_eikonal_desc {
   int32_t use_graph;      /* 1: capture check_interval passes once as a hipGraph and replay it */
   int32_t order;          /* 1 (or 0) or 2: second-order upwind differences
                              where the solution is smooth (HFM.h:91-134) */
-  int32_t reserved;       /* 0                                           */
+  int32_t convex;         /* curvature models ReedsSheppForward2, Elastica2,
+                             Dubins2: 1 for their Convex variants, whose
+                             vehicle turns left only (kernel_traits.py:129-130) */
   double order2_threshold;/* order 2: |u-2v+v2| < threshold * max(u-v) (0.3) */
   /* Source factorization (EuclideanFactor.h:8-50, Constants.h:50-61,
    * TTI_.h:139-212): the cells with |x - factor_origin|^2 < factor_radius2
@@ -205,6 +207,15 @@ int agd_solver_solve_batched(agd_solver* solver, const agd_eikonal_desc* desc, i
                              agd_eikonal_stats* stats);
 int agd_solver_check(agd_solver* solver, const agd_eikonal_desc* desc, const agd_eikonal_io* io,
                      double atol, double rtol, int64_t* violations);
+/* Cells a solve left above their scheme update (no counterpart in the
+ * reference, whose schedule has the same gap): counts the non-seed cells whose
+ * update lies below their value by more than atol + rtol |u|, and sets their
+ * bits in stale_bits (tiled, 1 bit per cell; accumulated into, not cleared).
+ * A solve with io->trigger = stale_bits continues from them.  A changed tile
+ * re-lists only its face neighbours (Propagation.h:18-35), so a stencil that
+ * reaches two tiles away (the curvature models') can leave such cells behind. */
+int agd_solver_check_stale(agd_solver* solver, const agd_eikonal_desc* desc, const agd_eikonal_io* io,
+                           double atol, double rtol, uint32_t* stale_bits, int64_t* stale);
 /* Turns io->values into a supersolution of the scheme before a warm-started
  * solve (no counterpart in the reference): each round counts the non-seed
  * cells whose scheme update exceeds their value by more than atol + rtol |u|

@@ -471,9 +471,14 @@ static void fejer(int l, int n, double& c, double& s, double& w) {
 }
 
 // Scheme of a curvature model at a cell with geometry g = [1/xi, kappa,
-// cos theta, sin theta] (grid units); relax = eps^2.
+// cos theta, sin theta] (grid units); relax = eps^2.  convex: the Convex
+// variants (kernel_traits.py:129-130), whose vehicle turns left only:
+// ReedsSheppForward2's rotation becomes a forward offset
+// (ReedsSheppForward2.h:10-12), Dubins2's right turn straight motion
+// (Dubins2.h:29-33), Elastica2's nodes weigh 1, 1/2, 0 for left turns,
+// straight motion and right turns (Elastica2.h:69-73).
 template <class S>
-static void curvature_scheme(int model, const S* g, S relax, Stencil<S>& st) {
+static void curvature_scheme(int model, const S* g, S relax, bool convex, Stencil<S>& st) {
   const S ixi = g[0], kappa = g[1], cT = g[2], sT = g[3];
   if (model == REEDSSHEPP2) {
     // ReedsShepp2.h:13-36: the angular coefficient raised to kappa^2 + 1/xi^2,
@@ -503,7 +508,7 @@ static void curvature_scheme(int model, const S* g, S relax, Stencil<S>& st) {
   } else if (model == DUBINS2) {
     // Dubins2.h:17-36: turning left or right at curvature 1/xi, the smaller
     // of the two updates
-    const S vl[3] = {cT, sT, kappa + ixi}, vr[3] = {cT, sT, kappa - ixi};
+    const S vl[3] = {cT, sT, kappa + ixi}, vr[3] = {cT, sT, convex ? kappa : kappa - ixi};
     decomp_v<S>(vl, relax, st.w, st.e);
     decomp_v<S>(vr, relax, st.w + 6, st.e + 6);
   } else {
@@ -513,7 +518,9 @@ static void curvature_scheme(int model, const S* g, S relax, Stencil<S>& st) {
     for (int l = 0; l < kNFejer; ++l) {
       double c, sn, wf;
       fejer(l, kNFejer, c, sn, wf);
-      const S cP = S(c), sP = S(sn), wF = S(wf);
+      const S cP = S(c), sP = S(sn);
+      S wF = S(wf);
+      if (convex) wF *= 2 * l < kNFejer - 1 ? S(1) : (2 * l == kNFejer - 1 ? S(0.5) : S(0));
       const S v[3] = {sP * cT, sP * sT, sP * kappa + cP * ixi};
       decomp_v<S>(v, relax, st.w + 6 * l, st.e + 6 * l);
       for (int i = 0; i < 6; ++i) st.w[6 * l + i] *= wF;
@@ -522,6 +529,7 @@ static void curvature_scheme(int model, const S* g, S relax, Stencil<S>& st) {
 }
 
 static double g_curvature_relax = 0.01;  // eps^2 (eps = 0.1, _Kernel.py:319-321)
+static bool g_curvature_convex = false;  // the Convex variants
 
 template <class S>
 static void make_stencil(int model, int d, const S* g, const S* iso_w, Stencil<S>& st) {
@@ -536,7 +544,8 @@ static void make_stencil(int model, int d, const S* g, const S* iso_w, Stencil<S
   }
   st.drift[0] = st.drift[1] = st.drift[2] = 0;
   if (curvature(model)) {
-    curvature_scheme<S>(model, g, S(g_curvature_relax), st);
+    curvature_scheme<S>(model, g, S(g_curvature_relax), g_curvature_convex, st);
+    if (g_curvature_convex && model == REEDSSHEPPFWD2) st.nfwd = st.nsym;  // the rotation too
     return;
   }
   if (model == TTI) {
@@ -879,9 +888,15 @@ static S cell_update(const Problem& P, const Stencil<S>& st, int64_t n, Get get,
     }
     return val;
   }
+  // curvature models: an offset of weight 0 takes no part in the update (a
+  // much smaller value of it would otherwise set the base of the quadratic
+  // and cancel its FP32 digits); exact arithmetic gives the same value
+  const bool drop0 = curvature(P.model);
   if (st.nmix == 1) {
     S v[kMaxAct];
     neighbours<S>(P, st, 0, n, get, v, side, nidx);
+    if (drop0)
+      for (int k = 0; k < ns; ++k) v[k] = st.w[k] > S(0) ? v[k] : Inf<S>();
     return hfm_solve<S>(ns, st.w, v, rhs, fw);
   }
   S best = st.mix_is_min ? Inf<S>() : -Inf<S>();
@@ -891,6 +906,8 @@ static S cell_update(const Problem& P, const Stencil<S>& st, int64_t n, Get get,
     int64_t ni[kMaxAct];
     neighbours<S>(P, st, m * ns, n, get, v, sd, ni, P.order == 2 ? v2 : nullptr);
     for (int k = 0; k < ns; ++k) w[k] = st.w[m * ns + k];
+    if (drop0)
+      for (int k = 0; k < ns; ++k) v[k] = w[k] > S(0) ? v[k] : Inf<S>();
     bool o2[kMaxAct];
     for (int k = 0; k < kMaxAct; ++k) o2[k] = false;
     if (P.order == 2) {
@@ -1082,7 +1099,8 @@ static int64_t tiles_omp(const Problem& P, int tile, int niter, double atol, dou
           for (int s2 = -1; s2 <= 1; s2 += 2) {
             int64_t y[3] = {to[0], to[1], to[2]};
             y[c] += s2;
-            if (y[c] < 0 || y[c] >= so[c]) continue;
+            if (P.G.periodic[c]) y[c] = (y[c] + so[c]) % so[c];  // Propagation.h:25-28
+            else if (y[c] < 0 || y[c] >= so[c]) continue;
             int64_t tn = 0;
             for (int k = 0; k < d; ++k) tn = tn * so[k] + y[k];
             __atomic_store(&next[tn], &one, __ATOMIC_RELAXED);
@@ -1130,6 +1148,9 @@ int orc_version() { return 8; }
 
 // Relaxation eps^2 of the curvature models' self outer products.
 void orc_set_curvature_relax(double relax) { g_curvature_relax = relax; }
+
+// 1: the Convex variants of ReedsSheppForward2, Elastica2 and Dubins2.
+void orc_set_curvature_convex(int convex) { g_curvature_convex = convex != 0; }
 
 // Node l of the n-point Fejer rule: out = (cos phi_l, sin phi_l, weight).
 void orc_fejer(int l, int n, double* out) { fejer(l, n, out[0], out[1], out[2]); }

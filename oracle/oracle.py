@@ -75,6 +75,7 @@ def _load():
     lib.orc_set_factor.argtypes = [c_double, P, P]
     lib.orc_set_curvature_relax.argtypes = [c_double]
     lib.orc_fejer.argtypes = [c_int, c_int, P]
+    lib.orc_set_curvature_convex.argtypes = [c_int]
     lib.orc_tti_factor.argtypes = [c_int, c_int, P, c_int64, P, P, P, P, P, P]
     lib.orc_wall_distance.argtypes = [c_int, P, P, P, c_int, P]
     lib.orc_visible.argtypes = [c_int, P, P, P, c_int64, P, P, P]
@@ -332,6 +333,17 @@ class curvature_relax:
 
     def __exit__(self, *exc):
         lib().orc_set_curvature_relax(0.01)
+
+
+class convex:
+    """Context: the Convex variants of the curvature models (left turns only)."""
+
+    def __enter__(self):
+        lib().orc_set_curvature_convex(1)
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_set_curvature_convex(0)
 
 
 def fejer(n=5):

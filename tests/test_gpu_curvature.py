@@ -3,6 +3,8 @@ Dubins2, Elastica2 on (x, y, theta) grids, theta periodic): the precomputed
 scheme bit for bit against the oracle on the same FP32 geometry, the solves
 against the oracle's fast marching, the flow of the forward offsets, and
 the geodesics and unoriented seeds and tips of the driver."""
+import contextlib
+
 import numpy as np
 import pytest
 
@@ -16,17 +18,30 @@ CURV = {'ReedsShepp2': (6, 0, 1), 'ReedsSheppForward2': (7, 6, 1), 'Dubins2': (6
 TOL32 = 1e-4
 
 
+def base(model):
+    """Model name without the Convex prefix, and the oracle's context for it."""
+    return (model[6:], True) if model.startswith('Convex') else (model, False)
+
+
+def variant(O, model):
+    return O.convex() if base(model)[1] else contextlib.nullcontext()
+
+
 def oracle_from_device(O, it, wl):
-    """Oracle problem on the device's own FP32 geometry and rhs."""
-    geom = cell_major(it, it.geom).astype(np.float64)
-    cost = np.asarray(it.cost, dtype=np.float32).astype(np.float64)
+    """Oracle problem on the device's own geometry and rhs (FP32 or FP64)."""
+    import torch
+    if it.geom.dtype == torch.float64:
+        geom = it._squeeze(it.geom.contiguous(), 4, np.float64).cpu().numpy()
+    else:
+        geom = cell_major(it, it.geom).astype(np.float64)
+    cost = np.asarray(it.cost, dtype=it.float_t).astype(np.float64)
     rhs = np.broadcast_to(cost, wl.dims).copy()
     seeds = np.zeros(wl.dims, np.uint8)
     idx = np.asarray(it.seedIndices)
     for s, v in zip(idx, it.seedValues):
         seeds[tuple(s)] = 1
         rhs[tuple(s)] = v
-    return O.Problem(wl.model, wl.dims, geom.reshape(4, -1), rhs, seeds, [0., 0., 0.],
+    return O.Problem(base(wl.model)[0], wl.dims, geom.reshape(4, -1), rhs, seeds, [0., 0., 0.],
                      periodic=[0, 0, 1]), idx
 
 
@@ -65,13 +80,16 @@ def code_mismatches(it, wl, sc, e_ref, ns, nfwd, nmix):
     return nbad
 
 
-@pytest.mark.parametrize("model", list(CURV))
+@pytest.mark.parametrize("model", list(CURV) + ['ConvexReedsSheppForward2', 'ConvexDubins2', 'ConvexElastica2'])
 def test_curvature_scheme_matches_oracle(O, model):
     wl = workloads.curvature2(model, n=24, ntheta=16, xi=0.4, kappa=0.3, seed=31)
     it = prepared(wl)
     P, _ = oracle_from_device(O, it, wl)
-    ns, nfwd, nmix = CURV[model]
-    w_ref, e_ref, mix_ref = P.stencils(use_float=True, with_mix=True)
+    ns, nfwd, nmix = CURV[base(model)[0]]
+    if model == 'ConvexReedsSheppForward2':
+        nfwd = ns                               # the rotation is a forward offset too
+    with variant(O, model):
+        w_ref, e_ref, mix_ref = P.stencils(use_float=True, with_mix=True)
     N = int(np.prod(wl.dims))
     sw = cell_major(it, it.scheme_w_t).reshape(nmix * ns, N).T
     sc = cell_major(it, it.scheme_c_t).reshape(nmix * 2 * ns + 1, N).T
@@ -85,18 +103,45 @@ def test_curvature_scheme_matches_oracle(O, model):
 
 
 @pytest.mark.parametrize("model,n,nt", [('ReedsShepp2', 48, 32), ('ReedsSheppForward2', 48, 32),
-                                        ('Dubins2', 40, 32), ('Elastica2', 32, 24)])
+                                        ('Dubins2', 40, 32), ('Elastica2', 32, 24),
+                                        ('ConvexReedsSheppForward2', 40, 32), ('ConvexDubins2', 40, 32),
+                                        ('ConvexElastica2', 32, 24)])
 def test_curvature_solve_matches_oracle(O, model, n, nt):
     wl = workloads.curvature2(model, n=n, ntheta=nt, xi=0.4, seed=32)
     out = wl.dict_in(exportValues=True).Run()
     u = out['values'].cpu().numpy().astype(np.float64)
     it = prepared(wl)
     P, _ = oracle_from_device(O, it, wl)
-    ref = P.solve_fm()
+    with variant(O, model):
+        ref = P.solve_fm()
     err = rel_err(u, ref)
+    fin = np.isfinite(ref)
+    core = fin & (ref <= 10 * np.median(ref[fin]))     # away from the far, barely reachable states
+    err_core = np.max(np.abs(u - ref)[core]) / np.max(ref[core])
     log(f"{model} {n}x{n}x{nt}: passes {out['stats']['eikonal']['launches']}, "
-        f"finite {int(np.isfinite(ref).sum())}/{ref.size}, rel err vs oracle FM {err:.2e}")
-    assert err <= TOL32
+        f"residual rounds {out['stats']['eikonal'].get('convergence_check_rounds')}, "
+        f"finite {int(fin.sum())}/{ref.size}, rel err vs oracle FM {err:.2e} (core {err_core:.2e})")
+    assert err_core <= 1e-5
+    # ConvexElastica2 reaches states ~1000x the median distance through offsets
+    # of weight ~1e-5; there the FP32 update loses digits: the oracle's own FP32
+    # iteration is 1.7e-4 of the maximum from its FP64 fixed point (the FP64
+    # device solve is checked to 1e-9 below)
+    assert err <= (5e-4 if model == 'ConvexElastica2' else TOL32)
+
+
+@pytest.mark.parametrize("model", ['Elastica2', 'ConvexElastica2', 'ConvexDubins2'])
+def test_curvature_fp64_solve_matches_oracle(O, model):
+    wl = workloads.curvature2(model, n=32, ntheta=24, xi=0.4, seed=32)
+    extra = dict(traits={'Scalar': np.float64}, atol=1e-12, rtol=0.)
+    out = wl.dict_in(exportValues=True, **extra).Run()
+    u = out['values'].cpu().numpy().astype(np.float64)
+    it = prepared(wl, **extra)
+    P, _ = oracle_from_device(O, it, wl)
+    with variant(O, model):
+        ref = P.solve_fm()
+    err = rel_err(u, ref)
+    log(f"{model} FP64 32x32x24: rel err vs oracle FM {err:.2e}")
+    assert err <= 1e-9
 
 
 @pytest.mark.parametrize("model", ['ReedsSheppForward2', 'Dubins2'])

@@ -234,3 +234,50 @@ def test_flow_weights_of_forward_offsets(O, model):
         y = cells[live] + e[live, k]
         y[:, 2] %= shape[2]
         np.testing.assert_array_equal(np.ravel_multi_index(tuple(y.T), P.shape), fl['indices'][live, k])
+
+
+def test_convex_variants(O):
+    """Convex variants (left turns only, kernel_traits.py:129-130): the
+    rotation of ReedsSheppForward2 turns left only, so turning right by k
+    cells costs (n_theta - k) xi h_theta; the right-turn mix of Dubins2 is
+    straight motion and the Elastica2 nodes weigh 1, 1/2, 0 (Hamiltonians as
+    above); fast marching equals the iteration."""
+    xi = 0.3
+    with O.convex():
+        P, h, c = grid_problem(O, 'ReedsSheppForward2', xi=xi)
+        u = P.solve_fm()
+        nt = P.shape[2]
+        k = np.arange(1, 8)
+        np.testing.assert_allclose(u[c, c, k], k * h[2] * xi, rtol=1e-13)
+        np.testing.assert_allclose(u[c, c, nt - k], (nt - k) * h[2] * xi, rtol=1e-13)
+        np.testing.assert_allclose(u[c + k, c, 0], k * h[0], rtol=1e-13)
+        rng = np.random.default_rng(8)
+        n = 200
+        ixi, kappa, th = rng.uniform(0.3, 2., n), rng.uniform(-0.5, 0.5, n), rng.uniform(0, 2 * np.pi, n)
+        dub = cell_problem(O, 'Dubins2', ixi, kappa, th).stencils()
+        ela = cell_problem(O, 'Elastica2', ixi, kappa, th).stencils()
+        for model in ('Dubins2', 'Elastica2', 'ReedsSheppForward2'):
+            Pm, _, _ = grid_problem(O, model, n=15, nt=16, xi=0.5)
+            a = Pm.solve_fm()
+            b, _ = Pm.solve_jacobi(1e-13, 0.)
+            fin = np.isfinite(a)
+            assert np.array_equal(fin, np.isfinite(b))
+            assert np.max(np.abs(a[fin] - b[fin])) <= 1e-12 * np.max(a[fin])
+    p = rng.normal(size=(n, 3))
+    fej = O.fejer(5)
+    factor = [1., 1., 0.5, 0., 0.]
+    worst = 0.
+    for cc in range(n):
+        v = np.array([np.cos(th[cc]), np.sin(th[cc]), kappa[cc]])
+        vm = v.copy()                         # the second Dubins mix: straight ahead
+        h_d = forward_hamiltonian(dub[0][cc, 6:], dub[1][cc, 6:], p[cc])
+        worst = max(worst, abs(h_d - max(0., p[cc] @ vm) ** 2) / ((p[cc] @ p[cc]) * (vm @ vm)))
+        ref = 0.
+        for (cP, sP, wF), f in zip(fej, factor):
+            vl = np.array([sP * v[0], sP * v[1], sP * kappa[cc] + cP * ixi[cc]])
+            ref += f * wF * max(0., p[cc] @ vl) ** 2
+        h_d = forward_hamiltonian(ela[0][cc], ela[1][cc], p[cc])
+        worst = max(worst, abs(h_d - ref) / ((p[cc] @ p[cc]) * (v @ v + ixi[cc] ** 2)))
+    assert worst <= 2 * 0.01, worst
+    # the Elastica2 stencil of the right-turn nodes is empty
+    assert np.all(ela[0][:, 18:] == 0.)
