@@ -648,6 +648,9 @@ class Interface:
         self.forwardAD = self.cost_coef is not None or self.seed_coef is not None
         self.values_coef = None
         self.SetChart()
+        self.convergence_check = bool(self.GetValue(
+            'convergence_check', default=self.curvature,
+            help="After the solve, find the cells left above their scheme update and resume from them"))
 
         self.wallDist_t = None
         if self.walls is not None:
@@ -832,9 +835,6 @@ class Interface:
         # the face neighbours a changed tile re-lists: cells left above
         # their update are found by a residual pass and the solve resumes
         # from them (no counterpart in the reference).
-        self.convergence_check = bool(self.GetValue(
-            'convergence_check', default=self.curvature,
-            help="After the solve, find the cells left above their scheme update and resume from them"))
         if self.convergence_check and status == _lib.AGD_OK:
             trig = zeros(((self.size_tot + 31) // 32,), np.int32)
             rounds = 0

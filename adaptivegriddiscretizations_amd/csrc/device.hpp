@@ -899,6 +899,28 @@ __device__ inline void cex(S& vi, S& wi, S& vj, S& wj) {
   vi = a; vj = b; wi = c; wj = d;
 }
 
+// Batcher's odd-even merge sort network on the power of two P >= N, without
+// the comparators that touch a padding position (index >= N, holding +inf,
+// which every comparator would leave in place): 7 offsets take 16 comparators
+// instead of 21 for odd-even transposition, 30 offsets 178 instead of 435.
+constexpr int next_pow2(int n) { return n <= 1 ? 1 : 2 * next_pow2((n + 1) / 2); }
+template <int N> struct MergeNet {
+  static constexpr int P = next_pow2(N);
+  int n;
+  int a[P * 8], b[P * 8];
+  constexpr MergeNet() : n(0), a{}, b{} {
+    for (int p = 1; p < P; p <<= 1)
+      for (int k = p; k >= 1; k >>= 1)
+        for (int j = k % p; j <= P - 1 - k; j += 2 * k)
+          for (int i = 0; i <= (k - 1 < P - j - k - 1 ? k - 1 : P - j - k - 1); ++i)
+            if ((i + j) / (2 * p) == (i + j + k) / (2 * p) && i + j + k < N) {
+              a[n] = i + j;
+              b[n] = i + j + k;
+              ++n;
+            }
+  }
+};
+
 template <class S, int N> __device__ inline void sort_pairs(S (&v)[N], S (&w)[N]) {
   if constexpr (N == 1) {
     // nothing to sort
@@ -914,12 +936,11 @@ template <class S, int N> __device__ inline void sort_pairs(S (&v)[N], S (&w)[N]
     cex(v[2], w[2], v[4], w[4]); cex(v[1], w[1], v[3], w[3]);
     cex(v[2], w[2], v[3], w[3]);
   } else {
-    // other lengths (the curvature models' 7 and 30): odd-even
-    // transposition, N rounds of the same compare-exchange
+    // other lengths (the curvature models' 7 and 30)
+    constexpr MergeNet<N> net{};
+    constexpr int count = net.n;
 #pragma unroll
-    for (int r = 0; r < N; ++r)
-#pragma unroll
-      for (int i = r & 1; i + 1 < N; i += 2) cex(v[i], w[i], v[i + 1], w[i + 1]);
+    for (int c = 0; c < count; ++c) cex(v[net.a[c]], w[net.a[c]], v[net.b[c]], w[net.b[c]]);
   }
 }
 

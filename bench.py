@@ -184,7 +184,7 @@ def main():
     ap.add_argument('--warmup', type=int, default=2)
     ap.add_argument('--config', default='riemann3',
                     choices=['riemann3', 'riemann2', 'isotropic2', 'batch256', 'warmstart', 'adjoint',
-                             'conv'] + list(CURVATURE_CONFIGS))
+                             'conv', 'chart'] + list(CURVATURE_CONFIGS))
     ap.add_argument('--train', type=int, default=4096, help='warmstart: training fields')
     ap.add_argument('--test', type=int, default=512, help='warmstart: held-out fields')
     ap.add_argument('--epochs', type=int, default=12, help='warmstart: training epochs')
@@ -226,6 +226,8 @@ def main():
         return bench_adjoint(args)
     if args.config == 'conv':
         return bench_conv(args)
+    if args.config == 'chart':
+        return bench_chart(args)
     t0 = time.time()
     wl = make_workload(args.config)
     log(f"workload {wl.name} {wl.dims} generated in {time.time() - t0:.1f} s")
@@ -399,6 +401,126 @@ def bench_batch(args):
         "batching_speedup": seq / (wall / args.steps),
         "max_rel_diff_vs_single": worst,
     }
+    print(json.dumps(result), flush=True)
+
+
+def bench_chart(args):
+    """The projective plane of Notebooks_Repro/Chart_GPU.ipynb (section 1) on
+    a 2001^2 grid: the unit sphere with antipodes identified, parametrised
+    from the equator (cost 2 / (1 + |x|^2), chart mapping x -> -x / |x|^2),
+    one seed.  A step is the whole solve: the cold solve, then up to
+    chart_nitermax (5) pastes between the charts, each followed by the solve
+    resumed from the lowered cells.  The roofline is that of the paste
+    (agd_chart_paste), timed alone on the converged values: per cell, the
+    evaluation reads u, the 2 mapping components and writes the pasted value
+    (16 B), the update reads both values (8 B), algorithmic 24 B per cell."""
+    import numpy as np
+    import torch
+    from adaptivegriddiscretizations_amd import Eikonal
+    from adaptivegriddiscretizations_amd.Eikonal import HFM_GPU
+    from adaptivegriddiscretizations_amd.Eikonal._device import empty, zeros
+    n, r = 2001, 1.1
+    hfm = Eikonal.dictIn({'model': 'Isotropic2', 'seed': [0.5, 0.7], 'exportValues': True, 'verbosity': 0})
+    hfm.SetRect([[-r, r], [-r, r]], dimx=n)
+    X = np.stack(np.meshgrid(*[np.asarray(a, dtype=np.float64) for a in hfm.Axes()], indexing='ij'))
+    e2 = np.sum(X ** 2, axis=0)
+    hfm['cost'] = (2. / (1. + e2)).astype(np.float32)
+    with np.errstate(divide='ignore', invalid='ignore'):
+        hfm['chart_mapping'] = (-X / e2).astype(np.float32)
+    iface = HFM_GPU.Interface(hfm)
+    iface.SetKernelTraits()
+    iface.SetGeometry()
+    iface.SetArgs()
+
+    def step():
+        iface.values_t.fill_(float('inf'))
+        iface.Solve()
+        return iface.stats['eikonal']
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        stats.append(dict(step()))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    # the paste alone, on the converged values (it lowers nothing more)
+    scratch = empty((iface.size_tot,), iface.float_t)
+    trig = zeros(((iface.size_tot + 31) // 32,), np.int32)
+    cnt = zeros((1,), np.int32)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 20
+    iface._chart_paste(scratch, trig, cnt)
+    torch.cuda.synchronize()
+    ev0.record()
+    for _ in range(reps):
+        from adaptivegriddiscretizations_amd import _lib
+        from adaptivegriddiscretizations_amd.Eikonal._device import ptr
+        _lib.check(_lib.lib.agd_chart_paste(iface._desc(), ptr(iface.values_t), ptr(iface.chart_mapping_t),
+                                            iface.chart_ndim_s, ptr(scratch), ptr(trig), ptr(cnt), iface.stream))
+    ev1.record()
+    torch.cuda.synchronize()
+    paste_us = ev0.elapsed_time(ev1) * 1e3 / reps
+    bpc = 24.
+    achieved = n * n * bpc / (paste_us * 1e-6) / 1e9
+    u = iface.values.cpu().numpy().astype(np.float64)
+
+    def to_sphere(x):
+        s2 = np.sum(x ** 2, axis=0)
+        return np.concatenate([(1. - s2)[None], 2 * x]) / (1. + s2)
+
+    seed = to_sphere(np.array([0.5, 0.7]).reshape(2, 1, 1))
+    XS = to_sphere(X)
+    dist = lambda p, q: 2 * np.arcsin(np.linalg.norm(p - q, axis=0) / 2)
+    err = float(np.max(np.abs(np.minimum(dist(seed, XS), dist(-seed, XS)) - u)))
+    result = {
+        "metric": "solved cells/s of a manifold given by two charts (solve, pastes, resumed solves)",
+        "value": n * n * args.steps / wall, "unit": "solved cells/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"projective plane from the equator, {n}x{n}, Isotropic2, "
+                               "cost 2/(1+|x|^2), one seed, chart mapping x -> -x/|x|^2",
+                   "parallelism": "single GPU"},
+        "pastes_per_step": float(np.mean([s_['chart_pastes'] for s_ in stats])),
+        "passes_per_step": float(np.mean([s_['launches'] for s_ in stats])),
+        "max_error_vs_exact_distance": err,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "agd_chart_paste "
+                     "(chart_eval_kernel + chart_apply_kernel)", "bytes_per_cell": bpc,
+                     "avg_paste_us": paste_us},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        # the oracle (FP64, one thread): fast marching, then pastes and fast
+        # marching resumed from the pasted values, on a 501^2 sample
+        sys.path.insert(0, os.path.join(ROOT, 'oracle'))
+        import oracle as O
+        ns_ = 501
+        hs = 2 * r / ns_
+        xs = -r + (np.arange(ns_) + 0.5) * hs
+        Xs = np.stack(np.meshgrid(xs, xs, indexing='ij'))
+        e2s = np.sum(Xs ** 2, axis=0)
+        seed_idx = [[int(round((0.5 + r) / hs - 0.5)), int(round((0.7 + r) / hs - 0.5))]]
+        P = O.make_problem('Isotropic', (ns_, ns_), (hs, hs), cost=2. / (1. + e2s), seeds_idx=seed_idx)
+        with np.errstate(divide='ignore', invalid='ignore'):
+            cont = (-Xs / e2s + r) / hs - 0.5
+        t0 = time.perf_counter()
+        us = P.solve_fm()
+        npaste = 0
+        for _ in range(5):
+            new, low = O.chart_paste(us, cont)
+            npaste += 1
+            if not low.any():
+                break
+            us = P.solve_fm(u0=new)
+        dt = time.perf_counter() - t0
+        result["cpu_baseline"] = {"value": ns_ * ns_ / dt, "unit": "solved cells/s", "cores": 1, "kind": "port",
+                                  "sample": f"projective plane {ns_}x{ns_}: FP64 fast marching and {npaste} "
+                                            f"pastes with resumed fast marching, {dt:.2f} s",
+                                  "host": _host_info()}
     print(json.dumps(result), flush=True)
 
 
