@@ -202,3 +202,35 @@ def test_unoriented_seeds_and_tips(O):
     ti = int(round((0.6 + 1) / (2. / n) - 0.5)), int(round((0.3 + 1) / (2. / n) - 0.5))
     k = int(round(geo[2, 0] / (2 * np.pi / nt))) % nt
     assert u[ti[0], ti[1], k] == np.min(u[ti[0], ti[1], :])
+
+
+def test_curvature_adjoint_and_online_geodesics(O):
+    """Reverse AD through the forward offsets (costSensitivity against the
+    oracle's exact adjoint on the oracle's flow from the device values), and
+    geodesics with the flow recomputed online equal to those from the
+    exported flow."""
+    n, nt = 32, 24
+    wl = workloads.curvature2('ReedsSheppForward2', n=n, ntheta=nt, xi=0.4, seed=36)
+    rng = np.random.default_rng(1)
+    sens = np.zeros(wl.dims + (1,), np.float32)
+    for _ in range(12):
+        sens[tuple(rng.integers(0, [n, n, nt]))] = 1.
+    tips = [[0.5, 0.3, 0.2], [-0.4, 0.6, 3.0], [0.7, -0.5, 5.5]]
+    out = wl.dict_in(exportValues=True, sensitivity=sens, tips=tips).Run()
+    u = out['values'].cpu().numpy().astype(np.float64)
+    it = prepared(wl)
+    P, idx = oracle_from_device(O, it, wl)
+    fl = P.flow(u, use_float=True)
+    lam = out['costSensitivity'].cpu().numpy().astype(np.float64)[..., 0]
+    lam_ref = P.adjoint(u, fl, sens.astype(np.float64))[..., 0]
+    for s in idx:
+        lam_ref[tuple(s)] = 0.
+    err = np.max(np.abs(lam - lam_ref)) / np.max(np.abs(lam_ref))
+    online = wl.dict_in(exportValues=True, tips=tips, geodesic_online_flow=True).Run()
+    gaps = [np.max(np.abs(a - b)) if a.shape == b.shape else np.inf
+            for a, b in zip(out['geodesics'], online['geodesics'])]
+    log(f"rsf adjoint: rel err {err:.2e}; online geodesics: max gap {max(gaps):.2e}, "
+        f"stops {out['geodesic_stopping_criteria']} / {online['geodesic_stopping_criteria']}")
+    assert err <= 1e-3
+    assert max(gaps) <= 1e-5
+    assert out['geodesic_stopping_criteria'] == online['geodesic_stopping_criteria']
