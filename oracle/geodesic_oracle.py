@@ -16,7 +16,12 @@ Follows (paths relative to the AGD source tree):
     Stationnary and PastSeed histories (including the reference's indexing of
     the PastSeed history by a boolean, i.e. slot 0 or 1);
   * _GetGeodesics.py:86-131: launches of `typical_len` points until every
-    geodesic stopped or max_len points were produced.
+    geodesic stopped or max_len points were produced;
+  * charts (_GetGeodesics.py:66-71, Kernel_GeodesicODE.h:317-320, 406,
+    GeodesicODE_Opt.h:43-99): a step taken from a cell marked `chart_jump`
+    keeps the previous distance to the seeds for the tests, then the point's
+    last coordinates are replaced by the multilinear interpolation of the
+    chart mapping at their cell's corners (clamped to the grid).
 
 Arrays are cell-major (not tiled), FP64 arithmetic.
 """
@@ -132,6 +137,40 @@ def _normalized_flow(F, x, c, causality):
     return 0, flow / nrm
 
 
+EUCL_CHART = 254
+
+
+def _chart_jump(F, x):
+    """x with its last ndim_s coordinates mapped to the other chart."""
+    m = F.chart
+    ns = m.shape[0]
+    nb = F.d - ns
+    shape_s = m.shape[1:]
+    xs = x[nb:]
+    q = np.floor(xs).astype(np.int64)
+    r = xs - q
+    out = np.zeros(ns)
+    vals = []
+    for corner in itertools.product((0, 1), repeat=ns):
+        w = 1.
+        idx = []
+        for i, e in enumerate(corner):
+            w *= r[i] if e else 1. - r[i]
+            idx.append(int(np.clip(q[i] + e, 0, shape_s[i] - 1)))
+        v = m[(slice(None),) + tuple(idx)]
+        out += w * v
+        vals.append(v)
+    vals = np.array(vals)
+    var = float(np.sum(np.mean(vals ** 2, axis=0) - np.mean(vals, axis=0) ** 2))
+    if var >= F.chart_var:
+        idx = tuple(int(np.clip(q[i] + (1 if r[i] > 0.5 else 0), 0, shape_s[i] - 1)) for i in range(ns))
+        out = m[(slice(None),) + idx]
+    if np.all(np.isfinite(out)):
+        x = x.copy()
+        x[nb:] = out
+    return x
+
+
 def _ode(F, x0, max_len, step, causality, eucl_delay, nymin_delay, pastseed_delay):
     """One launch from x0: the points 1 .. len-1, the stop code."""
     hlen = 1 + max(eucl_delay, nymin_delay)
@@ -150,6 +189,9 @@ def _ode(F, x0, max_len, step, causality, eucl_delay, nymin_delay, pastseed_dela
             break
         nymin_p[l] = c.nymin
         eucl_p[l] = int(F.eucl[c.nymin])
+        chart = F.chart is not None and eucl_p[l] == EUCL_CHART
+        if chart:
+            eucl_p[l] = eucl_p[(l - 1 + hlen) % hlen]
         if c.nymin == nymin_p[(l - nymin_delay + hlen) % hlen]:
             stop = 3
             break
@@ -164,12 +206,14 @@ def _ode(F, x0, max_len, step, causality, eucl_delay, nymin_delay, pastseed_dela
         x = step * flow + xp
         pts.append(x.copy())
         ln += 1
+        if chart:
+            x = _chart_jump(F, x)
     return np.array(pts).reshape(-1, F.d), stop
 
 
 def geodesics(u, flow, wsum, seeds, tips, step=0.25, causality=4., eucl_delay=None,
               nymin_delay=None, max_len=None, typical_len=None, target_tol=6, pastseed_delay=0,
-              periodic=None):
+              periodic=None, chart_mapping=None, chart_jump=None, chart_var=np.inf):
     """Geodesics (continuous grid indices, (len, d) each) from `tips`
     (continuous indices) and their stopping criteria."""
     u = np.asarray(u, dtype=np.float64)
@@ -181,7 +225,12 @@ def geodesics(u, flow, wsum, seeds, tips, step=0.25, causality=4., eucl_delay=No
     typical_len = min(max_len, int(max(40, 0.5 * smax / step))) if typical_len is None else typical_len
     eucl = eucl_distance(np.asarray(seeds, bool), int(np.ceil(target_tol)), 5, int(5 * target_tol),
                          periodic)
+    if chart_mapping is not None:
+        eucl = np.where(eucl == EUCL_CHART, 255, eucl)
+        eucl = np.where(np.broadcast_to(np.asarray(chart_jump, bool), eucl.shape), EUCL_CHART, eucl)
     F = _Field(u, np.asarray(flow, dtype=np.float64), np.asarray(wsum, dtype=np.float64), eucl, periodic)
+    F.chart = None if chart_mapping is None else np.asarray(chart_mapping, dtype=np.float64)
+    F.chart_var = chart_var
     out, stops = [], []
     for tip in np.asarray(tips, dtype=np.float64).reshape(-1, d):
         parts = [tip.reshape(1, -1)]

@@ -199,3 +199,39 @@ def test_sphere_from_two_charts():
     err = np.max(np.abs(np.where(walls, 0., u - dist)))
     log(f"sphere from two charts: pastes {out['stats']['eikonal'].get('chart_pastes')}, max error {err:.3e}")
     assert err < 0.03
+
+
+def test_band_geodesics_match_oracle():
+    """Geodesics jumping across the glued band: the device against the
+    geodesic oracle's chart jump on the same values and flow (same stopping
+    criteria, lengths within 2 points, points within 2e-2 pixel)."""
+    import geodesic_oracle as G
+    from test_gpu_geodesic import oracle_inputs
+    r = 1.1
+    hfm = band_input(r=r)
+    del hfm['factoringRadius']
+    _, aY = hfm.Axes()
+    aY = np.asarray(aY, dtype=np.float64)
+    hfm['chart_mapping'] = np.expand_dims(np.where(aY > 0, aY - 2, aY + 2), axis=0)
+    hfm['chart_jump'] = np.abs(aY) > (1 + r) / 2
+    hfm['tips'] = np.array([[0.5, -0.8], [-0.6, -0.9], [0.2, 0.95], [0.9, -0.3]])
+    hfm['retself'] = True
+    out, iface = hfm.Run()
+    u, fv, ws, seeds = oracle_inputs(hfm, iface)
+    mapping = iface.chart_mapping_t.cpu().numpy().astype(np.float64)
+    jump = np.broadcast_to(np.asarray(hfm['chart_jump'], bool), u.shape)
+    tips = hfm.PointFromIndex(np.asarray(hfm['tips']).reshape(-1, 2), to=True)
+    geos_o, stops_o = G.geodesics(u, fv, ws, seeds, tips, chart_mapping=mapping, chart_jump=jump)
+    worst, dlen, jumps = 0., 0, 0
+    for geo, go, so, sg in zip(out['geodesics'], geos_o, stops_o, out['geodesic_stopping_criteria']):
+        gi = hfm.PointFromIndex(geo.T, to=True)
+        assert sg == so
+        m = min(len(gi), len(go))
+        dlen = max(dlen, abs(len(gi) - len(go)))
+        worst = max(worst, float(np.abs(gi[:m] - go[:m]).max()))
+        jumps += int(np.sum(np.abs(np.diff(gi[:, 1])) > 50))
+    log(f"band geodesics: stops {stops_o}, jumps {jumps}, max point difference {worst:.2e} px, "
+        f"max length difference {dlen}")
+    assert jumps >= 2
+    assert dlen <= 2
+    assert worst <= 2e-2

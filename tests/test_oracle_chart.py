@@ -145,3 +145,40 @@ def test_projective_plane_exact_distance(O):
     exact = np.minimum(dist(s, XS), dist(-s, XS))
     assert np.max(np.abs(u - exact)) < 0.02
     assert np.max(np.abs(u0 - exact)) > 0.3
+
+
+def test_geodesic_jumps_across_the_glued_band(O):
+    """The geodesic oracle's chart jump on the band glued by a whole-cell
+    shift: a tip whose shortest path crosses the seam reaches the seed, jumps
+    by the shift once, and has the length of the geodesic of the periodic
+    solve."""
+    import geodesic_oracle as G
+    n, band = 100, 5
+    P, cont, seed = chart_band(O, n, band)
+    u = P.solve_fm()
+    for _ in range(40):
+        new, low = O.chart_paste(u, cont)
+        if not low.any():
+            break
+        u = P.solve_fm(u0=new)
+    fl = P.flow(u)
+    seeds = np.zeros(u.shape, bool)
+    seeds[seed] = True
+    jump = np.zeros(u.shape, bool)
+    jump[:, :band - 2] = jump[:, n + band + 2:] = True      # deep in the bands
+    tip = np.array([[float(seed[0]), float(seed[1] - 60)]])   # closer across the seam
+    geos, stops = G.geodesics(u, fl['vector'], fl['weightsum'], seeds, tip, chart_mapping=cont,
+                              chart_jump=jump)
+    g = geos[0]
+    steps = np.abs(np.diff(g[:, 1]))
+    assert stops[0] == 'AtSeed'
+    assert np.sum(steps > n / 2) == 1                           # one jump, by the period
+    Pp = O.make_problem('Isotropic', (n, n), (2. / n, 2. / n), seeds_idx=[(seed[0], seed[1] - band)],
+                        periodic=[0, 1])
+    up = Pp.solve_fm()
+    flp = Pp.flow(up)
+    seeds_p = np.zeros(up.shape, bool)
+    seeds_p[seed[0], seed[1] - band] = True
+    gp, sp_ = G.geodesics(up, flp['vector'], flp['weightsum'], seeds_p, tip - [0, band], periodic=[0, 1])
+    assert sp_[0] == 'AtSeed'
+    assert abs(len(g) - len(gp[0])) <= 3
