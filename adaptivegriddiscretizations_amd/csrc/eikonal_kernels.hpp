@@ -347,8 +347,12 @@ enum { VAR_FAST = 0, VAR_GENERAL = 1, VAR_PRE = 2, VAR_BATCH = 3, VAR_ORDER2 = 4
 template <int NI> constexpr int tiles_per_block() { return NI < 64 ? 64 / NI : 1; }
 template <int NI> constexpr int block_lanes() { return NI < 64 ? 64 : NI; }
 
+// (Elastica2 asks for two resident waves per SIMD: its 30 offsets per cell
+// otherwise hold 326 registers and one wave; measured 42.6 -> 37.4 ms on
+// 192 x 192 x 64 despite 300 bytes of spills per lane)
 template <class S, int MODEL, int D, int T0, int T1, int T2, int VAR>
-__global__ void __launch_bounds__(block_lanes<T0 * T1 * T2>()) eik_update(EikArgs<S> a) {
+__global__ void __launch_bounds__(block_lanes<T0 * T1 * T2>(), MODEL == AGD_ELASTICA2 ? 2 : 1)
+    eik_update(EikArgs<S> a) {
   constexpr bool ORDER2 = VAR == VAR_ORDER2;
   constexpr bool GENERAL = VAR == VAR_GENERAL || ORDER2;
   constexpr bool WALLS = GENERAL;
@@ -541,7 +545,7 @@ __global__ void __launch_bounds__(block_lanes<T0 * T1 * T2>()) eik_update(EikArg
         S best = mix_min ? inf_v<S>() : -inf_v<S>();
 #pragma unroll 1
         for (int m = 0; m < MT::NMIX; ++m) {
-          S v[NS], w[NS], v2[NS];
+          S v[NS], w[NS], v2[MT::CURV ? 1 : NS];
 #pragma unroll
           for (int k = 0; k < NS; ++k) {
             w[k] = ld32(a.sw + int64_t(m * NS + k) * N, uint32_t(n_t));
@@ -550,7 +554,9 @@ __global__ void __launch_bounds__(block_lanes<T0 * T1 * T2>()) eik_update(EikArg
             for (int sd = 0; sd < 2; ++sd) {
               const int c = ld32(a.scode + int64_t(m * NJ + 2 * k + sd) * N, uint32_t(n_t));
               side[sd] = c <= -2 ? us[-2 - c] : (c >= 0 ? ld32(a.u, uint32_t(c)) : inf_v<S>());
-              if (fb >= 0) side[sd] += a.fac.tab[int64_t(m * NJ + 2 * k + sd) * nbox + fb];
+              // (the curvature models have no source factorization)
+              if constexpr (!MT::CURV)
+                if (fb >= 0) side[sd] += a.fac.tab[int64_t(m * NJ + 2 * k + sd) * nbox + fb];
             }
             const int sd = (side[0] > side[1]) ? 1 : 0;  // Greater(v,-) keeps the -e side on ties
             v[k] = side[sd];
@@ -559,14 +565,17 @@ __global__ void __launch_bounds__(block_lanes<T0 * T1 * T2>()) eik_update(EikArg
             // much smaller value of it would set the base of the quadratic
             // and cancel its FP32 digits
             if constexpr (MT::CURV) v[k] = w[k] > S(0) ? v[k] : inf_v<S>();
-            if (a.scode2) {
-              // the further neighbour on the chosen side (second order)
-              const int c2 = ld32(a.scode2 + int64_t(m * NJ + 2 * k + sd) * N, uint32_t(n_t));
-              v2[k] = c2 <= -2 ? us[-2 - c2] : (c2 >= 0 ? ld32(a.u, uint32_t(c2)) : inf_v<S>());
-              if (fb >= 0) v2[k] += a.fac.tab[int64_t(m * NJ + 2 * k + sd) * nbox + fb2];
+            if constexpr (!MT::CURV) {  // (the curvature models are first order)
+              if (a.scode2) {
+                // the further neighbour on the chosen side (second order)
+                const int c2 = ld32(a.scode2 + int64_t(m * NJ + 2 * k + sd) * N, uint32_t(n_t));
+                v2[k] = c2 <= -2 ? us[-2 - c2] : (c2 >= 0 ? ld32(a.u, uint32_t(c2)) : inf_v<S>());
+                if (fb >= 0) v2[k] += a.fac.tab[int64_t(m * NJ + 2 * k + sd) * nbox + fb2];
+              }
             }
           }
-          if (a.scode2) order2_correct<S, NS>(us[lt], a.order2_thr, v, v2, w);
+          if constexpr (!MT::CURV)
+            if (a.scode2) order2_correct<S, NS>(us[lt], a.order2_thr, v, v2, w);
           const S um = hfm_solve<S, NS>(v, w, rhs);
           if (mix_min == (best > um)) best = um;
         }

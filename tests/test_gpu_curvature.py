@@ -234,3 +234,32 @@ def test_curvature_adjoint_and_online_geodesics(O):
     assert err <= 1e-3
     assert max(gaps) <= 1e-5
     assert out['geodesic_stopping_criteria'] == online['geodesic_stopping_criteria']
+
+
+@pytest.mark.parametrize("model", ['ReedsShepp2', 'Dubins2'])
+def test_curvature_walls_match_oracle(O, model):
+    """A disk obstacle in the plane, broadcast over the angle
+    (_SetGeometry.py:265): the wall distance and visibility walks with the
+    periodic angle, against the oracle's fast marching on the same FP32
+    geometry."""
+    n, nt = 40, 24
+    wl = workloads.curvature2(model, n=n, ntheta=nt, xi=0.4, seed=37)
+    x = -1 + (np.arange(n) + 0.5) * (2. / n)
+    X, Y = np.meshgrid(x, x, indexing='ij')
+    disk = (X + 0.45) ** 2 + (Y - 0.35) ** 2 < 0.25 ** 2
+    out = wl.dict_in(exportValues=True, walls=disk).Run()
+    u = out['values'].cpu().numpy().astype(np.float64)
+    it = prepared(wl, walls=disk)
+    P0, _ = oracle_from_device(O, it, wl)
+    walls3 = np.broadcast_to(disk[:, :, None], wl.dims)
+    rhs = P0.rhs.reshape(wl.dims).copy()
+    seeds = P0.seeds.reshape(wl.dims).copy()
+    rhs[walls3] = np.inf
+    seeds[walls3] = 1
+    wd = O.wall_distance(walls3.astype(np.uint8), bound=10, periodic=[0, 0, 1])
+    P = O.Problem(model, wl.dims, P0.geom, rhs, seeds, [0., 0., 0.], wall_dist=wd, periodic=[0, 0, 1])
+    ref = P.solve_fm()
+    err = rel_err(u, ref)
+    fin = np.isfinite(ref)
+    log(f"{model} with a disk wall: finite {int(fin.sum())}/{ref.size}, rel err vs oracle FM {err:.2e}")
+    assert err <= TOL32
