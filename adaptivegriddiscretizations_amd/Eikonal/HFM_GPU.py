@@ -798,13 +798,20 @@ class Interface:
         m = (m - lo.reshape(shp)) / h.reshape(shp) - 0.5
         self.chart_ndim_s = ndim_s
         self.chart_mapping_t = device_tensor(np.ascontiguousarray(m), self.float_t)
+        # A paste lowers a value only by more than the solve's own tolerance:
+        # smaller decreases are the rounding of the interpolation, and each
+        # would cost a resumed solve (the reference pastes any decrease).
+        self.chart_atol, self.chart_rtol = (float(t) for t in self.GetValue(
+            'chart_tolerance', default=(self.atol, self.rtol),
+            help="(atol, rtol): decreases a paste between charts must exceed (0, 0: any decrease)"))
 
     def _chart_paste(self, scratch, trig, cnt):
         """One paste of the values between charts; the number of cells lowered."""
         trig.zero_()
         cnt.zero_()
         _lib.check(_lib.lib.agd_chart_paste(self._desc(), ptr(self.values_t), ptr(self.chart_mapping_t),
-                                            self.chart_ndim_s, ptr(scratch), ptr(trig), ptr(cnt), self.stream))
+                                            self.chart_ndim_s, self.chart_atol, self.chart_rtol, ptr(scratch),
+                                            ptr(trig), ptr(cnt), self.stream))
         return int(cnt.item())
 
     # ------------------------------------------------------------ solve

@@ -127,8 +127,8 @@ SIGNATURES = {
                                    c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "agd_geodesic_ode_online": (c_int32, [POINTER(EikonalDesc), POINTER(EikonalIO), POINTER(GeodesicDesc),
                                           c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
-    "agd_chart_paste": (c_int32, [POINTER(EikonalDesc), c_void_p, c_void_p, c_int32, c_void_p, c_void_p,
-                                  c_void_p, c_void_p]),
+    "agd_chart_paste": (c_int32, [POINTER(EikonalDesc), c_void_p, c_void_p, c_int32, c_double, c_double,
+                                  c_void_p, c_void_p, c_void_p, c_void_p]),
     "agd_block_expand": (c_int32, [c_int32, c_int64_p, c_int32_p, c_int32, c_int32, c_void_p,
                                    c_int64, c_void_p, c_double, c_void_p]),
     "agd_block_squeeze": (c_int32, [c_int32, c_int64_p, c_int32_p, c_int32, c_int32, c_void_p,

@@ -91,12 +91,13 @@ template <class S, int D> __global__ void chart_eval_kernel(ChartArgs<S> a) {
 
 // ChartPaste.h:151-165: lower the cells whose pasted value is smaller.
 template <class S>
-__global__ void chart_apply_kernel(int64_t size_tot, S* u, const S* cand, uint32_t* trigger, int32_t* count) {
+__global__ void chart_apply_kernel(int64_t size_tot, S* u, const S* cand, S atol, S rtol, uint32_t* trigger,
+                                   int32_t* count) {
   const int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   bool upd = false;
   if (n < size_tot) {
     const S c = cand[n];
-    if (c < u[n]) {  // false for NaN and +inf
+    if (c < u[n] - (atol + rtol * abs_(c))) {  // false for NaN and +inf
       u[n] = c;
       atomicOr(trigger + (n >> 5), 1u << (n & 31));
       upd = true;
@@ -107,8 +108,8 @@ __global__ void chart_apply_kernel(int64_t size_tot, S* u, const S* cand, uint32
 }
 
 template <class S>
-int chart_paste_impl(const agd_eikonal_desc* d, void* values, const void* mapping, int32_t ndim_s,
-                     void* scratch, uint32_t* trigger, int32_t* count, hipStream_t s) {
+int chart_paste_impl(const agd_eikonal_desc* d, void* values, const void* mapping, int32_t ndim_s, double atol,
+                     double rtol, void* scratch, uint32_t* trigger, int32_t* count, hipStream_t s) {
   ChartArgs<S> a{};
   a.g = geo_grid(d);
   a.g.periodic = 0;
@@ -122,7 +123,7 @@ int chart_paste_impl(const agd_eikonal_desc* d, void* values, const void* mappin
   if (d->ndim == 2) hipLaunchKernelGGL((chart_eval_kernel<S, 2>), grid, block, 0, s, a);
   else hipLaunchKernelGGL((chart_eval_kernel<S, 3>), grid, block, 0, s, a);
   hipLaunchKernelGGL(chart_apply_kernel<S>, grid, block, 0, s, a.g.size_tot, static_cast<S*>(values),
-                     static_cast<const S*>(scratch), trigger, count);
+                     static_cast<const S*>(scratch), S(atol), S(rtol), trigger, count);
   AGD_HIP_TRY(hipGetLastError());
   return AGD_OK;
 }
@@ -134,13 +135,17 @@ using namespace agd;
 extern "C" {
 
 int agd_chart_paste(const agd_eikonal_desc* desc, void* values, const void* mapping, int32_t ndim_s,
-                    void* scratch, uint32_t* trigger, int32_t* count, void* stream) {
+                    double atol, double rtol, void* scratch, uint32_t* trigger, int32_t* count,
+                    void* stream) {
   if (int st = check_geo_desc(desc)) return st;
   AGD_REQUIRE(ndim_s >= 1 && ndim_s <= desc->ndim, "ndim_s must be in [1, ndim]");
+  AGD_REQUIRE(atol >= 0. && rtol >= 0., "atol and rtol must be >= 0");
   AGD_REQUIRE(values && mapping && scratch && trigger && count, "missing buffers");
   if (desc->scalar == AGD_F32)
-    return chart_paste_impl<float>(desc, values, mapping, ndim_s, scratch, trigger, count, as_stream(stream));
-  return chart_paste_impl<double>(desc, values, mapping, ndim_s, scratch, trigger, count, as_stream(stream));
+    return chart_paste_impl<float>(desc, values, mapping, ndim_s, atol, rtol, scratch, trigger, count,
+                                   as_stream(stream));
+  return chart_paste_impl<double>(desc, values, mapping, ndim_s, atol, rtol, scratch, trigger, count,
+                                  as_stream(stream));
 }
 
 }  // extern "C"

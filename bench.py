@@ -459,7 +459,8 @@ def bench_chart(args):
         from adaptivegriddiscretizations_amd import _lib
         from adaptivegriddiscretizations_amd.Eikonal._device import ptr
         _lib.check(_lib.lib.agd_chart_paste(iface._desc(), ptr(iface.values_t), ptr(iface.chart_mapping_t),
-                                            iface.chart_ndim_s, ptr(scratch), ptr(trig), ptr(cnt), iface.stream))
+                                            iface.chart_ndim_s, 0., 0., ptr(scratch), ptr(trig), ptr(cnt),
+                                            iface.stream))
     ev1.record()
     torch.cuda.synchronize()
     paste_us = ev0.elapsed_time(ev1) * 1e3 / reps

@@ -392,7 +392,8 @@ int agd_geodesic_ode_online(const agd_eikonal_desc* desc, const agd_eikonal_io* 
  * multilinear interpolation over the 2^ndim_s surrounding cells; points within
  * 1e-4 of a cell of the last layer are moved onto it, points beyond and NaN
  * mappings are skipped; the axes are not wrapped (as the reference's paste).
- * Where the interpolated value is smaller the cell takes it, its trigger bit
+ * Where the interpolated value is smaller by more than atol + rtol |value|
+ * (0, 0: by any amount, as the reference) the cell takes it, its trigger bit
  * is set and *count (a device int32) is incremented.  Evaluation and update are
  * two passes, so the result does not depend on the order of the cells.
  * values: tiled, desc->scalar.  mapping: cell-major [ndim_s][n_s] in
@@ -401,7 +402,8 @@ int agd_geodesic_ode_online(const agd_eikonal_desc* desc, const agd_eikonal_io* 
  * (tiled bits) and *count are accumulated into, not cleared.  A solve with
  * io->trigger = trigger then continues from the pasted values. */
 int agd_chart_paste(const agd_eikonal_desc* desc, void* values, const void* mapping, int32_t ndim_s,
-                    void* scratch, uint32_t* trigger, int32_t* count, void* stream);
+                    double atol, double rtol, void* scratch, uint32_t* trigger, int32_t* count,
+                    void* stream);
 
 /* ------------------------------------------------------------ layout */
 /* block_expand (FiniteDifferences.py:164-195): src cell-major

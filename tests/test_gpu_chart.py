@@ -39,8 +39,8 @@ def tiled_paste(u, mapping, shape_i, scalar):
     scratch = empty((size_tot,), dt)
     trig = zeros(((size_tot + 31) // 32,), np.int32)
     cnt = zeros((1,), np.int32)
-    _lib.check(_lib.lib.agd_chart_paste(desc, ptr(u_t), ptr(m), mapping.shape[0], ptr(scratch), ptr(trig),
-                                        ptr(cnt), None))
+    _lib.check(_lib.lib.agd_chart_paste(desc, ptr(u_t), ptr(m), mapping.shape[0], 0., 0., ptr(scratch),
+                                        ptr(trig), ptr(cnt), None))
     out = empty(shape, dt)
     _lib.check(_lib.lib.agd_block_squeeze(d, _lib.i64(shape), _lib.i32(shape_i), 1, np.dtype(dt).itemsize,
                                           ptr(u_t), ptr(out), None))
