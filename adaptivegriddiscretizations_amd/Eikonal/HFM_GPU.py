@@ -60,9 +60,10 @@ def default_traits(model):
         # Reeds-Shepp models take 2 sweeps and FIM (front width 5) instead of
         # the reference's 6 sweeps: measured on 192 x 192 x 64, 4.91 -> 3.64 ms
         # (ReedsShepp2, AGSI before) and 4.17 -> 3.64 ms (ReedsSheppForward2),
-        # the same fixed point
+        # the same fixed point; Dubins2 takes FIM (front width 4) instead of
+        # AGSI: 6.44 -> 5.22 ms
         'ReedsShepp2': ((4, 4, 4), 2, 5), 'ReedsSheppForward2': ((4, 4, 4), 2, 5),
-        'Elastica2': ((4, 4, 4), 1, 4), 'Dubins2': ((4, 4, 4), 1, None),
+        'Elastica2': ((4, 4, 4), 1, 4), 'Dubins2': ((4, 4, 4), 1, 4),
     }
     if model not in table:
         raise NotImplementedError(

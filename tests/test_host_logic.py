@@ -250,7 +250,7 @@ def test_curvature_grid_geometry():
 def test_curvature_traits_and_refusals():
     from adaptivegriddiscretizations_amd import _lib
     from adaptivegriddiscretizations_amd.Eikonal import HFM_GPU
-    for model, fim in (('ReedsShepp2', 5), ('ReedsSheppForward2', 5), ('Elastica2', 4), ('Dubins2', None)):
+    for model, fim in (('ReedsShepp2', 5), ('ReedsSheppForward2', 5), ('Elastica2', 4), ('Dubins2', 4)):
         traits, f = HFM_GPU.default_traits(model)
         assert traits['shape_i'] == (4, 4, 4) and f == fim
         mid = HFM_GPU._MODELS[model[:-1]]
