@@ -27,13 +27,29 @@ template <class S> struct ChartArgs {
   S* out;            // interpolated value per tiled cell, +inf where none
 };
 
+// Position of tiled index n (< 2^31) in 32-bit arithmetic; false for the
+// padding of partial tiles.
+template <int D> __device__ inline bool chart_pos(uint32_t n, const GeoGrid& g, int* x) {
+  uint32_t n_o = n / uint32_t(g.size_i), n_i = n % uint32_t(g.size_i);
+  bool in = true;
+#pragma unroll
+  for (int k = D - 1; k >= 0; --k) {
+    const uint32_t so = uint32_t(g.shape_o[k]), si = uint32_t(g.shape_i[k]);
+    x[k] = int((n_o % so) * si + n_i % si);
+    n_o /= so;
+    n_i /= si;
+    in = in && x[k] < g.shape[k];
+  }
+  return in;
+}
+
 // ChartPaste.h:99-149 for the cell at tiled index n.
 template <class S, int D> __global__ void chart_eval_kernel(ChartArgs<S> a) {
   const int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (n >= a.g.size_tot) return;
   const S inf = inf_v<S>();
-  int x[3];
-  if (!geo_pos<D>(n, a.g, x)) {
+  int x[3] = {0, 0, 0};
+  if (!chart_pos<D>(uint32_t(n), a.g, x)) {
     a.out[n] = inf;
     return;
   }
@@ -138,6 +154,7 @@ int agd_chart_paste(const agd_eikonal_desc* desc, void* values, const void* mapp
                     double atol, double rtol, void* scratch, uint32_t* trigger, int32_t* count,
                     void* stream) {
   if (int st = check_geo_desc(desc)) return st;
+  AGD_REQUIRE(geo_grid(desc).size_tot < (int64_t(1) << 31), "grids of 2^31 tiled cells or more are not supported");
   AGD_REQUIRE(ndim_s >= 1 && ndim_s <= desc->ndim, "ndim_s must be in [1, ndim]");
   AGD_REQUIRE(atol >= 0. && rtol >= 0., "atol and rtol must be >= 0");
   AGD_REQUIRE(values && mapping && scratch && trigger && count, "missing buffers");
